@@ -1,0 +1,141 @@
+/*
+ * rrtmgsw.h -- C-ABI of the MI355X shortwave library (librrtmgsw_hip.so).
+ *
+ * Drop-in boundary for RRTMG_SW's GCM-mode McICA path. Every array is a plain pointer to
+ * column-major FP64 data in the reference's Fortran layout (the column index fastest), exactly
+ * the dummies of the reference routines each entry point replaces, in the same order; integer
+ * flags are int32 (kind_im). A Fortran host binds these through iso_c_binding
+ * (rrtmg_sw_amd/fortran/rrtmg_sw_hip.f90 keeps the reference's module and routine names).
+ *
+ * Return value: 0 on success, otherwise an error code; rrtmgsw_last_error() returns the
+ * reference's STOP message for the condition (e.g. "ICE RADIUS OUT OF BOUNDS"). Flags the
+ * reference clamps silently are clamped the same way and written back (icld, iaer, irng).
+ *
+ * Host-array entry points copy inputs to the GPU and outputs back (synchronous). The *_dev
+ * entry points take device pointers and a hipStream_t (passed as void*, NULL = default stream)
+ * and are asynchronous except for the final status check, which synchronizes the stream.
+ * All entry points are reentrant after rrtmgsw_init; calls on different streams may overlap.
+ */
+#ifndef RRTMGSW_H
+#define RRTMGSW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes */
+#define RRTMGSW_OK 0
+#define RRTMGSW_ERR_NOT_INITIALIZED 1
+#define RRTMGSW_ERR_ARGUMENT 2
+#define RRTMGSW_ERR_INFLAG 3       /* "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA" */
+#define RRTMGSW_ERR_RADIUS 4       /* ice / liquid particle size out of the table bounds */
+#define RRTMGSW_ERR_OPTICS 5       /* cloud optical property outside its physical limits */
+#define RRTMGSW_ERR_ICLD 10        /* "MCICA_SUBCOL: INVALID ICLD" */
+#define RRTMGSW_ERR_KISS_SEED 11   /* "... REQUIRES PMID FROM BOTTOM FOUR LAYERS." */
+#define RRTMGSW_ERR_HIP 20         /* a HIP runtime call failed */
+
+/* Precision forms of the device-resident entry points. */
+#define RRTMGSW_FP64 0
+#define RRTMGSW_MIXED 1
+
+/* Replaces rrtmg_sw_ini(cpdair), src/rrtmg_sw_init.f90:47-173: builds the 112-point k-tables
+ * (224 -> 112 reduction), the exponential lookup table and the cloud optics tables, and
+ * uploads them to the current GPU. Must precede every other call. */
+int rrtmgsw_init(double cpdair);
+
+/* Frees the device tables (no counterpart in the reference, whose tables are module state). */
+int rrtmgsw_finalize(void);
+
+/* Message for the last non-zero return on this thread. */
+const char *rrtmgsw_last_error(void);
+
+/* Replaces rrtmg_sw(...), src/rrtmg_sw_rad.f90:99-111 (dummies :220-388). bndsolvar (14),
+ * indsolvar (2) and solcycfrac (1) are the optional arguments: NULL when absent. */
+int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer,
+               const double *play, const double *plev, const double *tlay, const double *tlev,
+               const double *tsfc, const double *h2ovmr, const double *o3vmr,
+               const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+               const double *o2vmr, const double *asdir, const double *asdif,
+               const double *aldir, const double *aldif, const double *coszen, double adjes,
+               int dyofyr, double scon, int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+               const double *cldfmcl, const double *taucmcl, const double *ssacmcl,
+               const double *asmcmcl, const double *fsfcmcl, const double *ciwpmcl,
+               const double *clwpmcl, const double *reicmcl, const double *relqmcl,
+               const double *tauaer, const double *ssaaer, const double *asmaer,
+               const double *ecaer, double *swuflx, double *swdflx, double *swhr,
+               double *swuflxc, double *swdflxc, double *swhrc, const double *bndsolvar,
+               const double *indsolvar, const double *solcycfrac);
+
+/* Same as rrtmgsw_sw with device pointers (outputs and inputs resident in HBM). The scalar
+ * optional inputs bndsolvar / indsolvar / solcycfrac stay host pointers (NULL = absent). */
+int rrtmgsw_sw_dev(int ncol, int nlay, int *icld, int *iaer,
+                   const double *play, const double *plev, const double *tlay, const double *tlev,
+                   const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                   const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                   const double *o2vmr, const double *asdir, const double *asdif,
+                   const double *aldir, const double *aldif, const double *coszen, double adjes,
+                   int dyofyr, double scon, int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                   const double *cldfmcl, const double *taucmcl, const double *ssacmcl,
+                   const double *asmcmcl, const double *fsfcmcl, const double *ciwpmcl,
+                   const double *clwpmcl, const double *reicmcl, const double *relqmcl,
+                   const double *tauaer, const double *ssaaer, const double *asmaer,
+                   const double *ecaer, double *swuflx, double *swdflx, double *swhr,
+                   double *swuflxc, double *swdflxc, double *swhrc, const double *bndsolvar,
+                   const double *indsolvar, const double *solcycfrac, void *stream);
+
+/* Replaces mcica_subcol_sw(...), src/mcica_subcol_gen_sw.f90:181-185 (dummies :200-246).
+ * Writes the seven (112, ncol, nlay) sub-column arrays and reicmcl / relqmcl (ncol, nlay).
+ * irng is clamped (non-zero -> 1) and written back, as src/mcica_subcol_gen_sw.f90:450 does.
+ * The overlap order of icld 4/5 with irng = 0 follows the compiling single-column generator
+ * (src/mcica_subcol_gen_sw.1col.f90:593-600): per column, CDF then CDF2 for each layer. */
+int rrtmgsw_mcica_subcol(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
+                         const double *play, const double *cldfrac, const double *ciwp,
+                         const double *clwp, const double *rei, const double *rel,
+                         const double *tauc, const double *ssac, const double *asmc,
+                         const double *fsfc, const double *alpha, double *cldfmcl,
+                         double *ciwpmcl, double *clwpmcl, double *reicmcl, double *relqmcl,
+                         double *taucmcl, double *ssacmcl, double *asmcmcl, double *fsfcmcl);
+
+/* Same as rrtmgsw_mcica_subcol with device pointers. */
+int rrtmgsw_mcica_subcol_dev(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
+                             const double *play, const double *cldfrac, const double *ciwp,
+                             const double *clwp, const double *rei, const double *rel,
+                             const double *tauc, const double *ssac, const double *asmc,
+                             const double *fsfc, const double *alpha, double *cldfmcl,
+                             double *ciwpmcl, double *clwpmcl, double *reicmcl,
+                             double *relqmcl, double *taucmcl, double *ssacmcl,
+                             double *asmcmcl, double *fsfcmcl, void *stream);
+
+/* Replaces get_alpha(...), src/mcica_subcol_gen_sw.f90:66-67 (host arrays; host-side work in
+ * the reference's API as well). The reference's missing THEN at :147 is read as intended. */
+int rrtmgsw_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, double decorr_con,
+                      const double *dz, const double *lat, int juldat, const double *cldfrac,
+                      double *alpha);
+
+/* mcica_subcol_sw followed by rrtmg_sw on device pointers, without materializing the seven
+ * (112, ncol, nlay) sub-column arrays: the sub-column masks are generated inside the radiative
+ * transfer kernel from the same KISS / Mersenne-Twister streams (no counterpart in the
+ * reference, which always materializes them; the results are those of calling
+ * rrtmgsw_mcica_subcol_dev then rrtmgsw_sw_dev). icld is the mcica_subcol_sw overlap flag
+ * (0..5); precision is RRTMGSW_FP64 or RRTMGSW_MIXED. */
+int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                         const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc, const double *h2ovmr,
+                         const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                         const double *n2ovmr, const double *o2vmr, const double *asdir,
+                         const double *asdif, const double *aldir, const double *aldif,
+                         const double *coszen, double adjes, int dyofyr, double scon,
+                         int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                         const double *cldfrac, const double *ciwp, const double *clwp,
+                         const double *rei, const double *rel, const double *tauc,
+                         const double *ssac, const double *asmc, const double *fsfc,
+                         const double *alpha, const double *tauaer, const double *ssaaer,
+                         const double *asmaer, const double *ecaer, double *swuflx,
+                         double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
+                         double *swhrc, const double *bndsolvar, const double *indsolvar,
+                         const double *solcycfrac, int precision, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""ctypes binding of librrtmgsw_hip.so (the C-ABI declared in include/rrtmgsw.h).
+
+The library is built in-tree by rrtmg_sw_amd.build; importing this module without it raises
+immediately -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librrtmgsw_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+# Argument lists, in the order of the reference's Fortran dummies.
+SW_PROFILE = ["play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr",
+              "n2ovmr", "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen"]
+SW_MCICA = ["cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl",
+            "reicmcl", "relqmcl"]
+SW_AER = ["tauaer", "ssaaer", "asmaer", "ecaer"]
+SW_OUT = ["swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"]
+MC_IN = ["play", "cldfrac", "ciwp", "clwp", "rei", "rel", "tauc", "ssac", "asmc", "fsfc", "alpha"]
+MC_OUT = ["cldfmcl", "ciwpmcl", "clwpmcl", "reicmcl", "relqmcl", "taucmcl", "ssacmcl",
+          "asmcmcl", "fsfcmcl"]
+FUSED_CLD = ["cldfrac", "ciwp", "clwp", "rei", "rel", "tauc", "ssac", "asmc", "fsfc", "alpha"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rrtmg_sw_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.rrtmgsw_init.argtypes = [C.c_double]
+    L.rrtmgsw_finalize.argtypes = []
+    L.rrtmgsw_last_error.restype = C.c_char_p
+    sw = ([C.c_int, C.c_int, _ip, _ip] + [_dp] * 16 + [C.c_double, C.c_int, C.c_double, C.c_int,
+          C.c_int, C.c_int, C.c_int] + [_dp] * 9 + [_dp] * 4 + [_dp] * 6 + [_dp] * 3)
+    L.rrtmgsw_sw.argtypes = sw
+    L.rrtmgsw_sw_dev.argtypes = sw + [_vp]
+    mc = [C.c_int] * 5 + [_ip] + [_dp] * 11 + [_dp] * 9
+    L.rrtmgsw_mcica_subcol.argtypes = mc
+    L.rrtmgsw_mcica_subcol_dev.argtypes = mc + [_vp]
+    L.rrtmgsw_get_alpha.argtypes = ([C.c_int] * 5 + [C.c_double, _dp, _dp, C.c_int, _dp, _dp])
+    L.rrtmgsw_mcica_sw_dev.argtypes = ([C.c_int] * 5 + [_ip] + [_dp] * 16 +
+                                       [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                        C.c_int] + [_dp] * 10 + [_dp] * 4 + [_dp] * 6 + [_dp] * 3 +
+                                       [C.c_int, _vp])
+    _lib = L
+    return L
+
+
+EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_sw",
+            "rrtmgsw_sw_dev", "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
+            "rrtmgsw_get_alpha", "rrtmgsw_mcica_sw_dev"]
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().rrtmgsw_last_error().decode()
+        raise RRTMGSWError(rc, msg)
+
+
+class RRTMGSWError(RuntimeError):
+    """A non-zero return of the C-ABI: the reference would STOP with `message`."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message

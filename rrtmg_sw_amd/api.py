@@ -1,0 +1,182 @@
+"""Python mirror of the reference's GCM-mode interface, running on the MI355X library.
+
+Names, argument order and meaning follow the reference routines:
+  rrtmg_sw_ini(cpdair)                          src/rrtmg_sw_init.f90:47
+  rrtmg_sw(ncol, nlay, icld, iaer, play, ...)   src/rrtmg_sw_rad.f90:99-111
+  mcica_subcol_sw(iplon, ncol, nlay, icld, ...) src/mcica_subcol_gen_sw.f90:181-185
+  get_alpha(iplon, ncol, nlayers, icld, ...)    src/mcica_subcol_gen_sw.f90:66-67
+plus mcica_rrtmg_sw(...), both calls fused on the GPU without the (112, ncol, nlay) arrays.
+
+Arrays are either numpy arrays in the reference's Fortran layout (column-major, column index
+fastest; the host-array path copies them to the GPU) or torch device tensors holding the same
+memory layout, i.e. with the Fortran dimensions reversed (a (ncol, nlay) Fortran array is a
+contiguous (nlay, ncol) tensor). Outputs come back in the same form as the inputs.
+Errors raise RRTMGSWError carrying the reference's STOP message.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import RRTMGSWError, check  # noqa: F401
+
+NGPT, NBND, NAER = 112, 14, 6
+
+try:  # torch is plumbing for device memory and streams only
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+def _is_dev(x):
+    return torch is not None and isinstance(x, torch.Tensor) and x.is_cuda
+
+
+def _ptr(x, n=None):
+    dp = C.POINTER(C.c_double)
+    if x is None:
+        return None
+    if _is_dev(x):
+        if x.dtype != torch.float64 or not x.is_contiguous():
+            raise ValueError("device arrays must be contiguous float64 tensors")
+        if n is not None and x.numel() != n:
+            raise ValueError(f"device array has {x.numel()} elements, expected {n}")
+        return C.cast(C.c_void_p(x.data_ptr()), dp)
+    a = np.asarray(x)
+    if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim <= 1):
+        raise ValueError("host arrays must be Fortran-ordered float64")
+    if n is not None and a.size != n:
+        raise ValueError(f"host array has {a.size} elements, expected {n}")
+    return a.ctypes.data_as(dp)
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream) if torch is not None else None
+
+
+def _new(like_dev, fshape):
+    """Output array of Fortran shape `fshape` on the side of `like_dev`."""
+    if like_dev:
+        return torch.zeros(tuple(reversed(fshape)), dtype=torch.float64, device="cuda")
+    return np.zeros(fshape, dtype=np.float64, order="F")
+
+
+def to_device(a):
+    """numpy Fortran-ordered array -> torch device tensor with the same memory layout."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    return torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+
+
+def to_host(t, fshape=None):
+    """torch device tensor (reversed Fortran dims) -> numpy Fortran-ordered array."""
+    a = t.detach().cpu().numpy()
+    return np.asfortranarray(a.T) if fshape is None else np.asfortranarray(a.T.reshape(fshape))
+
+
+def rrtmg_sw_ini(cpdair=1004.64):
+    """Build and upload the 112-point tables (src/rrtmg_sw_init.f90:47)."""
+    check(_lib.lib().rrtmgsw_init(C.c_double(cpdair)))
+
+
+def rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr,
+             ch4vmr, n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon,
+             isolvar, inflgsw, iceflgsw, liqflgsw, cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl,
+             ciwpmcl, clwpmcl, reicmcl, relqmcl, tauaer, ssaaer, asmaer, ecaer,
+             bndsolvar=None, indsolvar=None, solcycfrac=None):
+    """rrtmg_sw (src/rrtmg_sw_rad.f90:99). Returns dict(swuflx, swdflx, swhr, swuflxc, swdflxc,
+    swhrc, icld, iaer) -- icld / iaer as clamped by the reference (intent(inout))."""
+    dev = _is_dev(play)
+    N, L, G = ncol, nlay, NGPT
+    out = {k: _new(dev, (N, L + 1)) for k in ("swuflx", "swdflx", "swuflxc", "swdflxc")}
+    out["swhr"] = _new(dev, (N, L))
+    out["swhrc"] = _new(dev, (N, L))
+    ic, ia = C.c_int(icld), C.c_int(iaer)
+    opt = [None if bndsolvar is None else np.ascontiguousarray(bndsolvar, dtype=np.float64),
+           None if indsolvar is None else np.ascontiguousarray(indsolvar, dtype=np.float64),
+           None if solcycfrac is None else np.array([solcycfrac], dtype=np.float64)]
+    cl = icld >= 1 and icld <= 3
+    args = [C.c_int(N), C.c_int(L), C.byref(ic), C.byref(ia),
+            _ptr(play, N * L), _ptr(plev, N * (L + 1)), _ptr(tlay, N * L), _ptr(tlev), _ptr(tsfc),
+            _ptr(h2ovmr, N * L), _ptr(o3vmr, N * L), _ptr(co2vmr, N * L), _ptr(ch4vmr, N * L),
+            _ptr(n2ovmr), _ptr(o2vmr, N * L), _ptr(asdir, N), _ptr(asdif, N), _ptr(aldir, N),
+            _ptr(aldif, N), _ptr(coszen, N), C.c_double(adjes), C.c_int(dyofyr), C.c_double(scon),
+            C.c_int(isolvar), C.c_int(inflgsw), C.c_int(iceflgsw), C.c_int(liqflgsw),
+            *[_ptr(x, G * N * L if cl else None) for x in (cldfmcl, taucmcl, ssacmcl, asmcmcl,
+                                                          fsfcmcl, ciwpmcl, clwpmcl)],
+            _ptr(reicmcl), _ptr(relqmcl), _ptr(tauaer), _ptr(ssaaer), _ptr(asmaer), _ptr(ecaer),
+            _ptr(out["swuflx"]), _ptr(out["swdflx"]), _ptr(out["swhr"]), _ptr(out["swuflxc"]),
+            _ptr(out["swdflxc"]), _ptr(out["swhrc"]), *[_ptr(o) for o in opt]]
+    if dev:
+        check(_lib.lib().rrtmgsw_sw_dev(*args, _stream()))
+    else:
+        check(_lib.lib().rrtmgsw_sw(*args))
+    out["icld"], out["iaer"] = ic.value, ia.value
+    return out
+
+
+def mcica_subcol_sw(iplon, ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei,
+                    rel, tauc, ssac, asmc, fsfc, alpha):
+    """mcica_subcol_sw (src/mcica_subcol_gen_sw.f90:181). Returns dict of the nine outputs
+    (cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl) and
+    irng as written back."""
+    dev = _is_dev(play)
+    N, L, G = ncol, nlay, NGPT
+    out = {k: _new(dev, (G, N, L)) for k in ("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl",
+                                              "ssacmcl", "asmcmcl", "fsfcmcl")}
+    out["reicmcl"] = _new(dev, (N, L))
+    out["relqmcl"] = _new(dev, (N, L))
+    ir = C.c_int(irng)
+    args = [C.c_int(iplon), C.c_int(N), C.c_int(L), C.c_int(icld), C.c_int(permuteseed),
+            C.byref(ir), _ptr(play, N * L), _ptr(cldfrac, N * L), _ptr(ciwp, N * L),
+            _ptr(clwp, N * L), _ptr(rei, N * L), _ptr(rel, N * L), _ptr(tauc, NBND * N * L),
+            _ptr(ssac, NBND * N * L), _ptr(asmc, NBND * N * L), _ptr(fsfc, NBND * N * L),
+            _ptr(alpha, N * L)] + [_ptr(out[k]) for k in _lib.MC_OUT]
+    if dev:
+        check(_lib.lib().rrtmgsw_mcica_subcol_dev(*args, _stream()))
+    else:
+        check(_lib.lib().rrtmgsw_mcica_subcol(*args))
+    out["irng"] = ir.value
+    return out
+
+
+def get_alpha(iplon, ncol, nlayers, icld, idcor, decorr_con, dz, lat, juldat, cldfrac):
+    """get_alpha (src/mcica_subcol_gen_sw.f90:66); host arrays."""
+    alpha = np.zeros((ncol, nlayers), order="F")
+    check(_lib.lib().rrtmgsw_get_alpha(iplon, ncol, nlayers, icld, idcor, C.c_double(decorr_con),
+                                       _ptr(dz), _ptr(np.ascontiguousarray(lat, dtype=np.float64)),
+                                       juldat, _ptr(cldfrac), _ptr(alpha)))
+    return alpha
+
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None):
+    """mcica_subcol_sw + rrtmg_sw fused on the GPU for the workload `cols` (a
+    workloads.Columns) whose arrays are already resident: `dev_arrays` maps input names to
+    device tensors (see device_inputs). Returns the output tensors."""
+    N, L = cols.ncol, cols.nlay
+    d = dev_arrays
+    if out is None:
+        out = {k: _new(True, (N, L + 1)) for k in ("swuflx", "swdflx", "swuflxc", "swdflxc")}
+        out["swhr"] = _new(True, (N, L))
+        out["swhrc"] = _new(True, (N, L))
+    ir = C.c_int(cols.irng)
+    args = [C.c_int(N), C.c_int(L), C.c_int(cols.icld), C.c_int(cols.iaer),
+            C.c_int(cols.permuteseed), C.byref(ir)]
+    args += [_ptr(d.get(k)) for k in _lib.SW_PROFILE]
+    args += [C.c_double(cols.adjes), C.c_int(cols.dyofyr), C.c_double(cols.scon),
+             C.c_int(cols.isolvar), C.c_int(cols.inflgsw), C.c_int(cols.iceflgsw),
+             C.c_int(cols.liqflgsw)]
+    args += [_ptr(d.get(k)) for k in _lib.FUSED_CLD]
+    args += [_ptr(d.get(k)) for k in _lib.SW_AER]
+    args += [_ptr(out[k]) for k in _lib.SW_OUT]
+    args += [None, None, None, C.c_int(PRECISION[precision]), _stream()]
+    check(_lib.lib().rrtmgsw_mcica_sw_dev(*args))
+    return out
+
+
+def device_inputs(cols):
+    """Copy every array input of `cols` to the GPU (reversed-dims tensors)."""
+    names = _lib.SW_PROFILE + _lib.FUSED_CLD + _lib.SW_AER
+    return {k: to_device(getattr(cols, k)) for k in names if getattr(cols, k) is not None}

@@ -1,0 +1,708 @@
+// rrtmgsw_api.cpp -- C-ABI entry points of librrtmgsw_hip.so (declared in include/rrtmgsw.h).
+//
+// Host orchestration only: flag handling and solar-variability scalars of the reference driver
+// (src/rrtmg_sw_rad.f90), device buffers and streams, KISS jump-ahead tables, error reporting.
+// All per-column, per-g-point work runs in rrtmgsw_kernels.hip; there is no CPU path.
+#include <algorithm>
+#include <cmath>
+#include <tuple>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rrtmgsw.h"
+#include "rrtmgsw_internal.h"
+#include "rrtmgsw_tables.h"
+
+using namespace rrtmgsw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return fail(RRTMGSW_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Scratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct Context {
+  bool ready = false;
+  int device = 0;
+  double heatfac = 0.0;
+  HostTables host;
+  DeviceTables dev{};
+  double *d_tab = nullptr, *d_exp = nullptr;
+  GPoint *d_gp = nullptr;
+  std::mutex mu;
+  std::map<hipStream_t, Scratch> scratch, status, mtbuf;
+  std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
+} ctx;
+
+// Grow-only per-stream device buffer.
+int stream_buffer(std::map<hipStream_t, Scratch> &m, hipStream_t s, size_t bytes, void **out) {
+  Scratch &b = m[s];
+  if (b.bytes < bytes) {
+    if (b.ptr) HIPCHK(hipFree(b.ptr));
+    b.ptr = nullptr;
+    HIPCHK(hipMalloc(&b.ptr, bytes));
+    b.bytes = bytes;
+  }
+  *out = b.ptr;
+  return 0;
+}
+
+// ---------------- KISS jump-ahead (host) ----------------
+// Sub-column g of a column starts its draws at position W + g*D of the column's stream
+// (W = permuteseed warm-up draws, D = draws per sub-column). Each KISS component has a
+// closed-form jump: the LCG s1 is affine, the xorshift s2 is linear over GF(2), and the
+// multiply-with-carry s3/s4 are multiplications by 2^-16 = a modulo a*2^16-1.
+struct Affine {
+  uint32_t a, c;
+};
+Affine affine_compose(Affine f, Affine g) {  // f(g(x))
+  return {f.a * g.a, f.a * g.c + f.c};
+}
+Affine affine_pow(Affine f, uint64_t n) {
+  Affine r = {1u, 0u};
+  while (n) {
+    if (n & 1) r = affine_compose(f, r);
+    f = affine_compose(f, f);
+    n >>= 1;
+  }
+  return r;
+}
+struct Gf2 {
+  uint32_t col[32];
+};
+uint32_t gf2_apply(const Gf2 &m, uint32_t x) {
+  uint32_t r = 0;
+  for (int i = 0; i < 32; i++)
+    if ((x >> i) & 1u) r ^= m.col[i];
+  return r;
+}
+Gf2 gf2_compose(const Gf2 &a, const Gf2 &b) {  // a(b(x))
+  Gf2 r;
+  for (int i = 0; i < 32; i++) r.col[i] = gf2_apply(a, b.col[i]);
+  return r;
+}
+Gf2 gf2_pow(Gf2 m, uint64_t n) {
+  Gf2 r;
+  for (int i = 0; i < 32; i++) r.col[i] = 1u << i;
+  while (n) {
+    if (n & 1) r = gf2_compose(m, r);
+    m = gf2_compose(m, m);
+    n >>= 1;
+  }
+  return r;
+}
+uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+  uint64_t r = 1 % m;
+  b %= m;
+  while (e) {
+    if (e & 1) r = (r * b) % m;
+    b = (b * b) % m;
+    e >>= 1;
+  }
+  return (uint32_t)r;
+}
+
+int kiss_jumps(int permuteseed, int nlay, int per_sub, hipStream_t s, const uint32_t **out) {
+  auto key = std::make_tuple(permuteseed, nlay, per_sub);
+  auto it = ctx.jumps.find(key);
+  if (it != ctx.jumps.end()) {
+    *out = it->second;
+    return 0;
+  }
+  Gf2 xs;
+  for (int i = 0; i < 32; i++) {
+    uint32_t k = 1u << i;
+    k ^= k << 13;
+    k ^= k >> 17;
+    k ^= k << 5;
+    xs.col[i] = k;
+  }
+  const Affine lcg = {69069u, 1327217885u};
+  std::vector<KissJump> J(NGPT);
+  uint64_t W = permuteseed > 0 ? (uint64_t)permuteseed : 0;
+  for (int g = 0; g < NGPT; g++) {
+    uint64_t n = W + (uint64_t)g * (uint64_t)per_sub;
+    Affine a = affine_pow(lcg, n);
+    Gf2 m = gf2_pow(xs, n);
+    J[g].a1 = a.a, J[g].c1 = a.c;
+    memcpy(J[g].m2, m.col, sizeof m.col);
+    J[g].p3 = powmod(18000, n, 1179647999ull);
+    J[g].p4 = powmod(30903, n, 2025259007ull);
+  }
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(KissJump) * NGPT));
+  HIPCHK(hipMemcpyAsync(d, J.data(), sizeof(KissJump) * NGPT, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  ctx.jumps[key] = d;
+  *out = d;
+  return 0;
+}
+
+// ---------------- solar variability and flux adjustment (host) ----------------
+// inatm_sw, src/rrtmg_sw_rad.f90:1226-1512; earth_sun :844-868.
+const double MGAVGCYC[134] = {
+    0.150737, 0.150746, 0.150733, 0.150718, 0.150725, 0.150762, 0.150828, 0.150918, 0.151017, 0.151113,
+    0.151201, 0.151292, 0.151403, 0.151557, 0.151766, 0.152023, 0.152322, 0.152646, 0.152969, 0.153277,
+    0.153579, 0.153899, 0.154252, 0.154651, 0.155104, 0.155608, 0.156144, 0.156681, 0.157178, 0.157605,
+    0.157971, 0.158320, 0.158702, 0.159133, 0.159583, 0.160018, 0.160408, 0.160725, 0.160960, 0.161131,
+    0.161280, 0.161454, 0.161701, 0.162034, 0.162411, 0.162801, 0.163186, 0.163545, 0.163844, 0.164029,
+    0.164054, 0.163910, 0.163621, 0.163239, 0.162842, 0.162525, 0.162344, 0.162275, 0.162288, 0.162369,
+    0.162500, 0.162671, 0.162878, 0.163091, 0.163251, 0.163320, 0.163287, 0.163153, 0.162927, 0.162630,
+    0.162328, 0.162083, 0.161906, 0.161766, 0.161622, 0.161458, 0.161266, 0.161014, 0.160666, 0.160213,
+    0.159690, 0.159190, 0.158831, 0.158664, 0.158634, 0.158605, 0.158460, 0.158152, 0.157691, 0.157152,
+    0.156631, 0.156180, 0.155827, 0.155575, 0.155406, 0.155280, 0.155145, 0.154972, 0.154762, 0.154554,
+    0.154388, 0.154267, 0.154152, 0.154002, 0.153800, 0.153567, 0.153348, 0.153175, 0.153044, 0.152923,
+    0.152793, 0.152652, 0.152510, 0.152384, 0.152282, 0.152194, 0.152099, 0.151980, 0.151844, 0.151706,
+    0.151585, 0.151496, 0.151437, 0.151390, 0.151347, 0.151295, 0.151220, 0.151115, 0.150993, 0.150883,
+    0.150802, 0.150752, 0.150729, 0.150737};
+const double SBAVGCYC[134] = {
+    50.3550, 44.1322, 52.0179, 59.2231, 66.3702, 71.7545, 76.8671, 83.4723, 91.1574, 98.4915,
+    105.3173, 115.1791, 130.9432, 155.0483, 186.5379, 221.5456, 256.9212, 291.5276, 325.2953, 356.4789,
+    387.2470, 422.8557, 466.1698, 521.5139, 593.2833, 676.6234, 763.6930, 849.1200, 928.4259, 994.9705,
+    1044.2605, 1087.5703, 1145.0623, 1224.3491, 1320.6497, 1413.0979, 1472.1591, 1485.7531, 1464.1610, 1439.1617,
+    1446.2449, 1496.4323, 1577.8394, 1669.5933, 1753.0408, 1821.9296, 1873.2789, 1906.5240, 1920.4482, 1904.6881,
+    1861.8397, 1802.7661, 1734.0215, 1665.0562, 1608.8999, 1584.8208, 1594.0162, 1616.1486, 1646.6031, 1687.1962,
+    1736.4778, 1787.2419, 1824.9084, 1835.5236, 1810.2161, 1768.6124, 1745.1085, 1748.7762, 1756.1239, 1738.9929,
+    1700.0656, 1658.2209, 1629.2925, 1620.9709, 1622.5157, 1623.4703, 1612.3083, 1577.3031, 1516.7953, 1430.0403,
+    1331.5112, 1255.5171, 1226.7653, 1241.4419, 1264.6549, 1255.5559, 1203.0286, 1120.2747, 1025.5101, 935.4602,
+    855.0434, 781.0189, 718.0328, 678.5850, 670.4219, 684.1906, 697.0376, 694.8083, 674.1456, 638.8199,
+    602.3454, 577.6292, 565.6213, 553.7846, 531.7452, 503.9732, 476.9708, 452.4296, 426.2826, 394.6636,
+    360.1086, 324.9731, 297.2957, 286.1536, 287.4195, 288.9029, 282.7594, 267.7211, 246.6594, 224.7318,
+    209.2318, 204.5217, 204.1653, 200.0440, 191.0689, 175.7699, 153.9869, 128.4389, 103.8445, 85.6083,
+    73.6264, 64.4393, 56.5779, 50.3550};
+
+void solar_scalars(double adjes, int dyofyr, double scon, int isolvar, const double *bndsolvar,
+                   const double *indsolvar, const double *solcycfrac, SwScalars &o) {
+  const double svar_f_avg = 0.1567652, svar_s_avg = 902.71260;
+  const double Iint = 1360.37, Fint = 0.996047, Sint = -0.511590;
+  const double Foffset = 0.14959542, Soffset = 0.00066696;
+  const double cyc_min = 0.0189, cyc_max = 0.3750, d_min2max = 0.3561, d_max2min = 0.6439;
+  const double rrsw_scon = 1.36822e+03;
+  const int nsolfrac = 134;
+  double solvar[NBND];
+  for (int b = 0; b < NBND; b++) {
+    solvar[b] = 1.0;
+    o.svar_bnd[0][b] = o.svar_bnd[1][b] = o.svar_bnd[2][b] = 1.0;
+  }
+  o.svar_f = o.svar_s = o.svar_i = 1.0;
+  double scl[2] = {1.0, 1.0}, ndx[2] = {svar_f_avg, svar_s_avg}, cycfr = 0.0;
+  if (isolvar == 1 && indsolvar && solcycfrac) {  // :1250-1276
+    cycfr = *solcycfrac;
+    if (indsolvar[0] != 1.0 || indsolvar[1] != 1.0) {
+      double f = *solcycfrac, w;
+      if (f >= 0.0 && f < cyc_min) {
+        w = (f + 1.0 - cyc_max) / d_max2min;
+        scl[0] = indsolvar[0] + w * (1.0 - indsolvar[0]);
+        scl[1] = indsolvar[1] + w * (1.0 - indsolvar[1]);
+      }
+      if (f >= cyc_min && f <= cyc_max) {
+        w = (f - cyc_min) / d_min2max;
+        scl[0] = 1.0 + w * (indsolvar[0] - 1.0);
+        scl[1] = 1.0 + w * (indsolvar[1] - 1.0);
+      }
+      if (f > cyc_max && f <= 1.0) {
+        w = (f - cyc_max) / d_max2min;
+        scl[0] = indsolvar[0] + w * (1.0 - indsolvar[0]);
+        scl[1] = indsolvar[1] + w * (1.0 - indsolvar[1]);
+      }
+    }
+  }
+  if (isolvar == 2 && indsolvar) ndx[0] = indsolvar[0], ndx[1] = indsolvar[1];  // :1279-1287
+  double adjflx = dyofyr > 0 ? 0.0 : adjes;                                      // :1291-1294
+  if (dyofyr > 0) {
+    double pi = 2.0 * asin(1.0), gamma = 2.0 * pi * (dyofyr - 1) / 365.0;
+    adjflx = 1.000110 + .034221 * cos(gamma) + .001289 * sin(gamma) + .000719 * cos(2.0 * gamma) +
+             .000077 * sin(2.0 * gamma);
+  }
+  double f0 = 0.0, s0 = 0.0;
+  if (isolvar == 1) {  // mean-cycle interpolation, :1323-1351 / :1411-1439
+    if (cycfr <= 0.0) {
+      f0 = MGAVGCYC[0], s0 = SBAVGCYC[0];
+    } else if (cycfr >= 1.0) {
+      f0 = MGAVGCYC[nsolfrac - 1], s0 = SBAVGCYC[nsolfrac - 1];
+    } else {
+      double len = 1.0 / (nsolfrac - 2), hf = 0.5 * len, lo = 0.0, hi = hf;
+      int sfid = 1;
+      if (cycfr > hf && cycfr < 1.0 - hf) {
+        sfid = (int)floor((cycfr - hf) * (nsolfrac - 2)) + 2;
+        lo = (sfid - 2) * len + hf;
+        hi = lo + len;
+      }
+      if (cycfr >= 1.0 - hf) sfid = nsolfrac - 1, lo = 1.0 - hf, hi = 1.0;
+      double fr = (cycfr - lo) / (hi - lo);
+      f0 = MGAVGCYC[sfid - 1] + fr * (MGAVGCYC[sfid] - MGAVGCYC[sfid - 1]);
+      s0 = SBAVGCYC[sfid - 1] + fr * (SBAVGCYC[sfid] - SBAVGCYC[sfid - 1]);
+    }
+  }
+  if (scon == 0.0) {  // :1301-1380
+    if (isolvar == -1 || isolvar == 3)
+      for (int b = 0; b < NBND; b++) solvar[b] = bndsolvar ? bndsolvar[b] : 1.0;
+    if (isolvar == 1) {
+      o.svar_f = scl[0] * (f0 - Foffset) / (svar_f_avg - Foffset);
+      o.svar_s = scl[1] * (s0 - Soffset) / (svar_s_avg - Soffset);
+    }
+    if (isolvar == 2) {
+      o.svar_f = (ndx[0] - Foffset) / (svar_f_avg - Foffset);
+      o.svar_s = (ndx[1] - Soffset) / (svar_s_avg - Soffset);
+    }
+  } else if (scon > 0.0) {  // :1384-1479
+    double cprim = Fint + Sint + Iint;
+    if (isolvar == -1)
+      for (int b = 0; b < NBND; b++) solvar[b] = bndsolvar ? bndsolvar[b] * scon / rrsw_scon : scon / rrsw_scon;
+    if (isolvar == 0) o.svar_f = o.svar_s = o.svar_i = scon / cprim;
+    if (isolvar == 1) {
+      o.svar_i = (scon - (scl[0] * Fint + scl[1] * Sint)) / Iint;
+      o.svar_f = scl[0] * (f0 - Foffset) / (svar_f_avg - Foffset);
+      o.svar_s = scl[1] * (s0 - Soffset) / (svar_s_avg - Soffset);
+    }
+    if (isolvar == 3)
+      for (int b = 0; b < NBND; b++) solvar[b] = bndsolvar ? bndsolvar[b] * scon / cprim : scon / cprim;
+  }
+  if (isolvar == 3)
+    for (int b = 0; b < NBND; b++) o.svar_bnd[0][b] = o.svar_bnd[1][b] = o.svar_bnd[2][b] = solvar[b];
+  for (int b = 0; b < NBND; b++) o.adjflux[b] = isolvar < 0 ? adjflx * solvar[b] : adjflx;
+  o.isolvar = isolvar;
+}
+
+__global__ void zero_int(int *p) { *p = 0; }
+
+int finish(hipStream_t s, int *d_status) {
+  int st = 0;
+  HIPCHK(hipMemcpyAsync(&st, d_status, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  switch (st) {
+    case 0: return 0;
+    case RRTMGSW_ERR_INFLAG: return fail(st, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+    case RRTMGSW_ERR_RADIUS: return fail(st, "CLOUD PARTICLE RADIUS OUT OF BOUNDS");
+    case RRTMGSW_ERR_OPTICS: return fail(st, "CLOUD OPTICAL PROPERTY OUT OF PHYSICAL LIMITS");
+    case RRTMGSW_ERR_KISS_SEED:
+      return fail(st, "MCICA_SUBCOL: KISSVEC SEED GENERATOR REQUIRES PMID FROM BOTTOM FOUR LAYERS.");
+    default: return fail(st, "kernel error");
+  }
+}
+
+// Columns per launch so that the per-column elimination coefficients stay within ~4 GiB.
+int chunk_columns(int ncol, int nlay) {
+  size_t per = (size_t)nlay * 8 * NGPT * sizeof(double);
+  size_t cap = (size_t)4 << 30;
+  size_t n = cap / per;
+  if (n < 64) n = 64;
+  return (int)std::min<size_t>((size_t)ncol, n);
+}
+
+struct SwCall {
+  int ncol, nlay, icld, iaer;
+  const double *play, *plev, *tlay, *h2ovmr, *o3vmr, *co2vmr, *ch4vmr, *o2vmr;
+  const double *asdir, *asdif, *aldir, *aldif, *coszen;
+  const double *cldfmcl, *taucmcl, *ssacmcl, *asmcmcl, *fsfcmcl, *ciwpmcl, *clwpmcl, *reicmcl, *relqmcl;
+  const double *tauaer, *ssaaer, *asmaer, *ecaer;
+  double *swuflx, *swdflx, *swhr, *swuflxc, *swdflxc, *swhrc;
+};
+
+int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s) {
+  int chunk = chunk_columns(base.ncol, base.nlay);
+  void *scr = nullptr, *st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * base.nlay * 8 * NGPT * sizeof(double), &scr)) return rc;
+    if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
+  }
+  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
+  RtArgs a = base;
+  a.scratch = (double *)scr;
+  a.status = (int *)st;
+  for (int c0 = 0; c0 < base.ncol; c0 += chunk) {
+    a.col0 = c0;
+    a.ncol_chunk = std::min(chunk, base.ncol - c0);
+    HIPCHK(launch_rt_fp64(a, tables, s));
+  }
+  return finish(s, (int *)st);
+}
+
+int check_init() {
+  if (!ctx.ready) return fail(RRTMGSW_ERR_NOT_INITIALIZED, "rrtmgsw_init has not been called");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rrtmgsw_last_error(void) { return g_err.c_str(); }
+
+int rrtmgsw_init(double cpdair) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  if (!(cpdair > 0.0)) return fail(RRTMGSW_ERR_ARGUMENT, "cpdair must be positive");
+  ctx.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);  // src/rrtmg_sw_init.f90:258
+  if (ctx.ready) return 0;
+  std::string err;
+  if (!build_tables(ctx.host, err)) return fail(RRTMGSW_ERR_ARGUMENT, err);
+  HIPCHK(hipGetDevice(&ctx.device));
+  const HostTables &H = ctx.host;
+  HIPCHK(hipMalloc(&ctx.d_tab, H.tab.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(ctx.d_tab, H.tab.data(), H.tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&ctx.d_exp, H.exp_tbl.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(ctx.d_exp, H.exp_tbl.data(), H.exp_tbl.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&ctx.d_gp, H.gp.size() * sizeof(GPoint)));
+  HIPCHK(hipMemcpy(ctx.d_gp, H.gp.data(), H.gp.size() * sizeof(GPoint), hipMemcpyHostToDevice));
+  DeviceTables &D = ctx.dev;
+  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.gp = ctx.d_gp;
+  D.extliq1 = H.extliq1, D.ssaliq1 = H.ssaliq1, D.asyliq1 = H.asyliq1;
+  D.extice2 = H.extice2, D.ssaice2 = H.ssaice2, D.asyice2 = H.asyice2;
+  D.extice3 = H.extice3, D.ssaice3 = H.ssaice3, D.asyice3 = H.asyice3, D.fdlice3 = H.fdlice3;
+  D.abari = H.abari, D.bbari = H.bbari, D.cbari = H.cbari, D.dbari = H.dbari;
+  D.ebari = H.ebari, D.fbari = H.fbari;
+  D.rsrtaua = H.rsrtaua, D.rsrpiza = H.rsrpiza, D.rsrasya = H.rsrasya;
+  D.kiss_jump = nullptr;
+  ctx.ready = true;
+  return 0;
+}
+
+int rrtmgsw_finalize(void) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  if (!ctx.ready) return 0;
+  hipFree(ctx.d_tab);
+  hipFree(ctx.d_exp);
+  hipFree(ctx.d_gp);
+  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf}) {
+    for (auto &kv : *m) hipFree(kv.second.ptr);
+    m->clear();
+  }
+  for (auto &kv : ctx.jumps) hipFree(kv.second);
+  ctx.jumps.clear();
+  ctx.ready = false;
+  return 0;
+}
+
+int rrtmgsw_sw_dev(int ncol, int nlay, int *icld, int *iaer, const double *play, const double *plev,
+                   const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                   const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                   const double *n2ovmr, const double *o2vmr, const double *asdir,
+                   const double *asdif, const double *aldir, const double *aldif,
+                   const double *coszen, double adjes, int dyofyr, double scon, int isolvar,
+                   int inflgsw, int iceflgsw, int liqflgsw, const double *cldfmcl,
+                   const double *taucmcl, const double *ssacmcl, const double *asmcmcl,
+                   const double *fsfcmcl, const double *ciwpmcl, const double *clwpmcl,
+                   const double *reicmcl, const double *relqmcl, const double *tauaer,
+                   const double *ssaaer, const double *asmaer, const double *ecaer, double *swuflx,
+                   double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc,
+                   const double *bndsolvar, const double *indsolvar, const double *solcycfrac,
+                   void *stream) {
+  (void)tlev, (void)tsfc, (void)n2ovmr;  // reach only unused setcoef locals / no taumol term
+  if (int rc = check_init()) return rc;
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  if (!icld || !iaer) return fail(RRTMGSW_ERR_ARGUMENT, "icld and iaer are required");
+  if (*icld < 0 || *icld > 3) *icld = 2;                    // src/rrtmg_sw_rad.f90:589
+  if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;  // :597
+  if (ncol == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  RtArgs a{};
+  a.ncol = ncol, a.nlay = nlay;
+  a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
+  a.co2vmr = co2vmr, a.ch4vmr = ch4vmr, a.o2vmr = o2vmr;
+  a.asdir = asdir, a.asdif = asdif, a.aldir = aldir, a.aldif = aldif, a.coszen = coszen;
+  a.cldfmcl = cldfmcl, a.taucmcl = taucmcl, a.ssacmcl = ssacmcl, a.asmcmcl = asmcmcl;
+  a.fsfcmcl = fsfcmcl, a.ciwpmcl = ciwpmcl, a.clwpmcl = clwpmcl, a.reicmcl = reicmcl, a.relqmcl = relqmcl;
+  a.tauaer = tauaer, a.ssaaer = ssaaer, a.asmaer = asmaer, a.ecaer = ecaer;
+  a.swuflx = swuflx, a.swdflx = swdflx, a.swhr = swhr, a.swuflxc = swuflxc, a.swdflxc = swdflxc;
+  a.swhrc = swhrc;
+  a.fused = 0;
+  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
+  a.sc.heatfac = ctx.heatfac;
+  a.sc.icld = *icld, a.sc.iaer = *iaer;
+  a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
+  if (*icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+  return run_rt(a, ctx.dev, s);
+}
+
+int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer, const double *play, const double *plev,
+               const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+               const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+               const double *n2ovmr, const double *o2vmr, const double *asdir, const double *asdif,
+               const double *aldir, const double *aldif, const double *coszen, double adjes,
+               int dyofyr, double scon, int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+               const double *cldfmcl, const double *taucmcl, const double *ssacmcl,
+               const double *asmcmcl, const double *fsfcmcl, const double *ciwpmcl,
+               const double *clwpmcl, const double *reicmcl, const double *relqmcl,
+               const double *tauaer, const double *ssaaer, const double *asmaer,
+               const double *ecaer, double *swuflx, double *swdflx, double *swhr,
+               double *swuflxc, double *swdflxc, double *swhrc, const double *bndsolvar,
+               const double *indsolvar, const double *solcycfrac) {
+  if (int rc = check_init()) return rc;
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  if (ncol == 0) {
+    if (*icld < 0 || *icld > 3) *icld = 2;
+    if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;
+    return 0;
+  }
+  const size_t C = ncol, L = nlay, G = NGPT;
+  struct In {
+    const double *h;
+    size_t n;
+  };
+  int ic = *icld, ia = *iaer;
+  bool clouds = ic >= 1 && ic <= 3;
+  In in[] = {{play, C * L}, {plev, C * (L + 1)}, {tlay, C * L}, {h2ovmr, C * L}, {o3vmr, C * L},
+             {co2vmr, C * L}, {ch4vmr, C * L}, {o2vmr, C * L}, {asdir, C}, {asdif, C}, {aldir, C},
+             {aldif, C}, {coszen, C},
+             {cldfmcl, clouds ? G * C * L : 0}, {taucmcl, clouds ? G * C * L : 0},
+             {ssacmcl, clouds ? G * C * L : 0}, {asmcmcl, clouds ? G * C * L : 0},
+             {fsfcmcl, clouds ? G * C * L : 0}, {ciwpmcl, clouds ? G * C * L : 0},
+             {clwpmcl, clouds ? G * C * L : 0}, {reicmcl, clouds ? C * L : 0},
+             {relqmcl, clouds ? C * L : 0},
+             {tauaer, ia == 10 ? C * L * NBND : 0}, {ssaaer, ia == 10 ? C * L * NBND : 0},
+             {asmaer, ia == 10 ? C * L * NBND : 0}, {ecaer, ia == 6 ? C * L * NAER : 0}};
+  const int NIN = sizeof in / sizeof *in;
+  size_t total = 0;
+  for (auto &x : in) total += x.n;
+  size_t nout = 4 * C * (L + 1) + 2 * C * L;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
+  const double *dp[NIN];
+  size_t off = 0;
+  for (int i = 0; i < NIN; i++) {
+    dp[i] = in[i].n ? d + off : nullptr;
+    if (in[i].n) {
+      if (!in[i].h) {
+        hipFree(d);
+        return fail(RRTMGSW_ERR_ARGUMENT, "a required input array is NULL");
+      }
+      hipError_t e = hipMemcpy(d + off, in[i].h, in[i].n * sizeof(double), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        hipFree(d);
+        return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+      }
+    }
+    off += in[i].n;
+  }
+  double *o = d + total;
+  double *ou = o, *od = o + C * (L + 1), *ouc = o + 2 * C * (L + 1), *odc = o + 3 * C * (L + 1);
+  double *oh = o + 4 * C * (L + 1), *ohc = oh + C * L;
+  int rc = rrtmgsw_sw_dev(ncol, nlay, icld, iaer, dp[0], dp[1], dp[2], nullptr, nullptr, dp[3], dp[4],
+                          dp[5], dp[6], nullptr, dp[7], dp[8], dp[9], dp[10], dp[11], dp[12], adjes,
+                          dyofyr, scon, isolvar, inflgsw, iceflgsw, liqflgsw, dp[13], dp[14], dp[15],
+                          dp[16], dp[17], dp[18], dp[19], dp[20], dp[21], dp[22], dp[23], dp[24], dp[25],
+                          ou, od, oh, ouc, odc, ohc, bndsolvar, indsolvar, solcycfrac, nullptr);
+  if (rc == 0) {
+    hipError_t e = hipMemcpy(swuflx, ou, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swdflx, od, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swuflxc, ouc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swdflxc, odc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swhr, oh, C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swhrc, ohc, C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+  }
+  hipFree(d);
+  return rc;
+}
+
+static int mcica_prepare(int ncol, int nlay, int icld, int permuteseed, int irng, hipStream_t s,
+                         const uint32_t **mt, const uint32_t **jumps) {
+  *mt = nullptr;
+  *jumps = nullptr;
+  int per_sub = icld == 3 ? 1 : (icld >= 4 ? 2 * nlay : nlay);
+  if (irng == 0) return kiss_jumps(permuteseed, nlay, per_sub, s, jumps);
+  long long ndraw = (long long)NGPT * ncol * (icld == 3 ? 1 : (icld >= 4 ? 2LL * nlay : nlay));
+  void *buf = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (int rc = stream_buffer(ctx.mtbuf, s, (size_t)ndraw * 4, &buf)) return rc;
+  }
+  HIPCHK(launch_mt_stream((uint32_t)permuteseed, ndraw, (uint32_t *)buf, s));
+  *mt = (const uint32_t *)buf;
+  return 0;
+}
+
+int rrtmgsw_mcica_subcol_dev(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
+                             const double *play, const double *cldfrac, const double *ciwp,
+                             const double *clwp, const double *rei, const double *rel,
+                             const double *tauc, const double *ssac, const double *asmc,
+                             const double *fsfc, const double *alpha, double *cldfmcl,
+                             double *ciwpmcl, double *clwpmcl, double *reicmcl, double *relqmcl,
+                             double *taucmcl, double *ssacmcl, double *asmcmcl, double *fsfcmcl,
+                             void *stream) {
+  (void)iplon;
+  if (int rc = check_init()) return rc;
+  if (icld == 0) return 0;  // src/mcica_subcol_gen_sw.f90:261
+  if (icld < 0 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "MCICA_SUBCOL: INVALID ICLD");
+  if (!irng) return fail(RRTMGSW_ERR_ARGUMENT, "irng is required");
+  if (*irng != 0) *irng = 1;  // :450
+  if (ncol <= 0) return 0;
+  if (*irng == 0 && nlay < 4) return fail(RRTMGSW_ERR_ARGUMENT, "KISS seeding needs at least four layers");
+  if (nlay > MAX_LAY) return fail(RRTMGSW_ERR_ARGUMENT, "nlay must not exceed 256");
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t *mt, *jumps;
+  if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, *irng, s, &mt, &jumps)) return rc;
+  void *st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
+  }
+  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
+  McicaArgs a{};
+  a.ncol = ncol, a.nlay = nlay, a.icld = icld, a.irng = *irng, a.permuteseed = permuteseed;
+  a.play = play, a.cldfrac = cldfrac, a.ciwp = ciwp, a.clwp = clwp, a.rei = rei, a.rel = rel;
+  a.tauc = tauc, a.ssac = ssac, a.asmc = asmc, a.fsfc = fsfc, a.alpha = alpha, a.mt_stream = mt;
+  a.cldfmcl = cldfmcl, a.ciwpmcl = ciwpmcl, a.clwpmcl = clwpmcl, a.reicmcl = reicmcl;
+  a.relqmcl = relqmcl, a.taucmcl = taucmcl, a.ssacmcl = ssacmcl, a.asmcmcl = asmcmcl, a.fsfcmcl = fsfcmcl;
+  a.status = (int *)st;
+  DeviceTables t = ctx.dev;
+  t.kiss_jump = jumps;
+  HIPCHK(launch_mcica(a, t, s));
+  return finish(s, (int *)st);
+}
+
+int rrtmgsw_mcica_subcol(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
+                         const double *play, const double *cldfrac, const double *ciwp,
+                         const double *clwp, const double *rei, const double *rel,
+                         const double *tauc, const double *ssac, const double *asmc,
+                         const double *fsfc, const double *alpha, double *cldfmcl,
+                         double *ciwpmcl, double *clwpmcl, double *reicmcl, double *relqmcl,
+                         double *taucmcl, double *ssacmcl, double *asmcmcl, double *fsfcmcl) {
+  if (int rc = check_init()) return rc;
+  if (icld == 0) return 0;
+  if (icld < 0 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "MCICA_SUBCOL: INVALID ICLD");
+  if (ncol <= 0) {
+    if (irng && *irng != 0) *irng = 1;
+    return 0;
+  }
+  const size_t C = ncol, L = nlay, G = NGPT;
+  const double *hin[] = {play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, alpha};
+  size_t nin[] = {C * L, C * L, C * L, C * L, C * L, C * L, NBND * C * L, NBND * C * L,
+                  NBND * C * L, NBND * C * L, C * L};
+  size_t total = 0;
+  for (size_t n : nin) total += n;
+  size_t nout = 7 * G * C * L + 2 * C * L;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
+  const double *dp[11];
+  size_t off = 0;
+  for (int i = 0; i < 11; i++) {
+    dp[i] = d + off;
+    hipError_t e = hipMemcpy(d + off, hin[i], nin[i] * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(d);
+      return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+    }
+    off += nin[i];
+  }
+  double *o = d + total;
+  double *oc = o, *oi = o + G * C * L, *ol = o + 2 * G * C * L, *ot = o + 3 * G * C * L;
+  double *os = o + 4 * G * C * L, *oa = o + 5 * G * C * L, *of = o + 6 * G * C * L;
+  double *ore = o + 7 * G * C * L, *orl = ore + C * L;
+  int rc = rrtmgsw_mcica_subcol_dev(iplon, ncol, nlay, icld, permuteseed, irng, dp[0], dp[1], dp[2],
+                                    dp[3], dp[4], dp[5], dp[6], dp[7], dp[8], dp[9], dp[10], oc, oi, ol,
+                                    ore, orl, ot, os, oa, of, nullptr);
+  if (rc == 0) {
+    double *hout[] = {cldfmcl, ciwpmcl, clwpmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl};
+    double *dout[] = {oc, oi, ol, ot, os, oa, of};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 7 && e == hipSuccess; i++)
+      e = hipMemcpy(hout[i], dout[i], G * C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(reicmcl, ore, C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(relqmcl, orl, C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+  }
+  hipFree(d);
+  return rc;
+}
+
+int rrtmgsw_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, double decorr_con,
+                      const double *dz, const double *lat, int juldat, const double *cldfrac,
+                      double *alpha) {
+  (void)iplon;
+  // src/mcica_subcol_gen_sw.f90:110-176 (default-real constants as double, the -r8 build)
+  const double am1 = 1.4315, am2 = 2.1219, am4 = -25.584, amr = 7.0;
+  if (!(icld == 4 || icld == 5)) return 0;
+  for (int i = 0; i < ncol; i++) {
+    double len = decorr_con;
+    if (idcor == 1) {
+      double am3 = juldat > 181 ? -4. * amr / 365. * (juldat - 272) : 4. * amr / 365. * (juldat - 91);
+      double d = lat[i] - am3;
+      len = (am1 + am2 * exp(-(d * d) / (am4 * am4))) * 1.e3;
+    }
+    double inv = len >= 0.0 ? 1.0 / len : 1.0;
+    alpha[i] = 0.0;
+    for (int k = 1; k < nlayers; k++) {
+      size_t q = i + (size_t)ncol * k, qm = i + (size_t)ncol * (k - 1);
+      alpha[q] = exp(-(0.5 * (dz[q] + dz[qm])) * inv);
+      if (icld == 5 && cldfrac[q] == 0.0 && cldfrac[qm] > 0.0) alpha[q] = 0.0;
+    }
+  }
+  return 0;
+}
+
+int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                         const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc, const double *h2ovmr,
+                         const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                         const double *n2ovmr, const double *o2vmr, const double *asdir,
+                         const double *asdif, const double *aldir, const double *aldif,
+                         const double *coszen, double adjes, int dyofyr, double scon,
+                         int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                         const double *cldfrac, const double *ciwp, const double *clwp,
+                         const double *rei, const double *rel, const double *tauc,
+                         const double *ssac, const double *asmc, const double *fsfc,
+                         const double *alpha, const double *tauaer, const double *ssaaer,
+                         const double *asmaer, const double *ecaer, double *swuflx,
+                         double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
+                         double *swhrc, const double *bndsolvar, const double *indsolvar,
+                         const double *solcycfrac, int precision, void *stream) {
+  (void)tlev, (void)tsfc, (void)n2ovmr;
+  if (int rc = check_init()) return rc;
+  if (precision != RRTMGSW_FP64) return fail(RRTMGSW_ERR_ARGUMENT, "precision form not available");
+  if (icld < 0 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "MCICA_SUBCOL: INVALID ICLD");
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  int ir = irng ? *irng : 0;
+  if (ir != 0) ir = 1;
+  if (irng) *irng = ir;
+  if (iaer != 0 && iaer != 6 && iaer != 10) iaer = 0;
+  if (ncol == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  RtArgs a{};
+  a.ncol = ncol, a.nlay = nlay;
+  a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
+  a.co2vmr = co2vmr, a.ch4vmr = ch4vmr, a.o2vmr = o2vmr;
+  a.asdir = asdir, a.asdif = asdif, a.aldir = aldir, a.aldif = aldif, a.coszen = coszen;
+  a.tauaer = tauaer, a.ssaaer = ssaaer, a.asmaer = asmaer, a.ecaer = ecaer;
+  a.swuflx = swuflx, a.swdflx = swdflx, a.swhr = swhr, a.swuflxc = swuflxc, a.swdflxc = swdflxc;
+  a.swhrc = swhrc;
+  a.fused = 1, a.mc_icld = icld, a.mc_irng = ir, a.permuteseed = permuteseed;
+  a.cldfrac = cldfrac, a.ciwp = ciwp, a.clwp = clwp, a.rei = rei, a.rel = rel;
+  a.tauc = tauc, a.ssac = ssac, a.asmc = asmc, a.fsfc = fsfc, a.alpha = alpha;
+  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
+  a.sc.heatfac = ctx.heatfac;
+  a.sc.icld = icld >= 1 ? 1 : 0, a.sc.iaer = iaer;
+  a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
+  if (icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+  const uint32_t *mt = nullptr, *jumps = nullptr;
+  if (icld >= 1)
+    if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, ir, s, &mt, &jumps)) return rc;
+  a.mt_stream = mt;
+  DeviceTables t = ctx.dev;
+  t.kiss_jump = jumps;
+  return run_rt(a, t, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// rrtmgsw_internal.h -- shared host/device definitions of the MI355X shortwave library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rrtmgsw {
+
+constexpr int NGPT = 112;      // reduced g-points (modules/parrrsw.f90:36)
+constexpr int NBND = 14;       // shortwave bands 16..29
+constexpr int NAER = 6;        // ECMWF aerosol types (modules/parrrsw.f90:31)
+constexpr int WAVE_G = 56;     // g-points per wavefront: wave 0 = bands 16-22, wave 1 = bands 23-29
+constexpr int MAX_LAY = 256;   // layer limit of the per-column LDS staging
+constexpr int NTBL = 10000;    // exp lookup table, modules/rrsw_tbl.f90:28
+constexpr double PADE = 0.278;
+constexpr double OD_LO = 0.06;
+
+// Key-species interpolation modes of the unified gas-optics template (one record per g-point
+// replaces the fourteen taumolNN routines of src/rrtmg_sw_taumol.f90).
+enum KeyMode : int8_t { KEY_NONE = 0, KEY_BIN = 1, KEY_MONO = 2 };
+// Species columns staged per layer: index into ColumnLds::col.
+enum Species : int8_t { SP_H2O = 0, SP_CO2 = 1, SP_O3 = 2, SP_CH4 = 3, SP_O2 = 4, SP_NONE = -1 };
+
+// Per-g-point parameters. Offsets are element offsets into the device table (doubles).
+// Table layouts are transposed so that lanes of one band read consecutive addresses:
+// absa (ind, ig) at absa + (ind-1)*ng + ig, and likewise for absb, selfref, forref, rayla,
+// and the solar tables (js, ig).
+struct GPoint {
+  int8_t band, ibnd, ig, ng;
+  int8_t lo_mode, lo_s1, lo_s2, lo_cont;   // lower atmosphere key species and continuum
+  int8_t hi_mode, hi_s1, hi_s2, hi_for;    // upper atmosphere
+  int8_t lo_minor, hi_minor;               // minor absorber species (SP_NONE if none)
+  int8_t rayl_js;                          // 1: lower Rayleigh interpolated in js (band 24)
+  int8_t sol_interp;                       // 1: solar source interpolated in js
+  int8_t sol_upper;                        // 1: laysolfr found in the upper loop
+  int8_t layreffr;                         // reference-layer index for laysolfr (0: none)
+  int8_t lo_nsp, hi_nsp;                   // binary-species steps (9 or 1 / 5 or 1)
+  int8_t nfor;
+  int8_t sa, sb, sself, sfor, ssf, sra;    // row strides of the tables (0: term unused -> zero pad)
+  int8_t pad[3];
+  double lo_strrat, lo_mult, hi_strrat, hi_mult;
+  double lo_minor_coef, hi_minor_coef;
+  double rayl_lo, rayl_hi, sol_scale;
+  int32_t absa, absb, selfref, forref, rayla;
+  int32_t sflux, irrad, facb, snsp;        // solar source tables (ng, nsf) transposed
+  int32_t nsf;
+  int32_t zero;                            // offset of a zero pad (targets of unused terms)
+};
+
+// Device copies of all read-only tables (built once by rrtmgsw_init).
+struct DeviceTables {
+  const double *tab;        // k-tables, solar tables, cloud optics
+  const double *exp_tbl;    // NTBL+1 entries
+  const GPoint *gp;         // NGPT records
+  // cloud optics offsets (tables (n, 16:29) column-major: value(i, band) at off + (i-1) + n*(band-16))
+  int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;
+  int32_t abari, bbari, cbari, dbari, ebari, fbari;
+  int32_t rsrtaua, rsrpiza, rsrasya;   // (14, 6)
+  const uint32_t *kiss_jump;           // per-call KISS jump-ahead records (see KissJump)
+};
+
+// Jump-ahead of one KISS stream to the first draw of sub-column g:
+// s1 -> a1*s1 + c1, s2 -> M*s2 over GF(2) (columns m2[0..31]), s3 -> s3*p3 mod m3,
+// s4 -> s4*p4 mod m4 (the multiply-with-carry components are multiplications by 2^-16 modulo
+// 18000*2^16-1 and 30903*2^16-1; seeds below those moduli stay below them).
+struct KissJump {
+  uint32_t a1, c1, p3, p4;
+  uint32_t m2[32];
+};
+
+// Scalar inputs of one rrtmg_sw call, prepared on the host (inatm_sw's solar variability and
+// flux adjustment, src/rrtmg_sw_rad.f90:1226-1512, and the driver's flag handling).
+struct SwScalars {
+  double adjflux[NBND];
+  double svar_f, svar_s, svar_i;
+  double svar_bnd[3][NBND];  // f, s, i by band
+  double heatfac;
+  int isolvar;
+  int icld;                  // after the driver's clamp (0..3)
+  int iaer;                  // 0, 6 or 10
+  int inflag, iceflag, liqflag;
+};
+
+// Arguments of the radiative-transfer kernel.
+struct RtArgs {
+  int ncol, nlay, col0, ncol_chunk;
+  // profiles (ncol, nlay) / (ncol, nlay+1) / (ncol)
+  const double *play, *plev, *tlay, *h2ovmr, *o3vmr, *co2vmr, *ch4vmr, *o2vmr;
+  const double *asdir, *asdif, *aldir, *aldif, *coszen;
+  // materialized McICA inputs (112, ncol, nlay) -- ABI path
+  const double *cldfmcl, *taucmcl, *ssacmcl, *asmcmcl, *fsfcmcl, *ciwpmcl, *clwpmcl;
+  const double *reicmcl, *relqmcl;
+  // fused path: physical cloud inputs of mcica_subcol_sw
+  int fused, mc_icld, mc_irng, permuteseed;
+  const double *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc, *ssac, *asmc, *fsfc, *alpha;
+  const uint32_t *mt_stream;     // Mersenne-Twister draws for irng = 1 (fused path)
+  // aerosol
+  const double *tauaer, *ssaaer, *asmaer, *ecaer;
+  // outputs (ncol, nlay+1), (ncol, nlay)
+  double *swuflx, *swdflx, *swhr, *swuflxc, *swdflxc, *swhrc;
+  double *scratch;               // per (column, layer, coefficient, g) elimination coefficients
+  int *status;                   // first error code raised inside a kernel
+  SwScalars sc;
+};
+
+struct McicaArgs {
+  int ncol, nlay, icld, irng, permuteseed;
+  const double *play, *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc, *ssac, *asmc, *fsfc, *alpha;
+  const uint32_t *mt_stream;
+  double *cldfmcl, *ciwpmcl, *clwpmcl, *reicmcl, *relqmcl, *taucmcl, *ssacmcl, *asmcmcl, *fsfcmcl;
+  int *status;
+};
+
+// Kernel launchers (rrtmgsw_kernels.hip)
+hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, hipStream_t s);
+
+}  // namespace rrtmgsw

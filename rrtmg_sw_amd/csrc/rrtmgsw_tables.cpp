@@ -1,0 +1,307 @@
+// rrtmgsw_tables.cpp -- host-side table construction for rrtmgsw_init.
+//
+// Replaces the initialisation half of src/rrtmg_sw_init.f90 (rrtmg_sw_ini :47-173, swcmbdat
+// :263-386, cmbgb16s..cmbgb29 :492-1689): reads the raw 16-point-per-band data (embedded blob,
+// produced from src/rrtmg_sw_k_g.f90 and swcldpr/swaerpr by rrtmg_sw_amd/data/extract_tables.py),
+// reduces it to 112 g-points, builds the exponential lookup table, and lays every table out for
+// the GPU (g fastest, so that the lanes of one band read consecutive words). It also builds the
+// per-g-point records of the unified gas-optics template that replaces the fourteen
+// taumolNN routines of src/rrtmg_sw_taumol.f90.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rrtmgsw_internal.h"
+#include "rrtmgsw_tables.h"
+
+extern "C" const unsigned char rrtmgsw_blob_start[];
+extern "C" const unsigned char rrtmgsw_blob_end[];
+
+namespace rrtmgsw {
+
+namespace {
+
+struct Rec {
+  int ndim, lb[4], ext[4];
+  const double *data;
+  int count() const {
+    int n = 1;
+    for (int d = 0; d < ndim; d++) n *= ext[d];
+    return n;
+  }
+};
+
+// g-point reduction bookkeeping, src/rrtmg_sw_init.f90:286-384 (swcmbdat).
+const int NGC[NBND] = {6, 12, 8, 8, 10, 10, 2, 10, 8, 6, 6, 8, 6, 12};
+const int NGM[NBND * 16] = {
+    1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6,  1, 2, 3, 4, 5, 6, 6, 7, 8, 8, 9, 10, 10, 11, 12, 12,
+    1, 2, 3, 4, 5, 5, 6, 6, 7, 7, 7, 7, 8, 8, 8, 8,  1, 2, 3, 4, 5, 5, 6, 6, 7, 7, 7, 7, 8, 8, 8, 8,
+    1, 2, 3, 4, 5, 6, 7, 8, 9, 9, 10, 10, 10, 10, 10, 10, 1, 2, 3, 4, 5, 6, 7, 8, 9, 9, 10, 10, 10, 10, 10, 10,
+    1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2,  1, 1, 2, 2, 3, 4, 5, 6, 7, 8, 9, 9, 10, 10, 10, 10,
+    1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 8, 8,  1, 2, 3, 3, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 6, 6,
+    1, 2, 3, 3, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 6, 6,  1, 2, 3, 4, 5, 6, 7, 7, 7, 7, 8, 8, 8, 8, 8, 8,
+    1, 2, 3, 3, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 6, 6,  1, 2, 3, 4, 5, 5, 6, 6, 7, 7, 8, 8, 9, 10, 11, 12};
+// number of original g-points merged into each reduced one (ngn), derived from NGM.
+const double WT[16] = {0.1527534276, 0.1491729617, 0.1420961469, 0.1316886544,
+                       0.1181945205, 0.1019300893, 0.0832767040, 0.0626720116,
+                       0.0424925000, 0.0046269894, 0.0038279891, 0.0030260086,
+                       0.0022199750, 0.0014140010, 0.0005330000, 0.0000750000};
+
+std::map<std::string, Rec> parse_blob() {
+  std::map<std::string, Rec> m;
+  const unsigned char *p = rrtmgsw_blob_start;
+  if (memcmp(p, "RRTMGSWT", 8)) return m;
+  uint32_t nrec;
+  memcpy(&nrec, p + 12, 4);
+  const unsigned char *h = p + 16;
+  const double *data = reinterpret_cast<const double *>(p + 16 + nrec * (32 + 36 + 8));
+  for (uint32_t i = 0; i < nrec; i++, h += 32 + 36 + 8) {
+    Rec r;
+    char name[33] = {0};
+    memcpy(name, h, 32);
+    int32_t hdr[9];
+    memcpy(hdr, h + 32, 36);
+    int64_t off;
+    memcpy(&off, h + 68, 8);
+    r.ndim = hdr[0];
+    for (int d = 0; d < 4; d++) r.lb[d] = hdr[1 + d], r.ext[d] = hdr[5 + d];
+    r.data = data + off;
+    m[name] = r;
+  }
+  return m;
+}
+
+}  // namespace
+
+// Weights of the original g-points inside their merged reduced g-point:
+// rwgt(ind) = wt(ig) / sum of wt over the merged group (src/rrtmg_sw_init.f90:130-154).
+static void reduction_weights(double rwgt[NBND * 16], int ngn[NGPT]) {
+  int gi = 0;
+  for (int b = 0; b < NBND; b++) {
+    double wsum[16] = {0};
+    for (int ig = 0; ig < 16; ig++) wsum[NGM[b * 16 + ig] - 1] += WT[ig];
+    for (int ig = 0; ig < 16; ig++) rwgt[b * 16 + ig] = WT[ig] / wsum[NGM[b * 16 + ig] - 1];
+    for (int k = 0; k < NGC[b]; k++) {
+      int n = 0;
+      for (int ig = 0; ig < 16; ig++) n += NGM[b * 16 + ig] == k + 1;
+      ngn[gi++] = n;
+    }
+  }
+}
+
+// Merge the 16-point dimension of a raw array. `gfirst`: the g index is the first dimension
+// (solar source arrays, (16, nsf)) instead of the last ((rows, 16), k-data). Weighted sums for
+// absorption data, plain sums for solar data (cmbgbNN, src/rrtmg_sw_init.f90:492-1689).
+// Output is transposed to (rows, ng) with g fastest.
+static std::vector<double> merge(const Rec &r, int b, bool gfirst, bool weighted,
+                                 const double *rwgt, int &rows) {
+  int n = r.count();
+  rows = n / 16;
+  int ng = NGC[b];
+  std::vector<double> out(rows * ng, 0.0);
+  for (int row = 0; row < rows; row++)
+    for (int ig = 0; ig < 16; ig++) {
+      double v = gfirst ? r.data[ig + 16 * row] : r.data[row + rows * ig];
+      int k = NGM[b * 16 + ig] - 1;
+      out[row * ng + k] += weighted ? v * rwgt[b * 16 + ig] : v;
+    }
+  return out;
+}
+
+bool build_tables(HostTables &H, std::string &err) {
+  auto recs = parse_blob();
+  if (recs.empty()) {
+    err = "embedded table blob is missing or corrupt";
+    return false;
+  }
+  auto get = [&](const std::string &n) -> const Rec * {
+    auto it = recs.find(n);
+    return it == recs.end() ? nullptr : &it->second;
+  };
+  double rwgt[NBND * 16];
+  int ngn[NGPT];
+  reduction_weights(rwgt, ngn);
+  H.rwgt.assign(rwgt, rwgt + NBND * 16);
+
+  std::vector<double> &T = H.tab;
+  T.assign(64, 0.0);  // zero pad: target of every term a g-point does not use
+  auto append = [&](const std::vector<double> &v) {
+    int off = (int)T.size();
+    T.insert(T.end(), v.begin(), v.end());
+    while (T.size() % 8) T.push_back(0.0);
+    return off;
+  };
+
+  struct BandOff {
+    int absa = 0, absb = 0, self = 0, forr = 0, nfor = 0, rayla = 0, sflux = 0, irrad = 0, facb = 0,
+        snsp = 0, nsf = 1;
+    std::vector<double> raylv, raylb, abso3a, abso3b, absch4, absh2o, absco2;
+    double rayl = 0.0;
+  } bo[NBND];
+
+  for (int b = 0; b < NBND; b++) {
+    char nm[64];
+    int band = b + 16, rows;
+    auto R = [&](const char *s) {
+      snprintf(nm, sizeof nm, "kg%d_%s", band, s);
+      return get(nm);
+    };
+    if (auto r = R("kao")) bo[b].absa = append(merge(*r, b, false, true, rwgt, rows));
+    if (auto r = R("kbo")) bo[b].absb = append(merge(*r, b, false, true, rwgt, rows));
+    if (auto r = R("selfrefo")) bo[b].self = append(merge(*r, b, false, true, rwgt, rows));
+    if (auto r = R("forrefo")) {
+      bo[b].forr = append(merge(*r, b, false, true, rwgt, rows));
+      bo[b].nfor = rows;
+    }
+    const Rec *sf = R("sfluxrefo");
+    if (!sf) {
+      err = "table blob lacks sfluxrefo";
+      return false;
+    }
+    bo[b].sflux = append(merge(*sf, b, true, false, rwgt, rows));
+    bo[b].nsf = rows;
+    bo[b].irrad = append(merge(*R("irradnceo"), b, true, false, rwgt, rows));
+    bo[b].facb = append(merge(*R("facbrghto"), b, true, false, rwgt, rows));
+    bo[b].snsp = append(merge(*R("snsptdrko"), b, true, false, rwgt, rows));
+    if (auto r = R("rayl")) bo[b].rayl = r->data[0];
+    if (auto r = R("raylo")) bo[b].raylv = merge(*r, b, true, true, rwgt, rows);
+    if (auto r = R("raylao")) bo[b].rayla = append(merge(*r, b, true, true, rwgt, rows));
+    if (auto r = R("raylbo")) bo[b].raylb = merge(*r, b, true, true, rwgt, rows);
+    if (auto r = R("abso3ao")) bo[b].abso3a = merge(*r, b, true, true, rwgt, rows);
+    if (auto r = R("abso3bo")) bo[b].abso3b = merge(*r, b, true, true, rwgt, rows);
+    if (auto r = R("absch4o")) bo[b].absch4 = merge(*r, b, true, true, rwgt, rows);
+    if (auto r = R("absh2oo")) bo[b].absh2o = merge(*r, b, true, true, rwgt, rows);
+    if (auto r = R("absco2o")) bo[b].absco2 = merge(*r, b, true, true, rwgt, rows);
+  }
+  // cloud optics and aerosol tables, unchanged layout
+  auto put = [&](const char *name, int32_t &off) {
+    const Rec *r = get(name);
+    if (!r) return false;
+    off = append(std::vector<double>(r->data, r->data + r->count()));
+    return true;
+  };
+  bool ok = put("extliq1", H.extliq1) && put("ssaliq1", H.ssaliq1) && put("asyliq1", H.asyliq1) &&
+            put("extice2", H.extice2) && put("ssaice2", H.ssaice2) && put("asyice2", H.asyice2) &&
+            put("extice3", H.extice3) && put("ssaice3", H.ssaice3) && put("asyice3", H.asyice3) &&
+            put("fdlice3", H.fdlice3) && put("abari", H.abari) && put("bbari", H.bbari) &&
+            put("cbari", H.cbari) && put("dbari", H.dbari) && put("ebari", H.ebari) &&
+            put("fbari", H.fbari) && put("rsrtaua", H.rsrtaua) && put("rsrpiza", H.rsrpiza) &&
+            put("rsrasya", H.rsrasya);
+  if (!ok) {
+    err = "table blob lacks cloud or aerosol optics";
+    return false;
+  }
+
+  // Unified gas-optics template: band constants of src/rrtmg_sw_taumol.f90 (strrat, layreffr,
+  // key species, continua, minor absorbers, Rayleigh form, solar-source form).
+  struct BandDef {
+    int8_t lo_mode, lo_s1, lo_s2, lo_cont, hi_mode, hi_s1, hi_s2, hi_for, lo_minor, hi_minor;
+    double lo_strrat, lo_mult, hi_strrat, hi_mult;
+    int8_t sol_interp, sol_upper, layreffr;
+  };
+  const double o2cont = 4.35e-4 / (350.0 * 2.0);  // :1044, :1114
+  const BandDef D[NBND] = {
+      // 16 :275-383
+      {KEY_BIN, SP_H2O, SP_CH4, 1, KEY_MONO, SP_CH4, SP_CH4, 0, SP_NONE, SP_NONE, 252.131, 1, 0, 1, 0, 1, 18},
+      // 17 :386-525
+      {KEY_BIN, SP_H2O, SP_CO2, 1, KEY_BIN, SP_H2O, SP_CO2, 1, SP_NONE, SP_NONE, 0.364641, 1, 0.364641, 1, 1, 1, 30},
+      // 18 :528-643
+      {KEY_BIN, SP_H2O, SP_CH4, 1, KEY_MONO, SP_CH4, SP_CH4, 0, SP_NONE, SP_NONE, 38.9589, 1, 0, 1, 1, 0, 6},
+      // 19 :646-761
+      {KEY_BIN, SP_H2O, SP_CO2, 1, KEY_MONO, SP_CO2, SP_CO2, 0, SP_NONE, SP_NONE, 5.49281, 1, 0, 1, 1, 0, 3},
+      // 20 :764-859
+      {KEY_MONO, SP_H2O, SP_H2O, 1, KEY_MONO, SP_H2O, SP_H2O, 1, SP_CH4, SP_CH4, 0, 1, 0, 1, 0, 0, 3},
+      // 21 :862-1000
+      {KEY_BIN, SP_H2O, SP_CO2, 1, KEY_BIN, SP_H2O, SP_CO2, 1, SP_NONE, SP_NONE, 0.0045321, 1, 0.0045321, 1, 1, 0, 8},
+      // 22 :1003-1128 (o2adj = 1.6 folded into the strrat and the upper multiplier)
+      {KEY_BIN, SP_H2O, SP_O2, 1, KEY_MONO, SP_O2, SP_O2, 0, SP_O2, SP_O2, 1.6 * 0.022708, 1, 0, 1.6, 1, 0, 2},
+      // 23 :1131-1214 (givfac = 1.029)
+      {KEY_MONO, SP_H2O, SP_H2O, 1, KEY_NONE, SP_H2O, SP_H2O, 0, SP_NONE, SP_NONE, 0, 1.029, 0, 1, 0, 0, 6},
+      // 24 :1217-1335
+      {KEY_BIN, SP_H2O, SP_O2, 1, KEY_MONO, SP_O2, SP_O2, 0, SP_O3, SP_O3, 0.124692, 1, 0, 1, 1, 0, 1},
+      // 25 :1338-1411
+      {KEY_MONO, SP_H2O, SP_H2O, 0, KEY_NONE, SP_H2O, SP_H2O, 0, SP_O3, SP_O3, 0, 1, 0, 1, 0, 0, 2},
+      // 26 :1414-1474 (Rayleigh only; solar source at laytrop)
+      {KEY_NONE, SP_H2O, SP_H2O, 0, KEY_NONE, SP_H2O, SP_H2O, 0, SP_NONE, SP_NONE, 0, 1, 0, 1, 0, 0, 0},
+      // 27 :1477-1565
+      {KEY_MONO, SP_O3, SP_O3, 0, KEY_MONO, SP_O3, SP_O3, 0, SP_NONE, SP_NONE, 0, 1, 0, 1, 0, 1, 32},
+      // 28 :1568-1692
+      {KEY_BIN, SP_O3, SP_O2, 0, KEY_BIN, SP_O3, SP_O2, 0, SP_NONE, SP_NONE, 6.67029e-07, 1, 6.67029e-07, 1, 1, 1, 42},
+      // 29 :1695-1785
+      {KEY_MONO, SP_H2O, SP_H2O, 1, KEY_MONO, SP_CO2, SP_CO2, 0, SP_CO2, SP_H2O, 0, 1, 0, 1, 0, 1, 49},
+  };
+  H.gp.resize(NGPT);
+  int iw = 0;
+  for (int b = 0; b < NBND; b++) {
+    const BandDef &d = D[b];
+    const BandOff &o = bo[b];
+    int ng = NGC[b];
+    for (int ig = 0; ig < ng; ig++, iw++) {
+      GPoint &g = H.gp[iw];
+      memset(&g, 0, sizeof g);
+      g.band = (int8_t)(b + 16), g.ibnd = (int8_t)b, g.ig = (int8_t)ig, g.ng = (int8_t)ng;
+      g.lo_mode = d.lo_mode, g.lo_s1 = d.lo_s1, g.lo_s2 = d.lo_s2, g.lo_cont = d.lo_cont;
+      g.hi_mode = d.hi_mode, g.hi_s1 = d.hi_s1, g.hi_s2 = d.hi_s2, g.hi_for = d.hi_for;
+      g.lo_minor = d.lo_minor == SP_NONE ? SP_H2O : d.lo_minor;
+      g.hi_minor = d.hi_minor == SP_NONE ? SP_H2O : d.hi_minor;
+      g.lo_strrat = d.lo_strrat, g.lo_mult = d.lo_mult;
+      g.hi_strrat = d.hi_strrat, g.hi_mult = d.hi_mult;
+      g.lo_nsp = (int8_t)(d.lo_mode == KEY_BIN ? 9 : d.lo_mode == KEY_MONO ? 1 : 0);
+      g.hi_nsp = (int8_t)(d.hi_mode == KEY_BIN ? 5 : d.hi_mode == KEY_MONO ? 1 : 0);
+      g.zero = 0;
+      // tables: element offset of this g-point's column; stride ng (0 -> zero pad)
+      g.absa = d.lo_mode != KEY_NONE ? o.absa + ig : 0;
+      g.absb = d.hi_mode != KEY_NONE ? o.absb + ig : 0;
+      g.selfref = d.lo_cont ? o.self + ig : 0;
+      g.forref = (d.lo_cont || d.hi_for) ? o.forr + ig : 0;
+      g.nfor = (int8_t)o.nfor;
+      g.sa = d.lo_mode != KEY_NONE ? (int8_t)ng : 0;
+      g.sb = d.hi_mode != KEY_NONE ? (int8_t)ng : 0;
+      g.sself = d.lo_cont ? (int8_t)ng : 0;
+      g.sfor = (d.lo_cont || d.hi_for) ? (int8_t)ng : 0;
+      // minor absorbers
+      auto coef = [&](const std::vector<double> &v) { return v.empty() ? 0.0 : v[ig]; };
+      switch (b + 16) {
+        case 20: g.lo_minor_coef = coef(o.absch4), g.hi_minor_coef = coef(o.absch4); break;
+        case 22: g.lo_minor_coef = o2cont, g.hi_minor_coef = o2cont; break;
+        case 24: g.lo_minor_coef = coef(o.abso3a), g.hi_minor_coef = coef(o.abso3b); break;
+        case 25: g.lo_minor_coef = coef(o.abso3a), g.hi_minor_coef = coef(o.abso3b); break;
+        case 29: g.lo_minor_coef = coef(o.absco2), g.hi_minor_coef = coef(o.absh2o); break;
+        default: break;
+      }
+      // Rayleigh
+      if (!o.raylv.empty()) g.rayl_lo = g.rayl_hi = o.raylv[ig];
+      else g.rayl_lo = g.rayl_hi = o.rayl;
+      if (b + 16 == 24) {
+        g.rayl_js = 1;
+        g.rayla = o.rayla + ig;
+        g.sra = (int8_t)ng;
+        g.rayl_hi = o.raylb[ig];
+      }
+      // solar source
+      g.sol_interp = d.sol_interp, g.sol_upper = d.sol_upper, g.layreffr = d.layreffr;
+      g.sol_scale = (b + 16 == 27) ? 50.15 / 48.37 : 1.0;
+      g.sflux = o.sflux + ig, g.irrad = o.irrad + ig, g.facb = o.facb + ig, g.snsp = o.snsp + ig;
+      g.nsf = o.nsf;
+      g.ssf = o.nsf > 1 ? (int8_t)ng : 0;
+    }
+  }
+
+  // exp lookup table, src/rrtmg_sw_init.f90:115-123
+  H.exp_tbl.resize(NTBL + 1);
+  H.exp_tbl[0] = 1.0;
+  H.exp_tbl[NTBL] = 1.e-20;
+  const double bpade = 1.0 / PADE;
+  for (int itr = 1; itr < NTBL; itr++) {
+    double tfn = (double)itr / (double)NTBL;
+    double tau_tbl = bpade * tfn / (1.0 - tfn);
+    H.exp_tbl[itr] = exp(-tau_tbl);
+    if (H.exp_tbl[itr] <= 1.e-20) H.exp_tbl[itr] = 1.e-20;
+  }
+  return true;
+}
+
+}  // namespace rrtmgsw

@@ -1,0 +1,21 @@
+// rrtmgsw_tables.h -- host copy of the reduced tables built by rrtmgsw_init.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "rrtmgsw_internal.h"
+
+namespace rrtmgsw {
+
+struct HostTables {
+  std::vector<double> tab;       // k-tables (g fastest), solar tables, cloud / aerosol optics
+  std::vector<double> exp_tbl;   // NTBL + 1
+  std::vector<GPoint> gp;        // NGPT
+  std::vector<double> rwgt;      // 224 reduction weights
+  int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;
+  int32_t abari, bbari, cbari, dbari, ebari, fbari, rsrtaua, rsrpiza, rsrasya;
+};
+
+bool build_tables(HostTables &h, std::string &err);
+
+}  // namespace rrtmgsw

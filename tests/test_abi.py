@@ -1,0 +1,27 @@
+"""The C-ABI library loads and exports every entry point declared in include/rrtmgsw.h
+(no GPU call is made)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "rrtmg_sw_amd", "librrtmgsw_hip.so")
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "rrtmgsw.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rrtmgsw_[a-z_]+)\s*\(", src)))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_exports_every_declared_symbol():
+    names = declared()
+    assert len(names) >= 9
+    lib = ctypes.CDLL(LIB)
+    for n in names:
+        assert hasattr(lib, n), n
+    from rrtmg_sw_amd import _lib
+    assert sorted(_lib.EXPORTED) == names

@@ -1,0 +1,104 @@
+"""The C oracle against the compiled reference Fortran (oracle/_ref, built from the reference
+sources by oracle/build_ref.sh): reduced tables, clear-sky and cloudy fluxes for every overlap
+method and generator, McICA sub-column arrays. Bit-for-bit equality is required."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from rrtmg_sw_amd.workloads import make_columns, workload
+
+pytestmark = pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built")
+
+OUTS = ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc")
+
+
+def test_reduced_tables_bitexact():
+    n = 0
+    for b in range(16, 30):
+        for k in ob.TABLE_KINDS:
+            r = ob.ref_table(b, k)
+            o = ob.orc_table(f"{k}{b}")
+            if r is None:
+                continue
+            assert o is not None and o.shape == r.shape, (b, k)
+            assert np.array_equal(o, r), (b, k, np.abs(o - r).max())
+            n += 1
+    assert n > 100
+
+
+def test_clear_sky_bitexact():
+    cols = workload("clear64x60")
+    r = ob.run_rrtmg(cols, which="ref")
+    o = ob.run_rrtmg(cols, which="orc")
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
+@pytest.mark.parametrize("icld", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("irng", [0, 1])
+def test_mcica_and_cloudy_fluxes_bitexact(icld, irng):
+    cols = make_columns(12, 40, icld=icld, iaer=10 if icld >= 4 else 0, seed=100 + icld)
+    cols.irng = irng
+    mo, ir_o = ob.run_mcica(cols, "orc")
+    mr, ir_r = ob.run_mcica(cols, "ref")
+    assert ir_o == ir_r == irng
+    for k in mo:
+        assert np.array_equal(mo[k], mr[k]), k
+    assert mo["cldfmcl"].any()
+    r = ob.run_rrtmg(cols, mr, which="ref", icld=min(icld, 3))
+    o = ob.run_rrtmg(cols, mr, which="orc", icld=min(icld, 3))
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
+@pytest.mark.parametrize("isolvar,scon,opts", [
+    (-1, 0.0, {}), (-1, 1361.0, {"bndsolvar": np.linspace(0.9, 1.1, 14)}),
+    (0, 0.0, {}), (1, 1361.0, {"indsolvar": [1.2, 0.8], "solcycfrac": 0.3}),
+    (1, 0.0, {"indsolvar": [0.9, 1.1], "solcycfrac": 0.7}), (2, 0.0, {"indsolvar": [0.16, 1200.0]}),
+    (3, 1361.0, {"bndsolvar": np.linspace(1.05, 0.95, 14)}), (3, 0.0, {})])
+def test_solar_variability_bitexact(isolvar, scon, opts):
+    cols = workload("clear64x60", ncol=6)
+    cols.isolvar, cols.scon, cols.dyofyr = isolvar, scon, 172
+    r = ob.run_rrtmg(cols, which="ref", **opts)
+    o = ob.run_rrtmg(cols, which="orc", **opts)
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
+def test_ecmwf_aerosol_and_flag_clamps_bitexact():
+    cols = workload("clear64x60", ncol=5)
+    rng = np.random.default_rng(7)
+    cols.ecaer = np.asfortranarray(rng.uniform(0, 0.05, (5, 60, 6)))
+    r = ob.run_rrtmg(cols, which="ref", iaer=6, icld=7)
+    o = ob.run_rrtmg(cols, which="orc", iaer=6, icld=7)
+    assert r["icld"] == o["icld"] == 2 and r["iaer"] == o["iaer"] == 6
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+    r = ob.run_rrtmg(cols, which="ref", iaer=3)
+    o = ob.run_rrtmg(cols, which="orc", iaer=3)
+    assert r["iaer"] == o["iaer"] == 0
+
+
+@pytest.mark.parametrize("iceflg", [1, 2, 3])
+def test_ice_parameterizations_bitexact(iceflg):
+    cols = make_columns(8, 30, icld=2, seed=31)
+    cols.iceflgsw = iceflg
+    cols.rei = np.asfortranarray(np.clip(cols.rei, 14.0, 120.0))
+    mr, _ = ob.run_mcica(cols, "ref")
+    r = ob.run_rrtmg(cols, mr, which="ref")
+    o = ob.run_rrtmg(cols, mr, which="orc")
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
+def test_get_alpha_bitexact():
+    cols = make_columns(10, 30, icld=5, seed=5)
+    lat = np.linspace(-80, 80, 10)
+    for icld, idcor in [(4, 0), (5, 0), (4, 1), (5, 1)]:
+        outs = []
+        for lib, fn in ((ob.orc(), "orc_get_alpha"), (ob.ref(), "ref_get_alpha")):
+            a = np.zeros((10, 30), order="F")
+            getattr(lib, fn)(1, 10, 30, icld, idcor, ob.C.c_double(2500.0), ob._p(cols.dz),
+                             ob._p(lat), 200, ob._p(cols.cldfrac), ob._p(a))
+            outs.append(a)
+        assert np.array_equal(outs[0], outs[1]), (icld, idcor)
