@@ -22,9 +22,11 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(cols_full, budget_s=12.0, chunk=128):
+def cpu_baseline(cols_full, budget_s=12.0, chunk=16):
     """Time the compiled reference (oracle/_ref) -- mcica_subcol_sw + rrtmg_sw -- on consecutive
-    column chunks of the same workload on one host core, until `budget_s` seconds of work."""
+    column chunks of the same workload on one host core, until `budget_s` seconds of work. The
+    reference keeps its (112, ncol, nlay) sub-column work arrays on the stack, so it is called
+    on 16-column chunks, as a GCM calls it per chunk."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     kind = "reference" if ob.ref_available() else "port"

@@ -431,6 +431,16 @@ int rrtmgsw_sw_dev(int ncol, int nlay, int *icld, int *iaer, const double *play,
   a.sc.icld = *icld, a.sc.iaer = *iaer;
   a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
   if (*icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+  const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
+                        aldir, aldif, coszen, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (*icld >= 1 && (!cldfmcl || !taucmcl || !ssacmcl || !asmcmcl || !fsfcmcl || !ciwpmcl ||
+                     !clwpmcl || !reicmcl || !relqmcl))
+    return fail(RRTMGSW_ERR_ARGUMENT, "McICA cloud arrays are required when icld >= 1");
+  if (*iaer == 10 && (!tauaer || !ssaaer || !asmaer))
+    return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer are required when iaer = 10");
+  if (*iaer == 6 && !ecaer) return fail(RRTMGSW_ERR_ARGUMENT, "ecaer is required when iaer = 6");
   return run_rt(a, ctx.dev, s);
 }
 
@@ -460,8 +470,10 @@ int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer, const double *play, con
     const double *h;
     size_t n;
   };
-  int ic = *icld, ia = *iaer;
-  bool clouds = ic >= 1 && ic <= 3;
+  // the driver's clamps (src/rrtmg_sw_rad.f90:589, :597) decide which arrays are read
+  int ic = (*icld < 0 || *icld > 3) ? 2 : *icld;
+  int ia = (*iaer != 0 && *iaer != 6 && *iaer != 10) ? 0 : *iaer;
+  bool clouds = ic >= 1;
   In in[] = {{play, C * L}, {plev, C * (L + 1)}, {tlay, C * L}, {h2ovmr, C * L}, {o3vmr, C * L},
              {co2vmr, C * L}, {ch4vmr, C * L}, {o2vmr, C * L}, {asdir, C}, {asdif, C}, {aldir, C},
              {aldif, C}, {coszen, C},
@@ -550,6 +562,11 @@ int rrtmgsw_mcica_subcol_dev(int iplon, int ncol, int nlay, int icld, int permut
   if (ncol <= 0) return 0;
   if (*irng == 0 && nlay < 4) return fail(RRTMGSW_ERR_ARGUMENT, "KISS seeding needs at least four layers");
   if (nlay > MAX_LAY) return fail(RRTMGSW_ERR_ARGUMENT, "nlay must not exceed 256");
+  const void *need[] = {play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, cldfmcl, ciwpmcl,
+                        clwpmcl, reicmcl, relqmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (icld >= 4 && !alpha) return fail(RRTMGSW_ERR_ARGUMENT, "alpha is required when icld = 4 or 5");
   hipStream_t s = (hipStream_t)stream;
   const uint32_t *mt, *jumps;
   if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, *irng, s, &mt, &jumps)) return rc;
@@ -696,6 +713,16 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
   a.sc.icld = icld >= 1 ? 1 : 0, a.sc.iaer = iaer;
   a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
   if (icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+  const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
+                        aldir, aldif, coszen, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (icld >= 1 && (!cldfrac || !ciwp || !clwp || !rei || !rel || !tauc || !ssac || !asmc || !fsfc))
+    return fail(RRTMGSW_ERR_ARGUMENT, "cloud inputs are required when icld >= 1");
+  if (icld >= 4 && !alpha) return fail(RRTMGSW_ERR_ARGUMENT, "alpha is required when icld = 4 or 5");
+  if (iaer == 10 && (!tauaer || !ssaaer || !asmaer))
+    return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer are required when iaer = 10");
+  if (iaer == 6 && !ecaer) return fail(RRTMGSW_ERR_ARGUMENT, "ecaer is required when iaer = 6");
   const uint32_t *mt = nullptr, *jumps = nullptr;
   if (icld >= 1)
     if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, ir, s, &mt, &jumps)) return rc;

@@ -221,7 +221,10 @@ __device__ inline double kinterp(const double *__restrict__ A, int st, int ind0,
 
 __device__ inline void gas_optics(const GPoint &g, const double *__restrict__ tab, ColView &v,
                                   int lay, bool lower, double &taug, double &taur) {
-  const int jp = v.jp[lay], jt = v.jt[lay], jt1 = v.jt1[lay];
+  // A layer below laytrop has jp <= 12 and one above has jp >= 13 whenever pressure decreases
+  // with height; the clamps keep table reads in bounds for profiles that break that order.
+  const int jp = lower ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
+  const int jt = v.jt[lay], jt1 = v.jt1[lay];
   const double fac00 = v.f(0, lay), fac01 = v.f(1, lay), fac10 = v.f(2, lay), fac11 = v.f(3, lay);
   const double colh2o = v.c(SP_H2O, lay), colmol = v.c(5, lay);
   if (lower) {
@@ -234,7 +237,7 @@ __device__ inline void gas_optics(const GPoint &g, const double *__restrict__ ta
     double k = kinterp(tab + g.absa, g.sa, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
     const double *S = tab + g.selfref;
     const double *F = tab + g.forref;
-    int is = v.inds[lay], jf = v.indf[lay];
+    int is = max(v.inds[lay], 1), jf = min(v.indf[lay], 2);
     double s0 = S[(is - 1) * g.sself], s1 = S[is * g.sself];
     double f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
     double cont = v.f(4, lay) * (s0 + v.f(5, lay) * (s1 - s0)) + v.f(6, lay) * (f0 + v.f(7, lay) * (f1 - f0));
@@ -257,7 +260,7 @@ __device__ inline void gas_optics(const GPoint &g, const double *__restrict__ ta
     double fo = 0.0;
     if (g.hi_for) {
       const double *F = tab + g.forref;
-      int jf = v.indf[lay];
+      int jf = min(max(v.indf[lay], 1), 3);
       double f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
       fo = colh2o * v.f(6, lay) * (f0 + v.f(7, lay) * (f1 - f0));
     }
@@ -424,6 +427,7 @@ __device__ inline double tbl_exp(double ze, const double *__restrict__ exp_tbl) 
   if (ze <= OD_LO) return 1. - ze + 0.5 * ze * ze;
   double tblind = ze / (BPADE + ze);
   int itind = (int)(TBLINT * tblind + 0.5);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);   // NaN or negative input stays in the table
   return exp_tbl[itind];
 }
 

@@ -1,0 +1,13 @@
+#!/usr/bin/env bash
+# Build the Fortran drop-in modules (rrtmg_sw_init, mcica_subcol_gen_sw, rrtmg_sw_rad) into
+# rrtmg_sw_amd/fortran/librrtmgsw_fortran.so (+ .mod files in rrtmg_sw_amd/fortran/mod), linked
+# against rrtmg_sw_amd/librrtmgsw_hip.so, and the example host program example_gcm.
+set -euo pipefail
+HERE=$(cd "$(dirname "$0")" && pwd)
+PKG=$(dirname "$HERE")
+FC=${FC:-amdflang}
+mkdir -p "$HERE/mod"
+$FC -O2 -fPIC -shared -J "$HERE/mod" -o "$HERE/librrtmgsw_fortran.so" "$HERE/rrtmg_sw_hip.f90" \
+    -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN/..'
+$FC -O2 -I "$HERE/mod" -o "$HERE/example_gcm" "$HERE/example_gcm.f90" \
+    -L"$HERE" -lrrtmgsw_fortran -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN' -Wl,-rpath,'$ORIGIN/..'

@@ -1,0 +1,54 @@
+#!/usr/bin/env python3
+"""Write golden fixtures from the compiled reference (oracle/_ref/librrtmg_ref.so, built from the
+reference's own sources by oracle/build_ref.sh). Each fixture holds the inputs (a workload name,
+column count and seed recipe are enough to regenerate them, so only outputs are stored) and the
+reference's outputs. Run in a container where the reference tree and oracle/_ref exist:
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
+import oracle_bind as ob  # noqa: E402
+from rrtmg_sw_amd.workloads import make_columns, workload  # noqa: E402
+
+CASES = {
+    # name: (builder, kwargs)
+    "clear64x60": ("workload", dict(name="clear64x60")),
+    "mcica16x72_icld2_kiss": ("workload", dict(name="mcica8192x72", ncol=16)),
+    "mcica8x60_icld5_kiss_aer": ("make", dict(ncol=8, nlay=60, icld=5, iaer=10, seed=3)),
+    "mcica4x40_icld2_mt": ("make", dict(ncol=4, nlay=40, icld=2, iaer=0, seed=9, irng=1)),
+}
+
+
+def build(kind, kw):
+    if kind == "workload":
+        return workload(kw["name"], ncol=kw.get("ncol"))
+    return make_columns(**kw)
+
+
+def main():
+    assert ob.ref_available(), "oracle/_ref not built"
+    for name, (kind, kw) in CASES.items():
+        cols = build(kind, kw)
+        rec = {}
+        if cols.icld:
+            mc, _ = ob.run_mcica(cols, "ref")
+            rec["cldfmcl_bits"] = np.packbits(mc["cldfmcl"].astype(bool).ravel(order="F"))
+        else:
+            mc = ob.empty_mcica(cols)
+        out = ob.run_rrtmg(cols, mc, which="ref", icld=min(cols.icld, 3))
+        for k in ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc"):
+            rec[k] = out[k]
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **rec)
+        print(path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    main()

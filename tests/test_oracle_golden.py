@@ -1,0 +1,28 @@
+"""The C oracle against golden fixtures written by the compiled reference
+(tests/golden/make_golden.py): bit-for-bit equal fluxes, heating rates and McICA masks.
+These pin the oracle where oracle/_ref is not built."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from golden.make_golden import CASES, build
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_oracle_matches_reference_golden(name):
+    kind, kw = CASES[name]
+    cols = build(kind, kw)
+    g = np.load(os.path.join(HERE, "golden", f"{name}.npz"))
+    if cols.icld:
+        mc, _ = ob.run_mcica(cols, "orc")
+        bits = np.packbits(mc["cldfmcl"].astype(bool).ravel(order="F"))
+        assert np.array_equal(bits, g["cldfmcl_bits"])
+    else:
+        mc = ob.empty_mcica(cols)
+    out = ob.run_rrtmg(cols, mc, which="orc", icld=min(cols.icld, 3))
+    for k in ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc"):
+        assert np.array_equal(out[k], g[k]), k
