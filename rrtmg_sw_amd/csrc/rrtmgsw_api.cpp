@@ -45,7 +45,7 @@ struct Context {
   double heatfac = 0.0;
   HostTables host;
   DeviceTables dev{};
-  double *d_tab = nullptr, *d_exp = nullptr;
+  double *d_tab = nullptr, *d_exp = nullptr, *d_rexp = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
   std::map<hipStream_t, Scratch> scratch, status, mtbuf;
@@ -301,7 +301,7 @@ int finish(hipStream_t s, int *d_status) {
 
 // Columns per launch so that the per-column elimination coefficients stay within ~4 GiB.
 int chunk_columns(int ncol, int nlay) {
-  size_t per = (size_t)nlay * 8 * NGPT * sizeof(double);
+  size_t per = (size_t)nlay * SCR_COEFS * NGPT * sizeof(double);
   size_t cap = (size_t)4 << 30;
   size_t n = cap / per;
   if (n < 64) n = 64;
@@ -322,7 +322,7 @@ int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s) {
   void *scr = nullptr, *st = nullptr;
   {
     std::lock_guard<std::mutex> lk(ctx.mu);
-    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * base.nlay * 8 * NGPT * sizeof(double), &scr)) return rc;
+    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * base.nlay * SCR_COEFS * NGPT * sizeof(double), &scr)) return rc;
     if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
   }
   hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
@@ -361,10 +361,16 @@ int rrtmgsw_init(double cpdair) {
   HIPCHK(hipMemcpy(ctx.d_tab, H.tab.data(), H.tab.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx.d_exp, H.exp_tbl.size() * sizeof(double)));
   HIPCHK(hipMemcpy(ctx.d_exp, H.exp_tbl.data(), H.exp_tbl.size() * sizeof(double), hipMemcpyHostToDevice));
+  {
+    std::vector<double> r(H.exp_tbl.size());
+    for (size_t i = 0; i < r.size(); i++) r[i] = 1.0 / H.exp_tbl[i];
+    HIPCHK(hipMalloc(&ctx.d_rexp, r.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx.d_rexp, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMalloc(&ctx.d_gp, H.gp.size() * sizeof(GPoint)));
   HIPCHK(hipMemcpy(ctx.d_gp, H.gp.data(), H.gp.size() * sizeof(GPoint), hipMemcpyHostToDevice));
   DeviceTables &D = ctx.dev;
-  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.gp = ctx.d_gp;
+  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.rexp_tbl = ctx.d_rexp, D.gp = ctx.d_gp;
   D.extliq1 = H.extliq1, D.ssaliq1 = H.ssaliq1, D.asyliq1 = H.asyliq1;
   D.extice2 = H.extice2, D.ssaice2 = H.ssaice2, D.asyice2 = H.asyice2;
   D.extice3 = H.extice3, D.ssaice3 = H.ssaice3, D.asyice3 = H.asyice3, D.fdlice3 = H.fdlice3;
@@ -381,6 +387,7 @@ int rrtmgsw_finalize(void) {
   if (!ctx.ready) return 0;
   hipFree(ctx.d_tab);
   hipFree(ctx.d_exp);
+  hipFree(ctx.d_rexp);
   hipFree(ctx.d_gp);
   for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf}) {
     for (auto &kv : *m) hipFree(kv.second.ptr);

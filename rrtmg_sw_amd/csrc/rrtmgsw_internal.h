@@ -11,6 +11,7 @@ constexpr int NAER = 6;        // ECMWF aerosol types (modules/parrrsw.f90:31)
 constexpr int WAVE_G = 56;     // g-points per wavefront: wave 0 = bands 16-22, wave 1 = bands 23-29
 constexpr int MAX_LAY = 256;   // layer limit of the per-column LDS staging
 constexpr int NTBL = 10000;    // exp lookup table, modules/rrsw_tbl.f90:28
+constexpr int SCR_COEFS = 6;   // elimination coefficients kept per (layer, g-point): B C E for two skies
 constexpr double PADE = 0.278;
 constexpr double OD_LO = 0.06;
 
@@ -50,6 +51,7 @@ struct GPoint {
 struct DeviceTables {
   const double *tab;        // k-tables, solar tables, cloud optics
   const double *exp_tbl;    // NTBL+1 entries
+  const double *rexp_tbl;   // 1 / exp_tbl, NTBL+1 entries
   const GPoint *gp;         // NGPT records
   // cloud optics offsets (tables (n, 16:29) column-major: value(i, band) at off + (i-1) + n*(band-16))
   int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;

@@ -37,7 +37,7 @@ constexpr double CLDMIN = 1.e-20;           // src/rrtmg_sw_cldprmc.f90:118
 struct ColLayout {
   int L;
   int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
-  int o_fac, o_col, o_red, o_mask, total;             // double blocks / mask words
+  int o_fac, o_col, o_red, o_reda, o_mask, total;     // double blocks / mask words
   __host__ __device__ ColLayout(int nlay, int nmaskw) {
     L = nlay;
     int ib = 0;
@@ -51,6 +51,7 @@ struct ColLayout {
     o_fac = bytes, bytes += 8 * L * 8;     // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac
     o_col = bytes, bytes += 6 * L * 8;     // h2o co2 o3 ch4 o2 colmol
     o_red = bytes, bytes += 2 * (L + 1) * 4 * 8;
+    o_reda = bytes, bytes += 2 * L * 2 * 8;
     o_mask = bytes, bytes += nmaskw * NGPT * 4;
     total = bytes;
   }
@@ -59,7 +60,7 @@ struct ColLayout {
 struct ColView {
   int L;
   int *jp, *jt, *jt1, *inds, *indf, *misc;
-  double *fac, *col, *red;
+  double *fac, *col, *red, *reda;
   uint32_t *mask;
   __device__ ColView(char *smem, const ColLayout &lo) {
     L = lo.L;
@@ -69,6 +70,7 @@ struct ColView {
     fac = reinterpret_cast<double *>(smem + lo.o_fac);
     col = reinterpret_cast<double *>(smem + lo.o_col);
     red = reinterpret_cast<double *>(smem + lo.o_red);
+    reda = reinterpret_cast<double *>(smem + lo.o_reda);
     mask = reinterpret_cast<uint32_t *>(smem + lo.o_mask);
   }
   __device__ double &f(int k, int lay) { return fac[k * L + lay]; }   // 0..7 as listed above
@@ -431,12 +433,34 @@ __device__ inline double tbl_exp(double ze, const double *__restrict__ exp_tbl) 
   return exp_tbl[itind];
 }
 
-// reftra_sw for one layer (src/rrtmg_sw_reftra.f90:151-327, PIFM kmodts = 2).
+// Transmittance and its reciprocal: the reciprocal comes from a table of 1/exp_tbl (built on
+// the host with IEEE division, so it equals the division the reference performs).
+struct Ex {
+  double e, r;
+};
+__device__ inline Ex tbl_exp2(double ze, const double *__restrict__ exp_tbl,
+                              const double *__restrict__ rexp_tbl) {
+#pragma clang fp contract(off)
+  if (ze <= OD_LO) {
+    double e = 1. - ze + 0.5 * ze * ze;
+    return {e, 1. / e};
+  }
+  double tblind = ze / (BPADE + ze);
+  int itind = (int)(TBLINT * tblind + 0.5);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
+  return {exp_tbl[itind], rexp_tbl[itind]};
+}
+
+// reftra_sw for one layer (src/rrtmg_sw_reftra.f90:151-327, PIFM kmodts = 2). Algebraically the
+// reference's expressions with fewer divisions: 1/mu0 is passed in (rmu), exponentials come
+// with their reciprocals, R and T share one reciprocal of the common denominator, and
+// 1/((1 - beta e^-2k tau)(k + g1)) is formed as 1/((k + g1) - (g1 - k) e^-2k tau).
 struct RT4 {
   double r, rd, t, td;
 };
-__device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz,
-                              const double *__restrict__ exp_tbl) {
+__device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz, double rmu,
+                              const double *__restrict__ exp_tbl,
+                              const double *__restrict__ rexp_tbl) {
   const double zwcrit = 0.9999995, eps = 1.e-08;
   RT4 o;
   double zg3 = 3. * zg;
@@ -444,23 +468,21 @@ __device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz,
   double zgamma2 = 3. * (zw * (1. - zg)) * 0.25;
   double zgamma3 = (2. - zg3 * prmuz) * 0.25;
   double zgamma4 = 1. - zgamma3;
-  double zwo = 0., denom = 1.;
-  if (zg != 1.) {
-    double r = zg / (1. - zg);
-    denom = 1. - (1. - zw) * (r * r);
-  }
-  if (zw > 0. && denom != 0.) zwo = zw / denom;
-  if (zwo >= zwcrit) {  // conservative scattering
+  // zwo = zw / (1 - (1-zw) (zg/(1-zg))^2) >= zwcrit, compared without dividing
+  double omg = 1. - zg;
+  double den2 = omg * omg - (1. - zw) * zg * zg;   // denom * (1-zg)^2
+  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+  if (cons) {  // conservative scattering, :188-228
     double za1 = zgamma1 * prmuz - zgamma3;
     double zgt = zgamma1 * zto1;
-    double ze2 = tbl_exp(fmin(zto1 / prmuz, 500.), exp_tbl);
+    double ze2 = tbl_exp2(fmin(zto1 * rmu, 500.), exp_tbl, rexp_tbl).e;
     double inv = 1. / (1. + zgt);
     o.r = (zgt - za1 * (1. - ze2)) * inv;
     o.t = 1. - o.r;
     o.rd = zgt * inv;
     o.td = 1. - o.rd;
     if (ze2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
-  } else {
+  } else {     // general two-stream solution, :233-322
     double za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
     double za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
     double zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
@@ -476,20 +498,20 @@ __device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz,
     double zt1 = zrp1 * (za1 + zrk * zgamma4);
     double zt2 = zrm1 * (za1 - zrk * zgamma4);
     double zt3 = zrk2 * (zgamma4 + za1 * prmuz);
-    double zbeta = (zgamma1 - zrk) / zrkg;
-    double zem1 = tbl_exp(fmin(zrk * zto1, 500.), exp_tbl), zep1 = 1. / zem1;
-    double zem2 = tbl_exp(fmin(zto1 / prmuz, 500.), exp_tbl), zep2 = 1. / zem2;
+    Ex e1 = tbl_exp2(fmin(zrk * zto1, 500.), exp_tbl, rexp_tbl);
+    Ex e2 = tbl_exp2(fmin(zto1 * rmu, 500.), exp_tbl, rexp_tbl);
+    double zem1 = e1.e, zep1 = e1.r, zem2 = e2.e, zep2 = e2.r;
     double zdenr = zr4 * zep1 + zr5 * zem1;
-    double zdent = zr4 * zep1 + zr5 * zem1;
     if (zdenr >= -eps && zdenr <= eps) {
       o.r = eps;
       o.t = zem2;
     } else {
-      o.r = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) / zdenr;
-      o.t = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) / zdent;
+      double rden = 1. / zdenr;
+      o.r = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) * rden;
+      o.t = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) * rden;
     }
     double zemm = zem1 * zem1;
-    double zdend = 1. / ((1. - zbeta * zemm) * zrkg);
+    double zdend = 1. / (zrkg - (zgamma1 - zrk) * zemm);
     o.rd = zgamma2 * (1. - zemm) * zdend;
     o.td = zrk2 * zem1 * zdend;
   }
@@ -530,6 +552,19 @@ __device__ inline double wave_sum4(double v0, double v1, double v2, double v3, i
   double w1 = kb + __shfl_xor(sb, 32, 64);
   double snd = b4 ? w0 : w1, kp = b4 ? w1 : w0;
   double x = kp + __shfl_xor(snd, 16, 64);
+  x += __shfl_xor(x, 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
+// Sum two values over the 64 lanes of a wavefront with 6 exchanges; lane 0 ends with the total
+// of v0, lane 32 with the total of v1.
+__device__ inline double wave_sum2(double v0, double v1, int lane) {
+  bool b5 = lane & 32;
+  double x = (b5 ? v1 : v0) + __shfl_xor(b5 ? v0 : v1, 32, 64);
+  x += __shfl_xor(x, 16, 64);
   x += __shfl_xor(x, 8, 64);
   x += __shfl_xor(x, 4, 64);
   x += __shfl_xor(x, 2, 64);
@@ -632,7 +667,7 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
 // ------------------------------------------------------------------------------------------
 // The radiative-transfer kernel.
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) rt_kernel_fp64(RtArgs a, DeviceTables T) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) rt_kernel_fp64(RtArgs a, DeviceTables T) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
   const int nmw = (L + 31) / 32;
@@ -675,8 +710,14 @@ __global__ void __launch_bounds__(128) rt_kernel_fp64(RtArgs a, DeviceTables T) 
   const double zinc = active ? incident_flux(gp, T.tab, v, a.sc, mu0) : 0.0;
 
   // ---- pass 1: top to bottom ----
+  // Per layer and sky the lane stores B = prdnd, C and E of the elimination (slots 0-2 clear,
+  // 3-5 total sky); the A = ztdn term of the downward flux is summed over g here. A sub-column
+  // stays identical to the clear sky until its first cloudy layer (jfirst): above it the total-sky
+  // sweep is not computed and nothing is stored for it.
+  const double rmu = 1.0 / mu0;
   Sweep sc_ = {1.0, 1.0, 0.0}, st_ = {1.0, 1.0, 0.0};
-  double *scr = a.scratch + (size_t)cl * L * 8 * NGPT;
+  int jfirst = L;                   // first layer (from the top) whose total sky differs
+  double *scr = a.scratch + (size_t)cl * L * SCR_COEFS * NGPT;
   for (int j = 0; j < L; j++) {
     const int lay = L - 1 - j;
     double taug, taur;
@@ -741,75 +782,111 @@ __global__ void __launch_bounds__(128) rt_kernel_fp64(RtArgs a, DeviceTables T) 
         ct = tin, cw = win, cg = gin;
       }
     }
-    RT4 rc = reftra1(tauc, omc, gcc, mu0, T.exp_tbl);
-    double dbc = tbl_exp(tauc / mu0, T.exp_tbl);
-    RT4 rt = rc;
-    double dbt = dbc;
-    if (cf > 0.0) {
+    RT4 rc = reftra1(tauc, omc, gcc, mu0, rmu, T.exp_tbl, T.rexp_tbl);
+    double dbc = tbl_exp2(tauc * rmu, T.exp_tbl, T.rexp_tbl).e;
+    Sweep pre = sc_;
+    double Bc, Cc, Ec, Ac;
+    sweep_layer(sc_, rc, dbc, Ac, Bc, Cc, Ec);
+    double At = Ac;
+    if (cf > 0.0 && jfirst == L) {
+      jfirst = j;
+      st_ = pre;
+    }
+    if (jfirst <= j) {
       double tauo = tauc + ct;
       double omo = tauc * omc + ct * cw;
       double go = (ct * cw * cg + tauc * omc * gcc) / omo;
       omo = omo / tauo;
-      double dbo = tbl_exp(tauo / mu0, T.exp_tbl);
-      RT4 ro = {0.0, 0.0, 1.0, 1.0};
-      if (cf > REPCLC) ro = reftra1(tauo, omo, go, mu0, T.exp_tbl);
-      double zcl = 1.0 - cf;
-      rt.r = zcl * rc.r + cf * ro.r;
-      rt.rd = zcl * rc.rd + cf * ro.rd;
-      rt.t = zcl * rc.t + cf * ro.t;
-      rt.td = zcl * rc.td + cf * ro.td;
-      dbt = zcl * dbc + cf * dbo;
+      RT4 rt = rc;
+      double dbt = dbc;
+      if (cf > 0.0) {
+        double dbo = tbl_exp2(tauo * rmu, T.exp_tbl, T.rexp_tbl).e;
+        RT4 ro = {0.0, 0.0, 1.0, 1.0};
+        if (cf > REPCLC) ro = reftra1(tauo, omo, go, mu0, rmu, T.exp_tbl, T.rexp_tbl);
+        double zcl = 1.0 - cf;
+        rt.r = zcl * rc.r + cf * ro.r;
+        rt.rd = zcl * rc.rd + cf * ro.rd;
+        rt.t = zcl * rc.t + cf * ro.t;
+        rt.td = zcl * rc.td + cf * ro.td;
+        dbt = zcl * dbc + cf * dbo;
+      }
+      double Bt, Ct, Et;
+      sweep_layer(st_, rt, dbt, At, Bt, Ct, Et);
+      if (active) {
+        double *p = scr + ((size_t)j * SCR_COEFS + 3) * NGPT + g;
+        p[0] = Bt, p[NGPT] = Ct, p[2 * NGPT] = Et;
+      }
     }
-    double A0, B0, C0, E0, A1, B1, C1, E1;
-    sweep_layer(sc_, rc, dbc, A0, B0, C0, E0);
-    sweep_layer(st_, rt, dbt, A1, B1, C1, E1);
     if (active) {
-      double *p = scr + (size_t)j * 8 * NGPT + g;
-      p[0 * NGPT] = A0, p[1 * NGPT] = B0, p[2 * NGPT] = C0, p[3 * NGPT] = E0;
-      p[4 * NGPT] = A1, p[5 * NGPT] = B1, p[6 * NGPT] = C1, p[7 * NGPT] = E1;
+      double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
+      p[0] = Bc, p[NGPT] = Cc, p[2 * NGPT] = Ec;
     }
+    // sum over g of the incident-flux-weighted ztdn of level j, both skies
+    double sa = wave_sum2(zinc * At, zinc * Ac, lane);
+    if ((lane & 31) == 0) v.reda[(wave * L + j) * 2 + (lane >> 5)] = sa;
   }
+  if (jfirst == L) st_ = sc_;
   if (err) atomicCAS(a.status, 0, err);
 
   // ---- pass 2: bottom to top ----
+  // surface (src/rrtmg_sw_vrtqdr.f90:163-169 with prup = albdir, prupd = albdif)
   double uc = (albdir * sc_.tdbt + albdif * (sc_.ztdn - sc_.tdbt)) / (1. - albdif * sc_.prdnd);
   double dc = sc_.ztdn + sc_.prdnd * uc;
   double ut = (albdir * st_.tdbt + albdif * (st_.ztdn - st_.tdbt)) / (1. - albdif * st_.prdnd);
   double dt = st_.ztdn + st_.prdnd * ut;
-  for (int j = L; j >= 0; j--) {
-    if (j < L && active) {
-      const double *p = scr + (size_t)j * 8 * NGPT + g;
-      uc = p[2 * NGPT] + p[3 * NGPT] * uc;
-      dc = p[0 * NGPT] + p[1 * NGPT] * uc;
-      ut = p[6 * NGPT] + p[7 * NGPT] * ut;
-      dt = p[4 * NGPT] + p[5 * NGPT] * ut;
-    }
+  {
     double s = wave_sum4(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
+    if ((lane & 15) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 4)] = s;
+  }
+  for (int j = L - 1; j >= 0; j--) {
+    double bu_c = 0.0, bu_t = 0.0;
+    if (active) {
+      const double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
+      double Bc = p[0], Cc = p[NGPT], Ec = p[2 * NGPT];
+      uc = Cc + Ec * uc;
+      bu_c = Bc * uc;
+      if (j >= jfirst) {
+        const double *q = p + 3 * NGPT;
+        ut = q[NGPT] + q[2 * NGPT] * ut;
+        bu_t = q[0] * ut;
+      } else {
+        ut = Cc + Ec * ut;
+        bu_t = Bc * ut;
+      }
+    }
+    // downward flux = sum(zinc ztdn) (pass 1) + sum(zinc prdnd U)
+    double s = wave_sum4(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
     if ((lane & 15) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 4)] = s;
   }
   __syncthreads();
 
   // ---- epilogue: broadband fluxes and heating rates (src/rrtmg_sw_rad.f90:805-836) ----
-  for (int j = tid; j <= L; j += blockDim.x) {
+  // level j (top-based): up = red[.][0], down = red[.][1] (+ reda total), clear in 2, 3 (+ reda clear)
+  auto level = [&](int j, double &fu, double &fd, double &cu, double &cd) {
     const double *r0 = v.red + (0 * (L + 1) + j) * 4, *r1 = v.red + (1 * (L + 1) + j) * 4;
+    fu = r0[0] + r1[0], fd = r0[1] + r1[1], cu = r0[2] + r1[2], cd = r0[3] + r1[3];
+    if (j < L) {
+      fd += v.reda[(0 * L + j) * 2 + 0] + v.reda[(1 * L + j) * 2 + 0];
+      cd += v.reda[(0 * L + j) * 2 + 1] + v.reda[(1 * L + j) * 2 + 1];
+    }
+  };
+  for (int j = tid; j <= L; j += blockDim.x) {
+    double fu, fd, cu, cd;
+    level(j, fu, fd, cu, cd);
     size_t q = c + (size_t)C * (L - j);
-    a.swuflx[q] = r0[0] + r1[0];
-    a.swdflx[q] = r0[1] + r1[1];
-    a.swuflxc[q] = r0[2] + r1[2];
-    a.swdflxc[q] = r0[3] + r1[3];
+    a.swuflx[q] = fu, a.swdflx[q] = fd, a.swuflxc[q] = cu, a.swdflxc[q] = cd;
   }
   for (int i = tid; i < L; i += blockDim.x) {
     // layer i (bottom-based) lies between output levels i and i+1 = top-based j and j-1
     int j = L - i;
-    const double *a0 = v.red + (0 * (L + 1) + j) * 4, *a1 = v.red + (1 * (L + 1) + j) * 4;
-    const double *b0 = v.red + (0 * (L + 1) + j - 1) * 4, *b1 = v.red + (1 * (L + 1) + j - 1) * 4;
-    double nf0 = (a0[1] + a1[1]) - (a0[0] + a1[0]), nf1 = (b0[1] + b1[1]) - (b0[0] + b1[0]);
-    double nc0 = (a0[3] + a1[3]) - (a0[2] + a1[2]), nc1 = (b0[3] + b1[3]) - (b0[2] + b1[2]);
+    double fu0, fd0, cu0, cd0, fu1, fd1, cu1, cd1;
+    level(j, fu0, fd0, cu0, cd0);
+    level(j - 1, fu1, fd1, cu1, cd1);
     double pdp = a.plev[c + (size_t)C * i] - a.plev[c + (size_t)C * (i + 1)];
     double zdpgcp = a.sc.heatfac / pdp;
     size_t q = c + (size_t)C * i;
-    a.swhr[q] = i == L - 1 ? 0.0 : (nf1 - nf0) * zdpgcp;
-    a.swhrc[q] = i == L - 1 ? 0.0 : (nc1 - nc0) * zdpgcp;
+    a.swhr[q] = i == L - 1 ? 0.0 : ((fd1 - fu1) - (fd0 - fu0)) * zdpgcp;
+    a.swhrc[q] = i == L - 1 ? 0.0 : ((cd1 - cu1) - (cd0 - cu0)) * zdpgcp;
   }
 }
 
