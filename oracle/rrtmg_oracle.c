@@ -1090,7 +1090,7 @@ static void vrtqdr(int klev, const double *pref, const double *prefd, const doub
  * Outputs pbb* (nlay+1, bottom->top). bandu/bandd: optional (nlay+1,14) total-sky blocks.
  * ---------------------------------------------------------------------------------- */
 typedef struct {
-  double *pbbfu, *pbbfd, *pbbcu, *pbbcd, *pbbfddir, *bandu, *bandd;
+  double *pbbfu, *pbbfd, *pbbcu, *pbbcd, *pbbfddir, *bandu, *bandd, *banddir;
 } spc_out_t;
 
 static void spcvmc(int nlay, int idelm, const double *palbd, const double *palbp,
@@ -1100,7 +1100,6 @@ static void spcvmc(int nlay, int idelm, const double *palbd, const double *palbp
                    const double *adjflux, const solar_t *sol, const coef_t *c,
                    spc_out_t *o) {
   const int klev = nlay;
-  (void)ptaormc; (void)idelm;
   const double repclc = 1.e-12;
   double *taug = (double *)malloc(sizeof(double) * nlay * NGPT);
   double *taur = (double *)malloc(sizeof(double) * nlay * NGPT);
@@ -1120,6 +1119,7 @@ static void spcvmc(int nlay, int idelm, const double *palbd, const double *palbp
   double *ztauc = calloc(n1, 8), *zomcc = calloc(n1, 8), *zgcc = calloc(n1, 8);
   double *ztauo = calloc(n1, 8), *zomco = calloc(n1, 8), *zgco = calloc(n1, 8);
   int *lclr = calloc(n1, sizeof(int)), *lcld = calloc(n1, sizeof(int));
+  double *ztdbt_nodel = calloc(n1, 8); /* unscaled direct beam (idelm = 0, :446-487) */
   int iw = 0;
   for (int jb = 16; jb <= 29; jb++) {
     int ibm = jb - 15;
@@ -1146,6 +1146,14 @@ static void spcvmc(int nlay, int idelm, const double *palbd, const double *palbp
         zomcc[jk] = taur[q] * 1.0 + ptaua[qa] * pomga[qa];
         zgcc[jk] = pasya[qa] * pomga[qa] * ptaua[qa] / zomcc[jk];
         zomcc[jk] = zomcc[jk] / ztauc[jk];
+        if (idelm == 0) { /* :449-487 (the clear-sky twin feeds only the dead clear direct sums) */
+          double zclear = 1.0 - pcldfmc[q], zcloud = pcldfmc[q];
+          double zdbtmc = tbl_exp(ztauc[jk] / prmu0);
+          double tauorig = ztauc[jk] + ptaormc[q];
+          double zdbtmo = tbl_exp(tauorig / prmu0);
+          if (jk == 0) ztdbt_nodel[0] = 1.0;
+          ztdbt_nodel[jk + 1] = (zclear * zdbtmc + zcloud * zdbtmo) * ztdbt_nodel[jk];
+        }
         double zf = zgcc[jk] * zgcc[jk];
         double zwf = zomcc[jk] * zf;
         ztauc[jk] = (1.0 - zwf) * ztauc[jk];
@@ -1182,9 +1190,11 @@ static void spcvmc(int nlay, int idelm, const double *palbd, const double *palbp
         o->pbbfd[ikl] = o->pbbfd[ikl] + zincflx * zfd[jk];
         o->pbbcu[ikl] = o->pbbcu[ikl] + zincflx * zcu[jk];
         o->pbbcd[ikl] = o->pbbcd[ikl] + zincflx * zcd[jk];
-        if (idelm == 1) o->pbbfddir[ikl] = o->pbbfddir[ikl] + zincflx * ztdbt[jk];
+        const double *dir = idelm == 0 ? ztdbt_nodel : ztdbt; /* :623-628 */
+        o->pbbfddir[ikl] = o->pbbfddir[ikl] + zincflx * dir[jk];
         if (o->bandu) o->bandu[ikl + n1 * (ibm - 1)] += zincflx * zfu[jk];
         if (o->bandd) o->bandd[ikl + n1 * (ibm - 1)] += zincflx * zfd[jk];
+        if (o->banddir) o->banddir[ikl + n1 * (ibm - 1)] += zincflx * dir[jk];
       }
     }
   }
@@ -1193,7 +1203,7 @@ static void spcvmc(int nlay, int idelm, const double *palbd, const double *palbp
   free(zref), free(zrefd), free(ztra), free(ztrad), free(zdbtc), free(ztdbtc), free(zdbt), free(ztdbt);
   free(zrupc), free(zrupdc), free(zrdndc), free(zrup), free(zrupd), free(zrdnd);
   free(zcd), free(zcu), free(zfd), free(zfu), free(ztauc), free(zomcc), free(zgcc), free(ztauo), free(zomco), free(zgco);
-  free(lclr), free(lcld);
+  free(lclr), free(lcld), free(ztdbt_nodel);
 }
 
 /* ------------------------------------------------------------------------------------
@@ -1492,7 +1502,7 @@ int orc_rrtmg_sw_ext(int ncol, int nlay, int *icld, int *iaer, const double *pla
           ztaua[t] = tauaer[s], zasya[t] = asmaer[s], zomga[t] = ssaaer[s];
         }
       }
-    spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], bu, bd};
+    spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], bu, bd, NULL};
     if (bu) memset(bu, 0, 8 * n1 * NBND);
     if (bd) memset(bd, 0, 8 * n1 * NBND);
     spcvmc(L, idelm, albdif, albdir, zcldfmc, ztaucmc, zasycmc, zomgcmc, ztaormc, ztaua, zasya, zomga,
@@ -1783,4 +1793,66 @@ int orc_cldprmc(int nlay, int inflag, int iceflag, int liqflag, const double *cl
                 double *asmcmc, const double *fsfcmc) {
   return cldprmc(nlay, inflag, iceflag, liqflag, cldfmc, ciwpmc, clwpmc, reicmc, relqmc, taormc,
                  taucmc, ssacmc, asmcmc, fsfcmc);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Column-mode driver: the reference's single-column program (src/rrtmg_sw.1col.f90) from
+ * setcoef_sw on, for clear skies, fed the layer amounts its readprof builds
+ * (:1695-1766: coldry and wkl in molecules/cm2, pz in hPa bottom-up from the surface).
+ * setcoef_sw :634-638; cossza = zenith (0 -> 1e-10, :489, :644-645); albedo = 1 - semiss
+ * (:650-653); iaer = 10 takes tauaer/ssaaer/asmaer (:739-747); spcvmc_sw :781-795; level
+ * outputs and heating rates :798-821 with heatfac from cpdair (src/rrtmg_sw_init.f90:258).
+ * out (6, nlay+1, 15): up, difdown, dirdown, down, net, heating rate; block 0 broadband,
+ * blocks 1..14 bands 16..29 (the IOUT=98 band loop runs spcvmc_sw on one band at a time,
+ * which is the band's share of the broadband sums).
+ * ---------------------------------------------------------------------------------- */
+int orc_column_sw(int nlay, const double *pavel, const double *tavel, const double *pz,
+                  const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                  int juldat, double scon, int isolvar, const double *bndsolvar,
+                  const double *indsolvar, const double *solcycfrac, int iaer,
+                  const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                  double cpdair, double *out) {
+  if (!g_ready) return fail(9, "orc_init not called");
+  const int L = nlay, n1 = nlay + 1;
+  solvar_t sv;
+  solar_setup(1.0, juldat, scon, isolvar, bndsolvar, indsolvar, solcycfrac, &sv);
+  solar_t sol = {isolvar, sv.svar_f, sv.svar_s, sv.svar_i, sv.svf_bnd + 15, sv.svs_bnd + 15, sv.svi_bnd + 15};
+  coef_t c = coef_alloc(L);
+  double *wkl = calloc((size_t)MXMOL * L, 8);
+  for (int l = 0; l < L; l++)
+    for (int m = 0; m < 7; m++) wkl[m + MXMOL * l] = wkl7[m + 7 * l];
+  setcoef(L, pavel, tavel, coldry, wkl, &c);
+  double cossza = zenith == 0. ? 1.e-10 : zenith;
+  double alb[NBND];
+  for (int ib = 0; ib < NBND; ib++) alb[ib] = 1. - semiss[ib];
+  double *zc0 = calloc((size_t)L * NGPT, 8), *zw1 = malloc(8 * (size_t)L * NGPT);
+  for (int k = 0; k < L * NGPT; k++) zw1[k] = 1.;
+  double *ta = calloc((size_t)L * NBND, 8), *ga = calloc((size_t)L * NBND, 8), *wa = malloc(8 * (size_t)L * NBND);
+  for (int k = 0; k < L * NBND; k++) wa[k] = 1.;
+  if (iaer == 10)
+    for (int k = 0; k < L * NBND; k++) ta[k] = tauaer[k], wa[k] = ssaaer[k], ga[k] = asmaer[k];
+  double *pbb[5];
+  for (int i = 0; i < 5; i++) pbb[i] = calloc(n1, 8);
+  double *bu = calloc((size_t)n1 * NBND, 8), *bd = calloc((size_t)n1 * NBND, 8), *bdir = calloc((size_t)n1 * NBND, 8);
+  spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], bu, bd, bdir};
+  spcvmc(L, idelm, alb, alb, zc0, zc0, zc0, zw1, zc0, ta, ga, wa, cossza, sv.adjflux, &sol, &c, &o);
+  const double heatfac = GRAV * SECDY / (cpdair * 1.e2);
+  for (int b = 0; b <= NBND; b++) {
+    double *up = out + (size_t)n1 * (0 + 6 * b), *dif = out + (size_t)n1 * (1 + 6 * b);
+    double *dir = out + (size_t)n1 * (2 + 6 * b), *dn = out + (size_t)n1 * (3 + 6 * b);
+    double *net = out + (size_t)n1 * (4 + 6 * b), *htr = out + (size_t)n1 * (5 + 6 * b);
+    for (int k = 0; k < n1; k++) {
+      up[k] = b == 0 ? pbb[0][k] : bu[k + n1 * (b - 1)];
+      dn[k] = b == 0 ? pbb[1][k] : bd[k + n1 * (b - 1)];
+      dir[k] = b == 0 ? pbb[4][k] : bdir[k + n1 * (b - 1)];
+      dif[k] = dn[k] - dir[k];
+      net[k] = dn[k] - up[k];
+    }
+    for (int k = 0; k < L; k++) htr[k] = (net[k + 1] - net[k]) * (heatfac / (pz[k] - pz[k + 1]));
+    htr[L] = 0.;
+  }
+  coef_free(&c);
+  free(wkl), free(zc0), free(zw1), free(ta), free(ga), free(wa), free(bu), free(bd), free(bdir);
+  for (int i = 0; i < 5; i++) free(pbb[i]);
+  return 0;
 }

@@ -59,6 +59,17 @@ int orc_rrtmg_sw_ext(int ncol, int nlay, int *icld, int *iaer,
                      const double *indsolvar, const double *solcycfrac, double *bandu,
                      double *bandd, double *dirdflx, int idelm);
 
+/* Column-mode driver (src/rrtmg_sw.1col.f90:634-821) for clear skies on layer amounts:
+ * pavel/tavel/coldry (nlay), pz (nlay+1, surface first), wkl7 (7, nlay) molecules/cm2,
+ * semiss (14). out: (6, nlay+1, 15) = up, difdown, dirdown, down, net, heating rate for the
+ * broadband (block 0) and bands 16..29 (blocks 1..14), levels from the surface. */
+int orc_column_sw(int nlay, const double *pavel, const double *tavel, const double *pz,
+                  const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                  int juldat, double scon, int isolvar, const double *bndsolvar,
+                  const double *indsolvar, const double *solcycfrac, int iaer,
+                  const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                  double cpdair, double *out);
+
 /* mcica_subcol_sw(...) -- src/mcica_subcol_gen_sw.f90:181-296 (+ :300-753). */
 int orc_mcica_subcol_sw(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
                         const double *play, const double *cldfrac, const double *ciwp,
