@@ -7,6 +7,10 @@ overlap (icld = 2) with the KISS generator, synthetic inputs of rrtmg_sw_amd.wor
 device entry point (rrtmgsw_mcica_sw_dev, FP64). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
+
+Both precision forms are timed in the same run; `value` is the one --precision names (FP64 by
+default), and `forms` carries both, their speed ratio and the mixed form's largest error
+against the FP64 form (fluxes W/m2, heating rates K/day).
 """
 import argparse
 import ctypes as C
@@ -70,7 +74,6 @@ def main():
     api.rrtmg_sw_ini(1004.64)
     cols = workload(args.workload)
     d = api.device_inputs(cols)
-    out = api.mcica_rrtmg_sw(cols, d, precision=args.precision)
 
     # cloudy (g, layer) cells of this workload, for the algorithmic FLOP count
     mc = api.mcica_subcol_sw(1, cols.ncol, cols.nlay, cols.icld, cols.permuteseed, cols.irng,
@@ -79,23 +82,31 @@ def main():
     cloudy = float(mc["cldfmcl"].sum().item())
     del mc
     torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        api.mcica_rrtmg_sw(cols, d, precision=args.precision, out=out)
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        api.mcica_rrtmg_sw(cols, d, precision=args.precision, out=out)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = wall / args.steps * 1e3
-    kern_ms = e0.elapsed_time(e1) / args.steps
-
     fl = flops.call_flops(cols.ncol, cols.nlay, cloudy)
+
+    def timed(precision):
+        """Warm-up, then args.steps calls bracketed by synchronisations; HIP events on the
+        stream the kernels are launched on (torch's current stream)."""
+        out = api.mcica_rrtmg_sw(cols, d, precision=precision)
+        for _ in range(args.warmup):
+            api.mcica_rrtmg_sw(cols, d, precision=precision, out=out)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            api.mcica_rrtmg_sw(cols, d, precision=precision, out=out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return out, wall / args.steps * 1e3, e0.elapsed_time(e1) / args.steps
+
+    other = "mixed" if args.precision == "fp64" else "fp64"
+    out, ms, kern_ms = timed(args.precision)
+    out_o, ms_o, kern_ms_o = timed(other)
+    o64, omx = (out, out_o) if args.precision == "fp64" else (out_o, out)
+    err = {k: float((omx[k] - o64[k]).abs().max().item()) for k in o64}
     achieved = fl / (kern_ms * 1e-3) / 1e12
     line = {
         "metric": "McICA columns/s at 72 layers, FP32 vs FP64 GPU at <=0.1 W/m2; adjoint/fwd ratio",
@@ -111,11 +122,20 @@ def main():
                    "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "rt_kernel_fp64",
+                     "kernel": f"rt_kernel<{'false' if args.precision == 'fp64' else 'true'}>",
                      "note": "FP64 VALU work (no dense contraction): algorithmic FLOPs of "
                              "rrtmg_sw_amd/flops.py over the per-step event time on the launch "
                              "stream; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s",
                      "flops_per_step": fl, "event_ms_per_step": kern_ms},
+        "forms": {
+            args.precision: {"columns_per_s": cols.ncol / (ms * 1e-3), "ms_per_step": ms,
+                             "event_ms_per_step": kern_ms},
+            other: {"columns_per_s": cols.ncol / (ms_o * 1e-3), "ms_per_step": ms_o,
+                    "event_ms_per_step": kern_ms_o},
+            "mixed_over_fp64_speed": (ms if args.precision == "fp64" else ms_o) /
+                                     (ms_o if args.precision == "fp64" else ms),
+            "mixed_max_abs_error_vs_fp64": err,
+        },
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cols)

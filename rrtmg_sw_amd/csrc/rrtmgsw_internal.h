@@ -52,6 +52,7 @@ struct DeviceTables {
   const double *tab;        // k-tables, solar tables, cloud optics
   const double *exp_tbl;    // NTBL+1 entries
   const double *rexp_tbl;   // 1 / exp_tbl, NTBL+1 entries
+  const float *tab32, *exp32, *rexp32;   // FP32 copies of tab, exp_tbl, rexp_tbl (mixed form)
   const GPoint *gp;         // NGPT records
   // cloud optics offsets (tables (n, 16:29) column-major: value(i, band) at off + (i-1) + n*(band-16))
   int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;
@@ -101,6 +102,7 @@ struct RtArgs {
   double *swuflx, *swdflx, *swhr, *swuflxc, *swdflxc, *swhrc;
   double *scratch;               // per (column, layer, coefficient, g) elimination coefficients
   int *status;                   // first error code raised inside a kernel
+  double thin_pdp;               // mixed form: layers thinner than this (hPa) run in FP64
   SwScalars sc;
 };
 
@@ -114,6 +116,7 @@ struct McicaArgs {
 
 // Kernel launchers (rrtmgsw_kernels.hip)
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, hipStream_t s);
 

@@ -37,7 +37,7 @@ constexpr double CLDMIN = 1.e-20;           // src/rrtmg_sw_cldprmc.f90:118
 struct ColLayout {
   int L;
   int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
-  int o_fac, o_col, o_red, o_reda, o_mask, total;     // double blocks / mask words
+  int o_fac, o_col, o_red, o_reda, o_thin, o_mask, total;   // double blocks / mask words
   __host__ __device__ ColLayout(int nlay, int nmaskw) {
     L = nlay;
     int ib = 0;
@@ -52,6 +52,7 @@ struct ColLayout {
     o_col = bytes, bytes += 6 * L * 8;     // h2o co2 o3 ch4 o2 colmol
     o_red = bytes, bytes += 2 * (L + 1) * 4 * 8;
     o_reda = bytes, bytes += 2 * L * 2 * 8;
+    o_thin = bytes, bytes += ((L + 31) / 32) * 4;
     o_mask = bytes, bytes += nmaskw * NGPT * 4;
     total = bytes;
   }
@@ -61,7 +62,7 @@ struct ColView {
   int L;
   int *jp, *jt, *jt1, *inds, *indf, *misc;
   double *fac, *col, *red, *reda;
-  uint32_t *mask;
+  uint32_t *mask, *thin;
   __device__ ColView(char *smem, const ColLayout &lo) {
     L = lo.L;
     int *ip = reinterpret_cast<int *>(smem);
@@ -72,6 +73,7 @@ struct ColView {
     red = reinterpret_cast<double *>(smem + lo.o_red);
     reda = reinterpret_cast<double *>(smem + lo.o_reda);
     mask = reinterpret_cast<uint32_t *>(smem + lo.o_mask);
+    thin = reinterpret_cast<uint32_t *>(smem + lo.o_thin);
   }
   __device__ double &f(int k, int lay) { return fac[k * L + lay]; }   // 0..7 as listed above
   __device__ double &c(int s, int lay) { return col[s * L + lay]; }   // 0..4 species, 5 colmol
@@ -188,86 +190,98 @@ __device__ static int find_laysolfr(const GPoint &g, const ColView &v, int laytr
 // ------------------------------------------------------------------------------------------
 // Gas optics template: taug and taur of one g-point at one layer.
 // ------------------------------------------------------------------------------------------
-struct Spec {
-  double speccomb, fs;
+template <typename R>
+struct SpecT {
+  R speccomb, fs;
   int js;
 };
-__device__ inline Spec key_spec(double c1, double c2, double strrat, double mult, bool bin) {
+using Spec = SpecT<double>;
+// Binary species parameter (e.g. src/rrtmg_sw_taumol.f90:287-292): the species ratio, clipped
+// at oneminus, its bin js and the fraction fs (mod(specmult, 1) = specmult - trunc(specmult)
+// exactly for the non-negative values that occur).
+template <typename R>
+__device__ inline SpecT<R> key_spec_t(R c1, R c2, R strrat, R mult, bool bin) {
 #pragma clang fp contract(off)
-  Spec s;
-  double speccomb = c1 + strrat * c2;
-  double specparm = c1 / speccomb;
-  if (specparm >= ONEMINUS) specparm = ONEMINUS;
-  double specmult = mult * specparm;
+  SpecT<R> s;
+  R speccomb = c1 + strrat * c2;
+  R specparm = c1 / speccomb;
+  if (specparm >= R(ONEMINUS)) specparm = R(ONEMINUS);
+  R specmult = mult * specparm;
   int jsb = 1 + (int)specmult;
-  double fsb = fmod(specmult, 1.0);
+  R fsb = specmult - trunc(specmult);
   s.speccomb = speccomb;
   s.js = bin ? jsb : 1;
-  s.fs = bin ? fsb : 0.0;
+  s.fs = bin ? fsb : R(0);
   return s;
+}
+__device__ inline Spec key_spec(double c1, double c2, double strrat, double mult, bool bin) {
+  return key_spec_t<double>(c1, c2, strrat, mult, bin);
 }
 
 // 8-corner interpolation of a key-species table. Corner strides: cjs (species step: 1 for
 // binary species, 0 otherwise) and nsp (temperature step); st = table row stride (0: unused).
-__device__ inline double kinterp(const double *__restrict__ A, int st, int ind0, int ind1, int cjs,
-                                 int nsp, double fs, double fac00, double fac10, double fac01,
-                                 double fac11) {
-  double f0 = 1.0 - fs;
-  double a0 = A[(ind0 - 1) * st], a1 = A[(ind0 - 1 + cjs) * st];
-  double a2 = A[(ind0 - 1 + nsp) * st], a3 = A[(ind0 - 1 + nsp + cjs) * st];
-  double b0 = A[(ind1 - 1) * st], b1 = A[(ind1 - 1 + cjs) * st];
-  double b2 = A[(ind1 - 1 + nsp) * st], b3 = A[(ind1 - 1 + nsp + cjs) * st];
+template <typename R>
+__device__ inline R kinterp(const R *__restrict__ A, int st, int ind0, int ind1, int cjs, int nsp,
+                            R fs, R fac00, R fac10, R fac01, R fac11) {
+  R f0 = R(1) - fs;
+  R a0 = A[(ind0 - 1) * st], a1 = A[(ind0 - 1 + cjs) * st];
+  R a2 = A[(ind0 - 1 + nsp) * st], a3 = A[(ind0 - 1 + nsp + cjs) * st];
+  R b0 = A[(ind1 - 1) * st], b1 = A[(ind1 - 1 + cjs) * st];
+  R b2 = A[(ind1 - 1 + nsp) * st], b3 = A[(ind1 - 1 + nsp + cjs) * st];
   return f0 * fac00 * a0 + fs * fac00 * a1 + f0 * fac10 * a2 + fs * fac10 * a3 +
          f0 * fac01 * b0 + fs * fac01 * b1 + f0 * fac11 * b2 + fs * fac11 * b3;
 }
 
-__device__ inline void gas_optics(const GPoint &g, const double *__restrict__ tab, ColView &v,
-                                  int lay, bool lower, double &taug, double &taur) {
+// taug and taur of one g-point at one layer. R = double reproduces the reference's FP64
+// arithmetic; R = float is the gas optics of the mixed form (FP32 copies of the tables).
+template <typename R>
+__device__ inline void gas_optics(const GPoint &g, const R *__restrict__ tab, ColView &v, int lay,
+                                  bool lower, R &taug, R &taur) {
   // A layer below laytrop has jp <= 12 and one above has jp >= 13 whenever pressure decreases
   // with height; the clamps keep table reads in bounds for profiles that break that order.
   const int jp = lower ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
   const int jt = v.jt[lay], jt1 = v.jt1[lay];
-  const double fac00 = v.f(0, lay), fac01 = v.f(1, lay), fac10 = v.f(2, lay), fac11 = v.f(3, lay);
-  const double colh2o = v.c(SP_H2O, lay), colmol = v.c(5, lay);
+  const R fac00 = R(v.f(0, lay)), fac01 = R(v.f(1, lay)), fac10 = R(v.f(2, lay)), fac11 = R(v.f(3, lay));
+  const R colh2o = R(v.c(SP_H2O, lay)), colmol = R(v.c(5, lay));
   if (lower) {
     bool bin = g.lo_mode == KEY_BIN;
-    Spec s = key_spec(v.c(g.lo_s1, lay), v.c(g.lo_s2, lay), g.lo_strrat, 8.0, bin);
-    double kmul = bin ? s.speccomb : v.c(g.lo_s1, lay) * g.lo_mult;
+    SpecT<R> s = key_spec_t<R>(R(v.c(g.lo_s1, lay)), R(v.c(g.lo_s2, lay)), R(g.lo_strrat), R(8), bin);
+    R kmul = bin ? s.speccomb : R(v.c(g.lo_s1, lay)) * R(g.lo_mult);
     int nsp = g.lo_nsp;
     int ind0 = ((jp - 1) * 5 + (jt - 1)) * nsp + s.js;
     int ind1 = (jp * 5 + (jt1 - 1)) * nsp + s.js;
-    double k = kinterp(tab + g.absa, g.sa, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
-    const double *S = tab + g.selfref;
-    const double *F = tab + g.forref;
+    R k = kinterp<R>(tab + g.absa, g.sa, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
+    const R *S = tab + g.selfref;
+    const R *F = tab + g.forref;
     int is = max(v.inds[lay], 1), jf = min(v.indf[lay], 2);
-    double s0 = S[(is - 1) * g.sself], s1 = S[is * g.sself];
-    double f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
-    double cont = v.f(4, lay) * (s0 + v.f(5, lay) * (s1 - s0)) + v.f(6, lay) * (f0 + v.f(7, lay) * (f1 - f0));
-    taug = kmul * k + colh2o * cont + v.c(g.lo_minor, lay) * g.lo_minor_coef;
-    double ray = g.rayl_lo;
+    R s0 = S[(is - 1) * g.sself], s1 = S[is * g.sself];
+    R f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
+    R cont = R(v.f(4, lay)) * (s0 + R(v.f(5, lay)) * (s1 - s0)) + R(v.f(6, lay)) * (f0 + R(v.f(7, lay)) * (f1 - f0));
+    taug = kmul * k + colh2o * cont + R(v.c(g.lo_minor, lay)) * R(g.lo_minor_coef);
+    R ray = R(g.rayl_lo);
     if (g.rayl_js) {
-      const double *RA = tab + g.rayla;
-      double r0 = RA[(s.js - 1) * g.sra], r1 = RA[s.js * g.sra];
+      const R *RA = tab + g.rayla;
+      R r0 = RA[(s.js - 1) * g.sra], r1 = RA[s.js * g.sra];
       ray = r0 + s.fs * (r1 - r0);
     }
     taur = colmol * ray;
   } else {
     bool bin = g.hi_mode == KEY_BIN;
-    Spec s = key_spec(v.c(g.hi_s1, lay), v.c(g.hi_s2, lay), g.hi_strrat, 4.0, bin);
-    double kmul = bin ? s.speccomb : v.c(g.hi_s1, lay) * g.hi_mult;
+    SpecT<R> s = key_spec_t<R>(R(v.c(g.hi_s1, lay)), R(v.c(g.hi_s2, lay)), R(g.hi_strrat), R(4), bin);
+    R kmul = bin ? s.speccomb : R(v.c(g.hi_s1, lay)) * R(g.hi_mult);
     int nsp = g.hi_nsp;
     int ind0 = ((jp - 13) * 5 + (jt - 1)) * nsp + s.js;
     int ind1 = ((jp - 12) * 5 + (jt1 - 1)) * nsp + s.js;
-    double k = kinterp(tab + g.absb, g.sb, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
-    double fo = 0.0;
+    R k = kinterp<R>(tab + g.absb, g.sb, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
+    R fo = R(0);
     if (g.hi_for) {
-      const double *F = tab + g.forref;
+      const R *F = tab + g.forref;
       int jf = min(max(v.indf[lay], 1), 3);
-      double f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
-      fo = colh2o * v.f(6, lay) * (f0 + v.f(7, lay) * (f1 - f0));
+      R f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
+      fo = colh2o * R(v.f(6, lay)) * (f0 + R(v.f(7, lay)) * (f1 - f0));
     }
-    taug = kmul * k + fo + v.c(g.hi_minor, lay) * g.hi_minor_coef;
-    taur = colmol * g.rayl_hi;
+    taug = kmul * k + fo + R(v.c(g.hi_minor, lay)) * R(g.hi_minor_coef);
+    taur = colmol * R(g.rayl_hi);
   }
 }
 
@@ -541,6 +555,125 @@ __device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A
   s.prdnd = prdnd;
 }
 
+// ------------------------------------------------------------------------------------------
+// FP32 pieces of the mixed form. The layer optics carry the co-albedo c = 1 - w explicitly
+// (from the absorbing optical depths, so it keeps full relative precision when w is close to
+// one), which removes the cancellations of the two-stream expressions in FP32:
+//   gamma1 - gamma2 = 2c (PIFM), so k = sqrt(2c (gamma1 + gamma2));
+//   (k + gamma1) / e - (gamma1 - k) e = gamma1 (1 - e^2)/e + k (1/e + e) with 1 - e from the
+//   expansion or the table, and likewise for the diffuse denominator;
+//   zwo >= zwcrit  <=>  c (1 - zwcrit h^2) <= 1 - zwcrit,  h = g / (1 - g)
+// (algebraically the reference's quantities, src/rrtmg_sw_reftra.f90:164-322).
+// ------------------------------------------------------------------------------------------
+__device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+struct Exf {
+  float e, r, om;   // exp(-x) from the table (or its expansion), 1/e, 1 - e
+};
+__device__ inline Exf tbl_expf(float ze, const float *__restrict__ et, const float *__restrict__ ret) {
+  if (ze <= float(OD_LO)) {
+    float om = ze - 0.5f * ze * ze;
+    float e = 1.f - om;
+    return {e, rcpf(e), om};
+  }
+  float tblind = ze * rcpf(float(BPADE) + ze);
+  int itind = (int)(10000.f * tblind + 0.5f);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
+  float e = et[itind];
+  return {e, ret[itind], 1.f - e};
+}
+
+struct RT4f {
+  float r, rd, t, td;
+};
+// |1 - k mu0| < 2e-3 for the layer's two-stream eigenvalue k (general branch only; the
+// conservative branch has no such denominator).
+__device__ inline bool near_singular(float zw, float zc, float zg, float mu0) {
+  float zgamma1 = (8.f - zw * (5.f + 3.f * zg)) * 0.25f;
+  float zgamma2 = 3.f * (zw * (1.f - zg)) * 0.25f;
+  float k = __builtin_sqrtf(2.f * zc * (zgamma1 + zgamma2));
+  return fabsf(1.f - k * mu0) < 2.e-3f;
+}
+__device__ inline RT4f reftra_f(float zto1, float zw, float zc, float zg, float prmuz, float rmu,
+                                const float *__restrict__ et, const float *__restrict__ ret) {
+  const float zwcrit = 0.9999995f, eps = 1.e-08f;
+  RT4f o;
+  float zg3 = 3.f * zg;
+  float zgamma1 = (8.f - zw * (5.f + zg3)) * 0.25f;
+  float zgamma2 = 3.f * (zw * (1.f - zg)) * 0.25f;
+  float zgamma3 = (2.f - zg3 * prmuz) * 0.25f;
+  float zgamma4 = 1.f - zgamma3;
+  bool cons;
+  if (zg != 1.f) {
+    float h = zg * rcpf(1.f - zg);
+    float h2 = h * h;
+    float den = 1.f - zc * h2;
+    cons = zw > 0.f && den > 0.f && zc * ((1.f - h2) + (1.f - zwcrit) * h2) <= (1.f - zwcrit);
+  } else {
+    cons = zw >= zwcrit;
+  }
+  if (cons) {
+    float za1 = zgamma1 * prmuz - zgamma3;
+    float zgt = zgamma1 * zto1;
+    Exf e2 = tbl_expf(fminf(zto1 * rmu, 500.f), et, ret);
+    float inv = rcpf(1.f + zgt);
+    o.r = (zgt - za1 * e2.om) * inv;
+    o.t = 1.f - o.r;
+    o.rd = zgt * inv;
+    o.td = 1.f - o.rd;
+    if (e2.e == 1.f) o.r = 0.f, o.t = 1.f, o.rd = 0.f, o.td = 1.f;
+  } else {
+    float za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+    float za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+    float zrk = __builtin_sqrtf(2.f * zc * (zgamma1 + zgamma2));
+    float zrp = zrk * prmuz;
+    float zrp1 = 1.f + zrp, zrm1 = 1.f - zrp, zrk2 = 2.f * zrk;
+    float zrpp = 1.f - zrp * zrp;
+    float zr1 = zrm1 * (za2 + zrk * zgamma3);
+    float zr2 = zrp1 * (za2 - zrk * zgamma3);
+    float zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+    float zt1 = zrp1 * (za1 + zrk * zgamma4);
+    float zt2 = zrm1 * (za1 - zrk * zgamma4);
+    float zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+    Exf e1 = tbl_expf(fminf(zrk * zto1, 500.f), et, ret);
+    Exf e2 = tbl_expf(fminf(zto1 * rmu, 500.f), et, ret);
+    // zr4 / e1 + zr5 e1 = zrpp (g1 (1/e1 - e1) + k (1/e1 + e1)), 1/e1 - e1 = (1 - e1)(1 + e1)/e1
+    float zdenr = zrpp * (zgamma1 * (e1.om * (1.f + e1.e) * e1.r) + zrk * (e1.r + e1.e));
+    if (zdenr >= -eps && zdenr <= eps) {
+      o.r = eps;
+      o.t = e2.e;
+    } else {
+      float rden = rcpf(zdenr);
+      o.r = zw * (zr1 * e1.r - zr2 * e1.e - zr3 * e2.e) * rden;
+      o.t = e2.e - e2.e * zw * (zt1 * e1.r - zt2 * e1.e - zt3 * e2.r) * rden;
+    }
+    // (k + g1) - (g1 - k) e1^2 = k (1 + e1^2) + g1 (1 - e1^2)
+    float om2 = e1.om * (1.f + e1.e);
+    float zdend = rcpf(zrk * (1.f + e1.e * e1.e) + zgamma1 * om2);
+    o.rd = zgamma2 * om2 * zdend;
+    o.td = zrk2 * e1.e * zdend;
+  }
+  return o;
+}
+
+struct SweepF {
+  float tdbt, ztdn, prdnd;
+};
+__device__ inline void sweep_layer_f(SweepF &s, const RT4f &l, float dbt, float &A, float &B,
+                                     float &Cc, float &E) {
+  A = s.ztdn;
+  B = s.prdnd;
+  float a = s.ztdn - s.tdbt;
+  float zreflect = rcpf(1.f - l.rd * s.prdnd);
+  Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
+  E = l.td * zreflect;
+  float ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;
+  float prdnd = l.rd + l.td * l.td * s.prdnd * zreflect;
+  s.tdbt = dbt * s.tdbt;
+  s.ztdn = ztdn;
+  s.prdnd = prdnd;
+}
+
 // Sum four values over the 64 lanes of a wavefront with 7 exchanges: after the first two
 // exchanges each lane carries one of the four partial sums; lanes 0, 16, 32, 48 end with
 // the totals of values 0, 1, 2, 3.
@@ -667,7 +800,8 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
 // ------------------------------------------------------------------------------------------
 // The radiative-transfer kernel.
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) rt_kernel_fp64(RtArgs a, DeviceTables T) {
+template <bool MIXED>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) rt_kernel(RtArgs a, DeviceTables T) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
   const int nmw = (L + 31) / 32;
@@ -685,7 +819,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
 
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band ----
   if (tid == 0) v.misc[M_LAYTROP] = 0;
+  if (MIXED)
+    for (int w = tid; w < (L + 31) / 32; w += blockDim.x) v.thin[w] = 0u;
   __syncthreads();
+  if (MIXED)   // layers (top-based j) thin enough in pressure to need FP64 flux divergences
+    for (int j = tid; j < L; j += blockDim.x) {
+      int lay = L - 1 - j;
+      double pdp = a.plev[c + (size_t)C * lay] - a.plev[c + (size_t)C * (lay + 1)];
+      if (pdp < a.thin_pdp) atomicOr(&v.thin[j >> 5], 1u << (j & 31));
+    }
   for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, c, lay, v, &v.misc[M_LAYTROP]);
   __syncthreads();
   const int laytrop = v.misc[M_LAYTROP];
@@ -720,8 +862,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
   double *scr = a.scratch + (size_t)cl * L * SCR_COEFS * NGPT;
   for (int j = 0; j < L; j++) {
     const int lay = L - 1 - j;
+    const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);   // uniform over the workgroup
     double taug, taur;
-    gas_optics(gp, T.tab, v, lay, lay < laytrop, taug, taur);
+    float taugf = 0.f, taurf = 0.f;
+    if (MIXED && !thin) {
+      gas_optics<float>(gp, T.tab32, v, lay, lay < laytrop, taugf, taurf);
+      taug = taugf, taur = taurf;
+    } else {
+      gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
+    }
     // aerosol (src/rrtmg_sw_rad.f90:709-765)
     double ta = 0.0, wa = 1.0, ga = 0.0;
     if (a.sc.iaer == 10) {
@@ -745,15 +894,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
         ta = t6;
       }
     }
-    // clear-sky optics and delta scaling (src/rrtmg_sw_spcvmc.f90:442-496)
-    double tauc = taur + taug + ta;
-    double omc = taur * 1.0 + ta * wa;
-    double gcc = ga * wa * ta / omc;
-    omc = omc / tauc;
-    double zf = gcc * gcc, zwf = omc * zf;
-    tauc = (1.0 - zwf) * tauc;
-    omc = (omc - zwf) / (1.0 - zwf);
-    gcc = (gcc - zf) / (1.0 - zf);
     // cloud optics of this sub-column (cldprmc_sw) and total-sky optics (:500-505)
     double cf = 0.0, ct = 0.0, cw = 1.0, cg = 0.0;
     if (a.sc.icld >= 1 && active) {
@@ -782,44 +922,142 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
         ct = tin, cw = win, cg = gin;
       }
     }
-    RT4 rc = reftra1(tauc, omc, gcc, mu0, rmu, T.exp_tbl, T.rexp_tbl);
-    double dbc = tbl_exp2(tauc * rmu, T.exp_tbl, T.rexp_tbl).e;
-    Sweep pre = sc_;
-    double Bc, Cc, Ec, Ac;
-    sweep_layer(sc_, rc, dbc, Ac, Bc, Cc, Ec);
-    double At = Ac;
-    if (cf > 0.0 && jfirst == L) {
-      jfirst = j;
-      st_ = pre;
-    }
-    if (jfirst <= j) {
-      double tauo = tauc + ct;
-      double omo = tauc * omc + ct * cw;
-      double go = (ct * cw * cg + tauc * omc * gcc) / omo;
-      omo = omo / tauo;
-      RT4 rt = rc;
-      double dbt = dbc;
-      if (cf > 0.0) {
-        double dbo = tbl_exp2(tauo * rmu, T.exp_tbl, T.rexp_tbl).e;
-        RT4 ro = {0.0, 0.0, 1.0, 1.0};
-        if (cf > REPCLC) ro = reftra1(tauo, omo, go, mu0, rmu, T.exp_tbl, T.rexp_tbl);
-        double zcl = 1.0 - cf;
-        rt.r = zcl * rc.r + cf * ro.r;
-        rt.rd = zcl * rc.rd + cf * ro.rd;
-        rt.t = zcl * rc.t + cf * ro.t;
-        rt.td = zcl * rc.td + cf * ro.td;
-        dbt = zcl * dbc + cf * dbo;
+    // Mixed form: FP32 layer optics with the co-albedo carried explicitly. Where k mu0 is close to
+    // one the two-stream solution divides by 1 - (k mu0)^2: the reference's removable singularity
+    // turns the exponential-table error into a large reflectance, so the result there depends on
+    // its inputs to the last bits. Such lanes redo the layer in FP64 from the FP64 gas optics.
+    float tauc32 = 0.f, omc32 = 0.f, ccl32 = 0.f, gcc32 = 0.f;
+    bool use64 = !MIXED || thin;
+    if (MIXED && !thin) {
+      float taf = float(ta), waf = float(wa), gaf = float(ga);
+      float tauc = taurf + taugf + taf;
+      float scat = taurf + taf * waf;
+      float coal = taugf + taf * (1.f - waf);
+      float gcc = gaf * waf * taf * rcpf(scat);
+      float rtau = rcpf(tauc);
+      float omc = scat * rtau, ccl = coal * rtau;
+      float zf = gcc * gcc, zwf = omc * zf;
+      float r1 = rcpf(1.f - zwf);
+      tauc = (1.f - zwf) * tauc;
+      omc = (omc - zwf) * r1;
+      ccl = ccl * r1;
+      gcc = gcc * rcpf(1.f + gcc);   // (g - g^2) / (1 - g^2)
+      tauc32 = tauc, omc32 = omc, ccl32 = ccl, gcc32 = gcc;
+      bool sing = near_singular(omc, ccl, gcc, float(mu0));
+      if (cf > REPCLC && !sing) {
+        float ctf = float(ct), cwf = float(cw), cgf = float(cg), ccw = float(1.0 - cw);
+        float scato = tauc * omc + ctf * cwf;
+        float coalo = tauc * ccl + ctf * ccw;
+        float go = (ctf * cwf * cgf + tauc * omc * gcc) * rcpf(scato);
+        float rto = rcpf(tauc + ctf);
+        sing = near_singular(scato * rto, coalo * rto, go, float(mu0));
       }
-      double Bt, Ct, Et;
-      sweep_layer(st_, rt, dbt, At, Bt, Ct, Et);
-      if (active) {
-        double *p = scr + ((size_t)j * SCR_COEFS + 3) * NGPT + g;
-        p[0] = Bt, p[NGPT] = Ct, p[2 * NGPT] = Et;
+      if (sing) {
+        use64 = true;
+        gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
       }
     }
-    if (active) {
-      double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
-      p[0] = Bc, p[NGPT] = Cc, p[2 * NGPT] = Ec;
+    double Ac, At;
+    if (use64) {
+      // clear-sky optics and delta scaling (src/rrtmg_sw_spcvmc.f90:442-496)
+      double tauc = taur + taug + ta;
+      double omc = taur * 1.0 + ta * wa;
+      double gcc = ga * wa * ta / omc;
+      omc = omc / tauc;
+      double zf = gcc * gcc, zwf = omc * zf;
+      tauc = (1.0 - zwf) * tauc;
+      omc = (omc - zwf) / (1.0 - zwf);
+      gcc = (gcc - zf) / (1.0 - zf);
+      RT4 rc = reftra1(tauc, omc, gcc, mu0, rmu, T.exp_tbl, T.rexp_tbl);
+      double dbc = tbl_exp2(tauc * rmu, T.exp_tbl, T.rexp_tbl).e;
+      Sweep pre = sc_;
+      double Bc, Cc, Ec;
+      sweep_layer(sc_, rc, dbc, Ac, Bc, Cc, Ec);
+      At = Ac;
+      if (cf > 0.0 && jfirst == L) {
+        jfirst = j;
+        st_ = pre;
+      }
+      if (jfirst <= j) {
+        double tauo = tauc + ct;
+        double omo = tauc * omc + ct * cw;
+        double go = (ct * cw * cg + tauc * omc * gcc) / omo;
+        omo = omo / tauo;
+        RT4 rt = rc;
+        double dbt = dbc;
+        if (cf > 0.0) {
+          double dbo = tbl_exp2(tauo * rmu, T.exp_tbl, T.rexp_tbl).e;
+          RT4 ro = {0.0, 0.0, 1.0, 1.0};
+          if (cf > REPCLC) ro = reftra1(tauo, omo, go, mu0, rmu, T.exp_tbl, T.rexp_tbl);
+          double zcl = 1.0 - cf;
+          rt.r = zcl * rc.r + cf * ro.r;
+          rt.rd = zcl * rc.rd + cf * ro.rd;
+          rt.t = zcl * rc.t + cf * ro.t;
+          rt.td = zcl * rc.td + cf * ro.td;
+          dbt = zcl * dbc + cf * dbo;
+        }
+        double Bt, Ct, Et;
+        sweep_layer(st_, rt, dbt, At, Bt, Ct, Et);
+        if (active && MIXED && !thin) {   // FP32 storage of the mixed form's thick layers
+          float *pf = reinterpret_cast<float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
+          pf[3 * NGPT] = float(Bt), pf[4 * NGPT] = float(Ct), pf[5 * NGPT] = float(Et);
+        } else if (active) {
+          double *p = scr + ((size_t)j * SCR_COEFS + 3) * NGPT + g;
+          p[0] = Bt, p[NGPT] = Ct, p[2 * NGPT] = Et;
+        }
+      }
+      if (active && MIXED && !thin) {
+        float *pf = reinterpret_cast<float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
+        pf[0] = float(Bc), pf[NGPT] = float(Cc), pf[2 * NGPT] = float(Ec);
+      } else if (active) {
+        double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
+        p[0] = Bc, p[NGPT] = Cc, p[2 * NGPT] = Ec;
+      }
+    } else {
+      const float tauc = tauc32, omc = omc32, ccl = ccl32, gcc = gcc32;
+      const float mu0f = float(mu0), rmuf = float(rmu);
+      RT4f rc = reftra_f(tauc, omc, ccl, gcc, mu0f, rmuf, T.exp32, T.rexp32);
+      float dbc = tbl_expf(tauc * rmuf, T.exp32, T.rexp32).e;
+      Sweep pre = sc_;
+      SweepF sf = {float(sc_.tdbt), float(sc_.ztdn), float(sc_.prdnd)};
+      float Af, Bc, Cc, Ec;
+      sweep_layer_f(sf, rc, dbc, Af, Bc, Cc, Ec);
+      sc_ = {double(sf.tdbt), double(sf.ztdn), double(sf.prdnd)};
+      Ac = Af, At = Af;
+      if (cf > 0.0 && jfirst == L) {
+        jfirst = j;
+        st_ = pre;
+      }
+      float *pf = reinterpret_cast<float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
+      if (jfirst <= j) {
+        RT4f rt = rc;
+        float dbt = dbc;
+        if (cf > 0.0) {
+          float ctf = float(ct), cwf = float(cw), cgf = float(cg), ccw = float(1.0 - cw);
+          float tauo = tauc + ctf;
+          float scato = tauc * omc + ctf * cwf;
+          float coalo = tauc * ccl + ctf * ccw;
+          float go = (ctf * cwf * cgf + tauc * omc * gcc) * rcpf(scato);
+          float rto = rcpf(tauo);
+          float omo = scato * rto, cco = coalo * rto;
+          float dbo = tbl_expf(tauo * rmuf, T.exp32, T.rexp32).e;
+          RT4f ro = {0.f, 0.f, 1.f, 1.f};
+          if (cf > REPCLC) ro = reftra_f(tauo, omo, cco, go, mu0f, rmuf, T.exp32, T.rexp32);
+          float zcl = float(1.0 - cf), cff = float(cf);
+          rt.r = zcl * rc.r + cff * ro.r;
+          rt.rd = zcl * rc.rd + cff * ro.rd;
+          rt.t = zcl * rc.t + cff * ro.t;
+          rt.td = zcl * rc.td + cff * ro.td;
+          dbt = zcl * dbc + cff * dbo;
+        }
+        SweepF tf = {float(st_.tdbt), float(st_.ztdn), float(st_.prdnd)};
+        float Atf, Bt, Ct, Et;
+        sweep_layer_f(tf, rt, dbt, Atf, Bt, Ct, Et);
+        st_ = {double(tf.tdbt), double(tf.ztdn), double(tf.prdnd)};
+        At = Atf;
+        if (active) pf[3 * NGPT] = Bt, pf[4 * NGPT] = Ct, pf[5 * NGPT] = Et;
+      }
+      if (active) pf[0] = Bc, pf[NGPT] = Cc, pf[2 * NGPT] = Ec;
     }
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
     double sa = wave_sum2(zinc * At, zinc * Ac, lane);
@@ -840,7 +1078,23 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
   }
   for (int j = L - 1; j >= 0; j--) {
     double bu_c = 0.0, bu_t = 0.0;
-    if (active) {
+    const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);
+    if (MIXED && !thin && active) {
+      const float *p = reinterpret_cast<const float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
+      float Bc = p[0], Cc = p[NGPT], Ec = p[2 * NGPT];
+      float ucf = Cc + Ec * float(uc);
+      uc = ucf;
+      bu_c = double(Bc * ucf);
+      float utf;
+      if (j >= jfirst) {
+        utf = p[4 * NGPT] + p[5 * NGPT] * float(ut);
+        bu_t = double(p[3 * NGPT] * utf);
+      } else {
+        utf = Cc + Ec * float(ut);
+        bu_t = double(Bc * utf);
+      }
+      ut = utf;
+    } else if (active) {
       const double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
       double Bc = p[0], Cc = p[NGPT], Ec = p[2 * NGPT];
       uc = Cc + Ec * uc;
@@ -890,11 +1144,18 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
   }
 }
 
-hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
+template <bool MIXED>
+static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   ColLayout lo(a.nlay, a.fused ? (a.nlay + 31) / 32 : 0);
   int per = (a.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL(rt_kernel_fp64, dim3(per * 8), dim3(128), lo.total, s, a, t);
+  hipLaunchKernelGGL(rt_kernel<MIXED>, dim3(per * 8), dim3(128), lo.total, s, a, t);
   return hipGetLastError();
+}
+hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
+  return launch_rt<false>(a, t, s);
+}
+hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
+  return launch_rt<true>(a, t, s);
 }
 
 // ------------------------------------------------------------------------------------------
