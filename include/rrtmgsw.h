@@ -37,6 +37,9 @@ extern "C" {
 /* Precision forms of the device-resident entry points. */
 #define RRTMGSW_FP64 0
 #define RRTMGSW_MIXED 1
+/* FP64 with exact exponentials in place of the reference's lookup table: the smooth forward that
+ * rrtmgsw_mcica_sw_adjoint_dev differentiates (for finite-difference checks). */
+#define RRTMGSW_FP64_EXP 2
 
 /* Replaces rrtmg_sw_ini(cpdair), src/rrtmg_sw_init.f90:47-173: builds the 112-point k-tables
  * (224 -> 112 reduction), the exponential lookup table and the cloud optics tables, and
@@ -117,7 +120,7 @@ int rrtmgsw_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, dou
  * transfer kernel from the same KISS / Mersenne-Twister streams (no counterpart in the
  * reference, which always materializes them; the results are those of calling
  * rrtmgsw_mcica_subcol_dev then rrtmgsw_sw_dev). icld is the mcica_subcol_sw overlap flag
- * (0..5); precision is RRTMGSW_FP64 or RRTMGSW_MIXED. */
+ * (0..5); precision is RRTMGSW_FP64, RRTMGSW_MIXED or RRTMGSW_FP64_EXP. */
 int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
                          const double *play, const double *plev, const double *tlay,
                          const double *tlev, const double *tsfc, const double *h2ovmr,
@@ -134,6 +137,31 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
                          double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
                          double *swhrc, const double *bndsolvar, const double *indsolvar,
                          const double *solcycfrac, int precision, void *stream);
+
+/* Adjoint of rrtmgsw_mcica_sw_dev (precision RRTMGSW_FP64_EXP) for the objective
+ *   J = sum_c seed_toa(c) * swuflx(c, nlay+1) + sum_(c,k) seed_hr(c, k) * swhr(c, k)
+ * (total sky; seeds are device arrays (ncol) and (ncol, nlay)). Writes dJ/d tlay, h2ovmr,
+ * o3vmr, clwp and ciwp (the in-cloud water paths of mcica_subcol_sw), each (ncol, nlay), and
+ * dJ/d (asdir, asdif, aldir, aldif) as d_alb (ncol, 4). McICA masks are held fixed. No
+ * counterpart in the reference; inputs as rrtmgsw_mcica_sw_dev (same argument order). */
+int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                                 const double *play, const double *plev, const double *tlay,
+                                 const double *tlev, const double *tsfc, const double *h2ovmr,
+                                 const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                                 const double *n2ovmr, const double *o2vmr, const double *asdir,
+                                 const double *asdif, const double *aldir, const double *aldif,
+                                 const double *coszen, double adjes, int dyofyr, double scon,
+                                 int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                                 const double *cldfrac, const double *ciwp, const double *clwp,
+                                 const double *rei, const double *rel, const double *tauc,
+                                 const double *ssac, const double *asmc, const double *fsfc,
+                                 const double *alpha, const double *tauaer, const double *ssaaer,
+                                 const double *asmaer, const double *ecaer,
+                                 const double *bndsolvar, const double *indsolvar,
+                                 const double *solcycfrac, const double *seed_toa,
+                                 const double *seed_hr, double *d_tlay, double *d_h2ovmr,
+                                 double *d_o3vmr, double *d_clwp, double *d_ciwp, double *d_alb,
+                                 void *stream);
 
 #ifdef __cplusplus
 }

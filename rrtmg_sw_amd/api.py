@@ -148,7 +148,7 @@ def get_alpha(iplon, ncol, nlayers, icld, idcor, decorr_con, dz, lat, juldat, cl
     return alpha
 
 
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "exact": 2}
 
 
 def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None):
@@ -173,6 +173,39 @@ def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None):
     args += [_ptr(out[k]) for k in _lib.SW_OUT]
     args += [None, None, None, C.c_int(PRECISION[precision]), _stream()]
     check(_lib.lib().rrtmgsw_mcica_sw_dev(*args))
+    return out
+
+
+def _fused_inputs(cols, d, irng):
+    args = [C.c_int(cols.ncol), C.c_int(cols.nlay), C.c_int(cols.icld), C.c_int(cols.iaer),
+            C.c_int(cols.permuteseed), C.byref(irng)]
+    args += [_ptr(d.get(k)) for k in _lib.SW_PROFILE]
+    args += [C.c_double(cols.adjes), C.c_int(cols.dyofyr), C.c_double(cols.scon),
+             C.c_int(cols.isolvar), C.c_int(cols.inflgsw), C.c_int(cols.iceflgsw),
+             C.c_int(cols.liqflgsw)]
+    args += [_ptr(d.get(k)) for k in _lib.FUSED_CLD]
+    args += [_ptr(d.get(k)) for k in _lib.SW_AER]
+    return args
+
+
+ADJ_INPUTS = ("tlay", "h2ovmr", "o3vmr", "clwp", "ciwp")
+
+
+def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None):
+    """Adjoint of the fused call (exact-exponential FP64 forward, masks fixed):
+    J = sum seed_toa * swuflx(:, top) + sum seed_hr * swhr. seed_toa is a device tensor of
+    shape (ncol,), seed_hr one of Fortran shape (ncol, nlay) (i.e. torch shape (nlay, ncol)).
+    Returns device tensors dJ/d{tlay, h2ovmr, o3vmr, clwp, ciwp} (Fortran (ncol, nlay)) and
+    dJ/d albedo ("alb", Fortran (ncol, 4): asdir, asdif, aldir, aldif)."""
+    N, L = cols.ncol, cols.nlay
+    if out is None:
+        out = {k: _new(True, (N, L)) for k in ADJ_INPUTS}
+        out["alb"] = _new(True, (N, 4))
+    ir = C.c_int(cols.irng)
+    args = _fused_inputs(cols, dev_arrays, ir)
+    args += [None, None, None, _ptr(seed_toa, N), _ptr(seed_hr, N * L)]
+    args += [_ptr(out[k]) for k in ADJ_INPUTS] + [_ptr(out["alb"]), _stream()]
+    check(_lib.lib().rrtmgsw_mcica_sw_adjoint_dev(*args))
     return out
 
 

@@ -49,7 +49,7 @@ struct Context {
   float *d_tab32 = nullptr, *d_exp32 = nullptr, *d_rexp32 = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
-  std::map<hipStream_t, Scratch> scratch, status, mtbuf;
+  std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork;
   std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
 } ctx;
 
@@ -326,7 +326,7 @@ struct SwCall {
 // 0.003 K/day at 3 hPa).
 constexpr double THIN_PDP_HPA = 3.0;
 
-int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, bool mixed = false) {
+int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int precision = RRTMGSW_FP64) {
   int chunk = chunk_columns(base.ncol, base.nlay);
   void *scr = nullptr, *st = nullptr;
   {
@@ -342,7 +342,9 @@ int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, bool m
   for (int c0 = 0; c0 < base.ncol; c0 += chunk) {
     a.col0 = c0;
     a.ncol_chunk = std::min(chunk, base.ncol - c0);
-    HIPCHK(mixed ? launch_rt_mixed(a, tables, s) : launch_rt_fp64(a, tables, s));
+    HIPCHK(precision == RRTMGSW_MIXED     ? launch_rt_mixed(a, tables, s)
+           : precision == RRTMGSW_FP64_EXP ? launch_rt_exact(a, tables, s)
+                                           : launch_rt_fp64(a, tables, s));
   }
   return finish(s, (int *)st);
 }
@@ -416,7 +418,7 @@ int rrtmgsw_finalize(void) {
   hipFree(ctx.d_exp32);
   hipFree(ctx.d_rexp32);
   hipFree(ctx.d_gp);
-  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf}) {
+  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.adjwork}) {
     for (auto &kv : *m) hipFree(kv.second.ptr);
     m->clear();
   }
@@ -703,6 +705,60 @@ int rrtmgsw_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, dou
   return 0;
 }
 
+// Inputs, flags and McICA generator state of the fused entry points (mcica_subcol_sw followed by
+// rrtmg_sw): validation as the two reference routines do it, clamps, solar scalars.
+static int fused_setup(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                       const double *play, const double *plev, const double *tlay,
+                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                       const double *ch4vmr, const double *o2vmr, const double *asdir,
+                       const double *asdif, const double *aldir, const double *aldif,
+                       const double *coszen, double adjes, int dyofyr, double scon, int isolvar,
+                       int inflgsw, int iceflgsw, int liqflgsw, const double *cldfrac,
+                       const double *ciwp, const double *clwp, const double *rei,
+                       const double *rel, const double *tauc, const double *ssac,
+                       const double *asmc, const double *fsfc, const double *alpha,
+                       const double *tauaer, const double *ssaaer, const double *asmaer,
+                       const double *ecaer, const double *bndsolvar, const double *indsolvar,
+                       const double *solcycfrac, hipStream_t s, RtArgs &a, DeviceTables &t) {
+  if (icld < 0 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "MCICA_SUBCOL: INVALID ICLD");
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  int ir = irng ? *irng : 0;
+  if (ir != 0) ir = 1;
+  if (irng) *irng = ir;
+  if (iaer != 0 && iaer != 6 && iaer != 10) iaer = 0;
+  a.ncol = ncol, a.nlay = nlay;
+  a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
+  a.co2vmr = co2vmr, a.ch4vmr = ch4vmr, a.o2vmr = o2vmr;
+  a.asdir = asdir, a.asdif = asdif, a.aldir = aldir, a.aldif = aldif, a.coszen = coszen;
+  a.tauaer = tauaer, a.ssaaer = ssaaer, a.asmaer = asmaer, a.ecaer = ecaer;
+  a.fused = 1, a.mc_icld = icld, a.mc_irng = ir, a.permuteseed = permuteseed;
+  a.cldfrac = cldfrac, a.ciwp = ciwp, a.clwp = clwp, a.rei = rei, a.rel = rel;
+  a.tauc = tauc, a.ssac = ssac, a.asmc = asmc, a.fsfc = fsfc, a.alpha = alpha;
+  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
+  a.sc.heatfac = ctx.heatfac;
+  a.sc.icld = icld >= 1 ? 1 : 0, a.sc.iaer = iaer;
+  a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
+  if (icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+  if (ncol == 0) return 0;
+  const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
+                        aldir, aldif, coszen};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (icld >= 1 && (!cldfrac || !ciwp || !clwp || !rei || !rel || !tauc || !ssac || !asmc || !fsfc))
+    return fail(RRTMGSW_ERR_ARGUMENT, "cloud inputs are required when icld >= 1");
+  if (icld >= 4 && !alpha) return fail(RRTMGSW_ERR_ARGUMENT, "alpha is required when icld = 4 or 5");
+  if (iaer == 10 && (!tauaer || !ssaaer || !asmaer))
+    return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer are required when iaer = 10");
+  if (iaer == 6 && !ecaer) return fail(RRTMGSW_ERR_ARGUMENT, "ecaer is required when iaer = 6");
+  const uint32_t *mt = nullptr, *jumps = nullptr;
+  if (icld >= 1)
+    if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, ir, s, &mt, &jumps)) return rc;
+  a.mt_stream = mt;
+  t.kiss_jump = jumps;
+  return 0;
+}
+
 int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
                          const double *play, const double *plev, const double *tlay,
                          const double *tlev, const double *tsfc, const double *h2ovmr,
@@ -719,52 +775,83 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
                          double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
                          double *swhrc, const double *bndsolvar, const double *indsolvar,
                          const double *solcycfrac, int precision, void *stream) {
-  (void)tlev, (void)tsfc, (void)n2ovmr;
   if (int rc = check_init()) return rc;
-  if (precision != RRTMGSW_FP64 && precision != RRTMGSW_MIXED)
-    return fail(RRTMGSW_ERR_ARGUMENT, "precision must be RRTMGSW_FP64 or RRTMGSW_MIXED");
-  if (icld < 0 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "MCICA_SUBCOL: INVALID ICLD");
-  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
-    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
-  int ir = irng ? *irng : 0;
-  if (ir != 0) ir = 1;
-  if (irng) *irng = ir;
-  if (iaer != 0 && iaer != 6 && iaer != 10) iaer = 0;
-  if (ncol == 0) return 0;
+  if (precision != RRTMGSW_FP64 && precision != RRTMGSW_MIXED && precision != RRTMGSW_FP64_EXP)
+    return fail(RRTMGSW_ERR_ARGUMENT, "precision must be RRTMGSW_FP64, RRTMGSW_MIXED or RRTMGSW_FP64_EXP");
   hipStream_t s = (hipStream_t)stream;
   RtArgs a{};
-  a.ncol = ncol, a.nlay = nlay;
-  a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
-  a.co2vmr = co2vmr, a.ch4vmr = ch4vmr, a.o2vmr = o2vmr;
-  a.asdir = asdir, a.asdif = asdif, a.aldir = aldir, a.aldif = aldif, a.coszen = coszen;
-  a.tauaer = tauaer, a.ssaaer = ssaaer, a.asmaer = asmaer, a.ecaer = ecaer;
+  DeviceTables t = ctx.dev;
+  const void *outs[] = {swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+  for (const void *p : outs)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (int rc = fused_setup(ncol, nlay, icld, iaer, permuteseed, irng, play, plev, tlay, h2ovmr, o3vmr,
+                           co2vmr, ch4vmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr,
+                           scon, isolvar, inflgsw, iceflgsw, liqflgsw, cldfrac, ciwp, clwp, rei, rel,
+                           tauc, ssac, asmc, fsfc, alpha, tauaer, ssaaer, asmaer, ecaer, bndsolvar,
+                           indsolvar, solcycfrac, s, a, t))
+    return rc;
+  (void)tlev, (void)tsfc, (void)n2ovmr;
+  if (ncol == 0) return 0;
   a.swuflx = swuflx, a.swdflx = swdflx, a.swhr = swhr, a.swuflxc = swuflxc, a.swdflxc = swdflxc;
   a.swhrc = swhrc;
-  a.fused = 1, a.mc_icld = icld, a.mc_irng = ir, a.permuteseed = permuteseed;
-  a.cldfrac = cldfrac, a.ciwp = ciwp, a.clwp = clwp, a.rei = rei, a.rel = rel;
-  a.tauc = tauc, a.ssac = ssac, a.asmc = asmc, a.fsfc = fsfc, a.alpha = alpha;
-  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
-  a.sc.heatfac = ctx.heatfac;
-  a.sc.icld = icld >= 1 ? 1 : 0, a.sc.iaer = iaer;
-  a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
-  if (icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
-  const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
-                        aldir, aldif, coszen, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+  return run_rt(a, t, s, precision);
+}
+
+int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                                 const double *play, const double *plev, const double *tlay,
+                                 const double *tlev, const double *tsfc, const double *h2ovmr,
+                                 const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                                 const double *n2ovmr, const double *o2vmr, const double *asdir,
+                                 const double *asdif, const double *aldir, const double *aldif,
+                                 const double *coszen, double adjes, int dyofyr, double scon,
+                                 int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                                 const double *cldfrac, const double *ciwp, const double *clwp,
+                                 const double *rei, const double *rel, const double *tauc,
+                                 const double *ssac, const double *asmc, const double *fsfc,
+                                 const double *alpha, const double *tauaer, const double *ssaaer,
+                                 const double *asmaer, const double *ecaer,
+                                 const double *bndsolvar, const double *indsolvar,
+                                 const double *solcycfrac, const double *seed_toa,
+                                 const double *seed_hr, double *d_tlay, double *d_h2ovmr,
+                                 double *d_o3vmr, double *d_clwp, double *d_ciwp, double *d_alb,
+                                 void *stream) {
+  if (int rc = check_init()) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  RtArgs a{};
+  DeviceTables t = ctx.dev;
+  const void *need[] = {seed_toa, seed_hr, d_tlay, d_h2ovmr, d_o3vmr, d_clwp, d_ciwp, d_alb};
   for (const void *p : need)
     if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
-  if (icld >= 1 && (!cldfrac || !ciwp || !clwp || !rei || !rel || !tauc || !ssac || !asmc || !fsfc))
-    return fail(RRTMGSW_ERR_ARGUMENT, "cloud inputs are required when icld >= 1");
-  if (icld >= 4 && !alpha) return fail(RRTMGSW_ERR_ARGUMENT, "alpha is required when icld = 4 or 5");
-  if (iaer == 10 && (!tauaer || !ssaaer || !asmaer))
-    return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer are required when iaer = 10");
-  if (iaer == 6 && !ecaer) return fail(RRTMGSW_ERR_ARGUMENT, "ecaer is required when iaer = 6");
-  const uint32_t *mt = nullptr, *jumps = nullptr;
-  if (icld >= 1)
-    if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, ir, s, &mt, &jumps)) return rc;
-  a.mt_stream = mt;
-  DeviceTables t = ctx.dev;
-  t.kiss_jump = jumps;
-  return run_rt(a, t, s, precision == RRTMGSW_MIXED);
+  if (int rc = fused_setup(ncol, nlay, icld, iaer, permuteseed, irng, play, plev, tlay, h2ovmr, o3vmr,
+                           co2vmr, ch4vmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr,
+                           scon, isolvar, inflgsw, iceflgsw, liqflgsw, cldfrac, ciwp, clwp, rei, rel,
+                           tauc, ssac, asmc, fsfc, alpha, tauaer, ssaaer, asmaer, ecaer, bndsolvar,
+                           indsolvar, solcycfrac, s, a, t))
+    return rc;
+  (void)tlev, (void)tsfc, (void)n2ovmr;
+  if (ncol == 0) return 0;
+  AdjArgs A{};
+  A.seed_toa = seed_toa, A.seed_hr = seed_hr;
+  A.d_tlay = d_tlay, A.d_h2ovmr = d_h2ovmr, A.d_o3vmr = d_o3vmr;
+  A.d_clwp = d_clwp, A.d_ciwp = d_ciwp, A.d_alb = d_alb;
+  const size_t per_col = ((size_t)ADJ_WORK_PER_LAYER * nlay + 1) * NGPT * sizeof(double);
+  int chunk = (int)std::min<size_t>((size_t)ncol, std::max<size_t>(64, ((size_t)4 << 30) / per_col));
+  void *work = nullptr, *st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (int rc = stream_buffer(ctx.adjwork, s, (size_t)chunk * per_col, &work)) return rc;
+    if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
+  }
+  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
+  a.status = (int *)st;
+  A.work = (double *)work;
+  for (int c0 = 0; c0 < ncol; c0 += chunk) {
+    a.col0 = c0;
+    a.ncol_chunk = std::min(chunk, ncol - c0);
+    A.f = a;
+    HIPCHK(launch_adjoint(A, t, s));
+  }
+  return finish(s, (int *)st);
 }
 
 }  // extern "C"

@@ -1,0 +1,987 @@
+// rrtmgsw_device.h -- device building blocks shared by the forward and adjoint kernels of the
+// GCM-mode McICA shortwave path (gfx950): per-column LDS staging, inatm_sw + setcoef_sw, the
+// gas-optics template over per-g-point records (taumol_sw), cldprmc_sw, exponential-table
+// transmittances, reftra_sw, the adding-method elimination step (vrtqdr_sw), wavefront
+// reductions, the FP32 pieces of the mixed form, and the McICA sub-column mask generators
+// (KISS, Mersenne Twister).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/rrtmgsw.h"
+#include "rrtmgsw_internal.h"
+
+namespace rrtmgsw {
+
+constexpr double BPADE = 1.0 / PADE;
+constexpr double TBLINT = 10000.0;
+constexpr double ONEMINUS = 1.0 - 1.e-06;   // src/rrtmg_sw_rad.f90:563-565
+constexpr double REPCLC = 1.e-12;           // src/rrtmg_sw_spcvmc.f90:295
+constexpr double CLDMIN = 1.e-20;           // src/rrtmg_sw_cldprmc.f90:118
+
+// ------------------------------------------------------------------------------------------
+// Per-column LDS staging. Offsets are computed for the call's nlay (dynamic shared memory).
+// ------------------------------------------------------------------------------------------
+struct ColLayout {
+  int L;
+  int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
+  int o_fac, o_col, o_red, o_reda, o_thin, o_mask, total;   // double blocks / mask words
+  __host__ __device__ ColLayout(int nlay, int nmaskw) {
+    L = nlay;
+    int ib = 0;
+    o_jp = ib, ib += L;
+    o_jt = ib, ib += L;
+    o_jt1 = ib, ib += L;
+    o_inds = ib, ib += L;
+    o_indf = ib, ib += L;
+    o_misc = ib, ib += 32;
+    int bytes = ((ib * 4 + 15) / 16) * 16;
+    o_fac = bytes, bytes += 8 * L * 8;     // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac
+    o_col = bytes, bytes += 6 * L * 8;     // h2o co2 o3 ch4 o2 colmol
+    o_red = bytes, bytes += 2 * (L + 1) * 4 * 8;
+    o_reda = bytes, bytes += 2 * L * 2 * 8;
+    o_thin = bytes, bytes += ((L + 31) / 32) * 4;
+    o_mask = bytes, bytes += nmaskw * NGPT * 4;
+    total = bytes;
+  }
+};
+
+struct ColView {
+  int L;
+  int *jp, *jt, *jt1, *inds, *indf, *misc;
+  double *fac, *col, *red, *reda;
+  uint32_t *mask, *thin;
+  __device__ ColView(char *smem, const ColLayout &lo) {
+    L = lo.L;
+    int *ip = reinterpret_cast<int *>(smem);
+    jp = ip + lo.o_jp, jt = ip + lo.o_jt, jt1 = ip + lo.o_jt1, inds = ip + lo.o_inds;
+    indf = ip + lo.o_indf, misc = ip + lo.o_misc;
+    fac = reinterpret_cast<double *>(smem + lo.o_fac);
+    col = reinterpret_cast<double *>(smem + lo.o_col);
+    red = reinterpret_cast<double *>(smem + lo.o_red);
+    reda = reinterpret_cast<double *>(smem + lo.o_reda);
+    mask = reinterpret_cast<uint32_t *>(smem + lo.o_mask);
+    thin = reinterpret_cast<uint32_t *>(smem + lo.o_thin);
+  }
+  __device__ double &f(int k, int lay) { return fac[k * L + lay]; }   // 0..7 as listed above
+  __device__ double &c(int s, int lay) { return col[s * L + lay]; }   // 0..4 species, 5 colmol
+};
+// misc slots
+constexpr int M_LAYTROP = 0, M_LAYSOL = 1;   // laysolfr of band b at misc[M_LAYSOL + b]
+
+// ------------------------------------------------------------------------------------------
+// inatm_sw + setcoef_sw for one layer (src/rrtmg_sw_rad.f90:1531-1586,
+// src/rrtmg_sw_setcoef.f90:163-303). Contraction is off so that the interpolation indices
+// round exactly as the reference's separate multiply and add do.
+// ------------------------------------------------------------------------------------------
+__device__ static const double PREFLOG[59] = {
+    6.9600e+00, 6.7600e+00, 6.5600e+00, 6.3600e+00, 6.1600e+00, 5.9600e+00, 5.7600e+00,
+    5.5600e+00, 5.3600e+00, 5.1600e+00, 4.9600e+00, 4.7600e+00, 4.5600e+00, 4.3600e+00,
+    4.1600e+00, 3.9600e+00, 3.7600e+00, 3.5600e+00, 3.3600e+00, 3.1600e+00, 2.9600e+00,
+    2.7600e+00, 2.5600e+00, 2.3600e+00, 2.1600e+00, 1.9600e+00, 1.7600e+00, 1.5600e+00,
+    1.3600e+00, 1.1600e+00, 9.6000e-01, 7.6000e-01, 5.6000e-01, 3.6000e-01, 1.6000e-01,
+    -4.0000e-02, -2.4000e-01, -4.4000e-01, -6.4000e-01, -8.4000e-01, -1.0400e+00, -1.2400e+00,
+    -1.4400e+00, -1.6400e+00, -1.8400e+00, -2.0400e+00, -2.2400e+00, -2.4400e+00, -2.6400e+00,
+    -2.8400e+00, -3.0400e+00, -3.2400e+00, -3.4400e+00, -3.6400e+00, -3.8400e+00, -4.0400e+00,
+    -4.2400e+00, -4.4400e+00, -4.6400e+00};
+__device__ static const double TREF[59] = {
+    2.9420e+02, 2.8799e+02, 2.7894e+02, 2.6925e+02, 2.5983e+02, 2.5017e+02, 2.4077e+02,
+    2.3179e+02, 2.2306e+02, 2.1578e+02, 2.1570e+02, 2.1570e+02, 2.1570e+02, 2.1706e+02,
+    2.1858e+02, 2.2018e+02, 2.2174e+02, 2.2328e+02, 2.2479e+02, 2.2655e+02, 2.2834e+02,
+    2.3113e+02, 2.3401e+02, 2.3703e+02, 2.4022e+02, 2.4371e+02, 2.4726e+02, 2.5085e+02,
+    2.5457e+02, 2.5832e+02, 2.6216e+02, 2.6606e+02, 2.6999e+02, 2.7340e+02, 2.7536e+02,
+    2.7568e+02, 2.7372e+02, 2.7163e+02, 2.6955e+02, 2.6593e+02, 2.6211e+02, 2.5828e+02,
+    2.5360e+02, 2.4854e+02, 2.4348e+02, 2.3809e+02, 2.3206e+02, 2.2603e+02, 2.2000e+02,
+    2.1435e+02, 2.0887e+02, 2.0340e+02, 1.9792e+02, 1.9290e+02, 1.8809e+02, 1.8329e+02,
+    1.7849e+02, 1.7394e+02, 1.7212e+02};
+
+__device__ static void setcoef_layer(const RtArgs &a, int c, int lay, ColView &v, int *ntrop) {
+#pragma clang fp contract(off)
+  const int C = a.ncol;
+  const double amd = 28.9660, amw = 18.0160, avogad = 6.02214199e+23, grav = 9.8066;
+  double p = a.play[c + (size_t)C * lay], t = a.tlay[c + (size_t)C * lay];
+  double pz0 = a.plev[c + (size_t)C * lay], pz1 = a.plev[c + (size_t)C * (lay + 1)];
+  double q = a.h2ovmr[c + (size_t)C * lay];
+  double amm = (1. - q) * amd + q * amw;
+  double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1. + q));
+  double w_h2o = coldry * q, w_co2 = coldry * a.co2vmr[c + (size_t)C * lay];
+  double w_o3 = coldry * a.o3vmr[c + (size_t)C * lay], w_ch4 = coldry * a.ch4vmr[c + (size_t)C * lay];
+  double w_o2 = coldry * a.o2vmr[c + (size_t)C * lay];
+
+  double plog = log(p);
+  int jp = (int)(36. - 5 * (plog + 0.04));
+  jp = jp < 1 ? 1 : (jp > 58 ? 58 : jp);
+  int jp1 = jp + 1;
+  double fp = 5. * (PREFLOG[jp - 1] - plog);
+  int jt = (int)(3. + (t - TREF[jp - 1]) / 15.);
+  jt = jt < 1 ? 1 : (jt > 4 ? 4 : jt);
+  double ft = ((t - TREF[jp - 1]) / 15.) - (double)(jt - 3);
+  int jt1 = (int)(3. + (t - TREF[jp1 - 1]) / 15.);
+  jt1 = jt1 < 1 ? 1 : (jt1 > 4 ? 4 : jt1);
+  double ft1 = ((t - TREF[jp1 - 1]) / 15.) - (double)(jt1 - 3);
+  double water = w_h2o / coldry;
+  double scalefac = p * (296. / 1013.) / t;
+  double forfac = scalefac / (1. + water), forfrac, selffac, selffrac;
+  int indfor, indself;
+  if (plog > 4.56) {
+    atomicAdd(ntrop, 1);
+    double factor = (332.0 - t) / 36.0;
+    indfor = (int)factor;
+    indfor = indfor < 1 ? 1 : (indfor > 2 ? 2 : indfor);
+    forfrac = factor - (double)indfor;
+    selffac = water * forfac;
+    factor = (t - 188.0) / 7.2;
+    indself = (int)factor - 7;
+    indself = indself < 1 ? 1 : (indself > 9 ? 9 : indself);
+    selffrac = factor - (double)(indself + 7);
+  } else {
+    double factor = (t - 188.0) / 36.0;
+    indfor = 3;
+    forfrac = factor - 1.0;
+    selffac = 0.;
+    selffrac = 0.;
+    indself = 0;
+  }
+  double colh2o = 1.e-20 * w_h2o, colco2 = 1.e-20 * w_co2, colo3 = 1.e-20 * w_o3;
+  double colch4 = 1.e-20 * w_ch4, colo2 = 1.e-20 * w_o2;
+  double colmol = 1.e-20 * coldry + colh2o;
+  if (colco2 == 0.) colco2 = 1.e-32 * coldry;
+  if (colch4 == 0.) colch4 = 1.e-32 * coldry;
+  if (colo2 == 0.) colo2 = 1.e-32 * coldry;
+  double compfp = 1. - fp;
+  v.jp[lay] = jp, v.jt[lay] = jt, v.jt1[lay] = jt1, v.inds[lay] = indself, v.indf[lay] = indfor;
+  v.f(0, lay) = compfp * (1. - ft);   // fac00
+  v.f(1, lay) = fp * (1. - ft1);      // fac01
+  v.f(2, lay) = compfp * ft;          // fac10
+  v.f(3, lay) = fp * ft1;             // fac11
+  v.f(4, lay) = selffac, v.f(5, lay) = selffrac, v.f(6, lay) = forfac, v.f(7, lay) = forfrac;
+  v.c(SP_H2O, lay) = colh2o, v.c(SP_CO2, lay) = colco2, v.c(SP_O3, lay) = colo3;
+  v.c(SP_CH4, lay) = colch4, v.c(SP_O2, lay) = colo2, v.c(5, lay) = colmol;
+}
+
+// laysolfr of one band: the layer whose species ratio sets the band's solar source
+// (e.g. src/rrtmg_sw_taumol.f90:351-356 upper loop, :559-564 lower loop). 1-based.
+__device__ static int find_laysolfr(const GPoint &g, const ColView &v, int laytrop, int L) {
+  int ref = g.layreffr;
+  if (!g.sol_upper) {
+    int ls = laytrop;
+    for (int lay = 1; lay <= laytrop; lay++)
+      if (lay + 1 <= L && v.jp[lay - 1] < ref && v.jp[lay] >= ref) ls = min(lay + 1, laytrop);
+    return ls < 1 ? 1 : ls;
+  }
+  int ls = L;
+  for (int lay = laytrop + 1; lay <= L; lay++)
+    if (lay >= 2 && v.jp[lay - 2] < ref && v.jp[lay - 1] >= ref) ls = lay;
+  return ls;
+}
+
+// ------------------------------------------------------------------------------------------
+// Gas optics template: taug and taur of one g-point at one layer.
+// ------------------------------------------------------------------------------------------
+template <typename R>
+struct SpecT {
+  R speccomb, fs;
+  int js;
+};
+using Spec = SpecT<double>;
+// Binary species parameter (e.g. src/rrtmg_sw_taumol.f90:287-292): the species ratio, clipped
+// at oneminus, its bin js and the fraction fs (mod(specmult, 1) = specmult - trunc(specmult)
+// exactly for the non-negative values that occur).
+template <typename R>
+__device__ inline SpecT<R> key_spec_t(R c1, R c2, R strrat, R mult, bool bin) {
+#pragma clang fp contract(off)
+  SpecT<R> s;
+  R speccomb = c1 + strrat * c2;
+  R specparm = c1 / speccomb;
+  if (specparm >= R(ONEMINUS)) specparm = R(ONEMINUS);
+  R specmult = mult * specparm;
+  int jsb = 1 + (int)specmult;
+  R fsb = specmult - trunc(specmult);
+  s.speccomb = speccomb;
+  s.js = bin ? jsb : 1;
+  s.fs = bin ? fsb : R(0);
+  return s;
+}
+__device__ inline Spec key_spec(double c1, double c2, double strrat, double mult, bool bin) {
+  return key_spec_t<double>(c1, c2, strrat, mult, bin);
+}
+
+// 8-corner interpolation of a key-species table. Corner strides: cjs (species step: 1 for
+// binary species, 0 otherwise) and nsp (temperature step); st = table row stride (0: unused).
+template <typename R>
+__device__ inline R kinterp(const R *__restrict__ A, int st, int ind0, int ind1, int cjs, int nsp,
+                            R fs, R fac00, R fac10, R fac01, R fac11) {
+  R f0 = R(1) - fs;
+  R a0 = A[(ind0 - 1) * st], a1 = A[(ind0 - 1 + cjs) * st];
+  R a2 = A[(ind0 - 1 + nsp) * st], a3 = A[(ind0 - 1 + nsp + cjs) * st];
+  R b0 = A[(ind1 - 1) * st], b1 = A[(ind1 - 1 + cjs) * st];
+  R b2 = A[(ind1 - 1 + nsp) * st], b3 = A[(ind1 - 1 + nsp + cjs) * st];
+  return f0 * fac00 * a0 + fs * fac00 * a1 + f0 * fac10 * a2 + fs * fac10 * a3 +
+         f0 * fac01 * b0 + fs * fac01 * b1 + f0 * fac11 * b2 + fs * fac11 * b3;
+}
+
+// taug and taur of one g-point at one layer. R = double reproduces the reference's FP64
+// arithmetic; R = float is the gas optics of the mixed form (FP32 copies of the tables).
+template <typename R>
+__device__ inline void gas_optics(const GPoint &g, const R *__restrict__ tab, ColView &v, int lay,
+                                  bool lower, R &taug, R &taur) {
+  // A layer below laytrop has jp <= 12 and one above has jp >= 13 whenever pressure decreases
+  // with height; the clamps keep table reads in bounds for profiles that break that order.
+  const int jp = lower ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
+  const int jt = v.jt[lay], jt1 = v.jt1[lay];
+  const R fac00 = R(v.f(0, lay)), fac01 = R(v.f(1, lay)), fac10 = R(v.f(2, lay)), fac11 = R(v.f(3, lay));
+  const R colh2o = R(v.c(SP_H2O, lay)), colmol = R(v.c(5, lay));
+  if (lower) {
+    bool bin = g.lo_mode == KEY_BIN;
+    SpecT<R> s = key_spec_t<R>(R(v.c(g.lo_s1, lay)), R(v.c(g.lo_s2, lay)), R(g.lo_strrat), R(8), bin);
+    R kmul = bin ? s.speccomb : R(v.c(g.lo_s1, lay)) * R(g.lo_mult);
+    int nsp = g.lo_nsp;
+    int ind0 = ((jp - 1) * 5 + (jt - 1)) * nsp + s.js;
+    int ind1 = (jp * 5 + (jt1 - 1)) * nsp + s.js;
+    R k = kinterp<R>(tab + g.absa, g.sa, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
+    const R *S = tab + g.selfref;
+    const R *F = tab + g.forref;
+    int is = max(v.inds[lay], 1), jf = min(v.indf[lay], 2);
+    R s0 = S[(is - 1) * g.sself], s1 = S[is * g.sself];
+    R f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
+    R cont = R(v.f(4, lay)) * (s0 + R(v.f(5, lay)) * (s1 - s0)) + R(v.f(6, lay)) * (f0 + R(v.f(7, lay)) * (f1 - f0));
+    taug = kmul * k + colh2o * cont + R(v.c(g.lo_minor, lay)) * R(g.lo_minor_coef);
+    R ray = R(g.rayl_lo);
+    if (g.rayl_js) {
+      const R *RA = tab + g.rayla;
+      R r0 = RA[(s.js - 1) * g.sra], r1 = RA[s.js * g.sra];
+      ray = r0 + s.fs * (r1 - r0);
+    }
+    taur = colmol * ray;
+  } else {
+    bool bin = g.hi_mode == KEY_BIN;
+    SpecT<R> s = key_spec_t<R>(R(v.c(g.hi_s1, lay)), R(v.c(g.hi_s2, lay)), R(g.hi_strrat), R(4), bin);
+    R kmul = bin ? s.speccomb : R(v.c(g.hi_s1, lay)) * R(g.hi_mult);
+    int nsp = g.hi_nsp;
+    int ind0 = ((jp - 13) * 5 + (jt - 1)) * nsp + s.js;
+    int ind1 = ((jp - 12) * 5 + (jt1 - 1)) * nsp + s.js;
+    R k = kinterp<R>(tab + g.absb, g.sb, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
+    R fo = R(0);
+    if (g.hi_for) {
+      const R *F = tab + g.forref;
+      int jf = min(max(v.indf[lay], 1), 3);
+      R f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
+      fo = colh2o * R(v.f(6, lay)) * (f0 + R(v.f(7, lay)) * (f1 - f0));
+    }
+    taug = kmul * k + fo + R(v.c(g.hi_minor, lay)) * R(g.hi_minor_coef);
+    taur = colmol * R(g.rayl_hi);
+  }
+}
+
+// Solar source of the lane's g-point at its band's laysolfr (src/rrtmg_sw_taumol.f90, e.g.
+// :366-375 plain, :495-520 interpolated in the binary species parameter), times
+// adjflux * cos(sza): zincflx of src/rrtmg_sw_spcvmc.f90:349-356.
+__device__ inline double incident_flux(const GPoint &g, const double *__restrict__ tab, ColView &v,
+                                       const SwScalars &sc, double mu0) {
+  int ls = v.misc[M_LAYSOL + g.ibnd] - 1;  // 0-based layer
+  int js = 1;
+  double fs = 0.0;
+  if (g.sol_interp) {
+    Spec s = g.sol_upper ? key_spec(v.c(g.hi_s1, ls), v.c(g.hi_s2, ls), g.hi_strrat, 4.0, true)
+                         : key_spec(v.c(g.lo_s1, ls), v.c(g.lo_s2, ls), g.lo_strrat, 8.0, true);
+    js = s.js, fs = s.fs;
+  }
+  auto interp = [&](int off) {
+    double t0 = tab[off + (js - 1) * g.ssf], t1 = tab[off + js * g.ssf];
+    return t0 + fs * (t1 - t0);
+  };
+  double src;
+  if (sc.isolvar < 0) {
+    src = g.sol_scale * interp(g.sflux);
+  } else {
+    double fb = interp(g.facb), sn = interp(g.snsp), ir = interp(g.irrad);
+    if (sc.isolvar == 3)
+      src = sc.svar_bnd[0][g.ibnd] * fb + sc.svar_bnd[1][g.ibnd] * sn + sc.svar_bnd[2][g.ibnd] * ir;
+    else
+      src = sc.svar_f * fb + sc.svar_s * sn + sc.svar_i * ir;
+  }
+  return sc.adjflux[g.ibnd] * src * mu0;
+}
+
+// ------------------------------------------------------------------------------------------
+// cldprmc_sw for one (g, layer): src/rrtmg_sw_cldprmc.f90:141-347. Returns the delta-scaled
+// cloud optical depth, single-scattering albedo and asymmetry; raises *err on a STOP condition.
+// ------------------------------------------------------------------------------------------
+struct CloudOpt {
+  double tau, ssa, asm_;
+};
+__device__ inline double tab2(const double *tab, int off, int n, int i, int band) {
+  return tab[off + (i - 1) + n * (band - 16)];
+}
+// Ice and liquid single-particle optics of one band at the layer's particle sizes
+// (src/rrtmg_sw_cldprmc.f90:176-295): extinction per unit water path, single-scattering albedo,
+// asymmetry and forward-scattering fraction; raises *err on the reference's STOP conditions.
+struct CloudProps {
+  double extcoice, ssacoice, gice, forwice, extcoliq, ssacoliq, gliq, forwliq;
+};
+__device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars &sc, int band,
+                                         double ciwp, double clwp, double rei, double rel, int &err) {
+  CloudProps p = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double *tab = T.tab;
+  if (ciwp != 0.0) {
+    double radice = rei;
+    if (sc.iceflag == 1) {  // :182-208
+      if (radice < 13.0 || radice > 130.) err = RRTMGSW_ERR_RADIUS;
+      const double w2[14] = {3250., 4000., 4650., 5150., 6150., 7700., 8050.,
+                             12850., 16000., 22650., 29000., 38000., 50000., 2600.};
+      double wn = w2[band - 16];
+      int icx = wn > 1.43e04 ? 1 : wn > 7.7e03 ? 2 : wn > 5.3e03 ? 3 : wn > 4.0e03 ? 4 : 5;
+      p.extcoice = tab[T.abari + icx - 1] + tab[T.bbari + icx - 1] / radice;
+      p.ssacoice = 1. - tab[T.cbari + icx - 1] - tab[T.dbari + icx - 1] * radice;
+      p.gice = tab[T.ebari + icx - 1] + tab[T.fbari + icx - 1] * radice;
+      if (p.gice >= 1.) p.gice = 1. - 1.e-06;
+      p.forwice = p.gice * p.gice;
+    } else if (sc.iceflag == 2) {  // :212-231
+      if (radice < 5.0 || radice > 131.0) err = RRTMGSW_ERR_RADIUS;
+      double factor = (radice - 2.) / 3.;
+      int index = (int)factor;
+      if (index == 43) index = 42;
+      index = index < 1 ? 1 : (index > 42 ? 42 : index);
+      double fint = factor - (double)index;
+      double e0 = tab2(tab, T.extice2, 43, index, band), e1 = tab2(tab, T.extice2, 43, index + 1, band);
+      double s0 = tab2(tab, T.ssaice2, 43, index, band), s1 = tab2(tab, T.ssaice2, 43, index + 1, band);
+      double g0 = tab2(tab, T.asyice2, 43, index, band), g1 = tab2(tab, T.asyice2, 43, index + 1, band);
+      p.extcoice = e0 + fint * (e1 - e0);
+      p.ssacoice = s0 + fint * (s1 - s0);
+      p.gice = g0 + fint * (g1 - g0);
+      p.forwice = p.gice * p.gice;
+    } else if (sc.iceflag == 3) {  // :235-260
+      if (radice < 5.0 || radice > 140.0) err = RRTMGSW_ERR_RADIUS;
+      double factor = (radice - 2.) / 3.;
+      int index = (int)factor;
+      if (index == 46) index = 45;
+      index = index < 1 ? 1 : (index > 45 ? 45 : index);
+      double fint = factor - (double)index;
+      double e0 = tab2(tab, T.extice3, 46, index, band), e1 = tab2(tab, T.extice3, 46, index + 1, band);
+      double s0 = tab2(tab, T.ssaice3, 46, index, band), s1 = tab2(tab, T.ssaice3, 46, index + 1, band);
+      double g0 = tab2(tab, T.asyice3, 46, index, band), g1 = tab2(tab, T.asyice3, 46, index + 1, band);
+      double d0 = tab2(tab, T.fdlice3, 46, index, band), d1 = tab2(tab, T.fdlice3, 46, index + 1, band);
+      p.extcoice = e0 + fint * (e1 - e0);
+      p.ssacoice = s0 + fint * (s1 - s0);
+      p.gice = g0 + fint * (g1 - g0);
+      double fdelta = d0 + fint * (d1 - d0);
+      if (fdelta < 0.0 || fdelta > 1.0) err = RRTMGSW_ERR_OPTICS;
+      p.forwice = fdelta + 0.5 / p.ssacoice;
+      if (p.forwice > p.gice) p.forwice = p.gice;
+    }
+    if (p.extcoice < 0.0 || p.ssacoice > 1.0 || p.ssacoice < 0.0 || p.gice > 1.0 || p.gice < 0.0)
+      err = err ? err : RRTMGSW_ERR_OPTICS;
+  }
+  if (clwp != 0.0 && sc.liqflag == 1) {  // :265-295
+    double radliq = rel;
+    if (radliq < 2.5 || radliq > 60.) err = RRTMGSW_ERR_RADIUS;
+    int index = (int)(radliq - 1.5);
+    if (index == 0) index = 1;
+    if (index == 58) index = 57;
+    index = index < 1 ? 1 : (index > 57 ? 57 : index);
+    double fint = radliq - 1.5 - (double)index;
+    double e0 = tab2(tab, T.extliq1, 58, index, band), e1 = tab2(tab, T.extliq1, 58, index + 1, band);
+    double s0 = tab2(tab, T.ssaliq1, 58, index, band), s1 = tab2(tab, T.ssaliq1, 58, index + 1, band);
+    double g0 = tab2(tab, T.asyliq1, 58, index, band), g1 = tab2(tab, T.asyliq1, 58, index + 1, band);
+    p.extcoliq = e0 + fint * (e1 - e0);
+    p.ssacoliq = s0 + fint * (s1 - s0);
+    if (fint < 0. && p.ssacoliq > 1.) p.ssacoliq = s0;
+    p.gliq = g0 + fint * (g1 - g0);
+    p.forwliq = p.gliq * p.gliq;
+    if (p.extcoliq < 0.0 || p.ssacoliq > 1.0 || p.ssacoliq < 0.0 || p.gliq > 1.0 || p.gliq < 0.0)
+      err = err ? err : RRTMGSW_ERR_OPTICS;
+  }
+  return p;
+}
+
+__device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &sc, int band,
+                                        double cldf, double ciwp, double clwp, double rei,
+                                        double rel, double tauc, double ssac, double asmc,
+                                        double fsfc, int &err) {
+  CloudOpt o = {tauc, ssac, asmc};
+  double cwp = ciwp + clwp;
+  if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauc >= CLDMIN))) return o;
+  if (sc.inflag == 0) {  // :150-163
+    double ffp = fsfc, ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssac;
+    o.ssa = ffp1 * ssac / ffpssa;
+    o.tau = ffpssa * tauc;
+    o.asm_ = (asmc - ffp) / ffp1;
+    return o;
+  }
+  if (sc.inflag != 2) {
+    err = RRTMGSW_ERR_INFLAG;
+    return o;
+  }
+  CloudProps p = cloud_props(T, sc, band, ciwp, clwp, rei, rel, err);
+  const double extcoice = p.extcoice, ssacoice = p.ssacoice, gice = p.gice, forwice = p.forwice;
+  const double extcoliq = p.extcoliq, ssacoliq = p.ssacoliq, gliq = p.gliq, forwliq = p.forwliq;
+  // :297-337
+  double tauliqorig = clwp * extcoliq, tauiceorig = ciwp * extcoice;
+  double ssaliq = ssacoliq * (1. - forwliq) / (1. - forwliq * ssacoliq);
+  double tauliq = (1. - forwliq * ssacoliq) * tauliqorig;
+  double ssaice = ssacoice * (1. - forwice) / (1. - forwice * ssacoice);
+  double tauice = (1. - forwice * ssacoice) * tauiceorig;
+  double scatliq = ssaliq * tauliq, scatice = ssaice * tauice;
+  double tau = tauliq + tauice;
+  if (tau == 0.) tau = CLDMIN;
+  if (scatice == 0.) scatice = CLDMIN;
+  o.tau = tau;
+  o.ssa = (scatliq + scatice) / tau;
+  if (sc.iceflag == 3)
+    o.asm_ = (1.0 / (scatliq + scatice)) *
+             (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * ((gice - forwice) / (1.0 - forwice)));
+  else
+    o.asm_ = (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * (gice - forwice) / (1.0 - forwice)) /
+             (scatliq + scatice);
+  return o;
+}
+
+// Adjoint of cloud_optics (inflag = 2) with respect to the in-cloud water paths, given the
+// adjoints of the returned (tau, ssa, asm). tau, scatliq and scatice are linear in the water
+// paths; ssa = (scatliq + scatice) / tau and asm is the scattering-weighted mean of the two
+// delta-scaled asymmetries.
+__device__ static void cloud_optics_adj(const DeviceTables &T, const SwScalars &sc, int band,
+                                        double cldf, double ciwp, double clwp, double rei,
+                                        double rel, double tau_b, double ssa_b, double asm_b,
+                                        double &clwp_b, double &ciwp_b) {
+  clwp_b = 0.0, ciwp_b = 0.0;
+  double cwp = ciwp + clwp;
+  if (!(cldf >= CLDMIN && cwp >= CLDMIN) || sc.inflag != 2) return;
+  int err = 0;
+  CloudProps p = cloud_props(T, sc, band, ciwp, clwp, rei, rel, err);
+  double al = (1. - p.forwliq * p.ssacoliq) * p.extcoliq;             // d tauliq / d clwp
+  double ai = (1. - p.forwice * p.ssacoice) * p.extcoice;             // d tauice / d ciwp
+  double bl = p.ssacoliq * (1. - p.forwliq) / (1. - p.forwliq * p.ssacoliq) * al;   // d scatliq / d clwp
+  double bi = p.ssacoice * (1. - p.forwice) / (1. - p.forwice * p.ssacoice) * ai;   // d scatice / d ciwp
+  double scatliq = bl * clwp, scatice = bi * ciwp;
+  double tau = al * clwp + ai * ciwp;
+  if (tau == 0.) return;
+  double gl = p.gliq != 0. || p.forwliq != 0. ? (p.gliq - p.forwliq) / (1.0 - p.forwliq) : 0.;
+  double gi = p.gice != 0. || p.forwice != 0. ? (p.gice - p.forwice) / (1.0 - p.forwice) : 0.;
+  double S = scatliq + (scatice == 0. ? CLDMIN : scatice);
+  double asmv = (scatliq * gl + scatice * gi) / S;
+  double S_b = ssa_b / tau;
+  double t_b = tau_b - ssa_b * S / (tau * tau);
+  double sl_b = S_b + asm_b * (gl - asmv) / S;
+  double si_b = S_b + asm_b * (gi - asmv) / S;
+  clwp_b = t_b * al + sl_b * bl;
+  ciwp_b = t_b * ai + si_b * bi;
+}
+
+// ------------------------------------------------------------------------------------------
+// Exponential-table transmittance (e.g. src/rrtmg_sw_spcvmc.f90:558-566, reftra :203-209).
+// ------------------------------------------------------------------------------------------
+__device__ inline double tbl_exp(double ze, const double *__restrict__ exp_tbl) {
+#pragma clang fp contract(off)
+  if (ze <= OD_LO) return 1. - ze + 0.5 * ze * ze;
+  double tblind = ze / (BPADE + ze);
+  int itind = (int)(TBLINT * tblind + 0.5);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);   // NaN or negative input stays in the table
+  return exp_tbl[itind];
+}
+
+// Transmittance and its reciprocal: the reciprocal comes from a table of 1/exp_tbl (built on
+// the host with IEEE division, so it equals the division the reference performs).
+struct Ex {
+  double e, r;
+};
+__device__ inline Ex tbl_exp2(double ze, const double *__restrict__ exp_tbl,
+                              const double *__restrict__ rexp_tbl) {
+#pragma clang fp contract(off)
+  if (ze <= OD_LO) {
+    double e = 1. - ze + 0.5 * ze * ze;
+    return {e, 1. / e};
+  }
+  double tblind = ze / (BPADE + ze);
+  int itind = (int)(TBLINT * tblind + 0.5);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
+  return {exp_tbl[itind], rexp_tbl[itind]};
+}
+
+// reftra_sw for one layer (src/rrtmg_sw_reftra.f90:151-327, PIFM kmodts = 2). Algebraically the
+// reference's expressions with fewer divisions: 1/mu0 is passed in (rmu), exponentials come
+// with their reciprocals, R and T share one reciprocal of the common denominator, and
+// 1/((1 - beta e^-2k tau)(k + g1)) is formed as 1/((k + g1) - (g1 - k) e^-2k tau).
+struct RT4 {
+  double r, rd, t, td;
+};
+__device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz, double rmu,
+                              const double *__restrict__ exp_tbl,
+                              const double *__restrict__ rexp_tbl) {
+  const double zwcrit = 0.9999995, eps = 1.e-08;
+  RT4 o;
+  double zg3 = 3. * zg;
+  double zgamma1 = (8. - zw * (5. + zg3)) * 0.25;
+  double zgamma2 = 3. * (zw * (1. - zg)) * 0.25;
+  double zgamma3 = (2. - zg3 * prmuz) * 0.25;
+  double zgamma4 = 1. - zgamma3;
+  // zwo = zw / (1 - (1-zw) (zg/(1-zg))^2) >= zwcrit, compared without dividing
+  double omg = 1. - zg;
+  double den2 = omg * omg - (1. - zw) * zg * zg;   // denom * (1-zg)^2
+  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+  if (cons) {  // conservative scattering, :188-228
+    double za1 = zgamma1 * prmuz - zgamma3;
+    double zgt = zgamma1 * zto1;
+    double ze2 = tbl_exp2(fmin(zto1 * rmu, 500.), exp_tbl, rexp_tbl).e;
+    double inv = 1. / (1. + zgt);
+    o.r = (zgt - za1 * (1. - ze2)) * inv;
+    o.t = 1. - o.r;
+    o.rd = zgt * inv;
+    o.td = 1. - o.rd;
+    if (ze2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
+  } else {     // general two-stream solution, :233-322
+    double za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+    double za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+    double zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
+    double zrp = zrk * prmuz;
+    double zrp1 = 1. + zrp, zrm1 = 1. - zrp, zrk2 = 2. * zrk;
+    double zrpp = 1. - zrp * zrp;
+    double zrkg = zrk + zgamma1;
+    double zr1 = zrm1 * (za2 + zrk * zgamma3);
+    double zr2 = zrp1 * (za2 - zrk * zgamma3);
+    double zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+    double zr4 = zrpp * zrkg;
+    double zr5 = zrpp * (zrk - zgamma1);
+    double zt1 = zrp1 * (za1 + zrk * zgamma4);
+    double zt2 = zrm1 * (za1 - zrk * zgamma4);
+    double zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+    Ex e1 = tbl_exp2(fmin(zrk * zto1, 500.), exp_tbl, rexp_tbl);
+    Ex e2 = tbl_exp2(fmin(zto1 * rmu, 500.), exp_tbl, rexp_tbl);
+    double zem1 = e1.e, zep1 = e1.r, zem2 = e2.e, zep2 = e2.r;
+    double zdenr = zr4 * zep1 + zr5 * zem1;
+    if (zdenr >= -eps && zdenr <= eps) {
+      o.r = eps;
+      o.t = zem2;
+    } else {
+      double rden = 1. / zdenr;
+      o.r = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) * rden;
+      o.t = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) * rden;
+    }
+    double zemm = zem1 * zem1;
+    double zdend = 1. / (zrkg - (zgamma1 - zrk) * zemm);
+    o.rd = zgamma2 * (1. - zemm) * zdend;
+    o.td = zrk2 * zem1 * zdend;
+  }
+  return o;
+}
+
+// ------------------------------------------------------------------------------------------
+// Forward-mode dual numbers (value + N partial derivatives) for the adjoint's local Jacobians.
+// ------------------------------------------------------------------------------------------
+template <int N>
+struct Dual {
+  double v;
+  double d[N];
+};
+template <int N>
+__device__ inline Dual<N> dconst(double x) {
+  Dual<N> r;
+  r.v = x;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = 0.0;
+  return r;
+}
+template <int N>
+__device__ inline Dual<N> dvar(double x, int k) {
+  Dual<N> r = dconst<N>(x);
+  r.d[k] = 1.0;
+  return r;
+}
+#define DUAL_BIN(OP, VEXPR, DEXPR)                                                      \
+  template <int N>                                                                      \
+  __device__ inline Dual<N> operator OP(const Dual<N> &a, const Dual<N> &b) {            \
+    Dual<N> r;                                                                          \
+    r.v = VEXPR;                                                                        \
+    _Pragma("unroll") for (int i = 0; i < N; i++) r.d[i] = DEXPR;                       \
+    return r;                                                                           \
+  }
+DUAL_BIN(+, a.v + b.v, a.d[i] + b.d[i])
+DUAL_BIN(-, a.v - b.v, a.d[i] - b.d[i])
+DUAL_BIN(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+#undef DUAL_BIN
+template <int N>
+__device__ inline Dual<N> operator/(const Dual<N> &a, const Dual<N> &b) {
+  Dual<N> r;
+  double rb = 1.0 / b.v;
+  r.v = a.v * rb;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * rb;
+  return r;
+}
+template <int N> __device__ inline Dual<N> operator+(const Dual<N> &a, double b) { Dual<N> r = a; r.v += b; return r; }
+template <int N> __device__ inline Dual<N> operator+(double a, const Dual<N> &b) { return b + a; }
+template <int N> __device__ inline Dual<N> operator-(const Dual<N> &a, double b) { Dual<N> r = a; r.v -= b; return r; }
+template <int N>
+__device__ inline Dual<N> operator-(double a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a - b.v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = -b.d[i];
+  return r;
+}
+template <int N>
+__device__ inline Dual<N> operator*(const Dual<N> &a, double b) {
+  Dual<N> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b;
+  return r;
+}
+template <int N> __device__ inline Dual<N> operator*(double a, const Dual<N> &b) { return b * a; }
+template <int N> __device__ inline Dual<N> operator/(const Dual<N> &a, double b) { return a * (1.0 / b); }
+template <int N> __device__ inline Dual<N> operator/(double a, const Dual<N> &b) { return dconst<N>(a) / b; }
+__device__ inline double dsqrt(double x) { return ::sqrt(x); }
+__device__ inline double dexp(double x) { return ::exp(x); }
+__device__ inline double dmin(double a, double b) { return ::fmin(a, b); }
+template <int N>
+__device__ inline Dual<N> dsqrt(const Dual<N> &a) {
+  Dual<N> r;
+  r.v = ::sqrt(a.v);
+  double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = a.d[i] * h;
+  return r;
+}
+template <int N>
+__device__ inline Dual<N> dexp(const Dual<N> &a) {
+  Dual<N> r;
+  r.v = ::exp(a.v);
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = a.d[i] * r.v;
+  return r;
+}
+template <int N>
+__device__ inline Dual<N> dmin(const Dual<N> &a, double b) { return a.v > b ? dconst<N>(b) : a; }
+__device__ inline double val(double x) { return x; }
+template <int N> __device__ inline double val(const Dual<N> &x) { return x.v; }
+
+// reftra_sw with exact exponentials in place of the lookup table and its expansion: the smooth
+// function of (tau, w, g) that the adjoint differentiates and that finite differences probe.
+// Same branches and expressions as the reference (src/rrtmg_sw_reftra.f90:164-322).
+template <typename Tn>
+struct RTt {
+  Tn r, rd, t, td;
+};
+template <typename Tn>
+__device__ inline RTt<Tn> reftra_exact(Tn zto1, Tn zw, Tn zg, double prmuz, double rmu) {
+  const double zwcrit = 0.9999995, eps = 1.e-08;
+  RTt<Tn> o;
+  Tn zg3 = 3. * zg;
+  Tn zgamma1 = (8. - zw * (5. + zg3)) * 0.25;
+  Tn zgamma2 = 3. * (zw * (1. - zg)) * 0.25;
+  Tn zgamma3 = (2. - zg3 * prmuz) * 0.25;
+  Tn zgamma4 = 1. - zgamma3;
+  double w = val(zw), gg = val(zg);
+  double omg = 1. - gg;
+  double den2 = omg * omg - (1. - w) * gg * gg;
+  bool cons = gg != 1. ? (w > 0. && den2 > 0. && w * (omg * omg) >= zwcrit * den2) : (w >= zwcrit);
+  if (cons) {
+    Tn za1 = zgamma1 * prmuz - zgamma3;
+    Tn zgt = zgamma1 * zto1;
+    Tn ze2 = dexp(0.0 - dmin(zto1 * rmu, 500.));
+    Tn inv = 1. / (1. + zgt);
+    o.r = (zgt - za1 * (1. - ze2)) * inv;
+    o.t = 1. - o.r;
+    o.rd = zgt * inv;
+    o.td = 1. - o.rd;
+    if (val(ze2) == 1.0) o.r = 0. * zto1, o.t = 0. * zto1 + 1., o.rd = 0. * zto1, o.td = 0. * zto1 + 1.;
+  } else {
+    Tn za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+    Tn za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+    Tn zrk = dsqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
+    Tn zrp = zrk * prmuz;
+    Tn zrp1 = 1. + zrp, zrm1 = 1. - zrp, zrk2 = 2. * zrk;
+    Tn zrpp = 1. - zrp * zrp;
+    Tn zrkg = zrk + zgamma1;
+    Tn zr1 = zrm1 * (za2 + zrk * zgamma3);
+    Tn zr2 = zrp1 * (za2 - zrk * zgamma3);
+    Tn zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+    Tn zr4 = zrpp * zrkg;
+    Tn zr5 = zrpp * (zrk - zgamma1);
+    Tn zt1 = zrp1 * (za1 + zrk * zgamma4);
+    Tn zt2 = zrm1 * (za1 - zrk * zgamma4);
+    Tn zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+    Tn x1 = dmin(zrk * zto1, 500.), x2 = dmin(zto1 * rmu, 500.);
+    Tn zem1 = dexp(0.0 - x1), zep1 = dexp(x1), zem2 = dexp(0.0 - x2), zep2 = dexp(x2);
+    Tn zdenr = zr4 * zep1 + zr5 * zem1;
+    if (val(zdenr) >= -eps && val(zdenr) <= eps) {
+      o.r = 0. * zto1 + eps;
+      o.t = zem2;
+    } else {
+      Tn rden = 1. / zdenr;
+      o.r = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) * rden;
+      o.t = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) * rden;
+    }
+    Tn zemm = zem1 * zem1;
+    Tn zdend = 1. / (zrkg - (zgamma1 - zrk) * zemm);
+    o.rd = zgamma2 * (1. - zemm) * zdend;
+    o.td = zrk2 * zem1 * zdend;
+  }
+  return o;
+}
+
+// Downward half of the adding method for one sky (vrtqdr_sw :144-159), written as forward
+// elimination: with D_dif(j) = (ztdn_j - tdbt_j) + prdnd_j U_j from the stack above and
+// U_j = R_j tdbt_j + Rd_j D_dif(j) + Td_j U_(j+1) for layer j,
+//   U_j = C_j + E_j U_(j+1),  C_j = (Rd_j a_j + R_j tdbt_j) / (1 - Rd_j prdnd_j),
+//   E_j = Td_j / (1 - Rd_j prdnd_j),  a_j = ztdn_j - tdbt_j,  D_j = ztdn_j + prdnd_j U_j.
+struct Sweep {
+  double tdbt, ztdn, prdnd;
+};
+__device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A, double &B,
+                                   double &Cc, double &E) {
+  A = s.ztdn;
+  B = s.prdnd;
+  double a = s.ztdn - s.tdbt;
+  double zreflect = 1. / (1. - l.rd * s.prdnd);
+  Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
+  E = l.td * zreflect;
+  double ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;
+  double prdnd = l.rd + l.td * l.td * s.prdnd * zreflect;
+  s.tdbt = dbt * s.tdbt;
+  s.ztdn = ztdn;
+  s.prdnd = prdnd;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP32 pieces of the mixed form. The layer optics carry the co-albedo c = 1 - w explicitly
+// (from the absorbing optical depths, so it keeps full relative precision when w is close to
+// one), which removes the cancellations of the two-stream expressions in FP32:
+//   gamma1 - gamma2 = 2c (PIFM), so k = sqrt(2c (gamma1 + gamma2));
+//   (k + gamma1) / e - (gamma1 - k) e = gamma1 (1 - e^2)/e + k (1/e + e) with 1 - e from the
+//   expansion or the table, and likewise for the diffuse denominator;
+//   zwo >= zwcrit  <=>  c (1 - zwcrit h^2) <= 1 - zwcrit,  h = g / (1 - g)
+// (algebraically the reference's quantities, src/rrtmg_sw_reftra.f90:164-322).
+// ------------------------------------------------------------------------------------------
+__device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+struct Exf {
+  float e, r, om;   // exp(-x) from the table (or its expansion), 1/e, 1 - e
+};
+__device__ inline Exf tbl_expf(float ze, const float *__restrict__ et, const float *__restrict__ ret) {
+  if (ze <= float(OD_LO)) {
+    float om = ze - 0.5f * ze * ze;
+    float e = 1.f - om;
+    return {e, rcpf(e), om};
+  }
+  float tblind = ze * rcpf(float(BPADE) + ze);
+  int itind = (int)(10000.f * tblind + 0.5f);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
+  float e = et[itind];
+  return {e, ret[itind], 1.f - e};
+}
+
+struct RT4f {
+  float r, rd, t, td;
+};
+// |1 - k mu0| < 2e-3 for the layer's two-stream eigenvalue k (general branch only; the
+// conservative branch has no such denominator).
+__device__ inline bool near_singular(float zw, float zc, float zg, float mu0) {
+  float zgamma1 = (8.f - zw * (5.f + 3.f * zg)) * 0.25f;
+  float zgamma2 = 3.f * (zw * (1.f - zg)) * 0.25f;
+  float k = __builtin_sqrtf(2.f * zc * (zgamma1 + zgamma2));
+  return fabsf(1.f - k * mu0) < 2.e-3f;
+}
+__device__ inline RT4f reftra_f(float zto1, float zw, float zc, float zg, float prmuz, float rmu,
+                                const float *__restrict__ et, const float *__restrict__ ret) {
+  const float zwcrit = 0.9999995f, eps = 1.e-08f;
+  RT4f o;
+  float zg3 = 3.f * zg;
+  float zgamma1 = (8.f - zw * (5.f + zg3)) * 0.25f;
+  float zgamma2 = 3.f * (zw * (1.f - zg)) * 0.25f;
+  float zgamma3 = (2.f - zg3 * prmuz) * 0.25f;
+  float zgamma4 = 1.f - zgamma3;
+  bool cons;
+  if (zg != 1.f) {
+    float h = zg * rcpf(1.f - zg);
+    float h2 = h * h;
+    float den = 1.f - zc * h2;
+    cons = zw > 0.f && den > 0.f && zc * ((1.f - h2) + (1.f - zwcrit) * h2) <= (1.f - zwcrit);
+  } else {
+    cons = zw >= zwcrit;
+  }
+  if (cons) {
+    float za1 = zgamma1 * prmuz - zgamma3;
+    float zgt = zgamma1 * zto1;
+    Exf e2 = tbl_expf(fminf(zto1 * rmu, 500.f), et, ret);
+    float inv = rcpf(1.f + zgt);
+    o.r = (zgt - za1 * e2.om) * inv;
+    o.t = 1.f - o.r;
+    o.rd = zgt * inv;
+    o.td = 1.f - o.rd;
+    if (e2.e == 1.f) o.r = 0.f, o.t = 1.f, o.rd = 0.f, o.td = 1.f;
+  } else {
+    float za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+    float za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+    float zrk = __builtin_sqrtf(2.f * zc * (zgamma1 + zgamma2));
+    float zrp = zrk * prmuz;
+    float zrp1 = 1.f + zrp, zrm1 = 1.f - zrp, zrk2 = 2.f * zrk;
+    float zrpp = 1.f - zrp * zrp;
+    float zr1 = zrm1 * (za2 + zrk * zgamma3);
+    float zr2 = zrp1 * (za2 - zrk * zgamma3);
+    float zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+    float zt1 = zrp1 * (za1 + zrk * zgamma4);
+    float zt2 = zrm1 * (za1 - zrk * zgamma4);
+    float zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+    Exf e1 = tbl_expf(fminf(zrk * zto1, 500.f), et, ret);
+    Exf e2 = tbl_expf(fminf(zto1 * rmu, 500.f), et, ret);
+    // zr4 / e1 + zr5 e1 = zrpp (g1 (1/e1 - e1) + k (1/e1 + e1)), 1/e1 - e1 = (1 - e1)(1 + e1)/e1
+    float zdenr = zrpp * (zgamma1 * (e1.om * (1.f + e1.e) * e1.r) + zrk * (e1.r + e1.e));
+    if (zdenr >= -eps && zdenr <= eps) {
+      o.r = eps;
+      o.t = e2.e;
+    } else {
+      float rden = rcpf(zdenr);
+      o.r = zw * (zr1 * e1.r - zr2 * e1.e - zr3 * e2.e) * rden;
+      o.t = e2.e - e2.e * zw * (zt1 * e1.r - zt2 * e1.e - zt3 * e2.r) * rden;
+    }
+    // (k + g1) - (g1 - k) e1^2 = k (1 + e1^2) + g1 (1 - e1^2)
+    float om2 = e1.om * (1.f + e1.e);
+    float zdend = rcpf(zrk * (1.f + e1.e * e1.e) + zgamma1 * om2);
+    o.rd = zgamma2 * om2 * zdend;
+    o.td = zrk2 * e1.e * zdend;
+  }
+  return o;
+}
+
+struct SweepF {
+  float tdbt, ztdn, prdnd;
+};
+__device__ inline void sweep_layer_f(SweepF &s, const RT4f &l, float dbt, float &A, float &B,
+                                     float &Cc, float &E) {
+  A = s.ztdn;
+  B = s.prdnd;
+  float a = s.ztdn - s.tdbt;
+  float zreflect = rcpf(1.f - l.rd * s.prdnd);
+  Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
+  E = l.td * zreflect;
+  float ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;
+  float prdnd = l.rd + l.td * l.td * s.prdnd * zreflect;
+  s.tdbt = dbt * s.tdbt;
+  s.ztdn = ztdn;
+  s.prdnd = prdnd;
+}
+
+// Sum four values over the 64 lanes of a wavefront with 7 exchanges: after the first two
+// exchanges each lane carries one of the four partial sums; lanes 0, 16, 32, 48 end with
+// the totals of values 0, 1, 2, 3.
+__device__ inline double wave_sum4(double v0, double v1, double v2, double v3, int lane) {
+  bool b5 = lane & 32, b4 = lane & 16;
+  double sa = b5 ? v0 : v2, sb = b5 ? v1 : v3;
+  double ka = b5 ? v2 : v0, kb = b5 ? v3 : v1;
+  double w0 = ka + __shfl_xor(sa, 32, 64);
+  double w1 = kb + __shfl_xor(sb, 32, 64);
+  double snd = b4 ? w0 : w1, kp = b4 ? w1 : w0;
+  double x = kp + __shfl_xor(snd, 16, 64);
+  x += __shfl_xor(x, 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
+// Sum two values over the 64 lanes of a wavefront with 6 exchanges; lane 0 ends with the total
+// of v0, lane 32 with the total of v1.
+__device__ inline double wave_sum2(double v0, double v1, int lane) {
+  bool b5 = lane & 32;
+  double x = (b5 ? v1 : v0) + __shfl_xor(b5 ? v0 : v1, 32, 64);
+  x += __shfl_xor(x, 16, 64);
+  x += __shfl_xor(x, 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
+// ------------------------------------------------------------------------------------------
+// KISS and the sub-column masks (src/mcica_subcol_gen_sw.f90:468-659, :721-753).
+// ------------------------------------------------------------------------------------------
+struct Kiss {
+  uint32_t s1, s2, s3, s4;
+};
+__device__ inline double kiss_draw(Kiss &k) {
+#pragma clang fp contract(off)
+  k.s1 = 69069u * k.s1 + 1327217885u;
+  uint32_t x = k.s2;
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  k.s2 = x;
+  k.s3 = 18000u * (k.s3 & 65535u) + (k.s3 >> 16);
+  k.s4 = 30903u * (k.s4 & 65535u) + (k.s4 >> 16);
+  int32_t kiss = (int32_t)(k.s1 + k.s2 + (k.s3 << 16) + k.s4);
+  return (double)kiss * 2.328306e-10 + 0.5;
+}
+// Seeds of a column: fractional Pa of the bottom four layers times 1e9 (:471-479).
+__device__ inline uint32_t kiss_seed(double play_hpa) {
+#pragma clang fp contract(off)
+  double p = play_hpa * 1.e2;
+  return (uint32_t)(int32_t)((p - (double)(int32_t)p) * 1000000000.0);
+}
+__device__ inline Kiss kiss_jump(Kiss k, const KissJump &j) {
+  Kiss o;
+  o.s1 = j.a1 * k.s1 + j.c1;
+  uint32_t r = 0;
+  for (int i = 0; i < 32; i++) r ^= ((k.s2 >> i) & 1u) ? j.m2[i] : 0u;
+  o.s2 = r;
+  o.s3 = (uint32_t)(((uint64_t)k.s3 * j.p3) % 1179647999ull);   // 18000*2^16 - 1
+  o.s4 = (uint32_t)(((uint64_t)k.s4 * j.p4) % 2025259007ull);   // 30903*2^16 - 1
+  return o;
+}
+__device__ inline double mt_real(uint32_t u) { return (double)u / (4294967296.0 - 1.0); }
+
+// Sub-column cloudiness of g-point g in every layer of column c (bottom to top), written as
+// bits into mask[w * NGPT + g] (bit l%32 of word l/32). Returns an error code.
+__device__ static int column_masks(int icld, int irng, int permuteseed, int C, int L, int c, int g,
+                                   const double *play, const double *cldfrac, const double *alpha,
+                                   const uint32_t *mt, const KissJump *jumps, uint32_t *mask) {
+#pragma clang fp contract(off)
+  Kiss k = {0, 0, 0, 0};
+  if (irng == 0) {
+    if (play[c] * 1.e2 < play[c + C] * 1.e2) return RRTMGSW_ERR_KISS_SEED;
+    k.s1 = kiss_seed(play[c]);
+    k.s2 = kiss_seed(play[c + (size_t)C]);
+    k.s3 = kiss_seed(play[c + 2 * (size_t)C]);
+    k.s4 = kiss_seed(play[c + 3 * (size_t)C]);
+    k = kiss_jump(k, jumps[g]);
+  }
+  double cdf_prev = 0.0, r3 = 0.0;
+  if (icld == 3) r3 = irng == 0 ? kiss_draw(k) : mt_real(mt[(size_t)g * C + c]);
+  uint32_t word = 0;
+  for (int l = 0; l < L; l++) {
+    double cldf = cldfrac[c + (size_t)C * l];
+    if (cldf < CLDMIN) cldf = 0.0;
+    double cdf, cdf2 = 0.0;
+    if (icld == 3) {
+      cdf = r3;
+    } else if (irng == 0) {
+      cdf = kiss_draw(k);
+      if (icld >= 4) cdf2 = kiss_draw(k);
+    } else {
+      size_t base = ((size_t)g * C + c) * L + l;
+      if (icld >= 4) {
+        cdf = mt_real(mt[2 * base]);
+        cdf2 = mt_real(mt[2 * base + 1]);
+      } else
+        cdf = mt_real(mt[base]);
+    }
+    if (l > 0) {
+      if (icld == 2) {  // :541-551
+        double cprev = cldfrac[c + (size_t)C * (l - 1)];
+        if (cprev < CLDMIN) cprev = 0.0;
+        if (cdf_prev > 1. - cprev) cdf = cdf_prev;
+        else cdf = cdf * (1. - cprev);
+      } else if (icld >= 4) {  // :607-611, :647-651
+        if (cdf2 < alpha[c + (size_t)C * l]) cdf = cdf_prev;
+      }
+    }
+    cdf_prev = cdf;
+    if (cdf >= 1. - cldf) word |= 1u << (l & 31);
+    if ((l & 31) == 31 || l == L - 1) {
+      mask[(l >> 5) * NGPT + g] = word;
+      word = 0;
+    }
+  }
+  return 0;
+}
+
+}  // namespace rrtmgsw

@@ -114,9 +114,23 @@ struct McicaArgs {
   int *status;
 };
 
+// Adjoint of the fused path (rrtmgsw_adjoint.hip): seeds of the total-sky TOA upward flux and
+// heating rates in, gradients with respect to T, H2O, O3, in-cloud water paths and albedos out.
+constexpr int ADJ_WORK_PER_LAYER = 7;   // doubles per (layer, g): sweep state 3, C, E, U, ubar
+struct AdjArgs {
+  RtArgs f;                       // forward inputs (fused form), scalars, status word
+  const double *seed_toa;         // (ncol)        cotangent of swuflx at the top level
+  const double *seed_hr;          // (ncol, nlay)  cotangent of swhr
+  double *d_tlay, *d_h2ovmr, *d_o3vmr, *d_clwp, *d_ciwp;   // (ncol, nlay)
+  double *d_alb;                  // (ncol, 4): asdir, asdif, aldir, aldif
+  double *work;                   // per column (ADJ_WORK_PER_LAYER * nlay + 1) * NGPT doubles
+};
+
 // Kernel launchers (rrtmgsw_kernels.hip)
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, hipStream_t s);
 
