@@ -142,8 +142,11 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
  *   J = sum_c seed_toa(c) * swuflx(c, nlay+1) + sum_(c,k) seed_hr(c, k) * swhr(c, k)
  * (total sky; seeds are device arrays (ncol) and (ncol, nlay)). Writes dJ/d tlay, h2ovmr,
  * o3vmr, clwp and ciwp (the in-cloud water paths of mcica_subcol_sw), each (ncol, nlay), and
- * dJ/d (asdir, asdif, aldir, aldif) as d_alb (ncol, 4). McICA masks are held fixed. No
- * counterpart in the reference; inputs as rrtmgsw_mcica_sw_dev (same argument order). */
+ * dJ/d (asdir, asdif, aldir, aldif) as d_alb (ncol, 4). McICA masks are held fixed. ckpt = 0
+ * keeps every layer's adding-method state (4 doubles per layer and g-point); ckpt = K > 0 keeps
+ * one state per K layers and recomputes each K-layer segment in the reverse sweep (1 + 3/K
+ * doubles per layer and g-point, one more forward layer evaluation per layer). No counterpart
+ * in the reference; inputs as rrtmgsw_mcica_sw_dev (same argument order). */
 int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
                                  const double *play, const double *plev, const double *tlay,
                                  const double *tlev, const double *tsfc, const double *h2ovmr,
@@ -161,7 +164,7 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
                                  const double *solcycfrac, const double *seed_toa,
                                  const double *seed_hr, double *d_tlay, double *d_h2ovmr,
                                  double *d_o3vmr, double *d_clwp, double *d_ciwp, double *d_alb,
-                                 void *stream);
+                                 int ckpt, void *stream);
 
 #ifdef __cplusplus
 }

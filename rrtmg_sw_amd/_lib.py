@@ -53,7 +53,7 @@ def lib():
     L.rrtmgsw_mcica_sw_adjoint_dev.argtypes = (
         [C.c_int] * 5 + [_ip] + [_dp] * 16 +
         [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int] + [_dp] * 10 +
-        [_dp] * 4 + [_dp] * 3 + [_dp] * 2 + [_dp] * 6 + [_vp])
+        [_dp] * 4 + [_dp] * 3 + [_dp] * 2 + [_dp] * 6 + [C.c_int, _vp])
     _lib = L
     return L
 

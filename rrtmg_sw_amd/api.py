@@ -191,12 +191,13 @@ def _fused_inputs(cols, d, irng):
 ADJ_INPUTS = ("tlay", "h2ovmr", "o3vmr", "clwp", "ciwp")
 
 
-def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None):
+def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None, ckpt=0):
     """Adjoint of the fused call (exact-exponential FP64 forward, masks fixed):
     J = sum seed_toa * swuflx(:, top) + sum seed_hr * swhr. seed_toa is a device tensor of
     shape (ncol,), seed_hr one of Fortran shape (ncol, nlay) (i.e. torch shape (nlay, ncol)).
     Returns device tensors dJ/d{tlay, h2ovmr, o3vmr, clwp, ciwp} (Fortran (ncol, nlay)) and
-    dJ/d albedo ("alb", Fortran (ncol, 4): asdir, asdif, aldir, aldif)."""
+    dJ/d albedo ("alb", Fortran (ncol, 4): asdir, asdif, aldir, aldif). ckpt = 0 stores every
+    layer's adding-method state, ckpt = K > 0 one per K layers (recomputing segments)."""
     N, L = cols.ncol, cols.nlay
     if out is None:
         out = {k: _new(True, (N, L)) for k in ADJ_INPUTS}
@@ -204,7 +205,7 @@ def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None):
     ir = C.c_int(cols.irng)
     args = _fused_inputs(cols, dev_arrays, ir)
     args += [None, None, None, _ptr(seed_toa, N), _ptr(seed_hr, N * L)]
-    args += [_ptr(out[k]) for k in ADJ_INPUTS] + [_ptr(out["alb"]), _stream()]
+    args += [_ptr(out[k]) for k in ADJ_INPUTS] + [_ptr(out["alb"]), C.c_int(ckpt), _stream()]
     check(_lib.lib().rrtmgsw_mcica_sw_adjoint_dev(*args))
     return out
 

@@ -11,18 +11,22 @@
 //
 // Mapping: as the forward, one workgroup per column, one lane per g-point. Per lane:
 //   F1  top -> bottom: layer optics and the total-sky downward elimination (vrtqdr_sw
-//       :144-159 as U_j = C_j + E_j U_(j+1), D_j = ztdn_j + prdnd_j U_j), storing the sweep
-//       state s_j = (tdbt, ztdn, prdnd) and C_j, E_j per layer;
-//   F2  bottom -> top: U_j, the objective per unit incident flux Phi;
-//   R2  top -> bottom: adjoint of the back substitution, ubar_j = dJ/dU_j (carried through
-//       E_j), stored per layer; surface adjoint (albedos, s_L);
-//   R1  bottom -> top: the layer's optics recomputed with forward-mode dual numbers over
-//       (taug, taur, cloud tau, ssa, asymmetry), the adjoint of the elimination step, then
+//       :144-159 as U_j = C_j + E_j U_(j+1), D_j = ztdn_j + prdnd_j U_j); since J reaches U_j
+//       through U_j itself, D_j and U_(j-1), ubar_j = dJ/dU_j = ubar_(j-1) E_(j-1) +
+//       zinc (wU_j + wD_j prdnd_j) is known as soon as layer j is: stored per layer, with the
+//       sweep state s_j = (tdbt, ztdn, prdnd) of every layer (store mode) or of every K-th
+//       layer (checkpoint mode); then the surface adjoint (albedos, s_L);
+//   R1  bottom -> top: s_j (loaded, or recomputed per K-layer segment from its checkpoint), the
+//       layer recomputed, U_j by the back substitution and the objective per unit incident
+//       flux Phi on the way up, the adjoint of the elimination step, reftra_sw's
+//       Jacobian in forward-mode dual numbers over (tau, w, g) chained through the layer
+//       optics' dual numbers over (taug, taur, cloud tau, ssa, asymmetry), then
 //       the hand-written adjoints of the gas optics (taumol template) and of cldprmc_sw;
 //       per-layer sums over g go to LDS accumulators;
 //   end the incident-flux adjoint (solar source interpolated at laysolfr), and per layer the
 //       adjoint of inatm_sw + setcoef_sw onto T, H2O, O3.
-// Memory per (column, layer, g-point): s_j (3), C_j, E_j (2), U_j, ubar_j (2): 7 doubles.
+// Memory per (column, layer, g-point): ubar_j and s_j, 4 doubles (store mode), or ubar_j and
+// 3/K for the checkpoints plus a 3K segment buffer per column (checkpoint mode).
 #include "rrtmgsw_device.h"
 
 namespace rrtmgsw {
@@ -131,16 +135,14 @@ __device__ static void gas_optics_adj(const GPoint &g, const double *__restrict_
   key_spec_adj(acc, s1, s2, c1, c2, strrat, lower ? 8.0 : 4.0, bin, fs_b, bin ? kmul_b : 0.0);
 }
 
-// Total-sky optics of one layer and sub-column (src/rrtmg_sw_spcvmc.f90:442-505, :539-587) with
-// exact exponentials; Tn = double or a dual number over (taug, taur, cloud tau, ssa, asym).
+// Layer optics of one sub-column (src/rrtmg_sw_spcvmc.f90:442-505): clear sky with delta
+// scaling, and the total sky (clear + delta-scaled cloud). Tn = double or a dual number.
 template <typename Tn>
-struct LayerT {
-  RTt<Tn> l;
-  Tn dbt;
+struct SkyOpt {
+  Tn tau, w, g;
 };
 template <typename Tn>
-__device__ inline LayerT<Tn> total_layer_exact(Tn taug, Tn taur, double ta, double wa, double ga,
-                                               double cf, Tn ct, Tn cw, Tn cg, double mu0, double rmu) {
+__device__ inline SkyOpt<Tn> clear_optics(Tn taug, Tn taur, double ta, double wa, double ga) {
   Tn tauc = taur + taug + ta;
   Tn omc = taur * 1.0 + ta * wa;
   Tn gcc = (ga * wa * ta) / omc;
@@ -149,29 +151,78 @@ __device__ inline LayerT<Tn> total_layer_exact(Tn taug, Tn taur, double ta, doub
   tauc = (1.0 - zwf) * tauc;
   omc = (omc - zwf) / (1.0 - zwf);
   gcc = (gcc - zf) / (1.0 - zf);
-  LayerT<Tn> o;
-  o.l = reftra_exact(tauc, omc, gcc, mu0, rmu);
-  o.dbt = dexp(0.0 - tauc * rmu);
+  return {tauc, omc, gcc};
+}
+template <typename Tn>
+__device__ inline SkyOpt<Tn> total_optics(const SkyOpt<Tn> &c, Tn ct, Tn cw, Tn cg) {
+  Tn tauo = c.tau + ct;
+  Tn omo = c.tau * c.w + ct * cw;
+  Tn go = (ct * cw * cg + c.tau * c.w * c.g) / omo;
+  omo = omo / tauo;
+  return {tauo, omo, go};
+}
+
+// Total-sky layer (r, rd, t, td, direct-beam transmittance) of the exact-exponential forward:
+// blend of the clear and cloudy solutions by the sub-column's cloud fraction (:539-549); for
+// McICA cloud fractions of 0 or 1 only one of the two is evaluated.
+struct LayerV {
+  double r, rd, t, td, dbt;
+};
+__device__ inline LayerV total_layer_exact(double taug, double taur, const double ta, double wa,
+                                           double ga, double cf, double ct, double cw, double cg,
+                                           double mu0, double rmu) {
+  SkyOpt<double> c = clear_optics<double>(taug, taur, ta, wa, ga);
+  LayerV o = {0., 0., 1., 1., 1.};
+  double zcl = 1.0 - cf;
+  if (zcl != 0.0) {
+    RTt<double> rc = reftra_exact<double>(c.tau, c.w, c.g, mu0, rmu);
+    o = {zcl * rc.r, zcl * rc.rd, zcl * rc.t, zcl * rc.td, zcl * rc.e2};
+  }
   if (cf > 0.0) {
-    Tn tauo = tauc + ct;
-    Tn omo = tauc * omc + ct * cw;
-    Tn go = (ct * cw * cg + tauc * omc * gcc) / omo;
-    omo = omo / tauo;
-    Tn dbo = dexp(0.0 - tauo * rmu);
-    RTt<Tn> ro;
-    if (cf > REPCLC) {
-      ro = reftra_exact(tauo, omo, go, mu0, rmu);
+    SkyOpt<double> t = total_optics<double>(c, ct, cw, cg);
+    RTt<double> ro = reftra_exact<double>(t.tau, t.w, t.g, mu0, rmu);
+    double dbo = ro.e2;
+    if (!(cf > REPCLC)) ro.r = 0., ro.rd = 0., ro.t = 1., ro.td = 1.;
+    if (zcl == 0.0) {
+      o = {ro.r, ro.rd, ro.t, ro.td, dbo};
     } else {
-      ro.r = 0.0 * taug, ro.rd = 0.0 * taug, ro.t = 0.0 * taug + 1.0, ro.td = 0.0 * taug + 1.0;
+      o.r += cf * ro.r, o.rd += cf * ro.rd, o.t += cf * ro.t, o.td += cf * ro.td, o.dbt += cf * dbo;
     }
-    double zcl = 1.0 - cf;
-    o.l.r = zcl * o.l.r + cf * ro.r;
-    o.l.rd = zcl * o.l.rd + cf * ro.rd;
-    o.l.t = zcl * o.l.t + cf * ro.t;
-    o.l.td = zcl * o.l.td + cf * ro.td;
-    o.dbt = zcl * o.dbt + cf * dbo;
   }
   return o;
+}
+
+// Adjoint of total_layer_exact: given the adjoints lb of (r, rd, t, td, dbt), returns in xb the
+// adjoints of (taug, taur, ct, cw, cg). reftra is differentiated in forward mode over
+// (tau, w, g) (three directions), the optics combinations over their own inputs.
+__device__ inline void total_layer_adj(double taug, double taur, double ta, double wa, double ga,
+                                       double cf, double ct, double cw, double cg, double mu0,
+                                       double rmu, const double *lb, double *xb) {
+  typedef Dual<3> D3;
+  typedef Dual<5> D5;
+  // sky optics as functions of (taug, taur, ct, cw, cg)
+  SkyOpt<D5> c = clear_optics<D5>(dvar<5>(taug, 0), dvar<5>(taur, 1), ta, wa, ga);
+#pragma unroll
+  for (int k = 0; k < 5; k++) xb[k] = 0.0;
+  auto sky = [&](const SkyOpt<D5> &o, double wgt, bool total) {
+    RTt<D3> r = reftra_exact<D3>(dvar<3>(o.tau.v, 0), dvar<3>(o.w.v, 1), dvar<3>(o.g.v, 2), mu0, rmu);
+    if (total && !(cf > REPCLC)) {   // no cloudy reftra: r = 0, t = 1; only the direct beam counts
+      r.r = dconst<3>(0.), r.rd = dconst<3>(0.), r.t = dconst<3>(1.), r.td = dconst<3>(1.);
+    }
+    double mb[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+      mb[m] = wgt * (lb[0] * r.r.d[m] + lb[1] * r.rd.d[m] + lb[2] * r.t.d[m] + lb[3] * r.td.d[m] +
+                     lb[4] * r.e2.d[m]);
+#pragma unroll
+    for (int k = 0; k < 5; k++) xb[k] += mb[0] * o.tau.d[k] + mb[1] * o.w.d[k] + mb[2] * o.g.d[k];
+  };
+  double zcl = 1.0 - cf;
+  if (zcl != 0.0) sky(c, zcl, false);
+  if (cf > 0.0) {
+    SkyOpt<D5> t = total_optics<D5>(c, dvar<5>(ct, 2), dvar<5>(cw, 3), dvar<5>(cg, 4));
+    sky(t, cf, true);
+  }
 }
 
 // Everything a layer of one sub-column needs besides the gas optics (aerosol and cloud optics),
@@ -327,6 +378,13 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
   q_b -= coldry_b * coldry * ((amw - amd) / amm + 1. / (1. + q));
 }
 
+// Doubles of work area per column per g-point stride group: UB (L) + states (L or ceil(L/K))
+// + segment buffer (K), each times 3 for states, times NGPT.
+__host__ __device__ inline size_t adj_work_doubles(int L, int K) {
+  size_t states = K == 0 ? (size_t)L * 3 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
+  return ((size_t)L + states) * NGPT;
+}
+
 __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const RtArgs &a = A.f;
@@ -388,58 +446,58 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
   const double albdif = vis ? a.asdif[c] : a.aldif[c];
   const double zinc = active ? incident_flux(gp, T.tab, v, a.sc, mu0) : 0.0;
 
-  // per column: S (L, 5, NGPT) = tdbt ztdn prdnd C E; U (L+1, NGPT); UB (L, NGPT)
-  double *S = A.work + (size_t)cl * ADJ_WORK_PER_LAYER * L * NGPT + (size_t)cl * NGPT;
-  double *U = S + (size_t)5 * L * NGPT;
-  double *UB = U + (size_t)(L + 1) * NGPT;
-  auto sp = [&](int j, int k) -> double & { return S[((size_t)j * 5 + k) * NGPT + g]; };
+  // Per-column work area: UB (L, NGPT) = dJ/dU_j, then either every sweep state
+  // S (L, 3, NGPT) (store mode) or one state per K layers CK (ceil(L/K), 3, NGPT) plus a
+  // segment buffer SG (K, 3, NGPT) (checkpoint mode, K = A.ckpt).
+  const int K = A.ckpt;
+  double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
+  double *S = UB + (size_t)L * NGPT;
+  double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
+  auto sp = [&](int j, int k) -> double & { return S[((size_t)j * 3 + k) * NGPT + g]; };
+  auto sg = [&](int j, int k) -> double & { return SG[((size_t)j * 3 + k) * NGPT + g]; };
 
-  // ---- F1: top -> bottom ----
-  Sweep st = {1.0, 1.0, 0.0};
-  for (int j = 0; j < L; j++) {
+  auto layer_at = [&](int j, LayerIn &in, double &taug, double &taur) {
     const int lay = L - 1 - j;
-    double taug, taur;
     gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-    LayerIn in = layer_inputs(a, T, gp, v, c, g, lay, active, err);
-    LayerT<double> ly = total_layer_exact<double>(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw,
-                                                  in.cg, mu0, rmu);
-    RT4 l = {ly.l.r, ly.l.rd, ly.l.t, ly.l.td};
-    double Aa, Bb, Cc, Ee;
-    if (active) sp(j, 0) = st.tdbt, sp(j, 1) = st.ztdn, sp(j, 2) = st.prdnd;
-    sweep_layer(st, l, ly.dbt, Aa, Bb, Cc, Ee);
-    if (active) sp(j, 3) = Cc, sp(j, 4) = Ee;
-  }
-  if (err) atomicCAS(a.status, 0, err);
+    in = layer_inputs(a, T, gp, v, c, g, lay, active, err);
+    return total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
+  };
 
-  // ---- F2: bottom -> top: U_j and Phi = J / zinc for this g-point ----
-  const double sig = 1. / (1. - albdif * st.prdnd);
-  const double Nsf = albdir * st.tdbt + albdif * (st.ztdn - st.tdbt);
-  double u = Nsf * sig;
-  if (active) U[(size_t)L * NGPT + g] = u;
-  double phi = wU[L] * u + wD[L] * (st.ztdn + st.prdnd * u);
-  for (int j = L - 1; j >= 0; j--) {
-    double Cc = active ? sp(j, 3) : 0.0, Ee = active ? sp(j, 4) : 0.0;
-    u = Cc + Ee * u;
-    if (active) U[(size_t)j * NGPT + g] = u;
-    double z = active ? sp(j, 1) : 0.0, p = active ? sp(j, 2) : 0.0;
-    phi += wU[j] * u + wD[j] * (z + p * u);
-  }
-
-  // ---- R2: top -> bottom: ubar_j = dJ/dU_j ----
+  // ---- F1: top -> bottom: sweep states and ubar_j = dJ/dU_j. The back substitution
+  // U_j = C_j + E_j U_(j+1) feeds J through U_j and D_j = ztdn_j + prdnd_j U_j, so
+  // ubar_j = ubar_(j-1) E_(j-1) + zinc (wU_j + wD_j prdnd_j), known as soon as layer j is. ----
+  Sweep st = {1.0, 1.0, 0.0};
   double carry = 0.0;
   for (int j = 0; j < L; j++) {
-    double p = active ? sp(j, 2) : 0.0, Ee = active ? sp(j, 4) : 0.0;
-    double ub = carry + zinc * (wU[j] + wD[j] * p);
+    if (active) {
+      if (K == 0) {
+        sp(j, 0) = st.tdbt, sp(j, 1) = st.ztdn, sp(j, 2) = st.prdnd;
+      } else if (j % K == 0) {
+        sp(j / K, 0) = st.tdbt, sp(j / K, 1) = st.ztdn, sp(j / K, 2) = st.prdnd;
+      }
+    }
+    LayerIn in;
+    double taug, taur;
+    LayerV ly = layer_at(j, in, taug, taur);
+    RT4 l = {ly.r, ly.rd, ly.t, ly.td};
+    double Aa, Bb, Cc, Ee;
+    sweep_layer(st, l, ly.dbt, Aa, Bb, Cc, Ee);
+    double ub = carry + zinc * (wU[j] + wD[j] * Bb);
     if (active) UB[(size_t)j * NGPT + g] = ub;
     carry = ub * Ee;
   }
-  // surface: U_L = (adir tdbt + adif (ztdn - tdbt)) / (1 - adif prdnd), D_L = ztdn + prdnd U_L
+  if (err) atomicCAS(a.status, 0, err);
+
+  // ---- surface: U_L = (adir tdbt + adif (ztdn - tdbt)) / (1 - adif prdnd), D_L = ztdn + prdnd U_L ----
+  const double sig = 1. / (1. - albdif * st.prdnd);
+  const double Nsf = albdir * st.tdbt + albdif * (st.ztdn - st.tdbt);
+  double u = Nsf * sig;                                   // U_L, then U_j going up
+  double phi = wU[L] * u + wD[L] * (st.ztdn + st.prdnd * u);
   Sweep sb = {0.0, 0.0, 0.0};
   {
-    const double uL = Nsf * sig;
     double ubL = carry + zinc * (wU[L] + wD[L] * st.prdnd);
     sb.ztdn += zinc * wD[L];
-    sb.prdnd += zinc * wD[L] * uL;
+    sb.prdnd += zinc * wD[L] * u;
     double N_b = ubL * sig, sig_b = ubL * Nsf;
     double adir_b = N_b * st.tdbt, adif_b = N_b * (st.ztdn - st.tdbt);
     sb.tdbt += N_b * (albdir - albdif);
@@ -454,24 +512,42 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
 
   // ---- R1: bottom -> top ----
   for (int j = L - 1; j >= 0; j--) {
+    if (K > 0 && (j == L - 1 || (j + 1) % K == 0)) {
+      // checkpoint mode: recompute the sweep states of this segment from its checkpoint
+      const int j0 = (j / K) * K;
+      Sweep ss = {active ? sp(j0 / K, 0) : 1.0, active ? sp(j0 / K, 1) : 1.0, active ? sp(j0 / K, 2) : 0.0};
+      for (int jj = j0; jj < j; jj++) {
+        if (active) sg(jj - j0, 0) = ss.tdbt, sg(jj - j0, 1) = ss.ztdn, sg(jj - j0, 2) = ss.prdnd;
+        LayerIn in;
+        double taug, taur;
+        LayerV ly = layer_at(jj, in, taug, taur);
+        RT4 l = {ly.r, ly.rd, ly.t, ly.td};
+        double Aa, Bb, Cc, Ee;
+        sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
+      }
+      if (active) sg(j - j0, 0) = ss.tdbt, sg(j - j0, 1) = ss.ztdn, sg(j - j0, 2) = ss.prdnd;
+    }
+    Sweep sj;
+    if (K == 0)
+      sj = {active ? sp(j, 0) : 1.0, active ? sp(j, 1) : 1.0, active ? sp(j, 2) : 0.0};
+    else
+      sj = {active ? sg(j % K, 0) : 1.0, active ? sg(j % K, 1) : 1.0, active ? sg(j % K, 2) : 0.0};
+    const double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
     const int lay = L - 1 - j;
-    Sweep sj = {active ? sp(j, 0) : 1.0, active ? sp(j, 1) : 1.0, active ? sp(j, 2) : 0.0};
-    double Uj = active ? U[(size_t)j * NGPT + g] : 0.0, Uj1 = active ? U[(size_t)(j + 1) * NGPT + g] : 0.0;
-    double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
+    LayerIn in;
     double taug, taur;
-    gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-    LayerIn in = layer_inputs(a, T, gp, v, c, g, lay, active, err);
-    typedef Dual<5> D5;
-    LayerT<D5> ly = total_layer_exact<D5>(dvar<5>(taug, 0), dvar<5>(taur, 1), in.ta, in.wa, in.ga, in.cf,
-                                          dvar<5>(in.ct, 2), dvar<5>(in.cw, 3), dvar<5>(in.cg, 4), mu0, rmu);
-    RT4 l = {ly.l.r.v, ly.l.rd.v, ly.l.t.v, ly.l.td.v};
-    double lb[5];
-    sweep_layer_adj(sj, l, ly.dbt.v, sb, ub, ub * Uj1, zinc * wD[j], zinc * wD[j] * Uj, lb);
-    double xb[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++)
-      xb[k] = lb[0] * ly.l.r.d[k] + lb[1] * ly.l.rd.d[k] + lb[2] * ly.l.t.d[k] + lb[3] * ly.l.td.d[k] +
-              lb[4] * ly.dbt.d[k];
+    LayerV ly = layer_at(j, in, taug, taur);
+    RT4 l = {ly.r, ly.rd, ly.t, ly.td};
+    // C_j, E_j and U_j from the recomputed layer
+    Sweep tmp = sj;
+    double Aa, Bb, Cc, Ee;
+    sweep_layer(tmp, l, ly.dbt, Aa, Bb, Cc, Ee);
+    const double Uj1 = u;
+    u = Cc + Ee * Uj1;
+    phi += wU[j] * u + wD[j] * (sj.ztdn + sj.prdnd * u);
+    double lb[5], xb[5];
+    sweep_layer_adj(sj, l, ly.dbt, sb, ub, ub * Uj1, zinc * wD[j], zinc * wD[j] * u, lb);
+    total_layer_adj(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu, lb, xb);
     double la[AC_N];
 #pragma unroll
     for (int k = 0; k < AC_N; k++) la[k] = 0.0;
@@ -533,6 +609,8 @@ size_t adj_lds_bytes(int nlay) {
   ColLayout lo(nlay, (nlay + 31) / 32);
   return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4);
 }
+
+size_t adj_work_bytes(int nlay, int ckpt) { return adj_work_doubles(nlay, ckpt) * sizeof(double); }
 
 hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s) {
   int per = (a.f.ncol_chunk + 7) / 8;

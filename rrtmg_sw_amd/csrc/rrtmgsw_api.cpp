@@ -814,7 +814,7 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
                                  const double *solcycfrac, const double *seed_toa,
                                  const double *seed_hr, double *d_tlay, double *d_h2ovmr,
                                  double *d_o3vmr, double *d_clwp, double *d_ciwp, double *d_alb,
-                                 void *stream) {
+                                 int ckpt, void *stream) {
   if (int rc = check_init()) return rc;
   hipStream_t s = (hipStream_t)stream;
   RtArgs a{};
@@ -834,7 +834,9 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
   A.seed_toa = seed_toa, A.seed_hr = seed_hr;
   A.d_tlay = d_tlay, A.d_h2ovmr = d_h2ovmr, A.d_o3vmr = d_o3vmr;
   A.d_clwp = d_clwp, A.d_ciwp = d_ciwp, A.d_alb = d_alb;
-  const size_t per_col = ((size_t)ADJ_WORK_PER_LAYER * nlay + 1) * NGPT * sizeof(double);
+  if (ckpt < 0 || ckpt > nlay) return fail(RRTMGSW_ERR_ARGUMENT, "ckpt must lie in 0..nlay");
+  A.ckpt = ckpt;
+  const size_t per_col = adj_work_bytes(nlay, ckpt);
   int chunk = (int)std::min<size_t>((size_t)ncol, std::max<size_t>(64, ((size_t)4 << 30) / per_col));
   void *work = nullptr, *st = nullptr;
   {

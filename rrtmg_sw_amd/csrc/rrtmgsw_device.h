@@ -656,10 +656,13 @@ template <int N> __device__ inline double val(const Dual<N> &x) { return x.v; }
 
 // reftra_sw with exact exponentials in place of the lookup table and its expansion: the smooth
 // function of (tau, w, g) that the adjoint differentiates and that finite differences probe.
-// Same branches and expressions as the reference (src/rrtmg_sw_reftra.f90:164-322).
+// Same branches and expressions as the reference (src/rrtmg_sw_reftra.f90:164-322); exp(+x) is
+// taken as 1 / exp(-x). The direct-beam transmittance of this smooth forward is e2 (the
+// reference evaluates the same exponential separately, src/rrtmg_sw_spcvmc.f90:558-585).
 template <typename Tn>
 struct RTt {
   Tn r, rd, t, td;
+  Tn e2;   // exp(-min(tau / mu0, 500)): also the layer's direct-beam transmittance
 };
 template <typename Tn>
 __device__ inline RTt<Tn> reftra_exact(Tn zto1, Tn zw, Tn zg, double prmuz, double rmu) {
@@ -678,6 +681,7 @@ __device__ inline RTt<Tn> reftra_exact(Tn zto1, Tn zw, Tn zg, double prmuz, doub
     Tn za1 = zgamma1 * prmuz - zgamma3;
     Tn zgt = zgamma1 * zto1;
     Tn ze2 = dexp(0.0 - dmin(zto1 * rmu, 500.));
+    o.e2 = ze2;
     Tn inv = 1. / (1. + zgt);
     o.r = (zgt - za1 * (1. - ze2)) * inv;
     o.t = 1. - o.r;
@@ -701,7 +705,9 @@ __device__ inline RTt<Tn> reftra_exact(Tn zto1, Tn zw, Tn zg, double prmuz, doub
     Tn zt2 = zrm1 * (za1 - zrk * zgamma4);
     Tn zt3 = zrk2 * (zgamma4 + za1 * prmuz);
     Tn x1 = dmin(zrk * zto1, 500.), x2 = dmin(zto1 * rmu, 500.);
-    Tn zem1 = dexp(0.0 - x1), zep1 = dexp(x1), zem2 = dexp(0.0 - x2), zep2 = dexp(x2);
+    Tn zem1 = dexp(0.0 - x1), zem2 = dexp(0.0 - x2);
+    Tn zep1 = 1. / zem1, zep2 = 1. / zem2;
+    o.e2 = zem2;
     Tn zdenr = zr4 * zep1 + zr5 * zem1;
     if (val(zdenr) >= -eps && val(zdenr) <= eps) {
       o.r = 0. * zto1 + eps;

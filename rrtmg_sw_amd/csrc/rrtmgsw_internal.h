@@ -116,15 +116,16 @@ struct McicaArgs {
 
 // Adjoint of the fused path (rrtmgsw_adjoint.hip): seeds of the total-sky TOA upward flux and
 // heating rates in, gradients with respect to T, H2O, O3, in-cloud water paths and albedos out.
-constexpr int ADJ_WORK_PER_LAYER = 7;   // doubles per (layer, g): sweep state 3, C, E, U, ubar
 struct AdjArgs {
   RtArgs f;                       // forward inputs (fused form), scalars, status word
   const double *seed_toa;         // (ncol)        cotangent of swuflx at the top level
   const double *seed_hr;          // (ncol, nlay)  cotangent of swhr
   double *d_tlay, *d_h2ovmr, *d_o3vmr, *d_clwp, *d_ciwp;   // (ncol, nlay)
   double *d_alb;                  // (ncol, 4): asdir, asdif, aldir, aldif
-  double *work;                   // per column (ADJ_WORK_PER_LAYER * nlay + 1) * NGPT doubles
+  double *work;                   // per column adj_work_bytes(nlay, ckpt)
+  int ckpt;                       // 0: store every sweep state; K > 0: checkpoint every K layers
 };
+size_t adj_work_bytes(int nlay, int ckpt);   // work area per column
 
 // Kernel launchers (rrtmgsw_kernels.hip)
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s);

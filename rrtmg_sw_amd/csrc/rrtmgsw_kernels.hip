@@ -25,16 +25,15 @@ namespace rrtmgsw {
 // Layer reflectance/transmittance and direct-beam transmittance with the reference's table or
 // with exact exponentials.
 template <bool EXACT>
-__device__ inline RT4 reftra_m(double tau, double w, double g, double mu0, double rmu, const DeviceTables &T) {
+__device__ inline RT4 reftra_m(double tau, double w, double g, double mu0, double rmu,
+                               const DeviceTables &T, double &dbt) {
   if (EXACT) {
     RTt<double> o = reftra_exact<double>(tau, w, g, mu0, rmu);
+    dbt = o.e2;
     return {o.r, o.rd, o.t, o.td};
   }
+  dbt = tbl_exp2(tau * rmu, T.exp_tbl, T.rexp_tbl).e;
   return reftra1(tau, w, g, mu0, rmu, T.exp_tbl, T.rexp_tbl);
-}
-template <bool EXACT>
-__device__ inline double dbt_m(double x, const DeviceTables &T) {
-  return EXACT ? ::exp(-x) : tbl_exp2(x, T.exp_tbl, T.rexp_tbl).e;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -212,8 +211,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
       tauc = (1.0 - zwf) * tauc;
       omc = (omc - zwf) / (1.0 - zwf);
       gcc = (gcc - zf) / (1.0 - zf);
-      RT4 rc = reftra_m<EXACT>(tauc, omc, gcc, mu0, rmu, T);
-      double dbc = dbt_m<EXACT>(tauc * rmu, T);
+      double dbc;
+      RT4 rc = reftra_m<EXACT>(tauc, omc, gcc, mu0, rmu, T, dbc);
       Sweep pre = sc_;
       double Bc, Cc, Ec;
       sweep_layer(sc_, rc, dbc, Ac, Bc, Cc, Ec);
@@ -230,9 +229,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) r
         RT4 rt = rc;
         double dbt = dbc;
         if (cf > 0.0) {
-          double dbo = dbt_m<EXACT>(tauo * rmu, T);
-          RT4 ro = {0.0, 0.0, 1.0, 1.0};
-          if (cf > REPCLC) ro = reftra_m<EXACT>(tauo, omo, go, mu0, rmu, T);
+          double dbo;
+          RT4 ro = reftra_m<EXACT>(tauo, omo, go, mu0, rmu, T, dbo);
+          if (!(cf > REPCLC)) ro = {0.0, 0.0, 1.0, 1.0};
           double zcl = 1.0 - cf;
           rt.r = zcl * rc.r + cf * ro.r;
           rt.rd = zcl * rc.rd + cf * ro.rd;
