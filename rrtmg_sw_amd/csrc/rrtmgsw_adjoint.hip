@@ -14,19 +14,21 @@
 //       :144-159 as U_j = C_j + E_j U_(j+1), D_j = ztdn_j + prdnd_j U_j); since J reaches U_j
 //       through U_j itself, D_j and U_(j-1), ubar_j = dJ/dU_j = ubar_(j-1) E_(j-1) +
 //       zinc (wU_j + wD_j prdnd_j) is known as soon as layer j is: stored per layer, with the
-//       sweep state s_j = (tdbt, ztdn, prdnd) of every layer (store mode) or of every K-th
-//       layer (checkpoint mode); then the surface adjoint (albedos, s_L);
+//       sweep state s_j = (tdbt, ztdn, prdnd), layer (r, rd, t, td, dbt) and gas optics
+//       (taug, taur) of every layer (store mode) or the state of every K-th layer (checkpoint mode); then the surface
+//       adjoint (albedos, s_L);
 //   R1  bottom -> top: s_j (loaded, or recomputed per K-layer segment from its checkpoint), the
 //       layer recomputed, U_j by the back substitution and the objective per unit incident
-//       flux Phi on the way up, the adjoint of the elimination step, reftra_sw's
-//       Jacobian in forward-mode dual numbers over (tau, w, g) chained through the layer
-//       optics' dual numbers over (taug, taur, cloud tau, ssa, asymmetry), then
+//       flux Phi on the way up, the adjoint of the elimination step, reftra_sw's reverse-mode
+//       derivative chained through the layer optics' dual numbers over (taug, taur, cloud
+//       tau, ssa, asymmetry), then
 //       the hand-written adjoints of the gas optics (taumol template) and of cldprmc_sw;
 //       per-layer sums over g go to LDS accumulators;
 //   end the incident-flux adjoint (solar source interpolated at laysolfr), and per layer the
 //       adjoint of inatm_sw + setcoef_sw onto T, H2O, O3.
-// Memory per (column, layer, g-point): ubar_j and s_j, 4 doubles (store mode), or ubar_j and
-// 3/K for the checkpoints plus a 3K segment buffer per column (checkpoint mode).
+// Memory per (column, layer, g-point): ubar_j, s_j, the layer's r, rd, t, td and direct-beam
+// transmittance and its gas and Rayleigh optical depths, 11 doubles (store mode), or ubar_j and 3/K for the checkpoints plus a 3K
+// segment buffer per column (checkpoint mode, every layer recomputed twice).
 #include "rrtmgsw_device.h"
 
 namespace rrtmgsw {
@@ -193,27 +195,22 @@ __device__ inline LayerV total_layer_exact(double taug, double taur, const doubl
 }
 
 // Adjoint of total_layer_exact: given the adjoints lb of (r, rd, t, td, dbt), returns in xb the
-// adjoints of (taug, taur, ct, cw, cg). reftra is differentiated in forward mode over
-// (tau, w, g) (three directions), the optics combinations over their own inputs.
+// adjoints of (taug, taur, ct, cw, cg). reftra in reverse mode (reftra_exact_adj), the optics
+// combinations in forward-mode dual numbers over their five inputs.
 __device__ inline void total_layer_adj(double taug, double taur, double ta, double wa, double ga,
                                        double cf, double ct, double cw, double cg, double mu0,
                                        double rmu, const double *lb, double *xb) {
-  typedef Dual<3> D3;
   typedef Dual<5> D5;
   // sky optics as functions of (taug, taur, ct, cw, cg)
   SkyOpt<D5> c = clear_optics<D5>(dvar<5>(taug, 0), dvar<5>(taur, 1), ta, wa, ga);
 #pragma unroll
   for (int k = 0; k < 5; k++) xb[k] = 0.0;
   auto sky = [&](const SkyOpt<D5> &o, double wgt, bool total) {
-    RTt<D3> r = reftra_exact<D3>(dvar<3>(o.tau.v, 0), dvar<3>(o.w.v, 1), dvar<3>(o.g.v, 2), mu0, rmu);
-    if (total && !(cf > REPCLC)) {   // no cloudy reftra: r = 0, t = 1; only the direct beam counts
-      r.r = dconst<3>(0.), r.rd = dconst<3>(0.), r.t = dconst<3>(1.), r.td = dconst<3>(1.);
-    }
+    // no cloudy reftra when the cloud fraction is tiny: r = 0, t = 1, only the direct beam counts
+    const bool rt = !(total && !(cf > REPCLC));
     double mb[3];
-#pragma unroll
-    for (int m = 0; m < 3; m++)
-      mb[m] = wgt * (lb[0] * r.r.d[m] + lb[1] * r.rd.d[m] + lb[2] * r.t.d[m] + lb[3] * r.td.d[m] +
-                     lb[4] * r.e2.d[m]);
+    reftra_exact_adj(o.tau.v, o.w.v, o.g.v, mu0, rmu, rt ? wgt * lb[0] : 0., rt ? wgt * lb[1] : 0.,
+                     rt ? wgt * lb[2] : 0., rt ? wgt * lb[3] : 0., wgt * lb[4], mb[0], mb[1], mb[2]);
 #pragma unroll
     for (int k = 0; k < 5; k++) xb[k] += mb[0] * o.tau.d[k] + mb[1] * o.w.d[k] + mb[2] * o.g.d[k];
   };
@@ -378,10 +375,10 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
   q_b -= coldry_b * coldry * ((amw - amd) / amm + 1. / (1. + q));
 }
 
-// Doubles of work area per column per g-point stride group: UB (L) + states (L or ceil(L/K))
-// + segment buffer (K), each times 3 for states, times NGPT.
+// Doubles of work area per column: (UB (L) + per-layer states and layers (10 L, store mode) or
+// checkpoints (3 ceil(L/K)) and a segment buffer (3 K)) times NGPT.
 __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
-  size_t states = K == 0 ? (size_t)L * 3 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
+  size_t states = K == 0 ? (size_t)L * 10 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
   return ((size_t)L + states) * NGPT;
 }
 
@@ -453,7 +450,8 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
   double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
   double *S = UB + (size_t)L * NGPT;
   double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
-  auto sp = [&](int j, int k) -> double & { return S[((size_t)j * 3 + k) * NGPT + g]; };
+  const int SW = K == 0 ? 10 : 3;   // store mode: tdbt ztdn prdnd r rd t td dbt taug taur per layer
+  auto sp = [&](int j, int k) -> double & { return S[((size_t)j * SW + k) * NGPT + g]; };
   auto sg = [&](int j, int k) -> double & { return SG[((size_t)j * 3 + k) * NGPT + g]; };
 
   auto layer_at = [&](int j, LayerIn &in, double &taug, double &taur) {
@@ -479,6 +477,10 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
     LayerIn in;
     double taug, taur;
     LayerV ly = layer_at(j, in, taug, taur);
+    if (active && K == 0) {
+      sp(j, 3) = ly.r, sp(j, 4) = ly.rd, sp(j, 5) = ly.t, sp(j, 6) = ly.td, sp(j, 7) = ly.dbt;
+      sp(j, 8) = taug, sp(j, 9) = taur;
+    }
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     double Aa, Bb, Cc, Ee;
     sweep_layer(st, l, ly.dbt, Aa, Bb, Cc, Ee);
@@ -536,9 +538,17 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
     const int lay = L - 1 - j;
     LayerIn in;
     double taug, taur;
-    LayerV ly = layer_at(j, in, taug, taur);
+    LayerV ly;
+    if (K == 0) {   // stored layer and gas optics; aerosol and cloud optics recomputed
+      taug = active ? sp(j, 8) : 0., taur = active ? sp(j, 9) : 0.;
+      in = layer_inputs(a, T, gp, v, c, g, lay, active, err);
+      ly = {active ? sp(j, 3) : 0., active ? sp(j, 4) : 0., active ? sp(j, 5) : 1., active ? sp(j, 6) : 1.,
+            active ? sp(j, 7) : 1.};
+    } else {
+      ly = layer_at(j, in, taug, taur);
+    }
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
-    // C_j, E_j and U_j from the recomputed layer
+    // C_j, E_j and U_j from the layer
     Sweep tmp = sj;
     double Aa, Bb, Cc, Ee;
     sweep_layer(tmp, l, ly.dbt, Aa, Bb, Cc, Ee);

@@ -629,8 +629,33 @@ __device__ inline Dual<N> operator*(const Dual<N> &a, double b) {
 template <int N> __device__ inline Dual<N> operator*(double a, const Dual<N> &b) { return b * a; }
 template <int N> __device__ inline Dual<N> operator/(const Dual<N> &a, double b) { return a * (1.0 / b); }
 template <int N> __device__ inline Dual<N> operator/(double a, const Dual<N> &b) { return dconst<N>(a) / b; }
+// exp(y) for |y| <= 700 (the exact-exponential forms clamp their arguments at 500): Cody-Waite
+// reduction y = k ln2 + r with ln2 split in a high part of 32 significant bits and a low part
+// (the fdlibm constants), |r| <= ln2/2, exp(r) by its Taylor series to degree 12 (truncation
+// below 2e-16 relative), scaled by 2^k. No special cases: arguments stay in range.
+__device__ inline double exp_cw(double y) {
+  const double LOG2E = 1.4426950408889634074;
+  const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
+  double k = rint(y * LOG2E);
+  double r = fma(-k, LN2_HI, y);
+  r = fma(-k, LN2_LO, r);
+  double p = 1.0 / 479001600.0;          // 1/12!
+  p = fma(p, r, 1.0 / 39916800.0);       // 1/11!
+  p = fma(p, r, 1.0 / 3628800.0);        // 1/10!
+  p = fma(p, r, 1.0 / 362880.0);         // 1/9!
+  p = fma(p, r, 1.0 / 40320.0);          // 1/8!
+  p = fma(p, r, 1.0 / 5040.0);           // 1/7!
+  p = fma(p, r, 1.0 / 720.0);            // 1/6!
+  p = fma(p, r, 1.0 / 120.0);            // 1/5!
+  p = fma(p, r, 1.0 / 24.0);             // 1/4!
+  p = fma(p, r, 1.0 / 6.0);              // 1/3!
+  p = fma(p, r, 0.5);                    // 1/2!
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
 __device__ inline double dsqrt(double x) { return ::sqrt(x); }
-__device__ inline double dexp(double x) { return ::exp(x); }
+__device__ inline double dexp(double x) { return exp_cw(x); }
 __device__ inline double dmin(double a, double b) { return ::fmin(a, b); }
 template <int N>
 __device__ inline Dual<N> dsqrt(const Dual<N> &a) {
@@ -644,7 +669,7 @@ __device__ inline Dual<N> dsqrt(const Dual<N> &a) {
 template <int N>
 __device__ inline Dual<N> dexp(const Dual<N> &a) {
   Dual<N> r;
-  r.v = ::exp(a.v);
+  r.v = exp_cw(a.v);
 #pragma unroll
   for (int i = 0; i < N; i++) r.d[i] = a.d[i] * r.v;
   return r;
@@ -723,6 +748,129 @@ __device__ inline RTt<Tn> reftra_exact(Tn zto1, Tn zw, Tn zg, double prmuz, doub
     o.td = zrk2 * zem1 * zdend;
   }
   return o;
+}
+
+// Reverse-mode derivative of reftra_exact: given the adjoints of (r, rd, t, td, e2), returns the
+// adjoints of (tau, w, g). The forward expressions are recomputed, then swept in reverse.
+__device__ inline void reftra_exact_adj(double zto1, double zw, double zg, double prmuz, double rmu,
+                                        double rb, double rdb, double tb, double tdb, double e2b,
+                                        double &taub, double &wb, double &gb) {
+  const double zwcrit = 0.9999995, eps = 1.e-08;
+  const double zg3 = 3. * zg;
+  const double g1 = (8. - zw * (5. + zg3)) * 0.25;
+  const double g2 = 3. * (zw * (1. - zg)) * 0.25;
+  const double g3 = (2. - zg3 * prmuz) * 0.25;
+  const double g4 = 1. - g3;
+  double omg = 1. - zg;
+  double den2 = omg * omg - (1. - zw) * zg * zg;
+  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+  double g1b = 0., g2b = 0., g3b = 0., g4b = 0.;
+  taub = 0., wb = 0., gb = 0.;
+  if (cons) {
+    double za1 = g1 * prmuz - g3;
+    double zgt = g1 * zto1;
+    double xa = zto1 * rmu;
+    double x2 = dmin(xa, 500.);
+    double ze2 = dexp(-x2);
+    double inv = 1. / (1. + zgt);
+    double ze2b = e2b;
+    if (ze2 != 1.0) {
+      double Reff = rb - tb, Rdeff = rdb - tdb;   // t = 1 - r, td = 1 - rd
+      double Rn = zgt - za1 * (1. - ze2);
+      double Rnb = Reff * inv, invb = Reff * Rn;
+      double zgtb = Rdeff * inv;
+      invb += Rdeff * zgt;
+      zgtb += Rnb;
+      double za1b = -Rnb * (1. - ze2);
+      ze2b += Rnb * za1;
+      zgtb -= invb * inv * inv;
+      g1b += zgtb * zto1;
+      taub += zgtb * g1;
+      g1b += za1b * prmuz;
+      g3b -= za1b;
+    }
+    double x2b = -ze2b * ze2;
+    if (xa < 500.) taub += x2b * rmu;
+  } else {
+    const double a1 = g1 * g4 + g2 * g3, a2 = g1 * g3 + g2 * g4;
+    const double k = dsqrt(g1 * g1 - g2 * g2);
+    const double rp = k * prmuz, rp1 = 1. + rp, rm1 = 1. - rp, k2 = 2. * k, rpp = 1. - rp * rp;
+    const double rkg = k + g1;
+    const double r1 = rm1 * (a2 + k * g3), r2 = rp1 * (a2 - k * g3), r3 = k2 * (g3 - a2 * prmuz);
+    const double r4 = rpp * rkg, r5 = rpp * (k - g1);
+    const double t1 = rp1 * (a1 + k * g4), t2 = rm1 * (a1 - k * g4), t3 = k2 * (g4 + a1 * prmuz);
+    const double xk = k * zto1, xa = zto1 * rmu;
+    const double x1 = dmin(xk, 500.), x2 = dmin(xa, 500.);
+    const double em1 = dexp(-x1), em2 = dexp(-x2);
+    const double ep1 = 1. / em1, ep2 = 1. / em2;
+    const double denr = r4 * ep1 + r5 * em1;
+    const double emm = em1 * em1;
+    const double Dd = rkg - (g1 - k) * emm, dend = 1. / Dd;
+    double kb = 0., k2b = 0., em1b = 0., em2b = e2b, emmb = 0., dendb = 0.;
+    double ep1b = 0., ep2b = 0., rkgb = 0.;
+    double r1b = 0., r2b = 0., r3b = 0., r4b = 0., r5b = 0., t1b = 0., t2b = 0., t3b = 0.;
+    // td = k2 em1 dend ; rd = g2 (1 - emm) dend
+    k2b += tdb * em1 * dend;
+    em1b += tdb * k2 * dend;
+    dendb += tdb * k2 * em1;
+    g2b += rdb * (1. - emm) * dend;
+    emmb -= rdb * g2 * dend;
+    dendb += rdb * g2 * (1. - emm);
+    double Ddb = -dendb * dend * dend;
+    rkgb += Ddb;
+    g1b -= Ddb * emm;
+    kb += Ddb * emm;
+    emmb -= Ddb * (g1 - k);
+    em1b += emmb * 2. * em1;
+    double denrb = 0.;
+    if (denr >= -eps && denr <= eps) {
+      em2b += tb;   // r = eps, t = em2
+    } else {
+      const double rden = 1. / denr;
+      const double NR = r1 * ep1 - r2 * em1 - r3 * em2;
+      const double NT = t1 * ep1 - t2 * em1 - t3 * ep2;
+      em2b += tb * (1. - zw * NT * rden);
+      wb -= tb * em2 * NT * rden;
+      double NTb = -tb * em2 * zw * rden;
+      double rdenb = -tb * em2 * zw * NT;
+      wb += rb * NR * rden;
+      double NRb = rb * zw * rden;
+      rdenb += rb * zw * NR;
+      t1b += NTb * ep1, ep1b += NTb * t1, t2b -= NTb * em1, em1b -= NTb * t2, t3b -= NTb * ep2, ep2b -= NTb * t3;
+      r1b += NRb * ep1, ep1b += NRb * r1, r2b -= NRb * em1, em1b -= NRb * r2, r3b -= NRb * em2, em2b -= NRb * r3;
+      denrb = -rdenb * rden * rden;
+    }
+    r4b += denrb * ep1, ep1b += denrb * r4, r5b += denrb * em1, em1b += denrb * r5;
+    em1b -= ep1b * ep1 * ep1;
+    em2b -= ep2b * ep2 * ep2;
+    double x1b = -em1b * em1, x2b = -em2b * em2;
+    if (xk < 500.) kb += x1b * zto1, taub += x1b * k;
+    if (xa < 500.) taub += x2b * rmu;
+    double a1b = 0., a2b = 0., rpb = 0., rppb = 0., rm1b = 0., rp1b = 0.;
+    k2b += t3b * (g4 + a1 * prmuz), g4b += t3b * k2, a1b += t3b * k2 * prmuz;
+    rm1b += t2b * (a1 - k * g4), a1b += t2b * rm1, kb -= t2b * rm1 * g4, g4b -= t2b * rm1 * k;
+    rp1b += t1b * (a1 + k * g4), a1b += t1b * rp1, kb += t1b * rp1 * g4, g4b += t1b * rp1 * k;
+    rppb += r5b * (k - g1), kb += r5b * rpp, g1b -= r5b * rpp;
+    rppb += r4b * rkg, rkgb += r4b * rpp;
+    k2b += r3b * (g3 - a2 * prmuz), g3b += r3b * k2, a2b -= r3b * k2 * prmuz;
+    rp1b += r2b * (a2 - k * g3), a2b += r2b * rp1, kb -= r2b * rp1 * g3, g3b -= r2b * rp1 * k;
+    rm1b += r1b * (a2 + k * g3), a2b += r1b * rm1, kb += r1b * rm1 * g3, g3b += r1b * rm1 * k;
+    kb += rkgb, g1b += rkgb;
+    rpb -= rppb * 2. * rp;
+    kb += 2. * k2b;
+    rpb += rp1b - rm1b;
+    kb += rpb * prmuz;
+    if (k > 0.) g1b += kb * g1 / k, g2b -= kb * g2 / k;
+    g1b += a2b * g3 + a1b * g4, g3b += a2b * g1 + a1b * g2;
+    g2b += a2b * g4 + a1b * g3, g4b += a2b * g2 + a1b * g1;
+  }
+  g3b -= g4b;
+  double g3p_b = -g3b * prmuz * 0.25;      // d/d(zg3) through gamma3
+  wb += g2b * 3. * (1. - zg) * 0.25;
+  gb -= g2b * 3. * zw * 0.25;
+  wb -= g1b * (5. + zg3) * 0.25;
+  g3p_b -= g1b * zw * 0.25;
+  gb += 3. * g3p_b;
 }
 
 // Downward half of the adding method for one sky (vrtqdr_sw :144-159), written as forward
