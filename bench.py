@@ -7,13 +7,20 @@ overlap (icld = 2) with the KISS generator, synthetic inputs of rrtmg_sw_amd.wor
 device entry point (rrtmgsw_mcica_sw_dev, FP64). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
+    python bench.py --adjoint [--workload expran65536x137]
 
 Both precision forms are timed in the same run; `value` is the one --precision names (FP64 by
 default), and `forms` carries both, their speed ratio and the mixed form's largest error
-against the FP64 form (fluxes W/m2, heating rates K/day).
+against the FP64 form (fluxes W/m2, heating rates K/day). `adjoint` carries the adjoint's cost
+against the forward on the 4096 x 60 exponential-random-overlap + aerosol workload.
+
+--adjoint prints instead one line for the adjoint on a large workload (default 65536 x 137):
+columns/s of the forward and of forward + adjoint, with full storage of every layer's terms
+and with a checkpoint every 16 layers, and each mode's work-area footprint in HBM.
 """
 import argparse
 import ctypes as C
+import gc
 import json
 import os
 import sys
@@ -52,6 +59,80 @@ def cpu_baseline(cols_full, budget_s=12.0, chunk=16):
                       f"chunks of {chunk}), {t:.1f} s on one host core"}
 
 
+def _time(fn, steps, warmup):
+    """Per-call milliseconds of `fn` over `steps` calls after `warmup`, synchronised."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def adjoint_ratio(args, name="expran4096x60"):
+    """Adjoint (total-sky TOA upward flux + heating-rate seeds, all five inputs + albedos)
+    against the forward on the same columns."""
+    import torch
+    from rrtmg_sw_amd import api
+    from rrtmg_sw_amd.workloads import workload
+    cols = workload(name)
+    d = api.device_inputs(cols)
+    st = torch.ones(cols.ncol, dtype=torch.float64, device="cuda")
+    sh = torch.ones(cols.nlay, cols.ncol, dtype=torch.float64, device="cuda")
+    res = {"workload": f"{name}: icld=5 KISS, iaer=10, {cols.ncol} columns x {cols.nlay} layers"}
+    for p in ("fp64", "exact"):
+        out = api.mcica_rrtmg_sw(cols, d, precision=p)
+        res[f"forward_{p}_ms"] = _time(lambda: api.mcica_rrtmg_sw(cols, d, precision=p, out=out),
+                                       args.steps, args.warmup)
+    for ck in (0, 16):
+        g = api.mcica_rrtmg_sw_adjoint(cols, d, st, sh, ckpt=ck)
+        res[f"adjoint_ckpt{ck}_ms"] = _time(
+            lambda: api.mcica_rrtmg_sw_adjoint(cols, d, st, sh, out=g, ckpt=ck), args.steps,
+            args.warmup)
+    res["adjoint_over_forward_fp64"] = res["adjoint_ckpt0_ms"] / res["forward_fp64_ms"]
+    res["adjoint_over_forward_exact"] = res["adjoint_ckpt0_ms"] / res["forward_exact_ms"]
+    return res
+
+
+def bench_adjoint(args):
+    """Forward and forward + adjoint on a large workload, store vs checkpoint-16."""
+    import torch
+    from rrtmg_sw_amd import api, _lib
+    from rrtmg_sw_amd.workloads import workload
+    name = args.workload if args.workload != "mcica8192x72" else "expran65536x137"
+    cols = workload(name)
+    d = api.device_inputs(cols)
+    N, L = cols.ncol, cols.nlay
+    st = torch.ones(N, dtype=torch.float64, device="cuda")
+    sh = torch.ones(L, N, dtype=torch.float64, device="cuda")
+    steps, warmup = max(2, args.steps // 4), 1
+    out = api.mcica_rrtmg_sw(cols, d, precision="fp64")
+    fwd = _time(lambda: api.mcica_rrtmg_sw(cols, d, precision="fp64", out=out), steps, warmup)
+    line = {"metric": "adjoint columns/s (forward + adjoint), store vs checkpoint-16",
+            "unit": "columns/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "higher_is_better": True, "data": "synthetic", "dtype": "f64",
+            "config": {"workload": f"{name}: icld=5 KISS, iaer=10, {N} columns x {L} layers"},
+            "forward_fp64": {"ms": fwd, "columns_per_s": N / (fwd * 1e-3)}}
+    for ck in (0, 16):
+        g = api.mcica_rrtmg_sw_adjoint(cols, d, st, sh, ckpt=ck)
+        adj = _time(lambda: api.mcica_rrtmg_sw_adjoint(cols, d, st, sh, out=g, ckpt=ck), steps,
+                    warmup)
+        per_col = _lib.lib().rrtmgsw_adjoint_work_bytes(C.c_int(L), C.c_int(ck))
+        chunk = min(N, max(64, (4 << 30) // per_col))   # columns per launch (library's rule)
+        line[f"ckpt{ck}"] = {
+            "adjoint_ms": adj, "forward_plus_adjoint_columns_per_s": N / ((fwd + adj) * 1e-3),
+            "adjoint_over_forward": adj / fwd, "work_bytes_per_column": per_col,
+            "work_gib_if_all_columns_at_once": per_col * N / 2 ** 30,
+            "work_gib_allocated": per_col * chunk / 2 ** 30, "columns_per_launch": chunk}
+        del g
+        torch.cuda.empty_cache()
+    line["value"] = line["ckpt16"]["forward_plus_adjoint_columns_per_s"]
+    print(json.dumps(line), flush=True)
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -60,6 +141,7 @@ def main():
     p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     p.add_argument("--workload", default="mcica8192x72")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--adjoint", action="store_true")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one GPU; the path shards "
@@ -72,6 +154,9 @@ def main():
     from rrtmg_sw_amd.workloads import workload
 
     api.rrtmg_sw_ini(1004.64)
+    gc.disable()   # a collector pause inside a timed region would be counted as GPU time
+    if args.adjoint:
+        return bench_adjoint(args)
     cols = workload(args.workload)
     d = api.device_inputs(cols)
 
@@ -137,6 +222,7 @@ def main():
             "mixed_max_abs_error_vs_fp64": err,
         },
     }
+    line["adjoint"] = adjoint_ratio(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cols)
     print(json.dumps(line), flush=True)

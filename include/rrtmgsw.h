@@ -19,6 +19,8 @@
 #ifndef RRTMGSW_H
 #define RRTMGSW_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -166,6 +168,11 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
                                  const double *seed_hr, double *d_tlay, double *d_h2ovmr,
                                  double *d_o3vmr, double *d_clwp, double *d_ciwp, double *d_alb,
                                  int ckpt, void *stream);
+
+/* Bytes of device work area per column that rrtmgsw_mcica_sw_adjoint_dev uses for nlay layers
+ * and checkpoint interval ckpt (it processes columns in chunks of at most 4 GiB of work area).
+ * No counterpart in the reference. */
+size_t rrtmgsw_adjoint_work_bytes(int nlay, int ckpt);
 
 #ifdef __cplusplus
 }

@@ -54,13 +54,15 @@ def lib():
         [C.c_int] * 5 + [_ip] + [_dp] * 16 +
         [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int] + [_dp] * 10 +
         [_dp] * 4 + [_dp] * 3 + [_dp] * 2 + [_dp] * 6 + [C.c_int, _vp])
+    L.rrtmgsw_adjoint_work_bytes.argtypes = [C.c_int, C.c_int]
+    L.rrtmgsw_adjoint_work_bytes.restype = C.c_size_t
     _lib = L
     return L
 
 
 EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_sw",
             "rrtmgsw_sw_dev", "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
-            "rrtmgsw_get_alpha", "rrtmgsw_mcica_sw_dev", "rrtmgsw_mcica_sw_adjoint_dev"]
+            "rrtmgsw_get_alpha", "rrtmgsw_mcica_sw_dev", "rrtmgsw_mcica_sw_adjoint_dev", "rrtmgsw_adjoint_work_bytes"]
 
 
 def check(rc):

@@ -195,31 +195,61 @@ __device__ inline LayerV total_layer_exact(double taug, double taur, const doubl
 }
 
 // Adjoint of total_layer_exact: given the adjoints lb of (r, rd, t, td, dbt), returns in xb the
-// adjoints of (taug, taur, ct, cw, cg). reftra in reverse mode (reftra_exact_adj), the optics
-// combinations in forward-mode dual numbers over their five inputs.
+// adjoints of (taug, taur, ct, cw, cg): reftra in reverse mode (reftra_exact_adj), then the
+// reverse of the total-sky combination and of the clear-sky delta scaling, written out:
+//   clear: tauc0 = taur + taug + ta, s = taur + ta wa, g0 = ga wa ta / s, w0 = s / tauc0,
+//          f = g0^2, tauc = (1 - w0 f) tauc0, wc = w0 (1 - f) / (1 - w0 f), gc = g0 / (1 + g0);
+//   total: tauo = tauc + ct, so = tauc wc + ct cw, go = (ct cw cg + tauc wc gc) / so,
+//          wo = so / tauo.
 __device__ inline void total_layer_adj(double taug, double taur, double ta, double wa, double ga,
                                        double cf, double ct, double cw, double cg, double mu0,
                                        double rmu, const double *lb, double *xb) {
-  typedef Dual<5> D5;
-  // sky optics as functions of (taug, taur, ct, cw, cg)
-  SkyOpt<D5> c = clear_optics<D5>(dvar<5>(taug, 0), dvar<5>(taur, 1), ta, wa, ga);
-#pragma unroll
-  for (int k = 0; k < 5; k++) xb[k] = 0.0;
-  auto sky = [&](const SkyOpt<D5> &o, double wgt, bool total) {
-    // no cloudy reftra when the cloud fraction is tiny: r = 0, t = 1, only the direct beam counts
-    const bool rt = !(total && !(cf > REPCLC));
-    double mb[3];
-    reftra_exact_adj(o.tau.v, o.w.v, o.g.v, mu0, rmu, rt ? wgt * lb[0] : 0., rt ? wgt * lb[1] : 0.,
-                     rt ? wgt * lb[2] : 0., rt ? wgt * lb[3] : 0., wgt * lb[4], mb[0], mb[1], mb[2]);
-#pragma unroll
-    for (int k = 0; k < 5; k++) xb[k] += mb[0] * o.tau.d[k] + mb[1] * o.w.d[k] + mb[2] * o.g.d[k];
-  };
-  double zcl = 1.0 - cf;
-  if (zcl != 0.0) sky(c, zcl, false);
-  if (cf > 0.0) {
-    SkyOpt<D5> t = total_optics<D5>(c, dvar<5>(ct, 2), dvar<5>(cw, 3), dvar<5>(cg, 4));
-    sky(t, cf, true);
+  const double tauc0 = taur + taug + ta;
+  const double sc = taur * 1.0 + ta * wa;
+  const double g0 = (ga * wa * ta) / sc;
+  const double w0 = sc / tauc0;
+  const double f = g0 * g0;
+  const double D = 1.0 - w0 * f;
+  SkyOpt<double> c = clear_optics<double>(taug, taur, ta, wa, ga);
+  double tcb = 0., wcb = 0., gcb = 0.;     // adjoints of the clear optics (tauc, wc, gc)
+  double ctb = 0., cwb = 0., cgb = 0.;
+  const double zcl = 1.0 - cf;
+  if (zcl != 0.0) {
+    double tb, wb, gb;
+    reftra_exact_adj(c.tau, c.w, c.g, mu0, rmu, zcl * lb[0], zcl * lb[1], zcl * lb[2], zcl * lb[3],
+                     zcl * lb[4], tb, wb, gb);
+    tcb += tb, wcb += wb, gcb += gb;
   }
+  if (cf > 0.0) {
+    SkyOpt<double> t = total_optics<double>(c, ct, cw, cg);
+    // no cloudy reftra when the cloud fraction is tiny: r = 0, t = 1, only the direct beam counts
+    const bool rt = cf > REPCLC;
+    double tob, wob, gob;
+    reftra_exact_adj(t.tau, t.w, t.g, mu0, rmu, rt ? cf * lb[0] : 0., rt ? cf * lb[1] : 0.,
+                     rt ? cf * lb[2] : 0., rt ? cf * lb[3] : 0., cf * lb[4], tob, wob, gob);
+    const double so = c.tau * c.w + ct * cw;
+    const double sob0 = wob / t.tau;
+    const double taob = tob - wob * t.w / t.tau;
+    const double Nb = gob / so;
+    const double sob = sob0 - gob * t.g / so;
+    ctb += taob + sob * cw + Nb * cw * cg;
+    cwb += sob * ct + Nb * ct * cg;
+    cgb += Nb * ct * cw;
+    tcb += taob + sob * c.w + Nb * c.w * c.g;
+    wcb += sob * c.tau + Nb * c.tau * c.g;
+    gcb += Nb * c.tau * c.w;
+  }
+  // clear-sky delta scaling
+  double g0b = gcb / ((1.0 + g0) * (1.0 + g0));
+  double w0b = wcb * (1.0 - f) / (D * D) - tcb * f * tauc0;
+  double fb = -wcb * w0 * (1.0 - w0) / (D * D) - tcb * w0 * tauc0;
+  double tauc0b = tcb * D;
+  g0b += fb * 2.0 * g0;
+  double scb = w0b / tauc0 - g0b * g0 / sc;
+  tauc0b -= w0b * w0 / tauc0;
+  xb[0] = tauc0b;            // taug
+  xb[1] = tauc0b + scb;      // taur
+  xb[2] = ctb, xb[3] = cwb, xb[4] = cgb;
 }
 
 // Everything a layer of one sub-column needs besides the gas optics (aerosol and cloud optics),
@@ -382,12 +412,12 @@ __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
   return ((size_t)L + states) * NGPT;
 }
 
-__global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) adj_kernel(AdjArgs A, DeviceTables T) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const RtArgs &a = A.f;
   const int L = a.nlay, C = a.ncol;
   const int nmw = (L + 31) / 32;
-  ColLayout lo(L, nmw);
+  ColLayout lo(L, nmw, false);
   ColView v(smem, lo);
   double *acc = reinterpret_cast<double *>(smem + ((lo.total + 15) & ~15));   // (L, AC_N)
   double *wU = acc + (size_t)L * AC_N, *wD = wU + (L + 1);     // level weights, top-based
@@ -435,7 +465,9 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
                        a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
   __syncthreads();
 
-  const GPoint gp = T.gp[active ? g : 0];
+  // the lane's g-point record is read field by field where used (a register copy held across
+  // the layer loops would take ~40 VGPRs per lane)
+  const GPoint &gp = T.gp[active ? g : 0];
   const double mu0 = fmax(a.coszen[c], 1.e-10);
   const double rmu = 1.0 / mu0;
   const bool vis = gp.ibnd >= 9 && gp.ibnd <= 12;
@@ -616,7 +648,7 @@ __global__ void __launch_bounds__(128) adj_kernel(AdjArgs A, DeviceTables T) {
 }
 
 size_t adj_lds_bytes(int nlay) {
-  ColLayout lo(nlay, (nlay + 31) / 32);
+  ColLayout lo(nlay, (nlay + 31) / 32, false);
   return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4);
 }
 

@@ -797,6 +797,11 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
   return run_rt(a, t, s, precision);
 }
 
+size_t rrtmgsw_adjoint_work_bytes(int nlay, int ckpt) {
+  if (nlay < 1 || ckpt < 0) return 0;
+  return adj_work_bytes(nlay, ckpt);
+}
+
 int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
                                  const double *play, const double *plev, const double *tlay,
                                  const double *tlev, const double *tsfc, const double *h2ovmr,

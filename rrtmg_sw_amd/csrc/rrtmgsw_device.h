@@ -25,7 +25,8 @@ struct ColLayout {
   int L;
   int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
   int o_fac, o_col, o_red, o_reda, o_thin, o_mask, total;   // double blocks / mask words
-  __host__ __device__ ColLayout(int nlay, int nmaskw) {
+  // fluxes = false leaves out the per-level flux sums (red, reda) of the forward kernels
+  __host__ __device__ ColLayout(int nlay, int nmaskw, bool fluxes = true) {
     L = nlay;
     int ib = 0;
     o_jp = ib, ib += L;
@@ -37,8 +38,8 @@ struct ColLayout {
     int bytes = ((ib * 4 + 15) / 16) * 16;
     o_fac = bytes, bytes += 8 * L * 8;     // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac
     o_col = bytes, bytes += 6 * L * 8;     // h2o co2 o3 ch4 o2 colmol
-    o_red = bytes, bytes += 2 * (L + 1) * 4 * 8;
-    o_reda = bytes, bytes += 2 * L * 2 * 8;
+    o_red = bytes, bytes += fluxes ? 2 * (L + 1) * 4 * 8 : 0;
+    o_reda = bytes, bytes += fluxes ? 2 * L * 2 * 8 : 0;
     o_thin = bytes, bytes += ((L + 31) / 32) * 4;
     o_mask = bytes, bytes += nmaskw * NGPT * 4;
     total = bytes;
