@@ -207,7 +207,7 @@ def main():
                    "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": f"rt_kernel<{'false' if args.precision == 'fp64' else 'true'}>",
+                     "kernel": f"rt_kernel<{0 if args.precision == 'fp64' else 1}>",
                      "note": "FP64 VALU work (no dense contraction): algorithmic FLOPs of "
                              "rrtmg_sw_amd/flops.py over the per-step event time on the launch "
                              "stream; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s",
