@@ -32,6 +32,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the timed kernel on mcica8192x72, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over this script, each scaled by what the counter reads for a 2 GiB device copy
+# (x2.0 and x1.0); profiles/pmc_rt_kernel_adj_kernel_mcica8192x72_expran4096x60.md.
+MEASURED_TRAFFIC = {"fp64": {"read": 1.993e9, "write": 1.905e9},
+                    "mixed": {"read": 1.645e9, "write": 1.548e9}}
+
 
 def cpu_baseline(cols_full, budget_s=12.0, chunk=16):
     """Time the compiled reference (oracle/_ref) -- mcica_subcol_sw + rrtmg_sw -- on consecutive
@@ -206,7 +212,11 @@ def main():
                                f"{cols.ncol} columns x {cols.nlay} layers, fused device entry",
                    "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": (sum(MEASURED_TRAFFIC[args.precision].values())
+                                 if args.workload == "mcica8192x72" else None),
+                     "traffic_read_write": MEASURED_TRAFFIC[args.precision]
+                                           if args.workload == "mcica8192x72" else None,
                      "kernel": f"rt_kernel<{0 if args.precision == 'fp64' else 1}>",
                      "note": "FP64 VALU work (no dense contraction): algorithmic FLOPs of "
                              "rrtmg_sw_amd/flops.py over the per-step event time on the launch "
