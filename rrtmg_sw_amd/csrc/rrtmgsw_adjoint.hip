@@ -540,8 +540,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     adif_b -= den_b * st.prdnd;
     sb.prdnd -= den_b * albdif;
     // reduce over g: vis bands -> asdir / asdif, the others -> aldir / aldif
-    double s = wave_sum4(vis ? adir_b : 0.0, vis ? adif_b : 0.0, vis ? 0.0 : adir_b, vis ? 0.0 : adif_b, lane);
-    if ((lane & 15) == 0) atomicAdd(&albacc[lane >> 4], s);
+    double s = wave_sum4d(vis ? adir_b : 0.0, vis ? adif_b : 0.0, vis ? 0.0 : adir_b, vis ? 0.0 : adif_b, lane);
+    if (lane < 16 && (lane & 3) == 0) atomicAdd(&albacc[lane >> 2], s);
   }
 
   // ---- R1: bottom -> top ----
@@ -603,8 +603,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     }
 #pragma unroll
     for (int q4 = 0; q4 < AC_N / 4; q4++) {
-      double s = wave_sum4(la[4 * q4], la[4 * q4 + 1], la[4 * q4 + 2], la[4 * q4 + 3], lane);
-      if ((lane & 15) == 0) atomicAdd(&acc[(size_t)lay * AC_N + 4 * q4 + (lane >> 4)], s);
+      double s = wave_sum4d(la[4 * q4], la[4 * q4 + 1], la[4 * q4 + 2], la[4 * q4 + 3], lane);
+      if (lane < 16 && (lane & 3) == 0) atomicAdd(&acc[(size_t)lay * AC_N + 4 * q4 + (lane >> 2)], s);
     }
   }
 

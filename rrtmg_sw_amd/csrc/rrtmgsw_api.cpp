@@ -302,7 +302,7 @@ int finish(hipStream_t s, int *d_status) {
 
 // Columns per launch so that the per-column elimination coefficients stay within ~4 GiB.
 int chunk_columns(int ncol, int nlay) {
-  size_t per = (size_t)nlay * SCR_COEFS * NGPT * sizeof(double);
+  size_t per = scr_col_doubles(nlay) * sizeof(double);
   size_t cap = (size_t)4 << 30;
   size_t n = cap / per;
   if (n < 64) n = 64;
@@ -331,7 +331,7 @@ int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int pr
   void *scr = nullptr, *st = nullptr;
   {
     std::lock_guard<std::mutex> lk(ctx.mu);
-    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * base.nlay * SCR_COEFS * NGPT * sizeof(double), &scr)) return rc;
+    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * scr_col_doubles(base.nlay) * sizeof(double), &scr)) return rc;
     if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
   }
   hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);

@@ -898,6 +898,233 @@ __device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A
 }
 
 // ------------------------------------------------------------------------------------------
+// Pieces of the FP64 forward kernel shaped for gfx950 issue:
+//  * reciprocals from the hardware estimate refined by two Newton-Raphson steps (an IEEE
+//    division adds operand rescaling for denormal and overflowing operands, which the
+//    denominators here never are, and a final correction: 11 instructions instead of 5);
+//  * table reads at 32-bit offsets from a uniform base (one full-rate add per corner instead of
+//    a quarter-rate 32-bit multiply and 64-bit address arithmetic);
+//  * wavefront sums by DPP row operations, with one LDS-pipe and one VALU exchange across rows.
+// ------------------------------------------------------------------------------------------
+__device__ inline double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+template <typename R>
+__device__ inline R ldt(const R *__restrict__ base, uint32_t i) {
+  return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) +
+                                      (size_t)(i * (uint32_t)sizeof(R)));
+}
+
+// Exponential-table transmittance alone (tbl_exp with the index quotient from frcp).
+__device__ inline double tbl_exp1(double ze, const double *__restrict__ exp_tbl) {
+  double tblind = ze * frcp(BPADE + ze);
+  int itind = (int)(TBLINT * tblind + 0.5);
+  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
+  double te = ldt(exp_tbl, (uint32_t)itind);
+  return ze <= OD_LO ? 1. - ze + 0.5 * ze * ze : te;
+}
+
+// reftra_sw (PIFM, table exponentials) from e1 = exp(-k tau) and e2 = exp(-tau / mu0) alone.
+// The reference's general solution (src/rrtmg_sw_reftra.f90:233-322) divides numerators holding
+// 1/e1 and 1/e2 by zdenr = (1 - k^2 mu0^2)((k + g1)/e1 + (k - g1) e1); multiplied through by
+// e1 (and, in T's last term, e2/e2) every term is a product of e1, e2 and the coefficients:
+//   D  = (k + g1) - (g1 - k) e1^2,  rD = 1 / ((1 - k^2 mu0^2) D)
+//   R  = w [(1 - k mu0)(a2 + k g3) - (1 + k mu0)(a2 - k g3) e1^2 - 2k (g3 - a2 mu0) e1 e2] rD
+//   T  = e2 - w [e2 ((1 + k mu0)(a1 + k g4) - (1 - k mu0)(a1 - k g4) e1^2) - 2k (g4 + a1 mu0) e1] rD
+//   Rd = g2 (1 - e1^2) (1 - k^2 mu0^2) rD,  Td = 2k e1 (1 - k^2 mu0^2) rD,
+// with the reference's |zdenr| <= 1e-8 test as |(1 - k^2 mu0^2) D| <= 1e-8 e1 (R = 1e-8, T = e2,
+// and then rD = 1 / D). e2 = table(min(tau / mu0, 500)) is also the layer's direct-beam
+// transmittance: every table entry past index 9275 is the floor 1e-20, reached at tau/mu0 > 46.
+__device__ inline RT4 reftra_e(double zto1, double zw, double zg, double prmuz, double e2,
+                               const double *__restrict__ exp_tbl) {
+  const double zwcrit = 0.9999995, eps = 1.e-08;
+  RT4 o;
+  double zg3 = 3. * zg;
+  double g1 = (8. - zw * (5. + zg3)) * 0.25;
+  double g2 = 3. * (zw * (1. - zg)) * 0.25;
+  double g3 = (2. - zg3 * prmuz) * 0.25;
+  double g4 = 1. - g3;
+  double omg = 1. - zg;
+  double den2 = omg * omg - (1. - zw) * zg * zg;
+  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+  if (cons) {  // conservative scattering, :188-228
+    double za1 = g1 * prmuz - g3;
+    double zgt = g1 * zto1;
+    double inv = frcp(1. + zgt);
+    o.r = (zgt - za1 * (1. - e2)) * inv;
+    o.t = 1. - o.r;
+    o.rd = zgt * inv;
+    o.td = 1. - o.rd;
+    if (e2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
+  } else {
+    double a1 = g1 * g4 + g2 * g3;
+    double a2 = g1 * g3 + g2 * g4;
+    double k = sqrt(g1 * g1 - g2 * g2);
+    double kmu = k * prmuz;
+    double e1 = tbl_exp1(fmin(k * zto1, 500.), exp_tbl);
+    double ee = e1 * e1;
+    double D = (k + g1) - (g1 - k) * ee;
+    double P = 1. - kmu * kmu;
+    bool tiny = fabs(P * D) <= eps * e1;
+    P = tiny ? 1. : P;
+    double rD = frcp(P * D);
+    double NR = (1. - kmu) * (a2 + k * g3) - (1. + kmu) * (a2 - k * g3) * ee - 2. * k * (g3 - a2 * prmuz) * (e1 * e2);
+    double NT = e2 * ((1. + kmu) * (a1 + k * g4) - (1. - kmu) * (a1 - k * g4) * ee) - 2. * k * (g4 + a1 * prmuz) * e1;
+    o.r = tiny ? eps : zw * NR * rD;
+    o.t = tiny ? e2 : e2 - zw * NT * rD;
+    double PrD = P * rD;
+    o.rd = g2 * (1. - ee) * PrD;
+    o.td = 2. * k * e1 * PrD;
+  }
+  return o;
+}
+
+// sweep_layer with a reciprocal from frcp; A (the level's ztdn) and B (prdnd) are the state
+// before the layer.
+__device__ inline void sweep_r(Sweep &s, const RT4 &l, double dbt, double &Cc, double &E) {
+  double a = s.ztdn - s.tdbt;
+  double zreflect = frcp(1. - l.rd * s.prdnd);
+  Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
+  E = l.td * zreflect;
+  double ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;
+  double prdnd = l.rd + l.td * l.td * s.prdnd * zreflect;
+  s.tdbt = dbt * s.tdbt;
+  s.ztdn = ztdn;
+  s.prdnd = prdnd;
+}
+
+// Gas optics of one regime (lower: layers below laytrop, upper: the rest) with the lane's
+// constants resolved once per kernel from its g-point record: element offsets of the k-table
+// corners and of the continuum, Rayleigh and species rows (the same taumol template as
+// gas_optics, e.g. src/rrtmg_sw_taumol.f90:287-375 lower and :357-383 upper).
+struct GasLane {
+  uint32_t kb, kjs, kt, kp, ks;   // key table: base, species step, temperature, pressure, row
+  int sp1, sp2, spm;              // LDS column offsets of the key pair and the minor absorber
+  double strrat, kmult, mcoef, rayl;
+  bool bin, rayl_js, cont;        // binary key; Rayleigh interpolated in js; continuum present
+  uint32_t selfb, selfs, forb, fors, rab, ras;
+};
+// key_spec_t with the species ratio from a reciprocal (frcp or the FP32 estimate): js and fs
+// can differ from the divided ratio's only where specmult lies within an ulp of an integer,
+// where the interpolation is continuous across the bin edge.
+template <typename R>
+__device__ inline SpecT<R> key_spec_r(R c1, R c2, R strrat, R mult, bool bin) {
+  SpecT<R> s;
+  R speccomb = c1 + strrat * c2;
+  R specparm = c1 * (sizeof(R) == 8 ? R(frcp(double(speccomb))) : R(__builtin_amdgcn_rcpf(float(speccomb))));
+  specparm = specparm >= R(ONEMINUS) ? R(ONEMINUS) : specparm;
+  R specmult = mult * specparm;
+  s.speccomb = speccomb;
+  s.js = bin ? 1 + (int)specmult : 1;
+  s.fs = bin ? specmult - trunc(specmult) : R(0);
+  return s;
+}
+template <bool LOWER>
+__device__ inline GasLane gas_lane(const GPoint &g, int L) {
+  GasLane q;
+  const int st = LOWER ? g.sa : g.sb, nsp = LOWER ? g.lo_nsp : g.hi_nsp;
+  q.bin = (LOWER ? g.lo_mode : g.hi_mode) == KEY_BIN;
+  q.kb = LOWER ? g.absa : g.absb;
+  q.ks = st;
+  q.kjs = q.bin ? st : 0;
+  q.kt = nsp * st;
+  q.kp = 5 * nsp * st;
+  q.sp1 = (LOWER ? g.lo_s1 : g.hi_s1) * L;
+  q.sp2 = (LOWER ? g.lo_s2 : g.hi_s2) * L;
+  q.spm = (LOWER ? g.lo_minor : g.hi_minor) * L;
+  q.strrat = LOWER ? g.lo_strrat : g.hi_strrat;
+  q.kmult = LOWER ? g.lo_mult : g.hi_mult;
+  q.mcoef = LOWER ? g.lo_minor_coef : g.hi_minor_coef;
+  q.rayl = LOWER ? g.rayl_lo : g.rayl_hi;
+  q.rayl_js = LOWER && g.rayl_js;
+  q.cont = LOWER || g.hi_for;
+  q.selfb = g.selfref, q.selfs = g.sself, q.forb = g.forref, q.fors = g.sfor;
+  q.rab = g.rayla, q.ras = g.sra;
+  return q;
+}
+template <bool LOWER, typename R>
+__device__ inline void gas_optics_r(const GasLane &q, const R *__restrict__ tab, ColView &v, int lay,
+                                    R &taug, R &taur) {
+  const int jp = LOWER ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
+  const int jt = v.jt[lay], jt1 = v.jt1[lay];
+  const R fac00 = R(v.f(0, lay)), fac01 = R(v.f(1, lay)), fac10 = R(v.f(2, lay)), fac11 = R(v.f(3, lay));
+  const R colh2o = R(v.c(SP_H2O, lay)), colmol = R(v.c(5, lay));
+  const R c1 = R(v.col[q.sp1 + lay]), c2 = R(v.col[q.sp2 + lay]);
+  SpecT<R> s = key_spec_r<R>(c1, c2, R(q.strrat), R(LOWER ? 8 : 4), q.bin);
+  const R kmul = q.bin ? s.speccomb : c1 * R(q.kmult);
+  const uint32_t jpr = LOWER ? jp - 1 : jp - 13;
+  const uint32_t o0 = q.kb + __umul24(jpr, q.kp) + __umul24(jt - 1, q.kt) + __umul24(s.js - 1, q.ks);
+  const uint32_t o1 = q.kb + __umul24(jpr + 1, q.kp) + __umul24(jt1 - 1, q.kt) + __umul24(s.js - 1, q.ks);
+  const R fs = s.fs, f0 = R(1) - fs;
+  const R a0 = ldt(tab, o0), a1 = ldt(tab, o0 + q.kjs), a2 = ldt(tab, o0 + q.kt), a3 = ldt(tab, o0 + q.kt + q.kjs);
+  const R b0 = ldt(tab, o1), b1 = ldt(tab, o1 + q.kjs), b2 = ldt(tab, o1 + q.kt), b3 = ldt(tab, o1 + q.kt + q.kjs);
+  const R k = f0 * fac00 * a0 + fs * fac00 * a1 + f0 * fac10 * a2 + fs * fac10 * a3 +
+              f0 * fac01 * b0 + fs * fac01 * b1 + f0 * fac11 * b2 + fs * fac11 * b3;
+  const int jf = LOWER ? min(v.indf[lay], 2) : min(max(v.indf[lay], 1), 3);
+  const R fo0 = ldt(tab, q.forb + __umul24(jf - 1, q.fors)), fo1 = ldt(tab, q.forb + __umul24(jf, q.fors));
+  R cont = R(v.f(6, lay)) * (fo0 + R(v.f(7, lay)) * (fo1 - fo0));
+  if (LOWER) {
+    const int is = max(v.inds[lay], 1);
+    const R s0 = ldt(tab, q.selfb + __umul24(is - 1, q.selfs)), s1 = ldt(tab, q.selfb + __umul24(is, q.selfs));
+    cont = R(v.f(4, lay)) * (s0 + R(v.f(5, lay)) * (s1 - s0)) + cont;
+  }
+  cont = q.cont ? cont : R(0);
+  taug = kmul * k + colh2o * cont + R(v.col[q.spm + lay]) * R(q.mcoef);
+  R ray = R(q.rayl);
+  if (LOWER) {
+    const R r0 = ldt(tab, q.rab + __umul24(s.js - 1, q.ras)), r1 = ldt(tab, q.rab + __umul24(s.js, q.ras));
+    ray = q.rayl_js ? r0 + fs * (r1 - r0) : ray;
+  }
+  taur = colmol * ray;
+}
+
+// Wavefront sums by DPP. Four values: two transposing exchanges inside each row of 16 lanes
+// (mirror: lanes i and 15-i, half mirror: i and 7-i) leave lane i with quantity (i >> 2) & 3
+// summed over 4 lanes of its row, two quad exchanges complete the row sums, and the rows are
+// combined by a swizzle (rows 0-1, 2-3) and a swap of the wave halves. Lanes 0, 4, 8, 12 end
+// with the totals of v0, v1, v2, v3 (every lane holds the total of its quantity).
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, false);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141;
+__device__ inline double cross_rows(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  x += __hiloint2double(__builtin_amdgcn_ds_swizzle(hi, 0x401F), __builtin_amdgcn_ds_swizzle(lo, 0x401F));
+  lo = __double2loint(x), hi = __double2hiint(x);
+  auto l2 = __builtin_amdgcn_permlane32_swap((unsigned)lo, (unsigned)lo, false, false);
+  auto h2 = __builtin_amdgcn_permlane32_swap((unsigned)hi, (unsigned)hi, false, false);
+  return __hiloint2double((int)h2[0], (int)l2[0]) + __hiloint2double((int)h2[1], (int)l2[1]);
+}
+__device__ inline double wave_sum4d(double v0, double v1, double v2, double v3, int lane) {
+  const bool b3 = lane & 8, b2 = lane & 4;
+  double ka = b3 ? v2 : v0, kb = b3 ? v3 : v1;
+  double sa = b3 ? v0 : v2, sb = b3 ? v1 : v3;
+  double w0 = ka + dpp_d<DPP_MIRROR>(sa);
+  double w1 = kb + dpp_d<DPP_MIRROR>(sb);
+  double x = (b2 ? w1 : w0) + dpp_d<DPP_HALF_MIRROR>(b2 ? w0 : w1);
+  x += dpp_d<DPP_XOR1>(x);
+  x += dpp_d<DPP_XOR2>(x);
+  return cross_rows(x);
+}
+// Two values: lanes 0 and 8 end with the totals of v0 and v1.
+__device__ inline double wave_sum2d(double v0, double v1, int lane) {
+  const bool b3 = lane & 8;
+  double x = (b3 ? v1 : v0) + dpp_d<DPP_MIRROR>(b3 ? v0 : v1);
+  x += dpp_d<DPP_HALF_MIRROR>(x);
+  x += dpp_d<DPP_XOR1>(x);
+  x += dpp_d<DPP_XOR2>(x);
+  return cross_rows(x);
+}
+
+// ------------------------------------------------------------------------------------------
 // FP32 pieces of the mixed form. The layer optics carry the co-albedo c = 1 - w explicitly
 // (from the absorbing optical depths, so it keeps full relative precision when w is close to
 // one), which removes the cancellations of the two-stream expressions in FP32:
@@ -1014,37 +1241,6 @@ __device__ inline void sweep_layer_f(SweepF &s, const RT4f &l, float dbt, float 
   s.tdbt = dbt * s.tdbt;
   s.ztdn = ztdn;
   s.prdnd = prdnd;
-}
-
-// Sum four values over the 64 lanes of a wavefront with 7 exchanges: after the first two
-// exchanges each lane carries one of the four partial sums; lanes 0, 16, 32, 48 end with
-// the totals of values 0, 1, 2, 3.
-__device__ inline double wave_sum4(double v0, double v1, double v2, double v3, int lane) {
-  bool b5 = lane & 32, b4 = lane & 16;
-  double sa = b5 ? v0 : v2, sb = b5 ? v1 : v3;
-  double ka = b5 ? v2 : v0, kb = b5 ? v3 : v1;
-  double w0 = ka + __shfl_xor(sa, 32, 64);
-  double w1 = kb + __shfl_xor(sb, 32, 64);
-  double snd = b4 ? w0 : w1, kp = b4 ? w1 : w0;
-  double x = kp + __shfl_xor(snd, 16, 64);
-  x += __shfl_xor(x, 8, 64);
-  x += __shfl_xor(x, 4, 64);
-  x += __shfl_xor(x, 2, 64);
-  x += __shfl_xor(x, 1, 64);
-  return x;
-}
-
-// Sum two values over the 64 lanes of a wavefront with 6 exchanges; lane 0 ends with the total
-// of v0, lane 32 with the total of v1.
-__device__ inline double wave_sum2(double v0, double v1, int lane) {
-  bool b5 = lane & 32;
-  double x = (b5 ? v1 : v0) + __shfl_xor(b5 ? v0 : v1, 32, 64);
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 8, 64);
-  x += __shfl_xor(x, 4, 64);
-  x += __shfl_xor(x, 2, 64);
-  x += __shfl_xor(x, 1, 64);
-  return x;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -12,6 +12,10 @@ constexpr int WAVE_G = 56;     // g-points per wavefront: wave 0 = bands 16-22, 
 constexpr int MAX_LAY = 256;   // layer limit of the per-column LDS staging
 constexpr int NTBL = 10000;    // exp lookup table, modules/rrsw_tbl.f90:28
 constexpr int SCR_COEFS = 6;   // elimination coefficients kept per (layer, g-point): B C E for two skies
+// Per-column scratch of the forward kernels, in doubles: the elimination coefficients
+// (nlay, SCR_COEFS, NGPT), then the cloud optics (tau, ssa, asymmetry) of each (layer, band) of
+// the fused entry (nlay, NBND, 3).
+constexpr size_t scr_col_doubles(int nlay) { return (size_t)nlay * (SCR_COEFS * NGPT + 3 * NBND); }
 constexpr double PADE = 0.278;
 constexpr double OD_LO = 0.06;
 

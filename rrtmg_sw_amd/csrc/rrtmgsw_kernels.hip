@@ -18,6 +18,8 @@
 //             weighted by the g-point's incident flux and summed over g with wavefront
 //             reductions (spcvmc_sw :607-622)
 //   epilogue  broadband fluxes and heating rates (src/rrtmg_sw_rad.f90:805-836)
+#include <type_traits>
+
 #include "rrtmgsw_device.h"
 
 namespace rrtmgsw {
@@ -36,14 +38,47 @@ __device__ inline RT4 reftra_m(double tau, double w, double g, double mu0, doubl
   return reftra1(tau, w, g, mu0, rmu, T.exp_tbl, T.rexp_tbl);
 }
 
+// Broadband fluxes at the nlay+1 levels and heating rates (src/rrtmg_sw_rad.f90:805-836,
+// top layer 0) from the per-level sums over g of both waves in LDS.
+__device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
+  const int L = a.nlay, C = a.ncol, tid = threadIdx.x;
+  // level j (top-based): up = red[.][0], down = red[.][1] (+ reda total), clear in 2, 3 (+ reda clear)
+  auto level = [&](int j, double &fu, double &fd, double &cu, double &cd) {
+    const double *r0 = v.red + (0 * (L + 1) + j) * 4, *r1 = v.red + (1 * (L + 1) + j) * 4;
+    fu = r0[0] + r1[0], fd = r0[1] + r1[1], cu = r0[2] + r1[2], cd = r0[3] + r1[3];
+    if (j < L) {
+      fd += v.reda[(0 * L + j) * 2 + 0] + v.reda[(1 * L + j) * 2 + 0];
+      cd += v.reda[(0 * L + j) * 2 + 1] + v.reda[(1 * L + j) * 2 + 1];
+    }
+  };
+  for (int j = tid; j <= L; j += blockDim.x) {
+    double fu, fd, cu, cd;
+    level(j, fu, fd, cu, cd);
+    size_t q = c + (size_t)C * (L - j);
+    a.swuflx[q] = fu, a.swdflx[q] = fd, a.swuflxc[q] = cu, a.swdflxc[q] = cd;
+  }
+  for (int i = tid; i < L; i += blockDim.x) {
+    // layer i (bottom-based) lies between output levels i and i+1 = top-based j and j-1
+    int j = L - i;
+    double fu0, fd0, cu0, cd0, fu1, fd1, cu1, cd1;
+    level(j, fu0, fd0, cu0, cd0);
+    level(j - 1, fu1, fd1, cu1, cd1);
+    double pdp = a.plev[c + (size_t)C * i] - a.plev[c + (size_t)C * (i + 1)];
+    double zdpgcp = a.sc.heatfac / pdp;
+    size_t q = c + (size_t)C * i;
+    a.swhr[q] = i == L - 1 ? 0.0 : ((fd1 - fu1) - (fd0 - fu0)) * zdpgcp;
+    a.swhrc[q] = i == L - 1 ? 0.0 : ((cd1 - cu1) - (cd0 - cu0)) * zdpgcp;
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // The radiative-transfer kernel.
 // ------------------------------------------------------------------------------------------
-// MODE 0: FP64 with the reference's exponential table; 1: mixed FP32/FP64; 2: FP64 with exact
-// exponentials (the smooth forward the adjoint differentiates).
+// The mixed FP32/FP64 form (MODE 1). Its FP64 branches are those of the earlier all-FP64
+// kernel; the FP64 forms run in rt_kernel below.
 template <int MODE>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
-rt_kernel(RtArgs a, DeviceTables T) {
+rt_kernel_m(RtArgs a, DeviceTables T) {
   constexpr bool MIXED = MODE == 1;
   constexpr bool EXACT = MODE == 2;
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -107,7 +142,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
   const double rmu = 1.0 / mu0;
   Sweep sc_ = {1.0, 1.0, 0.0}, st_ = {1.0, 1.0, 0.0};
   int jfirst = L;                   // first layer (from the top) whose total sky differs
-  double *scr = a.scratch + (size_t)cl * L * SCR_COEFS * NGPT;
+  double *scr = a.scratch + (size_t)cl * scr_col_doubles(L);
   for (int j = 0; j < L; j++) {
     const int lay = L - 1 - j;
     const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);   // uniform over the workgroup
@@ -308,8 +343,8 @@ rt_kernel(RtArgs a, DeviceTables T) {
       if (active) pf[0] = Bc, pf[NGPT] = Cc, pf[2 * NGPT] = Ec;
     }
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
-    double sa = wave_sum2(zinc * At, zinc * Ac, lane);
-    if ((lane & 31) == 0) v.reda[(wave * L + j) * 2 + (lane >> 5)] = sa;
+    double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
+    if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
   }
   if (jfirst == L) st_ = sc_;
   if (err) atomicCAS(a.status, 0, err);
@@ -321,8 +356,8 @@ rt_kernel(RtArgs a, DeviceTables T) {
   double ut = (albdir * st_.tdbt + albdif * (st_.ztdn - st_.tdbt)) / (1. - albdif * st_.prdnd);
   double dt = st_.ztdn + st_.prdnd * ut;
   {
-    double s = wave_sum4(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
-    if ((lane & 15) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 4)] = s;
+    double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
+    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 2)] = s;
   }
   for (int j = L - 1; j >= 0; j--) {
     double bu_c = 0.0, bu_t = 0.0;
@@ -357,56 +392,291 @@ rt_kernel(RtArgs a, DeviceTables T) {
       }
     }
     // downward flux = sum(zinc ztdn) (pass 1) + sum(zinc prdnd U)
-    double s = wave_sum4(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
-    if ((lane & 15) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 4)] = s;
+    double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
+    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 2)] = s;
   }
   __syncthreads();
+  rt_epilogue(a, v, c);
+}
 
-  // ---- epilogue: broadband fluxes and heating rates (src/rrtmg_sw_rad.f90:805-836) ----
-  // level j (top-based): up = red[.][0], down = red[.][1] (+ reda total), clear in 2, 3 (+ reda clear)
-  auto level = [&](int j, double &fu, double &fd, double &cu, double &cd) {
-    const double *r0 = v.red + (0 * (L + 1) + j) * 4, *r1 = v.red + (1 * (L + 1) + j) * 4;
-    fu = r0[0] + r1[0], fd = r0[1] + r1[1], cu = r0[2] + r1[2], cd = r0[3] + r1[3];
-    if (j < L) {
-      fd += v.reda[(0 * L + j) * 2 + 0] + v.reda[(1 * L + j) * 2 + 0];
-      cd += v.reda[(0 * L + j) * 2 + 1] + v.reda[(1 * L + j) * 2 + 1];
+// ------------------------------------------------------------------------------------------
+// FP64 forms: MODE 0 = the reference's exponential table, MODE 2 = exact exponentials (the
+// smooth forward the adjoint differentiates). FUSED: McICA masks generated here from the
+// physical cloud inputs; otherwise the materialised (112, ncol, nlay) sub-column arrays are read.
+//
+// Pass 1 runs the layers above laytrop and those below it as two loops, each with only its
+// regime's gas-optics constants (GasLane) live. Per layer and g: gas optics, aerosol, cloud
+// optics, delta-scaled clear optics (two reciprocals instead of four divisions), reftra from the
+// two transmittances alone (the direct-beam one shared), the elimination step, and the flux sum
+// over g by DPP.
+// Pass 2 requests the coefficients of layer j-1 while layer j is processed.
+// ------------------------------------------------------------------------------------------
+template <int MODE, bool FUSED>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
+rt_kernel(RtArgs a, DeviceTables T) {
+  constexpr bool EXACT = MODE == 2;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int L = a.nlay, C = a.ncol;
+  ColLayout lo(L, FUSED ? (L + 31) / 32 : 0);
+  ColView v(smem, lo);
+  // XCD-aware column order: consecutive columns run on the same XCD (its own L2), so the
+  // strided (ncol, nlay) reads and writes of neighbouring columns share cache lines.
+  const int per = (a.ncol_chunk + 7) / 8;
+  const int cl = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (cl >= a.ncol_chunk) return;
+  const int c = a.col0 + cl;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = wave * WAVE_G + lane;
+  const bool active = lane < WAVE_G;
+  const int gi = active ? g : 0;
+
+  // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
+  if (tid == 0) v.misc[M_LAYTROP] = 0;
+  __syncthreads();
+  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, c, lay, v, &v.misc[M_LAYTROP]);
+  __syncthreads();
+  const int laytrop = v.misc[M_LAYTROP];
+  if (tid < NBND) {
+    const int first[NBND] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100};
+    v.misc[M_LAYSOL + tid] = find_laysolfr(T.gp[first[tid]], v, laytrop, L);
+  }
+  int err = 0;
+  if (FUSED && a.sc.icld >= 1 && active)
+    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
+                       a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
+  __syncthreads();
+  // Fused entry: every cloudy sub-column cell of a band sees the same cloud inputs (the layer's
+  // water paths and radii, the band's tauc/ssac/asmc/fsfc), so cldprmc_sw's result depends on
+  // (layer, band) only. It is evaluated here for each (layer, band) with at least one cloudy
+  // sub-column (the cells cldprmc_sw visits, so its STOP conditions fire as in the reference).
+  double *csc = a.scratch + (size_t)cl * scr_col_doubles(L) + (size_t)L * SCR_COEFS * NGPT;
+  if (FUSED && a.sc.icld >= 1) {
+    const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};
+    for (int idx = tid; idx < L * NBND; idx += blockDim.x) {
+      const int lay = idx / NBND, b = idx - lay * NBND;
+      uint32_t any = 0;
+      for (int gg = first[b]; gg < first[b + 1]; gg++) any |= v.mask[(lay >> 5) * NGPT + gg];
+      if ((any >> (lay & 31)) & 1u) {
+        const size_t q1 = c + (size_t)C * lay, qb = b + NBND * q1;
+        CloudOpt co = cloud_optics(T, a.sc, b + 16, 1.0, a.ciwp[q1], a.clwp[q1], a.rei[q1], a.rel[q1],
+                                   a.tauc[qb], a.ssac[qb], a.asmc[qb], a.fsfc[qb], err);
+        double *o = csc + (size_t)idx * 3;
+        o[0] = co.tau, o[1] = co.ssa, o[2] = co.asm_;
+      }
     }
+    __syncthreads();
+  }
+
+  const GPoint &gp = T.gp[gi];
+  const int band = gp.band, ibnd = gp.ibnd;
+  const double mu0 = fmax(a.coszen[c], 1.e-10);
+  const double zinc = active ? incident_flux(gp, T.tab, v, a.sc, mu0) : 0.0;
+  const double rmu = 1.0 / mu0;
+
+  // ---- pass 1: top to bottom ----
+  // Per layer and sky the lane stores B = prdnd, C and E of the elimination (slots 0-2 clear,
+  // 3-5 total sky); the A = ztdn term of the downward flux is summed over g here. A sub-column
+  // stays identical to the clear sky until its first cloudy layer (jfirst): above it the
+  // total-sky sweep is not computed and nothing is stored for it.
+  Sweep sc_ = {1.0, 1.0, 0.0}, st_ = {1.0, 1.0, 0.0};
+  int jfirst = L;
+  double *scr = a.scratch + (size_t)cl * scr_col_doubles(L) + gi;
+  auto layer = [&](auto lower_tag, const GasLane &q, int j) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
+    const int lay = L - 1 - j;
+    double taug, taur;
+    gas_optics_r<LOWER, double>(q, T.tab, v, lay, taug, taur);
+    // aerosol (src/rrtmg_sw_rad.f90:709-765)
+    double ta = 0.0, wa = 1.0, ga = 0.0;
+    if (a.sc.iaer == 10) {
+      size_t qa = c + (size_t)C * (lay + (size_t)L * ibnd);
+      ta = a.tauaer[qa], wa = a.ssaaer[qa], ga = a.asmaer[qa];
+    } else if (a.sc.iaer == 6) {
+      double t6 = 0.0, w6 = 0.0, g6 = 0.0;
+      for (int ia = 0; ia < NAER; ia++) {
+        double ec = a.ecaer[c + (size_t)C * (lay + (size_t)L * ia)];
+        double rt = T.tab[T.rsrtaua + ibnd + NBND * ia], rp = T.tab[T.rsrpiza + ibnd + NBND * ia];
+        double rg = T.tab[T.rsrasya + ibnd + NBND * ia];
+        t6 = t6 + rt * ec;
+        w6 = w6 + rt * ec * rp;
+        g6 = g6 + rt * ec * rp * rg;
+      }
+      if (t6 == 0.) {
+        ta = 0., ga = 0., wa = 1.;
+      } else {
+        ga = w6 != 0. ? g6 / w6 : g6;
+        wa = w6 / t6;
+        ta = t6;
+      }
+    }
+    // cloud optics of this sub-column (cldprmc_sw) and total-sky inputs (:500-505)
+    double cf = 0.0, ct = 0.0, cw = 1.0, cg = 0.0;
+    if (FUSED && a.sc.icld >= 1 && active) {
+      if ((v.mask[(lay >> 5) * NGPT + g] >> (lay & 31)) & 1u) {
+        const double *o = csc + ((size_t)lay * NBND + ibnd) * 3;
+        cf = 1.0, ct = o[0], cw = o[1], cg = o[2];
+      }
+    } else if (!FUSED && a.sc.icld >= 1 && active) {
+      const size_t q1 = c + (size_t)C * lay, qg = (size_t)g + NGPT * q1;
+      cf = a.cldfmcl[qg];
+      const double ciwp = a.ciwpmcl[qg], clwp = a.clwpmcl[qg];
+      const double tin = a.taucmcl[qg], win = a.ssacmcl[qg], gin = a.asmcmcl[qg], fin = a.fsfcmcl[qg];
+      if (cf >= CLDMIN) {
+        CloudOpt co = cloud_optics(T, a.sc, band, cf, ciwp, clwp, a.reicmcl[q1], a.relqmcl[q1], tin, win,
+                                   gin, fin, err);
+        ct = co.tau, cw = co.ssa, cg = co.asm_;
+      } else {
+        ct = tin, cw = win, cg = gin;
+      }
+    }
+    // clear-sky optics and delta scaling (src/rrtmg_sw_spcvmc.f90:442-496):
+    // g0 = ga wa ta / s and w0 = s / tau0 share 1 / (s tau0); omc = (w0 - w0 f)/(1 - w0 f) and
+    // gcc = (g0 - f)/(1 - f) = g0 / (1 + g0) share 1 / ((1 - w0 f)(1 + g0)), f = g0^2.
+    const double tau0 = taur + taug + ta;
+    const double sct = taur * 1.0 + ta * wa;
+    const double r1 = frcp(sct * tau0);
+    const double g0 = ga * wa * ta * tau0 * r1;
+    const double w0 = sct * sct * r1;
+    const double zwf = w0 * (g0 * g0);
+    const double r2 = frcp((1.0 - zwf) * (1.0 + g0));
+    const double tauc = (1.0 - zwf) * tau0;
+    const double omc = (w0 - zwf) * (1.0 + g0) * r2;
+    const double gcc = g0 * (1.0 - zwf) * r2;
+    RT4 rc;
+    double dbc;
+    if (EXACT) {
+      RTt<double> o = reftra_exact<double>(tauc, omc, gcc, mu0, rmu);
+      rc = {o.r, o.rd, o.t, o.td};
+      dbc = o.e2;
+    } else {
+      const double e2 = tbl_exp1(fmin(tauc * rmu, 500.), T.exp_tbl);
+      rc = reftra_e(tauc, omc, gcc, mu0, e2, T.exp_tbl);
+      dbc = e2;
+    }
+    const double Ac = sc_.ztdn, Bc = sc_.prdnd;
+    if (cf > 0.0 && jfirst == L) {
+      jfirst = j;
+      st_ = sc_;
+    }
+    double Cc, Ec;
+    sweep_r(sc_, rc, dbc, Cc, Ec);
+    double At = Ac, Bt = Bc, Ct = Cc, Et = Ec;
+    if (jfirst <= j) {
+      At = st_.ztdn, Bt = st_.prdnd;
+      RT4 rt = rc;
+      double dbt = dbc;
+      if (cf > 0.0) {
+        // total-sky optics (:500-505): go = (ct cw cg + tauc omc gcc) / so, omo = so / tauo
+        const double tauo = tauc + ct;
+        const double so = tauc * omc + ct * cw;
+        const double r3 = frcp(so * tauo);
+        const double go = (ct * cw * cg + tauc * omc * gcc) * tauo * r3;
+        const double omo = so * so * r3;
+        RT4 ro;
+        double dbo;
+        if (EXACT) {
+          RTt<double> o = reftra_exact<double>(tauo, omo, go, mu0, rmu);
+          ro = {o.r, o.rd, o.t, o.td};
+          dbo = o.e2;
+        } else {
+          const double e2 = tbl_exp1(fmin(tauo * rmu, 500.), T.exp_tbl);
+          ro = reftra_e(tauo, omo, go, mu0, e2, T.exp_tbl);
+          dbo = e2;
+        }
+        if (!(cf > REPCLC)) ro = {0.0, 0.0, 1.0, 1.0};
+        const double zcl = 1.0 - cf;
+        rt.r = zcl * rc.r + cf * ro.r;
+        rt.rd = zcl * rc.rd + cf * ro.rd;
+        rt.t = zcl * rc.t + cf * ro.t;
+        rt.td = zcl * rc.td + cf * ro.td;
+        dbt = zcl * dbc + cf * dbo;
+      }
+      sweep_r(st_, rt, dbt, Ct, Et);
+    }
+    if (active) {
+      double *p = scr + (size_t)j * SCR_COEFS * NGPT;
+      p[0] = Bc, p[NGPT] = Cc, p[2 * NGPT] = Ec;
+      if (jfirst <= j) p[3 * NGPT] = Bt, p[4 * NGPT] = Ct, p[5 * NGPT] = Et;
+    }
+    // sum over g of the incident-flux-weighted ztdn of level j, both skies
+    const double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
+    if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
   };
-  for (int j = tid; j <= L; j += blockDim.x) {
-    double fu, fd, cu, cd;
-    level(j, fu, fd, cu, cd);
-    size_t q = c + (size_t)C * (L - j);
-    a.swuflx[q] = fu, a.swdflx[q] = fd, a.swuflxc[q] = cu, a.swdflxc[q] = cd;
+  const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
+  {
+    const GasLane q = gas_lane<false>(gp, L);
+    for (int j = 0; j < jsplit; j++) layer(std::integral_constant<bool, false>(), q, j);
   }
-  for (int i = tid; i < L; i += blockDim.x) {
-    // layer i (bottom-based) lies between output levels i and i+1 = top-based j and j-1
-    int j = L - i;
-    double fu0, fd0, cu0, cd0, fu1, fd1, cu1, cd1;
-    level(j, fu0, fd0, cu0, cd0);
-    level(j - 1, fu1, fd1, cu1, cd1);
-    double pdp = a.plev[c + (size_t)C * i] - a.plev[c + (size_t)C * (i + 1)];
-    double zdpgcp = a.sc.heatfac / pdp;
-    size_t q = c + (size_t)C * i;
-    a.swhr[q] = i == L - 1 ? 0.0 : ((fd1 - fu1) - (fd0 - fu0)) * zdpgcp;
-    a.swhrc[q] = i == L - 1 ? 0.0 : ((cd1 - cu1) - (cd0 - cu0)) * zdpgcp;
+  {
+    const GasLane q = gas_lane<true>(gp, L);
+    for (int j = jsplit; j < L; j++) layer(std::integral_constant<bool, true>(), q, j);
   }
+  if (jfirst == L) st_ = sc_;
+  if (err) atomicCAS(a.status, 0, err);
+
+  // ---- pass 2: bottom to top ----
+  // surface (src/rrtmg_sw_vrtqdr.f90:163-169 with prup = albdir, prupd = albdif)
+  const bool vis = ibnd >= 9 && ibnd <= 12;
+  const double albdir = vis ? a.asdir[c] : a.aldir[c];
+  const double albdif = vis ? a.asdif[c] : a.aldif[c];
+  double uc = (albdir * sc_.tdbt + albdif * (sc_.ztdn - sc_.tdbt)) / (1. - albdif * sc_.prdnd);
+  double dc = sc_.ztdn + sc_.prdnd * uc;
+  double ut = (albdir * st_.tdbt + albdif * (st_.ztdn - st_.tdbt)) / (1. - albdif * st_.prdnd);
+  double dt = st_.ztdn + st_.prdnd * ut;
+  {
+    const double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
+    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 2)] = s;
+  }
+  // Coefficients of layer j - 1 are requested before layer j is processed; the total-sky slots
+  // only by lanes whose sub-column is cloudy at or above that layer (the others were never
+  // written).
+  const double *p = scr + (size_t)(L - 1) * SCR_COEFS * NGPT;
+  double cB = p[0], cC = p[NGPT], cE = p[2 * NGPT], tB = 0.0, tC = 0.0, tE = 0.0;
+  if (L - 1 >= jfirst) tB = p[3 * NGPT], tC = p[4 * NGPT], tE = p[5 * NGPT];
+  for (int j = L - 1; j >= 0; j--) {
+    const int jn = j > 0 ? j - 1 : 0;
+    const double *pn = scr + (size_t)jn * SCR_COEFS * NGPT;
+    const double nB = pn[0], nC = pn[NGPT], nE = pn[2 * NGPT];
+    double mB = 0.0, mC = 0.0, mE = 0.0;
+    if (jn >= jfirst) mB = pn[3 * NGPT], mC = pn[4 * NGPT], mE = pn[5 * NGPT];
+    const bool tot = j >= jfirst;
+    const double ut_t = tC + tE * ut, ut_c = cC + cE * ut;
+    ut = tot ? ut_t : ut_c;
+    const double bu_t = (tot ? tB : cB) * ut;
+    uc = cC + cE * uc;
+    const double bu_c = cB * uc;
+    // downward flux = sum(zinc ztdn) (pass 1) + sum(zinc prdnd U)
+    const double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
+    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 2)] = s;
+    cB = nB, cC = nC, cE = nE, tB = mB, tC = mC, tE = mE;
+  }
+  __syncthreads();
+  rt_epilogue(a, v, c);
 }
 
 template <int MODE>
-static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
+static hipError_t launch_rt_m(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   ColLayout lo(a.nlay, a.fused ? (a.nlay + 31) / 32 : 0);
   int per = (a.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL(rt_kernel<MODE>, dim3(per * 8), dim3(128), lo.total, s, a, t);
+  hipLaunchKernelGGL(rt_kernel_m<MODE>, dim3(per * 8), dim3(128), lo.total, s, a, t);
+  return hipGetLastError();
+}
+template <int MODE, bool FUSED>
+static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
+  ColLayout lo(a.nlay, FUSED ? (a.nlay + 31) / 32 : 0);
+  int per = (a.ncol_chunk + 7) / 8;
+  hipLaunchKernelGGL((rt_kernel<MODE, FUSED>), dim3(per * 8), dim3(128), lo.total, s, a, t);
   return hipGetLastError();
 }
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  return launch_rt<0>(a, t, s);
+  return a.fused ? launch_rt<0, true>(a, t, s) : launch_rt<0, false>(a, t, s);
 }
 hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  return launch_rt<1>(a, t, s);
+  return launch_rt_m<1>(a, t, s);
 }
 hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  return launch_rt<2>(a, t, s);
+  // the exact-exponential form is offered by the fused entry point only
+  return a.fused ? launch_rt<2, true>(a, t, s) : hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------------------------------------
