@@ -56,10 +56,11 @@ __device__ inline void add_col(double *acc, int s, double x) {
 __device__ inline void key_spec_adj(double *acc, int s1, int s2, double c1, double c2, double strrat,
                                     double mult, bool bin, double fs_b, double speccomb_b) {
   double speccomb = c1 + strrat * c2;
-  double specparm = c1 / speccomb;
+  const double rsc = frcp(speccomb);
+  double specparm = c1 * rsc;
   double specparm_b = (bin && specparm < ONEMINUS) ? fs_b * mult : 0.0;
-  add_col(acc, s1, specparm_b / speccomb);
-  speccomb_b -= specparm_b * c1 / (speccomb * speccomb);
+  add_col(acc, s1, specparm_b * rsc);
+  speccomb_b -= specparm_b * c1 * rsc * rsc;
   add_col(acc, s1, speccomb_b);
   add_col(acc, s2, speccomb_b * strrat);
 }
@@ -138,30 +139,27 @@ __device__ static void gas_optics_adj(const GPoint &g, const double *__restrict_
 }
 
 // Layer optics of one sub-column (src/rrtmg_sw_spcvmc.f90:442-505): clear sky with delta
-// scaling, and the total sky (clear + delta-scaled cloud). Tn = double or a dual number.
-template <typename Tn>
+// scaling (g0 = ga wa ta / s and w0 = s / tau0 share 1 / (s tau0); (w0 - w0 f)/(1 - w0 f) and
+// (g0 - f)/(1 - f) = g0 / (1 + g0) share 1 / ((1 - w0 f)(1 + g0)), f = g0^2), and the total sky
+// (clear + delta-scaled cloud), as the forward kernel evaluates them.
 struct SkyOpt {
-  Tn tau, w, g;
+  double tau, w, g;
 };
-template <typename Tn>
-__device__ inline SkyOpt<Tn> clear_optics(Tn taug, Tn taur, double ta, double wa, double ga) {
-  Tn tauc = taur + taug + ta;
-  Tn omc = taur * 1.0 + ta * wa;
-  Tn gcc = (ga * wa * ta) / omc;
-  omc = omc / tauc;
-  Tn zf = gcc * gcc, zwf = omc * zf;
-  tauc = (1.0 - zwf) * tauc;
-  omc = (omc - zwf) / (1.0 - zwf);
-  gcc = (gcc - zf) / (1.0 - zf);
-  return {tauc, omc, gcc};
+__device__ inline SkyOpt clear_optics(double taug, double taur, double ta, double wa, double ga) {
+  const double tau0 = taur + taug + ta;
+  const double sct = taur * 1.0 + ta * wa;
+  const double r1 = frcp(sct * tau0);
+  const double g0 = ga * wa * ta * tau0 * r1;
+  const double w0 = sct * sct * r1;
+  const double zwf = w0 * (g0 * g0);
+  const double r2 = frcp((1.0 - zwf) * (1.0 + g0));
+  return {(1.0 - zwf) * tau0, (w0 - zwf) * (1.0 + g0) * r2, g0 * (1.0 - zwf) * r2};
 }
-template <typename Tn>
-__device__ inline SkyOpt<Tn> total_optics(const SkyOpt<Tn> &c, Tn ct, Tn cw, Tn cg) {
-  Tn tauo = c.tau + ct;
-  Tn omo = c.tau * c.w + ct * cw;
-  Tn go = (ct * cw * cg + c.tau * c.w * c.g) / omo;
-  omo = omo / tauo;
-  return {tauo, omo, go};
+__device__ inline SkyOpt total_optics(const SkyOpt &c, double ct, double cw, double cg) {
+  const double tauo = c.tau + ct;
+  const double so = c.tau * c.w + ct * cw;
+  const double r = frcp(so * tauo);
+  return {tauo, so * so * r, (ct * cw * cg + c.tau * c.w * c.g) * tauo * r};
 }
 
 // Total-sky layer (r, rd, t, td, direct-beam transmittance) of the exact-exponential forward:
@@ -173,16 +171,16 @@ struct LayerV {
 __device__ inline LayerV total_layer_exact(double taug, double taur, const double ta, double wa,
                                            double ga, double cf, double ct, double cw, double cg,
                                            double mu0, double rmu) {
-  SkyOpt<double> c = clear_optics<double>(taug, taur, ta, wa, ga);
+  SkyOpt c = clear_optics(taug, taur, ta, wa, ga);
   LayerV o = {0., 0., 1., 1., 1.};
   double zcl = 1.0 - cf;
   if (zcl != 0.0) {
-    RTt<double> rc = reftra_exact<double>(c.tau, c.w, c.g, mu0, rmu);
+    RTx rc = reftra_x(c.tau, c.w, c.g, mu0, rmu);
     o = {zcl * rc.r, zcl * rc.rd, zcl * rc.t, zcl * rc.td, zcl * rc.e2};
   }
   if (cf > 0.0) {
-    SkyOpt<double> t = total_optics<double>(c, ct, cw, cg);
-    RTt<double> ro = reftra_exact<double>(t.tau, t.w, t.g, mu0, rmu);
+    SkyOpt t = total_optics(c, ct, cw, cg);
+    RTx ro = reftra_x(t.tau, t.w, t.g, mu0, rmu);
     double dbo = ro.e2;
     if (!(cf > REPCLC)) ro.r = 0., ro.rd = 0., ro.t = 1., ro.td = 1.;
     if (zcl == 0.0) {
@@ -206,11 +204,12 @@ __device__ inline void total_layer_adj(double taug, double taur, double ta, doub
                                        double rmu, const double *lb, double *xb) {
   const double tauc0 = taur + taug + ta;
   const double sc = taur * 1.0 + ta * wa;
-  const double g0 = (ga * wa * ta) / sc;
-  const double w0 = sc / tauc0;
+  const double r1 = frcp(sc * tauc0), rsc = tauc0 * r1, rtau0 = sc * r1;
+  const double g0 = ga * wa * ta * rsc;
+  const double w0 = sc * rtau0;
   const double f = g0 * g0;
   const double D = 1.0 - w0 * f;
-  SkyOpt<double> c = clear_optics<double>(taug, taur, ta, wa, ga);
+  SkyOpt c = clear_optics(taug, taur, ta, wa, ga);
   double tcb = 0., wcb = 0., gcb = 0.;     // adjoints of the clear optics (tauc, wc, gc)
   double ctb = 0., cwb = 0., cgb = 0.;
   const double zcl = 1.0 - cf;
@@ -221,17 +220,18 @@ __device__ inline void total_layer_adj(double taug, double taur, double ta, doub
     tcb += tb, wcb += wb, gcb += gb;
   }
   if (cf > 0.0) {
-    SkyOpt<double> t = total_optics<double>(c, ct, cw, cg);
+    SkyOpt t = total_optics(c, ct, cw, cg);
     // no cloudy reftra when the cloud fraction is tiny: r = 0, t = 1, only the direct beam counts
     const bool rt = cf > REPCLC;
     double tob, wob, gob;
     reftra_exact_adj(t.tau, t.w, t.g, mu0, rmu, rt ? cf * lb[0] : 0., rt ? cf * lb[1] : 0.,
                      rt ? cf * lb[2] : 0., rt ? cf * lb[3] : 0., cf * lb[4], tob, wob, gob);
     const double so = c.tau * c.w + ct * cw;
-    const double sob0 = wob / t.tau;
-    const double taob = tob - wob * t.w / t.tau;
-    const double Nb = gob / so;
-    const double sob = sob0 - gob * t.g / so;
+    const double rts = frcp(t.tau * so), rtt = so * rts, rso = t.tau * rts;
+    const double sob0 = wob * rtt;
+    const double taob = tob - wob * t.w * rtt;
+    const double Nb = gob * rso;
+    const double sob = sob0 - gob * t.g * rso;
     ctb += taob + sob * cw + Nb * cw * cg;
     cwb += sob * ct + Nb * ct * cg;
     cgb += Nb * ct * cw;
@@ -240,37 +240,41 @@ __device__ inline void total_layer_adj(double taug, double taur, double ta, doub
     gcb += Nb * c.tau * c.w;
   }
   // clear-sky delta scaling
-  double g0b = gcb / ((1.0 + g0) * (1.0 + g0));
-  double w0b = wcb * (1.0 - f) / (D * D) - tcb * f * tauc0;
-  double fb = -wcb * w0 * (1.0 - w0) / (D * D) - tcb * w0 * tauc0;
+  const double rg = frcp(1.0 + g0), rD = frcp(D);
+  double g0b = gcb * rg * rg;
+  double w0b = wcb * (1.0 - f) * rD * rD - tcb * f * tauc0;
+  double fb = -wcb * w0 * (1.0 - w0) * rD * rD - tcb * w0 * tauc0;
   double tauc0b = tcb * D;
   g0b += fb * 2.0 * g0;
-  double scb = w0b / tauc0 - g0b * g0 / sc;
-  tauc0b -= w0b * w0 / tauc0;
+  double scb = w0b * rtau0 - g0b * g0 * rsc;
+  tauc0b -= w0b * w0 * rtau0;
   xb[0] = tauc0b;            // taug
   xb[1] = tauc0b + scb;      // taur
   xb[2] = ctb, xb[3] = cwb, xb[4] = cgb;
 }
 
 // Everything a layer of one sub-column needs besides the gas optics (aerosol and cloud optics),
-// as the forward computes it.
+// as the forward computes it. Cloud optics come from the per-(layer, band) table CJ built in the
+// prologue: every cloudy sub-column cell of a band has the same cloud inputs.
 struct LayerIn {
   double ta, wa, ga, cf, ct, cw, cg;
-  double ciwp, clwp, rei, rel;   // cloudy cells with inflag = 2 (for the adjoint)
 };
-__device__ static LayerIn layer_inputs(const RtArgs &a, const DeviceTables &T, const GPoint &gp,
-                                       ColView &v, int c, int g, int lay, bool active, int &err) {
+// per (layer, band): cloud tau, ssa, asymmetry, then their derivatives with respect to the
+// in-cloud liquid water path and to the ice water path
+constexpr int CJ_N = 9;
+__device__ static LayerIn layer_inputs(const RtArgs &a, const DeviceTables &T, int ibnd, ColView &v,
+                                       const double *CJ, int c, int g, int lay, bool active) {
   const int C = a.ncol, L = a.nlay;
-  LayerIn in = {0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  LayerIn in = {0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0};
   if (a.sc.iaer == 10) {
-    size_t q = c + (size_t)C * (lay + (size_t)L * gp.ibnd);
+    size_t q = c + (size_t)C * (lay + (size_t)L * ibnd);
     in.ta = a.tauaer[q], in.wa = a.ssaaer[q], in.ga = a.asmaer[q];
   } else if (a.sc.iaer == 6) {
     double t6 = 0.0, w6 = 0.0, g6 = 0.0;
     for (int ia = 0; ia < NAER; ia++) {
       double ec = a.ecaer[c + (size_t)C * (lay + (size_t)L * ia)];
-      double rt = T.tab[T.rsrtaua + gp.ibnd + NBND * ia], rp = T.tab[T.rsrpiza + gp.ibnd + NBND * ia];
-      double rg = T.tab[T.rsrasya + gp.ibnd + NBND * ia];
+      double rt = T.tab[T.rsrtaua + ibnd + NBND * ia], rp = T.tab[T.rsrpiza + ibnd + NBND * ia];
+      double rg = T.tab[T.rsrasya + ibnd + NBND * ia];
       t6 = t6 + rt * ec;
       w6 = w6 + rt * ec * rp;
       g6 = g6 + rt * ec * rp * rg;
@@ -283,21 +287,9 @@ __device__ static LayerIn layer_inputs(const RtArgs &a, const DeviceTables &T, c
       in.ta = t6;
     }
   }
-  if (a.sc.icld >= 1 && active) {
-    bool cloudy = (v.mask[(lay >> 5) * NGPT + g] >> (lay & 31)) & 1u;
-    size_t q = c + (size_t)C * lay, qb = gp.ibnd + NBND * q;
-    in.cf = cloudy ? 1.0 : 0.0;
-    double tin = cloudy ? a.tauc[qb] : 0.0, win = cloudy ? a.ssac[qb] : 1.0;
-    double gin = cloudy ? a.asmc[qb] : 0.0, fin = cloudy ? a.fsfc[qb] : 0.0;
-    in.ciwp = cloudy ? a.ciwp[q] : 0.0;
-    in.clwp = cloudy ? a.clwp[q] : 0.0;
-    in.rei = a.rei[q], in.rel = a.rel[q];
-    if (in.cf >= CLDMIN) {
-      CloudOpt co = cloud_optics(T, a.sc, gp.band, in.cf, in.ciwp, in.clwp, in.rei, in.rel, tin, win, gin, fin, err);
-      in.ct = co.tau, in.cw = co.ssa, in.cg = co.asm_;
-    } else {
-      in.ct = tin, in.cw = win, in.cg = gin;
-    }
+  if (a.sc.icld >= 1 && active && ((v.mask[(lay >> 5) * NGPT + g] >> (lay & 31)) & 1u)) {
+    const double *o = CJ + ((size_t)lay * NBND + ibnd) * CJ_N;
+    in.cf = 1.0, in.ct = o[0], in.cw = o[1], in.cg = o[2];
   }
   return in;
 }
@@ -309,7 +301,7 @@ __device__ inline void sweep_layer_adj(const Sweep &s, const RT4 &l, double dbt,
                                        double C_b, double E_b, double A_b, double B_b, double *lb) {
   const double t = s.tdbt, z = s.ztdn, p = s.prdnd;
   const double a = z - t;
-  const double den = 1. - l.rd * p, rho = 1. / den;
+  const double den = 1. - l.rd * p, rho = frcp(den);
   const double P = l.rd * a + l.r * t;
   const double Q = a + t * l.r * p;
   double t_b = 0., z_b = A_b, p_b = B_b, a_b = 0., rho_b = 0.;
@@ -406,10 +398,11 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
 }
 
 // Doubles of work area per column: (UB (L) + per-layer states and layers (10 L, store mode) or
-// checkpoints (3 ceil(L/K)) and a segment buffer (3 K)) times NGPT.
+// checkpoints (3 ceil(L/K)) and a segment buffer (3 K)) times NGPT, then the cloud table CJ
+// (L, NBND, CJ_N).
 __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
   size_t states = K == 0 ? (size_t)L * 10 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
-  return ((size_t)L + states) * NGPT;
+  return ((size_t)L + states) * NGPT + (size_t)L * NBND * CJ_N;
 }
 
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) adj_kernel(AdjArgs A, DeviceTables T) {
@@ -464,6 +457,33 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
                        a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
   __syncthreads();
+  // cloud optics and their water-path derivatives for each (layer, band) with a cloudy sub-column
+  const int K = A.ckpt;
+  double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
+  double *CJ = UB + adj_work_doubles(L, K) - (size_t)L * NBND * CJ_N;
+  if (a.sc.icld >= 1) {
+    const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};
+    for (int idx = tid; idx < L * NBND; idx += blockDim.x) {
+      const int lay = idx / NBND, b = idx - lay * NBND;
+      uint32_t any = 0;
+      for (int gg = first[b]; gg < first[b + 1]; gg++) any |= v.mask[(lay >> 5) * NGPT + gg];
+      if ((any >> (lay & 31)) & 1u) {
+        const size_t q1 = c + (size_t)C * lay, qb = b + NBND * q1;
+        const double ciwp = a.ciwp[q1], clwp = a.clwp[q1], rei = a.rei[q1], rel = a.rel[q1];
+        CloudOpt co = cloud_optics(T, a.sc, b + 16, 1.0, ciwp, clwp, rei, rel, a.tauc[qb], a.ssac[qb],
+                                   a.asmc[qb], a.fsfc[qb], err);
+        double *o = CJ + (size_t)idx * CJ_N;
+        o[0] = co.tau, o[1] = co.ssa, o[2] = co.asm_;
+        for (int k = 0; k < 3; k++) {
+          double lw_b, iw_b;
+          cloud_optics_adj(T, a.sc, b + 16, 1.0, ciwp, clwp, rei, rel, k == 0 ? 1. : 0., k == 1 ? 1. : 0.,
+                           k == 2 ? 1. : 0., lw_b, iw_b);
+          o[3 + k] = lw_b, o[6 + k] = iw_b;
+        }
+      }
+    }
+    __syncthreads();
+  }
 
   // the lane's g-point record is read field by field where used (a register copy held across
   // the layer loops would take ~40 VGPRs per lane)
@@ -478,8 +498,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   // Per-column work area: UB (L, NGPT) = dJ/dU_j, then either every sweep state
   // S (L, 3, NGPT) (store mode) or one state per K layers CK (ceil(L/K), 3, NGPT) plus a
   // segment buffer SG (K, 3, NGPT) (checkpoint mode, K = A.ckpt).
-  const int K = A.ckpt;
-  double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
   double *S = UB + (size_t)L * NGPT;
   double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
   const int SW = K == 0 ? 10 : 3;   // store mode: tdbt ztdn prdnd r rd t td dbt taug taur per layer
@@ -489,7 +507,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   auto layer_at = [&](int j, LayerIn &in, double &taug, double &taur) {
     const int lay = L - 1 - j;
     gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-    in = layer_inputs(a, T, gp, v, c, g, lay, active, err);
+    in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
     return total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
   };
 
@@ -573,7 +591,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     LayerV ly;
     if (K == 0) {   // stored layer and gas optics; aerosol and cloud optics recomputed
       taug = active ? sp(j, 8) : 0., taur = active ? sp(j, 9) : 0.;
-      in = layer_inputs(a, T, gp, v, c, g, lay, active, err);
+      in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
       ly = {active ? sp(j, 3) : 0., active ? sp(j, 4) : 0., active ? sp(j, 5) : 1., active ? sp(j, 6) : 1.,
             active ? sp(j, 7) : 1.};
     } else {
@@ -596,9 +614,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     if (active) {
       gas_optics_adj(gp, T.tab, v, lay, lay < laytrop, xb[0], xb[1], la);
       if (in.cf > 0.0) {
-        double lw_b, iw_b;
-        cloud_optics_adj(T, a.sc, gp.band, in.cf, in.ciwp, in.clwp, in.rei, in.rel, xb[2], xb[3], xb[4], lw_b, iw_b);
-        la[AC_CLWP] = lw_b, la[AC_CIWP] = iw_b;
+        const double *o = CJ + ((size_t)lay * NBND + gp.ibnd) * CJ_N;
+        la[AC_CLWP] = xb[2] * o[3] + xb[3] * o[4] + xb[4] * o[5];
+        la[AC_CIWP] = xb[2] * o[6] + xb[3] * o[7] + xb[4] * o[8];
       }
     }
 #pragma unroll

@@ -468,6 +468,17 @@ __device__ static void cloud_optics_adj(const DeviceTables &T, const SwScalars &
   ciwp_b = t_b * ai + si_b * bi;
 }
 
+// FP64 reciprocal from the hardware estimate refined by two Newton-Raphson steps (an IEEE
+// division adds operand rescaling for denormal and overflowing operands, which the
+// denominators it replaces never are, and a final correction: 11 instructions instead of 5).
+__device__ inline double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // ------------------------------------------------------------------------------------------
 // Exponential-table transmittance (e.g. src/rrtmg_sw_spcvmc.f90:558-566, reftra :203-209).
 // ------------------------------------------------------------------------------------------
@@ -566,70 +577,9 @@ __device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz, d
 }
 
 // ------------------------------------------------------------------------------------------
-// Forward-mode dual numbers (value + N partial derivatives) for the adjoint's local Jacobians.
+// Exact-exponential reftra_sw: the smooth function of (tau, w, g) that the adjoint
+// differentiates and that finite differences probe.
 // ------------------------------------------------------------------------------------------
-template <int N>
-struct Dual {
-  double v;
-  double d[N];
-};
-template <int N>
-__device__ inline Dual<N> dconst(double x) {
-  Dual<N> r;
-  r.v = x;
-#pragma unroll
-  for (int i = 0; i < N; i++) r.d[i] = 0.0;
-  return r;
-}
-template <int N>
-__device__ inline Dual<N> dvar(double x, int k) {
-  Dual<N> r = dconst<N>(x);
-  r.d[k] = 1.0;
-  return r;
-}
-#define DUAL_BIN(OP, VEXPR, DEXPR)                                                      \
-  template <int N>                                                                      \
-  __device__ inline Dual<N> operator OP(const Dual<N> &a, const Dual<N> &b) {            \
-    Dual<N> r;                                                                          \
-    r.v = VEXPR;                                                                        \
-    _Pragma("unroll") for (int i = 0; i < N; i++) r.d[i] = DEXPR;                       \
-    return r;                                                                           \
-  }
-DUAL_BIN(+, a.v + b.v, a.d[i] + b.d[i])
-DUAL_BIN(-, a.v - b.v, a.d[i] - b.d[i])
-DUAL_BIN(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
-#undef DUAL_BIN
-template <int N>
-__device__ inline Dual<N> operator/(const Dual<N> &a, const Dual<N> &b) {
-  Dual<N> r;
-  double rb = 1.0 / b.v;
-  r.v = a.v * rb;
-#pragma unroll
-  for (int i = 0; i < N; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * rb;
-  return r;
-}
-template <int N> __device__ inline Dual<N> operator+(const Dual<N> &a, double b) { Dual<N> r = a; r.v += b; return r; }
-template <int N> __device__ inline Dual<N> operator+(double a, const Dual<N> &b) { return b + a; }
-template <int N> __device__ inline Dual<N> operator-(const Dual<N> &a, double b) { Dual<N> r = a; r.v -= b; return r; }
-template <int N>
-__device__ inline Dual<N> operator-(double a, const Dual<N> &b) {
-  Dual<N> r;
-  r.v = a - b.v;
-#pragma unroll
-  for (int i = 0; i < N; i++) r.d[i] = -b.d[i];
-  return r;
-}
-template <int N>
-__device__ inline Dual<N> operator*(const Dual<N> &a, double b) {
-  Dual<N> r;
-  r.v = a.v * b;
-#pragma unroll
-  for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b;
-  return r;
-}
-template <int N> __device__ inline Dual<N> operator*(double a, const Dual<N> &b) { return b * a; }
-template <int N> __device__ inline Dual<N> operator/(const Dual<N> &a, double b) { return a * (1.0 / b); }
-template <int N> __device__ inline Dual<N> operator/(double a, const Dual<N> &b) { return dconst<N>(a) / b; }
 // exp(y) for |y| <= 700 (the exact-exponential forms clamp their arguments at 500): Cody-Waite
 // reduction y = k ln2 + r with ln2 split in a high part of 32 significant bits and a low part
 // (the fdlibm constants), |r| <= ln2/2, exp(r) by its Taylor series to degree 12 (truncation
@@ -658,100 +608,59 @@ __device__ inline double exp_cw(double y) {
 __device__ inline double dsqrt(double x) { return ::sqrt(x); }
 __device__ inline double dexp(double x) { return exp_cw(x); }
 __device__ inline double dmin(double a, double b) { return ::fmin(a, b); }
-template <int N>
-__device__ inline Dual<N> dsqrt(const Dual<N> &a) {
-  Dual<N> r;
-  r.v = ::sqrt(a.v);
-  double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
-#pragma unroll
-  for (int i = 0; i < N; i++) r.d[i] = a.d[i] * h;
-  return r;
-}
-template <int N>
-__device__ inline Dual<N> dexp(const Dual<N> &a) {
-  Dual<N> r;
-  r.v = exp_cw(a.v);
-#pragma unroll
-  for (int i = 0; i < N; i++) r.d[i] = a.d[i] * r.v;
-  return r;
-}
-template <int N>
-__device__ inline Dual<N> dmin(const Dual<N> &a, double b) { return a.v > b ? dconst<N>(b) : a; }
-__device__ inline double val(double x) { return x; }
-template <int N> __device__ inline double val(const Dual<N> &x) { return x.v; }
-
-// reftra_sw with exact exponentials in place of the lookup table and its expansion: the smooth
-// function of (tau, w, g) that the adjoint differentiates and that finite differences probe.
-// Same branches and expressions as the reference (src/rrtmg_sw_reftra.f90:164-322); exp(+x) is
-// taken as 1 / exp(-x). The direct-beam transmittance of this smooth forward is e2 (the
+// reftra_sw with exp() in place of the lookup table and its expansion: the reference's branches
+// (src/rrtmg_sw_reftra.f90:164-322), the general solution multiplied through by e1 as in
+// reftra_e. e2 = exp(-min(tau / mu0, 500)) is also the layer's direct-beam transmittance (the
 // reference evaluates the same exponential separately, src/rrtmg_sw_spcvmc.f90:558-585).
-template <typename Tn>
-struct RTt {
-  Tn r, rd, t, td;
-  Tn e2;   // exp(-min(tau / mu0, 500)): also the layer's direct-beam transmittance
+struct RTx {
+  double r, rd, t, td, e2;
 };
-template <typename Tn>
-__device__ inline RTt<Tn> reftra_exact(Tn zto1, Tn zw, Tn zg, double prmuz, double rmu) {
+__device__ inline RTx reftra_x(double zto1, double zw, double zg, double prmuz, double rmu) {
   const double zwcrit = 0.9999995, eps = 1.e-08;
-  RTt<Tn> o;
-  Tn zg3 = 3. * zg;
-  Tn zgamma1 = (8. - zw * (5. + zg3)) * 0.25;
-  Tn zgamma2 = 3. * (zw * (1. - zg)) * 0.25;
-  Tn zgamma3 = (2. - zg3 * prmuz) * 0.25;
-  Tn zgamma4 = 1. - zgamma3;
-  double w = val(zw), gg = val(zg);
-  double omg = 1. - gg;
-  double den2 = omg * omg - (1. - w) * gg * gg;
-  bool cons = gg != 1. ? (w > 0. && den2 > 0. && w * (omg * omg) >= zwcrit * den2) : (w >= zwcrit);
+  RTx o;
+  double zg3 = 3. * zg;
+  double g1 = (8. - zw * (5. + zg3)) * 0.25;
+  double g2 = 3. * (zw * (1. - zg)) * 0.25;
+  double g3 = (2. - zg3 * prmuz) * 0.25;
+  double g4 = 1. - g3;
+  double omg = 1. - zg;
+  double den2 = omg * omg - (1. - zw) * zg * zg;
+  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+  const double e2 = exp_cw(-fmin(zto1 * rmu, 500.));
+  o.e2 = e2;
   if (cons) {
-    Tn za1 = zgamma1 * prmuz - zgamma3;
-    Tn zgt = zgamma1 * zto1;
-    Tn ze2 = dexp(0.0 - dmin(zto1 * rmu, 500.));
-    o.e2 = ze2;
-    Tn inv = 1. / (1. + zgt);
-    o.r = (zgt - za1 * (1. - ze2)) * inv;
+    double za1 = g1 * prmuz - g3;
+    double zgt = g1 * zto1;
+    double inv = frcp(1. + zgt);
+    o.r = (zgt - za1 * (1. - e2)) * inv;
     o.t = 1. - o.r;
     o.rd = zgt * inv;
     o.td = 1. - o.rd;
-    if (val(ze2) == 1.0) o.r = 0. * zto1, o.t = 0. * zto1 + 1., o.rd = 0. * zto1, o.td = 0. * zto1 + 1.;
+    if (e2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
   } else {
-    Tn za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
-    Tn za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
-    Tn zrk = dsqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
-    Tn zrp = zrk * prmuz;
-    Tn zrp1 = 1. + zrp, zrm1 = 1. - zrp, zrk2 = 2. * zrk;
-    Tn zrpp = 1. - zrp * zrp;
-    Tn zrkg = zrk + zgamma1;
-    Tn zr1 = zrm1 * (za2 + zrk * zgamma3);
-    Tn zr2 = zrp1 * (za2 - zrk * zgamma3);
-    Tn zr3 = zrk2 * (zgamma3 - za2 * prmuz);
-    Tn zr4 = zrpp * zrkg;
-    Tn zr5 = zrpp * (zrk - zgamma1);
-    Tn zt1 = zrp1 * (za1 + zrk * zgamma4);
-    Tn zt2 = zrm1 * (za1 - zrk * zgamma4);
-    Tn zt3 = zrk2 * (zgamma4 + za1 * prmuz);
-    Tn x1 = dmin(zrk * zto1, 500.), x2 = dmin(zto1 * rmu, 500.);
-    Tn zem1 = dexp(0.0 - x1), zem2 = dexp(0.0 - x2);
-    Tn zep1 = 1. / zem1, zep2 = 1. / zem2;
-    o.e2 = zem2;
-    Tn zdenr = zr4 * zep1 + zr5 * zem1;
-    if (val(zdenr) >= -eps && val(zdenr) <= eps) {
-      o.r = 0. * zto1 + eps;
-      o.t = zem2;
-    } else {
-      Tn rden = 1. / zdenr;
-      o.r = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) * rden;
-      o.t = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) * rden;
-    }
-    Tn zemm = zem1 * zem1;
-    Tn zdend = 1. / (zrkg - (zgamma1 - zrk) * zemm);
-    o.rd = zgamma2 * (1. - zemm) * zdend;
-    o.td = zrk2 * zem1 * zdend;
+    double a1 = g1 * g4 + g2 * g3;
+    double a2 = g1 * g3 + g2 * g4;
+    double k = sqrt(g1 * g1 - g2 * g2);
+    double kmu = k * prmuz;
+    double e1 = exp_cw(-fmin(k * zto1, 500.));
+    double ee = e1 * e1;
+    double D = (k + g1) - (g1 - k) * ee;
+    double P = 1. - kmu * kmu;
+    bool tiny = fabs(P * D) <= eps * e1;
+    P = tiny ? 1. : P;
+    double rD = frcp(P * D);
+    double NR = (1. - kmu) * (a2 + k * g3) - (1. + kmu) * (a2 - k * g3) * ee - 2. * k * (g3 - a2 * prmuz) * (e1 * e2);
+    double NT = e2 * ((1. + kmu) * (a1 + k * g4) - (1. - kmu) * (a1 - k * g4) * ee) - 2. * k * (g4 + a1 * prmuz) * e1;
+    o.r = tiny ? eps : zw * NR * rD;
+    o.t = tiny ? e2 : e2 - zw * NT * rD;
+    double PrD = P * rD;
+    o.rd = g2 * (1. - ee) * PrD;
+    o.td = 2. * k * e1 * PrD;
   }
   return o;
 }
 
-// Reverse-mode derivative of reftra_exact: given the adjoints of (r, rd, t, td, e2), returns the
+// Reverse-mode derivative of reftra_x: given the adjoints of (r, rd, t, td, e2), returns the
 // adjoints of (tau, w, g). The forward expressions are recomputed, then swept in reverse.
 __device__ inline void reftra_exact_adj(double zto1, double zw, double zg, double prmuz, double rmu,
                                         double rb, double rdb, double tb, double tdb, double e2b,
@@ -773,7 +682,7 @@ __device__ inline void reftra_exact_adj(double zto1, double zw, double zg, doubl
     double xa = zto1 * rmu;
     double x2 = dmin(xa, 500.);
     double ze2 = dexp(-x2);
-    double inv = 1. / (1. + zgt);
+    double inv = frcp(1. + zgt);
     double ze2b = e2b;
     if (ze2 != 1.0) {
       double Reff = rb - tb, Rdeff = rdb - tdb;   // t = 1 - r, td = 1 - rd
@@ -803,10 +712,10 @@ __device__ inline void reftra_exact_adj(double zto1, double zw, double zg, doubl
     const double xk = k * zto1, xa = zto1 * rmu;
     const double x1 = dmin(xk, 500.), x2 = dmin(xa, 500.);
     const double em1 = dexp(-x1), em2 = dexp(-x2);
-    const double ep1 = 1. / em1, ep2 = 1. / em2;
+    const double ep1 = frcp(em1), ep2 = frcp(em2);
     const double denr = r4 * ep1 + r5 * em1;
     const double emm = em1 * em1;
-    const double Dd = rkg - (g1 - k) * emm, dend = 1. / Dd;
+    const double Dd = rkg - (g1 - k) * emm, dend = frcp(Dd);
     double kb = 0., k2b = 0., em1b = 0., em2b = e2b, emmb = 0., dendb = 0.;
     double ep1b = 0., ep2b = 0., rkgb = 0.;
     double r1b = 0., r2b = 0., r3b = 0., r4b = 0., r5b = 0., t1b = 0., t2b = 0., t3b = 0.;
@@ -827,7 +736,7 @@ __device__ inline void reftra_exact_adj(double zto1, double zw, double zg, doubl
     if (denr >= -eps && denr <= eps) {
       em2b += tb;   // r = eps, t = em2
     } else {
-      const double rden = 1. / denr;
+      const double rden = frcp(denr);
       const double NR = r1 * ep1 - r2 * em1 - r3 * em2;
       const double NT = t1 * ep1 - t2 * em1 - t3 * ep2;
       em2b += tb * (1. - zw * NT * rden);
@@ -861,7 +770,10 @@ __device__ inline void reftra_exact_adj(double zto1, double zw, double zg, doubl
     kb += 2. * k2b;
     rpb += rp1b - rm1b;
     kb += rpb * prmuz;
-    if (k > 0.) g1b += kb * g1 / k, g2b -= kb * g2 / k;
+    if (k > 0.) {
+      const double rk = frcp(k);
+      g1b += kb * g1 * rk, g2b -= kb * g2 * rk;
+    }
     g1b += a2b * g3 + a1b * g4, g3b += a2b * g1 + a1b * g2;
     g2b += a2b * g4 + a1b * g3, g4b += a2b * g2 + a1b * g1;
   }
@@ -887,7 +799,7 @@ __device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A
   A = s.ztdn;
   B = s.prdnd;
   double a = s.ztdn - s.tdbt;
-  double zreflect = 1. / (1. - l.rd * s.prdnd);
+  double zreflect = frcp(1. - l.rd * s.prdnd);
   Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
   E = l.td * zreflect;
   double ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;
@@ -899,20 +811,11 @@ __device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A
 
 // ------------------------------------------------------------------------------------------
 // Pieces of the FP64 forward kernel shaped for gfx950 issue:
-//  * reciprocals from the hardware estimate refined by two Newton-Raphson steps (an IEEE
-//    division adds operand rescaling for denormal and overflowing operands, which the
-//    denominators here never are, and a final correction: 11 instructions instead of 5);
+//  * reciprocals from frcp;
 //  * table reads at 32-bit offsets from a uniform base (one full-rate add per corner instead of
 //    a quarter-rate 32-bit multiply and 64-bit address arithmetic);
 //  * wavefront sums by DPP row operations, with one LDS-pipe and one VALU exchange across rows.
 // ------------------------------------------------------------------------------------------
-__device__ inline double frcp(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
 
 template <typename R>
 __device__ inline R ldt(const R *__restrict__ base, uint32_t i) {
@@ -1047,39 +950,63 @@ __device__ inline GasLane gas_lane(const GPoint &g, int L) {
   q.rab = g.rayla, q.ras = g.sra;
   return q;
 }
+// A value that is the same in every lane (read from LDS at a uniform address), moved to scalar
+// registers so that it occupies no vector registers while the table reads are in flight.
+__device__ inline double uni(double x) {
+  int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+  int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float uni(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// The gas optics in two halves so that a layer's table reads can be issued one layer ahead:
+// gas_fetch computes the key-species parameter and issues every read, gas_finish combines them.
+template <typename R>
+struct GasFetch {
+  R a0, a1, a2, a3, b0, b1, b2, b3, fo0, fo1, s0, s1, r0, r1;
+  R fs, kmul;
+};
 template <bool LOWER, typename R>
-__device__ inline void gas_optics_r(const GasLane &q, const R *__restrict__ tab, ColView &v, int lay,
-                                    R &taug, R &taur) {
-  const int jp = LOWER ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
-  const int jt = v.jt[lay], jt1 = v.jt1[lay];
-  const R fac00 = R(v.f(0, lay)), fac01 = R(v.f(1, lay)), fac10 = R(v.f(2, lay)), fac11 = R(v.f(3, lay));
-  const R colh2o = R(v.c(SP_H2O, lay)), colmol = R(v.c(5, lay));
+__device__ inline GasFetch<R> gas_fetch(const GasLane &q, const R *__restrict__ tab, ColView &v, int lay) {
+  GasFetch<R> f;
+  const int jp = uni(LOWER ? min(v.jp[lay], 12) : max(v.jp[lay], 13));
+  const int jt = uni(v.jt[lay]), jt1 = uni(v.jt1[lay]);
   const R c1 = R(v.col[q.sp1 + lay]), c2 = R(v.col[q.sp2 + lay]);
   SpecT<R> s = key_spec_r<R>(c1, c2, R(q.strrat), R(LOWER ? 8 : 4), q.bin);
-  const R kmul = q.bin ? s.speccomb : c1 * R(q.kmult);
+  f.kmul = q.bin ? s.speccomb : c1 * R(q.kmult);
+  f.fs = s.fs;
   const uint32_t jpr = LOWER ? jp - 1 : jp - 13;
   const uint32_t o0 = q.kb + __umul24(jpr, q.kp) + __umul24(jt - 1, q.kt) + __umul24(s.js - 1, q.ks);
   const uint32_t o1 = q.kb + __umul24(jpr + 1, q.kp) + __umul24(jt1 - 1, q.kt) + __umul24(s.js - 1, q.ks);
-  const R fs = s.fs, f0 = R(1) - fs;
-  const R a0 = ldt(tab, o0), a1 = ldt(tab, o0 + q.kjs), a2 = ldt(tab, o0 + q.kt), a3 = ldt(tab, o0 + q.kt + q.kjs);
-  const R b0 = ldt(tab, o1), b1 = ldt(tab, o1 + q.kjs), b2 = ldt(tab, o1 + q.kt), b3 = ldt(tab, o1 + q.kt + q.kjs);
-  const R k = f0 * fac00 * a0 + fs * fac00 * a1 + f0 * fac10 * a2 + fs * fac10 * a3 +
-              f0 * fac01 * b0 + fs * fac01 * b1 + f0 * fac11 * b2 + fs * fac11 * b3;
-  const int jf = LOWER ? min(v.indf[lay], 2) : min(max(v.indf[lay], 1), 3);
-  const R fo0 = ldt(tab, q.forb + __umul24(jf - 1, q.fors)), fo1 = ldt(tab, q.forb + __umul24(jf, q.fors));
-  R cont = R(v.f(6, lay)) * (fo0 + R(v.f(7, lay)) * (fo1 - fo0));
+  f.a0 = ldt(tab, o0), f.a1 = ldt(tab, o0 + q.kjs), f.a2 = ldt(tab, o0 + q.kt), f.a3 = ldt(tab, o0 + q.kt + q.kjs);
+  f.b0 = ldt(tab, o1), f.b1 = ldt(tab, o1 + q.kjs), f.b2 = ldt(tab, o1 + q.kt), f.b3 = ldt(tab, o1 + q.kt + q.kjs);
+  const int jf = uni(LOWER ? min(v.indf[lay], 2) : min(max(v.indf[lay], 1), 3));
+  f.fo0 = ldt(tab, q.forb + __umul24(jf - 1, q.fors)), f.fo1 = ldt(tab, q.forb + __umul24(jf, q.fors));
+  f.s0 = R(0), f.s1 = R(0), f.r0 = R(0), f.r1 = R(0);
   if (LOWER) {
-    const int is = max(v.inds[lay], 1);
-    const R s0 = ldt(tab, q.selfb + __umul24(is - 1, q.selfs)), s1 = ldt(tab, q.selfb + __umul24(is, q.selfs));
-    cont = R(v.f(4, lay)) * (s0 + R(v.f(5, lay)) * (s1 - s0)) + cont;
+    const int is = uni(max(v.inds[lay], 1));
+    f.s0 = ldt(tab, q.selfb + __umul24(is - 1, q.selfs)), f.s1 = ldt(tab, q.selfb + __umul24(is, q.selfs));
+    f.r0 = ldt(tab, q.rab + __umul24(s.js - 1, q.ras)), f.r1 = ldt(tab, q.rab + __umul24(s.js, q.ras));
   }
+  return f;
+}
+template <bool LOWER, typename R>
+__device__ inline void gas_finish(const GasLane &q, const GasFetch<R> &f, ColView &v, int lay, R &taug,
+                                  R &taur) {
+  const R fac00 = uni(R(v.f(0, lay))), fac01 = uni(R(v.f(1, lay))), fac10 = uni(R(v.f(2, lay))), fac11 = uni(R(v.f(3, lay)));
+  const R colh2o = uni(R(v.c(SP_H2O, lay))), colmol = uni(R(v.c(5, lay)));
+  const R fs = f.fs, f0 = R(1) - fs;
+  const R k = f0 * fac00 * f.a0 + fs * fac00 * f.a1 + f0 * fac10 * f.a2 + fs * fac10 * f.a3 +
+              f0 * fac01 * f.b0 + fs * fac01 * f.b1 + f0 * fac11 * f.b2 + fs * fac11 * f.b3;
+  R cont = uni(R(v.f(6, lay))) * (f.fo0 + uni(R(v.f(7, lay))) * (f.fo1 - f.fo0));
+  if (LOWER) cont = uni(R(v.f(4, lay))) * (f.s0 + uni(R(v.f(5, lay))) * (f.s1 - f.s0)) + cont;
   cont = q.cont ? cont : R(0);
-  taug = kmul * k + colh2o * cont + R(v.col[q.spm + lay]) * R(q.mcoef);
+  taug = f.kmul * k + colh2o * cont + R(v.col[q.spm + lay]) * R(q.mcoef);
   R ray = R(q.rayl);
-  if (LOWER) {
-    const R r0 = ldt(tab, q.rab + __umul24(s.js - 1, q.ras)), r1 = ldt(tab, q.rab + __umul24(s.js, q.ras));
-    ray = q.rayl_js ? r0 + fs * (r1 - r0) : ray;
-  }
+  if (LOWER) ray = q.rayl_js ? f.r0 + fs * (f.r1 - f.r0) : ray;
   taur = colmol * ray;
 }
 

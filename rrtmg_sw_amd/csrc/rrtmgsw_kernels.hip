@@ -24,16 +24,9 @@
 
 namespace rrtmgsw {
 
-// Layer reflectance/transmittance and direct-beam transmittance with the reference's table or
-// with exact exponentials.
-template <bool EXACT>
+// Layer reflectance/transmittance and direct-beam transmittance with the reference's table.
 __device__ inline RT4 reftra_m(double tau, double w, double g, double mu0, double rmu,
                                const DeviceTables &T, double &dbt) {
-  if (EXACT) {
-    RTt<double> o = reftra_exact<double>(tau, w, g, mu0, rmu);
-    dbt = o.e2;
-    return {o.r, o.rd, o.t, o.td};
-  }
   dbt = tbl_exp2(tau * rmu, T.exp_tbl, T.rexp_tbl).e;
   return reftra1(tau, w, g, mu0, rmu, T.exp_tbl, T.rexp_tbl);
 }
@@ -80,7 +73,6 @@ template <int MODE>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
 rt_kernel_m(RtArgs a, DeviceTables T) {
   constexpr bool MIXED = MODE == 1;
-  constexpr bool EXACT = MODE == 2;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
   const int nmw = (L + 31) / 32;
@@ -252,7 +244,7 @@ rt_kernel_m(RtArgs a, DeviceTables T) {
       omc = (omc - zwf) / (1.0 - zwf);
       gcc = (gcc - zf) / (1.0 - zf);
       double dbc;
-      RT4 rc = reftra_m<EXACT>(tauc, omc, gcc, mu0, rmu, T, dbc);
+      RT4 rc = reftra_m(tauc, omc, gcc, mu0, rmu, T, dbc);
       Sweep pre = sc_;
       double Bc, Cc, Ec;
       sweep_layer(sc_, rc, dbc, Ac, Bc, Cc, Ec);
@@ -270,7 +262,7 @@ rt_kernel_m(RtArgs a, DeviceTables T) {
         double dbt = dbc;
         if (cf > 0.0) {
           double dbo;
-          RT4 ro = reftra_m<EXACT>(tauo, omo, go, mu0, rmu, T, dbo);
+          RT4 ro = reftra_m(tauo, omo, go, mu0, rmu, T, dbo);
           if (!(cf > REPCLC)) ro = {0.0, 0.0, 1.0, 1.0};
           double zcl = 1.0 - cf;
           rt.r = zcl * rc.r + cf * ro.r;
@@ -411,8 +403,10 @@ rt_kernel_m(RtArgs a, DeviceTables T) {
 // over g by DPP.
 // Pass 2 requests the coefficients of layer j-1 while layer j is processed.
 // ------------------------------------------------------------------------------------------
+// Occupancy target: four waves per SIMD for the fused table form (measured fastest, a few VGPRs
+// spilled outside the layer loop), three for the others.
 template <int MODE, bool FUSED>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && FUSED ? 4 : 3)))
 rt_kernel(RtArgs a, DeviceTables T) {
   constexpr bool EXACT = MODE == 2;
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -481,11 +475,24 @@ rt_kernel(RtArgs a, DeviceTables T) {
   Sweep sc_ = {1.0, 1.0, 0.0}, st_ = {1.0, 1.0, 0.0};
   int jfirst = L;
   double *scr = a.scratch + (size_t)cl * scr_col_doubles(L) + gi;
-  auto layer = [&](auto lower_tag, const GasLane &q, int j) {
+  // A layer's coefficients are stored once the next layer's table reads have been issued, so that
+  // the waits for those reads do not include the stores (the vector memory counter retires in
+  // issue order).
+  double pend[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int pend_j = -1;
+  bool pend_t = false;
+  auto flush = [&]() {
+    if (active && pend_j >= 0) {
+      double *p = scr + (size_t)pend_j * SCR_COEFS * NGPT;
+      p[0] = pend[0], p[NGPT] = pend[1], p[2 * NGPT] = pend[2];
+      if (pend_t) p[3 * NGPT] = pend[3], p[4 * NGPT] = pend[4], p[5 * NGPT] = pend[5];
+    }
+  };
+  auto layer = [&](auto lower_tag, const GasLane &q, const GasFetch<double> &gf, int j) {
     constexpr bool LOWER = decltype(lower_tag)::value;
     const int lay = L - 1 - j;
     double taug, taur;
-    gas_optics_r<LOWER, double>(q, T.tab, v, lay, taug, taur);
+    gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
     // aerosol (src/rrtmg_sw_rad.f90:709-765)
     double ta = 0.0, wa = 1.0, ga = 0.0;
     if (a.sc.iaer == 10) {
@@ -545,7 +552,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
     RT4 rc;
     double dbc;
     if (EXACT) {
-      RTt<double> o = reftra_exact<double>(tauc, omc, gcc, mu0, rmu);
+      RTx o = reftra_x(tauc, omc, gcc, mu0, rmu);
       rc = {o.r, o.rd, o.t, o.td};
       dbc = o.e2;
     } else {
@@ -575,7 +582,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
         RT4 ro;
         double dbo;
         if (EXACT) {
-          RTt<double> o = reftra_exact<double>(tauo, omo, go, mu0, rmu);
+          RTx o = reftra_x(tauo, omo, go, mu0, rmu);
           ro = {o.r, o.rd, o.t, o.td};
           dbo = o.e2;
         } else {
@@ -593,24 +600,27 @@ rt_kernel(RtArgs a, DeviceTables T) {
       }
       sweep_r(st_, rt, dbt, Ct, Et);
     }
-    if (active) {
-      double *p = scr + (size_t)j * SCR_COEFS * NGPT;
-      p[0] = Bc, p[NGPT] = Cc, p[2 * NGPT] = Ec;
-      if (jfirst <= j) p[3 * NGPT] = Bt, p[4 * NGPT] = Ct, p[5 * NGPT] = Et;
-    }
+    pend[0] = Bc, pend[1] = Cc, pend[2] = Ec, pend[3] = Bt, pend[4] = Ct, pend[5] = Et;
+    pend_j = j, pend_t = jfirst <= j;
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
     const double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
     if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
   };
+  // Each layer issues its gas-optics table reads, then stores the coefficients of the layer
+  // above.
   const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
-  {
-    const GasLane q = gas_lane<false>(gp, L);
-    for (int j = 0; j < jsplit; j++) layer(std::integral_constant<bool, false>(), q, j);
-  }
-  {
-    const GasLane q = gas_lane<true>(gp, L);
-    for (int j = jsplit; j < L; j++) layer(std::integral_constant<bool, true>(), q, j);
-  }
+  auto regime = [&](auto lower_tag, int j0, int j1) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
+    const GasLane q = gas_lane<LOWER>(gp, L);
+    for (int j = j0; j < j1; j++) {
+      const GasFetch<double> cur = gas_fetch<LOWER, double>(q, T.tab, v, L - 1 - j);
+      flush();
+      layer(lower_tag, q, cur, j);
+    }
+  };
+  regime(std::integral_constant<bool, false>(), 0, jsplit);
+  regime(std::integral_constant<bool, true>(), jsplit, L);
+  flush();
   if (jfirst == L) st_ = sc_;
   if (err) atomicCAS(a.status, 0, err);
 
@@ -627,29 +637,48 @@ rt_kernel(RtArgs a, DeviceTables T) {
     const double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
     if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 2)] = s;
   }
-  // Coefficients of layer j - 1 are requested before layer j is processed; the total-sky slots
-  // only by lanes whose sub-column is cloudy at or above that layer (the others were never
-  // written).
-  const double *p = scr + (size_t)(L - 1) * SCR_COEFS * NGPT;
-  double cB = p[0], cC = p[NGPT], cE = p[2 * NGPT], tB = 0.0, tC = 0.0, tE = 0.0;
-  if (L - 1 >= jfirst) tB = p[3 * NGPT], tC = p[4 * NGPT], tE = p[5 * NGPT];
-  for (int j = L - 1; j >= 0; j--) {
-    const int jn = j > 0 ? j - 1 : 0;
-    const double *pn = scr + (size_t)jn * SCR_COEFS * NGPT;
-    const double nB = pn[0], nC = pn[NGPT], nE = pn[2 * NGPT];
-    double mB = 0.0, mC = 0.0, mE = 0.0;
-    if (jn >= jfirst) mB = pn[3 * NGPT], mC = pn[4 * NGPT], mE = pn[5 * NGPT];
+  // Coefficients are requested four layers ahead (a ring of four register sets); the total-sky
+  // slots only by lanes whose sub-column is cloudy at or above that layer (the others were
+  // never written).
+  struct Co {
+    double cB, cC, cE, tB, tC, tE;
+  };
+  auto fetch = [&](int j) {
+    Co o;
+    const int jj = j < 0 ? 0 : j;
+    const double *p = scr + (size_t)jj * SCR_COEFS * NGPT;
+    o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
+    o.tB = 0.0, o.tC = 0.0, o.tE = 0.0;
+    if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
+    return o;
+  };
+  auto step = [&](int j, const Co &o) {
     const bool tot = j >= jfirst;
-    const double ut_t = tC + tE * ut, ut_c = cC + cE * ut;
+    const double ut_t = o.tC + o.tE * ut, ut_c = o.cC + o.cE * ut;
     ut = tot ? ut_t : ut_c;
-    const double bu_t = (tot ? tB : cB) * ut;
-    uc = cC + cE * uc;
-    const double bu_c = cB * uc;
+    const double bu_t = (tot ? o.tB : o.cB) * ut;
+    uc = o.cC + o.cE * uc;
+    const double bu_c = o.cB * uc;
     // downward flux = sum(zinc ztdn) (pass 1) + sum(zinc prdnd U)
     const double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
     if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 2)] = s;
-    cB = nB, cC = nC, cE = nE, tB = mB, tC = mC, tE = mE;
+  };
+  Co r0 = fetch(L - 1), r1 = fetch(L - 2), r2 = fetch(L - 3), r3 = fetch(L - 4);
+  int j = L - 1;
+  for (; j >= 3; j -= 4) {
+    step(j, r0);
+    r0 = fetch(j - 4);
+    step(j - 1, r1);
+    r1 = fetch(j - 5);
+    step(j - 2, r2);
+    r2 = fetch(j - 6);
+    step(j - 3, r3);
+    r3 = fetch(j - 7);
   }
+  // at most three layers remain: j, j - 1, j - 2 in r0, r1, r2
+  if (j >= 0) step(j, r0);
+  if (j >= 1) step(j - 1, r1);
+  if (j >= 2) step(j - 2, r2);
   __syncthreads();
   rt_epilogue(a, v, c);
 }
