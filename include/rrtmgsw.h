@@ -145,10 +145,10 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
  * (total sky; seeds are device arrays (ncol) and (ncol, nlay)). Writes dJ/d tlay, h2ovmr,
  * o3vmr, clwp and ciwp (the in-cloud water paths of mcica_subcol_sw), each (ncol, nlay), and
  * dJ/d (asdir, asdif, aldir, aldif) as d_alb (ncol, 4). McICA masks are held fixed. ckpt = 0
- * stores every layer's reflectances, transmittances, direct-beam transmittance, gas and Rayleigh
- * optical depths and adding-method state (11 doubles per layer and g-point); ckpt = K > 0 keeps one state per K
- * layers and recomputes each K-layer segment in the reverse sweep (1 + 3/K doubles per layer
- * and g-point, two more layer evaluations per layer). No counterpart
+ * stores every layer's gas and Rayleigh optical depths, adding-method state and state adjoint
+ * (6 doubles per layer and g-point); ckpt = K > 0 keeps one state per K layers and recomputes
+ * each K-layer segment in the reverse sweep (1 + 3/K doubles per layer and g-point, two more
+ * layer evaluations per layer). Both add 9 doubles per layer and band. No counterpart
  * in the reference; inputs as rrtmgsw_mcica_sw_dev (same argument order). */
 int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
                                  const double *play, const double *plev, const double *tlay,

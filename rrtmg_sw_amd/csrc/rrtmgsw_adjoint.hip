@@ -14,21 +14,22 @@
 //       :144-159 as U_j = C_j + E_j U_(j+1), D_j = ztdn_j + prdnd_j U_j); since J reaches U_j
 //       through U_j itself, D_j and U_(j-1), ubar_j = dJ/dU_j = ubar_(j-1) E_(j-1) +
 //       zinc (wU_j + wD_j prdnd_j) is known as soon as layer j is: stored per layer, with the
-//       sweep state s_j = (tdbt, ztdn, prdnd), layer (r, rd, t, td, dbt) and gas optics
-//       (taug, taur) of every layer (store mode) or the state of every K-th layer (checkpoint mode); then the surface
+//       sweep state s_j = (tdbt, ztdn, prdnd) and gas optics (taug, taur) of every layer
+//       (store mode) or the state of every K-th layer (checkpoint mode); then the surface
 //       adjoint (albedos, s_L);
 //   R1  bottom -> top: s_j (loaded, or recomputed per K-layer segment from its checkpoint), the
-//       layer recomputed, U_j by the back substitution and the objective per unit incident
-//       flux Phi on the way up, the adjoint of the elimination step, reftra_sw's reverse-mode
-//       derivative chained through the layer optics' dual numbers over (taug, taur, cloud
-//       tau, ssa, asymmetry), then
-//       the hand-written adjoints of the gas optics (taumol template) and of cldprmc_sw;
-//       per-layer sums over g go to LDS accumulators;
+//       layer optics and reftra recomputed (cheaper than storing and reloading them), U_j by
+//       the back substitution and the objective per unit incident flux Phi on the way up, the
+//       adjoint of the elimination step, reftra_sw's reverse-mode derivative, the reverse of
+//       the layer-optics combination, then the hand-written adjoints of the gas optics
+//       (taumol template) and of cldprmc_sw (per-(layer, band) water-path Jacobians from the
+//       prologue); per-layer sums over g go to LDS accumulators;
 //   end the incident-flux adjoint (solar source interpolated at laysolfr), and per layer the
 //       adjoint of inatm_sw + setcoef_sw onto T, H2O, O3.
-// Memory per (column, layer, g-point): ubar_j, s_j, the layer's r, rd, t, td and direct-beam
-// transmittance and its gas and Rayleigh optical depths, 11 doubles (store mode), or ubar_j and 3/K for the checkpoints plus a 3K
-// segment buffer per column (checkpoint mode, every layer recomputed twice).
+// Memory per (column, layer, g-point): ubar_j, s_j and the gas and Rayleigh optical depths, 6
+// doubles (store mode), or ubar_j and 3/K for the checkpoints plus a 3K segment buffer per
+// column (checkpoint mode, every layer's gas optics and sweep recomputed); plus 9 doubles per
+// (layer, band) for the cloud table.
 #include "rrtmgsw_device.h"
 
 namespace rrtmgsw {
@@ -46,28 +47,25 @@ enum : int {
   AC_N = 16
 };
 
-__device__ inline void add_col(double *acc, int s, double x) {
-#pragma unroll
-  for (int i = 0; i < 6; i++) acc[i] += (i == s) ? x : 0.0;
-}
-
 // Adjoint of the key-species parameter (specparm = c1 / (c1 + strrat c2), fs = frac(mult *
-// specparm), clipped at oneminus) given d fs and d speccomb.
-__device__ inline void key_spec_adj(double *acc, int s1, int s2, double c1, double c2, double strrat,
-                                    double mult, bool bin, double fs_b, double speccomb_b) {
+// specparm), clipped at oneminus) given d fs and d speccomb: adds to the adjoints of c1 and c2.
+__device__ inline void key_spec_adj(double c1, double c2, double strrat, double mult, bool bin,
+                                    double fs_b, double speccomb_b, double &c1_b, double &c2_b) {
   double speccomb = c1 + strrat * c2;
   const double rsc = frcp(speccomb);
   double specparm = c1 * rsc;
   double specparm_b = (bin && specparm < ONEMINUS) ? fs_b * mult : 0.0;
-  add_col(acc, s1, specparm_b * rsc);
   speccomb_b -= specparm_b * c1 * rsc * rsc;
-  add_col(acc, s1, speccomb_b);
-  add_col(acc, s2, speccomb_b * strrat);
+  c1_b += specparm_b * rsc + speccomb_b;
+  c2_b += speccomb_b * strrat;
 }
 
-// Adjoint of gas_optics<double> (the taumol template): accumulates d/d(setcoef outputs).
+// Adjoint of gas_optics<double> (the taumol template): accumulates d/d(setcoef outputs). The
+// column adjoints are gathered per role (key species 1 and 2, minor absorber, H2O, colmol) and
+// added to the species slots once.
 __device__ static void gas_optics_adj(const GPoint &g, const double *__restrict__ tab, ColView &v,
                                       int lay, bool lower, double tg_b, double tr_b, double *acc) {
+  double c1_b = 0.0, c2_b = 0.0, cm_b = 0.0, h2o_b = 0.0, mol_b = 0.0;
   const int jp = lower ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
   const int jt = v.jt[lay], jt1 = v.jt1[lay];
   const double fac00 = v.f(0, lay), fac01 = v.f(1, lay), fac10 = v.f(2, lay), fac11 = v.f(3, lay);
@@ -106,13 +104,13 @@ __device__ static void gas_optics_adj(const GPoint &g, const double *__restrict_
     double selffac = v.f(4, lay), selffrac = v.f(5, lay), forfac = v.f(6, lay), forfrac = v.f(7, lay);
     double selfi = sa + selffrac * (sb - sa), fori = fa + forfrac * (fb - fa);
     double cont = selffac * selfi + forfac * fori;
-    add_col(acc, SP_H2O, tg_b * cont);
+    h2o_b += tg_b * cont;
     double cont_b = tg_b * colh2o;
     acc[AC_SELFFAC] += cont_b * selfi;
     acc[AC_SELFFRAC] += cont_b * selffac * (sb - sa);
     acc[AC_FORFAC] += cont_b * fori;
     acc[AC_FORFRAC] += cont_b * forfac * (fb - fa);
-    add_col(acc, g.lo_minor, tg_b * g.lo_minor_coef);
+    cm_b += tg_b * g.lo_minor_coef;
     double ray = g.rayl_lo;
     if (g.rayl_js) {
       const double *RA = tab + g.rayla;
@@ -120,22 +118,28 @@ __device__ static void gas_optics_adj(const GPoint &g, const double *__restrict_
       ray = r0 + fs * (r1 - r0);
       fs_b += tr_b * colmol * (r1 - r0);
     }
-    add_col(acc, 5, tr_b * ray);
+    mol_b += tr_b * ray;
   } else {
     if (g.hi_for) {
       int jf = min(max(v.indf[lay], 1), 3);
       double fa = F[(jf - 1) * g.sfor], fb = F[jf * g.sfor];
       double forfac = v.f(6, lay), forfrac = v.f(7, lay);
       double fori = fa + forfrac * (fb - fa);
-      add_col(acc, SP_H2O, tg_b * forfac * fori);
+      h2o_b += tg_b * forfac * fori;
       acc[AC_FORFAC] += tg_b * colh2o * fori;
       acc[AC_FORFRAC] += tg_b * colh2o * forfac * (fb - fa);
     }
-    add_col(acc, g.hi_minor, tg_b * g.hi_minor_coef);
-    add_col(acc, 5, tr_b * g.rayl_hi);
+    cm_b += tg_b * g.hi_minor_coef;
+    mol_b += tr_b * g.rayl_hi;
   }
-  if (!bin) add_col(acc, s1, kmul_b * mult);
-  key_spec_adj(acc, s1, s2, c1, c2, strrat, lower ? 8.0 : 4.0, bin, fs_b, bin ? kmul_b : 0.0);
+  if (!bin) c1_b += kmul_b * mult;
+  key_spec_adj(c1, c2, strrat, lower ? 8.0 : 4.0, bin, fs_b, bin ? kmul_b : 0.0, c1_b, c2_b);
+  const int sm = lower ? g.lo_minor : g.hi_minor;
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+    acc[AC_COL + i] += (s1 == i ? c1_b : 0.0) + (s2 == i ? c2_b : 0.0) + (sm == i ? cm_b : 0.0);
+  acc[AC_COL + SP_H2O] += h2o_b;
+  acc[AC_COL + 5] += mol_b;
 }
 
 // Layer optics of one sub-column (src/rrtmg_sw_spcvmc.f90:442-505): clear sky with delta
@@ -397,11 +401,11 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
   q_b -= coldry_b * coldry * ((amw - amd) / amm + 1. / (1. + q));
 }
 
-// Doubles of work area per column: (UB (L) + per-layer states and layers (10 L, store mode) or
+// Doubles of work area per column: (UB (L) + per-layer states and gas optics (5 L, store mode) or
 // checkpoints (3 ceil(L/K)) and a segment buffer (3 K)) times NGPT, then the cloud table CJ
 // (L, NBND, CJ_N).
 __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
-  size_t states = K == 0 ? (size_t)L * 10 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
+  size_t states = K == 0 ? (size_t)L * 5 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
   return ((size_t)L + states) * NGPT + (size_t)L * NBND * CJ_N;
 }
 
@@ -500,7 +504,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   // segment buffer SG (K, 3, NGPT) (checkpoint mode, K = A.ckpt).
   double *S = UB + (size_t)L * NGPT;
   double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
-  const int SW = K == 0 ? 10 : 3;   // store mode: tdbt ztdn prdnd r rd t td dbt taug taur per layer
+  const int SW = K == 0 ? 5 : 3;   // store mode: tdbt ztdn prdnd taug taur per layer
   auto sp = [&](int j, int k) -> double & { return S[((size_t)j * SW + k) * NGPT + g]; };
   auto sg = [&](int j, int k) -> double & { return SG[((size_t)j * 3 + k) * NGPT + g]; };
 
@@ -527,10 +531,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     LayerIn in;
     double taug, taur;
     LayerV ly = layer_at(j, in, taug, taur);
-    if (active && K == 0) {
-      sp(j, 3) = ly.r, sp(j, 4) = ly.rd, sp(j, 5) = ly.t, sp(j, 6) = ly.td, sp(j, 7) = ly.dbt;
-      sp(j, 8) = taug, sp(j, 9) = taur;
-    }
+    if (active && K == 0) sp(j, 3) = taug, sp(j, 4) = taur;
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     double Aa, Bb, Cc, Ee;
     sweep_layer(st, l, ly.dbt, Aa, Bb, Cc, Ee);
@@ -589,14 +590,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     LayerIn in;
     double taug, taur;
     LayerV ly;
-    if (K == 0) {   // stored layer and gas optics; aerosol and cloud optics recomputed
-      taug = active ? sp(j, 8) : 0., taur = active ? sp(j, 9) : 0.;
-      in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
-      ly = {active ? sp(j, 3) : 0., active ? sp(j, 4) : 0., active ? sp(j, 5) : 1., active ? sp(j, 6) : 1.,
-            active ? sp(j, 7) : 1.};
-    } else {
-      ly = layer_at(j, in, taug, taur);
-    }
+    // gas optics stored (store mode) or recomputed; aerosol, cloud and layer optics recomputed
+    // at one call site for both modes (identical rounding)
+    if (K == 0)
+      taug = active ? sp(j, 3) : 1., taur = active ? sp(j, 4) : 0.1;
+    else
+      gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
+    in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
+    ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     // C_j, E_j and U_j from the layer
     Sweep tmp = sj;
@@ -644,10 +645,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
       dsrc = a.sc.svar_f * dif(gp.facb) + a.sc.svar_s * dif(gp.snsp) + a.sc.svar_i * dif(gp.irrad);
     }
     double fs_b = phi * a.sc.adjflux[gp.ibnd] * mu0 * dsrc;
-    double la[6] = {0, 0, 0, 0, 0, 0};
-    key_spec_adj(la, s1, s2, v.c(s1, ls), v.c(s2, ls), strrat, mult, true, fs_b, 0.0);
-    for (int k = 0; k < 6; k++)
-      if (la[k] != 0.0) atomicAdd(&acc[(size_t)ls * AC_N + AC_COL + k], la[k]);
+    double c1_b = 0.0, c2_b = 0.0;
+    key_spec_adj(v.c(s1, ls), v.c(s2, ls), strrat, mult, true, fs_b, 0.0, c1_b, c2_b);
+    if (c1_b != 0.0) atomicAdd(&acc[(size_t)ls * AC_N + AC_COL + s1], c1_b);
+    if (c2_b != 0.0) atomicAdd(&acc[(size_t)ls * AC_N + AC_COL + s2], c2_b);
   }
   __syncthreads();
 
