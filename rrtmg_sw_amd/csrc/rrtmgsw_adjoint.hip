@@ -30,6 +30,8 @@
 // doubles (store mode), or ubar_j and 3/K for the checkpoints plus a 3K segment buffer per
 // column (checkpoint mode, every layer's gas optics and sweep recomputed); plus 9 doubles per
 // (layer, band) for the cloud table.
+#include <type_traits>
+
 #include "rrtmgsw_device.h"
 
 namespace rrtmgsw {
@@ -60,84 +62,55 @@ __device__ inline void key_spec_adj(double c1, double c2, double strrat, double 
   c2_b += speccomb_b * strrat;
 }
 
-// Adjoint of gas_optics<double> (the taumol template): accumulates d/d(setcoef outputs). The
+// Adjoint of gas_fetch + gas_finish for one regime (the lane's constants in q): re-reads the
+// table entries and accumulates d/d(setcoef outputs) given the adjoints of taug and taur. The
 // column adjoints are gathered per role (key species 1 and 2, minor absorber, H2O, colmol) and
 // added to the species slots once.
-__device__ static void gas_optics_adj(const GPoint &g, const double *__restrict__ tab, ColView &v,
-                                      int lay, bool lower, double tg_b, double tr_b, double *acc) {
-  double c1_b = 0.0, c2_b = 0.0, cm_b = 0.0, h2o_b = 0.0, mol_b = 0.0;
-  const int jp = lower ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
-  const int jt = v.jt[lay], jt1 = v.jt1[lay];
-  const double fac00 = v.f(0, lay), fac01 = v.f(1, lay), fac10 = v.f(2, lay), fac11 = v.f(3, lay);
-  const double colh2o = v.c(SP_H2O, lay), colmol = v.c(5, lay);
-  const bool bin = (lower ? g.lo_mode : g.hi_mode) == KEY_BIN;
-  const int s1 = lower ? g.lo_s1 : g.hi_s1, s2 = lower ? g.lo_s2 : g.hi_s2;
-  const double strrat = lower ? g.lo_strrat : g.hi_strrat, mult = lower ? g.lo_mult : g.hi_mult;
-  const double c1 = v.c(s1, lay), c2 = v.c(s2, lay);
-  Spec s = key_spec(c1, c2, strrat, lower ? 8.0 : 4.0, bin);
-  const double kmul = bin ? s.speccomb : c1 * mult;
-  const int nsp = lower ? g.lo_nsp : g.hi_nsp;
-  const int ind0 = lower ? ((jp - 1) * 5 + (jt - 1)) * nsp + s.js : ((jp - 13) * 5 + (jt - 1)) * nsp + s.js;
-  const int ind1 = lower ? (jp * 5 + (jt1 - 1)) * nsp + s.js : ((jp - 12) * 5 + (jt1 - 1)) * nsp + s.js;
-  const double *A = tab + (lower ? g.absa : g.absb);
-  const int st = lower ? g.sa : g.sb, cjs = bin ? 1 : 0;
-  double a0 = A[(ind0 - 1) * st], a1 = A[(ind0 - 1 + cjs) * st];
-  double a2 = A[(ind0 - 1 + nsp) * st], a3 = A[(ind0 - 1 + nsp + cjs) * st];
-  double b0 = A[(ind1 - 1) * st], b1 = A[(ind1 - 1 + cjs) * st];
-  double b2 = A[(ind1 - 1 + nsp) * st], b3 = A[(ind1 - 1 + nsp + cjs) * st];
-  const double fs = s.fs, f0 = 1.0 - fs;
-  double k = f0 * fac00 * a0 + fs * fac00 * a1 + f0 * fac10 * a2 + fs * fac10 * a3 +
-             f0 * fac01 * b0 + fs * fac01 * b1 + f0 * fac11 * b2 + fs * fac11 * b3;
-  // taug = kmul k + continua + minor
-  double kmul_b = tg_b * k, k_b = tg_b * kmul;
-  double fs_b = k_b * (fac00 * (a1 - a0) + fac10 * (a3 - a2) + fac01 * (b1 - b0) + fac11 * (b3 - b2));
-  acc[AC_FAC + 0] += k_b * (f0 * a0 + fs * a1);
-  acc[AC_FAC + 2] += k_b * (f0 * a2 + fs * a3);
-  acc[AC_FAC + 1] += k_b * (f0 * b0 + fs * b1);
-  acc[AC_FAC + 3] += k_b * (f0 * b2 + fs * b3);
-  const double *F = tab + g.forref;
-  if (lower) {
-    const double *S = tab + g.selfref;
-    int is = max(v.inds[lay], 1), jf = min(v.indf[lay], 2);
-    double sa = S[(is - 1) * g.sself], sb = S[is * g.sself];
-    double fa = F[(jf - 1) * g.sfor], fb = F[jf * g.sfor];
-    double selffac = v.f(4, lay), selffrac = v.f(5, lay), forfac = v.f(6, lay), forfrac = v.f(7, lay);
-    double selfi = sa + selffrac * (sb - sa), fori = fa + forfrac * (fb - fa);
-    double cont = selffac * selfi + forfac * fori;
-    h2o_b += tg_b * cont;
-    double cont_b = tg_b * colh2o;
+template <bool LOWER>
+__device__ inline void gas_adj_r(const GasLane &q, const double *__restrict__ tab, ColView &v, int lay,
+                                 double tg_b, double tr_b, double *acc) {
+  const GasFetch<double> f = gas_fetch<LOWER, double>(q, tab, v, lay);
+  const double fac00 = uni(v.f(0, lay)), fac01 = uni(v.f(1, lay)), fac10 = uni(v.f(2, lay)), fac11 = uni(v.f(3, lay));
+  const double colh2o = uni(v.c(SP_H2O, lay)), colmol = uni(v.c(5, lay));
+  const double forfac = uni(v.f(6, lay)), forfrac = uni(v.f(7, lay));
+  const double fs = f.fs, f0 = 1.0 - fs;
+  const double k = f0 * fac00 * f.a0 + fs * fac00 * f.a1 + f0 * fac10 * f.a2 + fs * fac10 * f.a3 +
+                   f0 * fac01 * f.b0 + fs * fac01 * f.b1 + f0 * fac11 * f.b2 + fs * fac11 * f.b3;
+  // taug = kmul k + colh2o cont + c_minor mcoef; taur = colmol ray
+  const double kmul_b = tg_b * k, k_b = tg_b * f.kmul;
+  double fs_b = k_b * (fac00 * (f.a1 - f.a0) + fac10 * (f.a3 - f.a2) + fac01 * (f.b1 - f.b0) + fac11 * (f.b3 - f.b2));
+  acc[AC_FAC + 0] += k_b * (f0 * f.a0 + fs * f.a1);
+  acc[AC_FAC + 2] += k_b * (f0 * f.a2 + fs * f.a3);
+  acc[AC_FAC + 1] += k_b * (f0 * f.b0 + fs * f.b1);
+  acc[AC_FAC + 3] += k_b * (f0 * f.b2 + fs * f.b3);
+  const double fori = f.fo0 + forfrac * (f.fo1 - f.fo0);
+  const double tgc_b = q.cont ? tg_b : 0.0;
+  double h2o_b, mol_b;
+  if (LOWER) {
+    const double selffac = uni(v.f(4, lay)), selffrac = uni(v.f(5, lay));
+    const double selfi = f.s0 + selffrac * (f.s1 - f.s0);
+    h2o_b = tgc_b * (selffac * selfi + forfac * fori);
+    const double cont_b = tgc_b * colh2o;
     acc[AC_SELFFAC] += cont_b * selfi;
-    acc[AC_SELFFRAC] += cont_b * selffac * (sb - sa);
+    acc[AC_SELFFRAC] += cont_b * selffac * (f.s1 - f.s0);
     acc[AC_FORFAC] += cont_b * fori;
-    acc[AC_FORFRAC] += cont_b * forfac * (fb - fa);
-    cm_b += tg_b * g.lo_minor_coef;
-    double ray = g.rayl_lo;
-    if (g.rayl_js) {
-      const double *RA = tab + g.rayla;
-      double r0 = RA[(s.js - 1) * g.sra], r1 = RA[s.js * g.sra];
-      ray = r0 + fs * (r1 - r0);
-      fs_b += tr_b * colmol * (r1 - r0);
-    }
-    mol_b += tr_b * ray;
+    acc[AC_FORFRAC] += cont_b * forfac * (f.fo1 - f.fo0);
+    const double ray = q.rayl_js ? f.r0 + fs * (f.r1 - f.r0) : q.rayl;
+    if (q.rayl_js) fs_b += tr_b * colmol * (f.r1 - f.r0);
+    mol_b = tr_b * ray;
   } else {
-    if (g.hi_for) {
-      int jf = min(max(v.indf[lay], 1), 3);
-      double fa = F[(jf - 1) * g.sfor], fb = F[jf * g.sfor];
-      double forfac = v.f(6, lay), forfrac = v.f(7, lay);
-      double fori = fa + forfrac * (fb - fa);
-      h2o_b += tg_b * forfac * fori;
-      acc[AC_FORFAC] += tg_b * colh2o * fori;
-      acc[AC_FORFRAC] += tg_b * colh2o * forfac * (fb - fa);
-    }
-    cm_b += tg_b * g.hi_minor_coef;
-    mol_b += tr_b * g.rayl_hi;
+    h2o_b = tgc_b * forfac * fori;
+    acc[AC_FORFAC] += tgc_b * colh2o * fori;
+    acc[AC_FORFRAC] += tgc_b * colh2o * forfac * (f.fo1 - f.fo0);
+    mol_b = tr_b * q.rayl;
   }
-  if (!bin) c1_b += kmul_b * mult;
-  key_spec_adj(c1, c2, strrat, lower ? 8.0 : 4.0, bin, fs_b, bin ? kmul_b : 0.0, c1_b, c2_b);
-  const int sm = lower ? g.lo_minor : g.hi_minor;
+  const double cm_b = tg_b * q.mcoef;
+  const double c1 = v.col[q.sp1 + lay], c2 = v.col[q.sp2 + lay];
+  double c1_b = q.bin ? 0.0 : kmul_b * q.kmult, c2_b = 0.0;
+  key_spec_adj(c1, c2, q.strrat, LOWER ? 8.0 : 4.0, q.bin, fs_b, q.bin ? kmul_b : 0.0, c1_b, c2_b);
 #pragma unroll
   for (int i = 0; i < 5; i++)
-    acc[AC_COL + i] += (s1 == i ? c1_b : 0.0) + (s2 == i ? c2_b : 0.0) + (sm == i ? cm_b : 0.0);
+    acc[AC_COL + i] += (q.s1 == i ? c1_b : 0.0) + (q.s2 == i ? c2_b : 0.0) + (q.sm == i ? cm_b : 0.0);
   acc[AC_COL + SP_H2O] += h2o_b;
   acc[AC_COL + 5] += mol_b;
 }
@@ -150,6 +123,7 @@ struct SkyOpt {
   double tau, w, g;
 };
 __device__ inline SkyOpt clear_optics(double taug, double taur, double ta, double wa, double ga) {
+#pragma clang fp contract(off)
   const double tau0 = taur + taug + ta;
   const double sct = taur * 1.0 + ta * wa;
   const double r1 = frcp(sct * tau0);
@@ -160,6 +134,7 @@ __device__ inline SkyOpt clear_optics(double taug, double taur, double ta, doubl
   return {(1.0 - zwf) * tau0, (w0 - zwf) * (1.0 + g0) * r2, g0 * (1.0 - zwf) * r2};
 }
 __device__ inline SkyOpt total_optics(const SkyOpt &c, double ct, double cw, double cg) {
+#pragma clang fp contract(off)
   const double tauo = c.tau + ct;
   const double so = c.tau * c.w + ct * cw;
   const double r = frcp(so * tauo);
@@ -175,6 +150,7 @@ struct LayerV {
 __device__ inline LayerV total_layer_exact(double taug, double taur, const double ta, double wa,
                                            double ga, double cf, double ct, double cw, double cg,
                                            double mu0, double rmu) {
+#pragma clang fp contract(off)
   SkyOpt c = clear_optics(taug, taur, ta, wa, ga);
   LayerV o = {0., 0., 1., 1., 1.};
   double zcl = 1.0 - cf;
@@ -268,6 +244,7 @@ struct LayerIn {
 constexpr int CJ_N = 9;
 __device__ static LayerIn layer_inputs(const RtArgs &a, const DeviceTables &T, int ibnd, ColView &v,
                                        const double *CJ, int c, int g, int lay, bool active) {
+#pragma clang fp contract(off)
   const int C = a.ncol, L = a.nlay;
   LayerIn in = {0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0};
   if (a.sc.iaer == 10) {
@@ -508,10 +485,23 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   auto sp = [&](int j, int k) -> double & { return S[((size_t)j * SW + k) * NGPT + g]; };
   auto sg = [&](int j, int k) -> double & { return SG[((size_t)j * 3 + k) * NGPT + g]; };
 
-  auto layer_at = [&](int j, LayerIn &in, double &taug, double &taur) {
+  // Gas optics of layer j with the lane's constants of the loop's regime (q), or of the other
+  // regime's, read from the g-point record, for a layer on the other side of laytrop (checkpoint
+  // segments that straddle it).
+  const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
+  auto gas_at = [&](auto lower_tag, const GasLane &q, int j, double &taug, double &taur) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
     const int lay = L - 1 - j;
-    gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-    in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
+    if ((j >= jsplit) == LOWER) {
+      gas_finish<LOWER, double>(q, gas_fetch<LOWER, double>(q, T.tab, v, lay), v, lay, taug, taur);
+    } else {
+      const GasLane o = gas_lane<!LOWER>(gp, L);
+      gas_finish<!LOWER, double>(o, gas_fetch<!LOWER, double>(o, T.tab, v, lay), v, lay, taug, taur);
+    }
+  };
+  auto layer_at = [&](auto lower_tag, const GasLane &q, int j, LayerIn &in, double &taug, double &taur) {
+    gas_at(lower_tag, q, j, taug, taur);
+    in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - j, active);
     return total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
   };
 
@@ -520,7 +510,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   // ubar_j = ubar_(j-1) E_(j-1) + zinc (wU_j + wD_j prdnd_j), known as soon as layer j is. ----
   Sweep st = {1.0, 1.0, 0.0};
   double carry = 0.0;
-  for (int j = 0; j < L; j++) {
+  auto f1_layer = [&](auto lower_tag, const GasLane &q, int j) {
     if (active) {
       if (K == 0) {
         sp(j, 0) = st.tdbt, sp(j, 1) = st.ztdn, sp(j, 2) = st.prdnd;
@@ -530,7 +520,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     }
     LayerIn in;
     double taug, taur;
-    LayerV ly = layer_at(j, in, taug, taur);
+    LayerV ly = layer_at(lower_tag, q, j, in, taug, taur);
     if (active && K == 0) sp(j, 3) = taug, sp(j, 4) = taur;
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     double Aa, Bb, Cc, Ee;
@@ -538,6 +528,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     double ub = carry + zinc * (wU[j] + wD[j] * Bb);
     if (active) UB[(size_t)j * NGPT + g] = ub;
     carry = ub * Ee;
+  };
+  {
+    const GasLane q = gas_lane<false>(gp, L);
+    for (int j = 0; j < jsplit; j++) f1_layer(std::integral_constant<bool, false>(), q, j);
+  }
+  {
+    const GasLane q = gas_lane<true>(gp, L);
+    for (int j = jsplit; j < L; j++) f1_layer(std::integral_constant<bool, true>(), q, j);
   }
   if (err) atomicCAS(a.status, 0, err);
 
@@ -563,7 +561,18 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     if (lane < 16 && (lane & 3) == 0) atomicAdd(&albacc[lane >> 2], s);
   }
 
-  // ---- R1: bottom -> top ----
+  // ---- R1: bottom -> top. The lane's gas-optics constants are read from its g-point record
+  // where used: held in registers across this sweep they would not fit beside its state. ----
+  auto gas_rec = [&](int j, double &taug, double &taur) {
+    const int lay = L - 1 - j;
+    if (j >= jsplit) {
+      const GasLane o = gas_lane<true>(gp, L);
+      gas_finish<true, double>(o, gas_fetch<true, double>(o, T.tab, v, lay), v, lay, taug, taur);
+    } else {
+      const GasLane o = gas_lane<false>(gp, L);
+      gas_finish<false, double>(o, gas_fetch<false, double>(o, T.tab, v, lay), v, lay, taug, taur);
+    }
+  };
   for (int j = L - 1; j >= 0; j--) {
     if (K > 0 && (j == L - 1 || (j + 1) % K == 0)) {
       // checkpoint mode: recompute the sweep states of this segment from its checkpoint
@@ -571,9 +580,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
       Sweep ss = {active ? sp(j0 / K, 0) : 1.0, active ? sp(j0 / K, 1) : 1.0, active ? sp(j0 / K, 2) : 0.0};
       for (int jj = j0; jj < j; jj++) {
         if (active) sg(jj - j0, 0) = ss.tdbt, sg(jj - j0, 1) = ss.ztdn, sg(jj - j0, 2) = ss.prdnd;
-        LayerIn in;
         double taug, taur;
-        LayerV ly = layer_at(jj, in, taug, taur);
+        gas_rec(jj, taug, taur);
+        const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - jj, active);
+        LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
         RT4 l = {ly.r, ly.rd, ly.t, ly.td};
         double Aa, Bb, Cc, Ee;
         sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
@@ -587,17 +597,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
       sj = {active ? sg(j % K, 0) : 1.0, active ? sg(j % K, 1) : 1.0, active ? sg(j % K, 2) : 0.0};
     const double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
     const int lay = L - 1 - j;
-    LayerIn in;
     double taug, taur;
-    LayerV ly;
     // gas optics stored (store mode) or recomputed; aerosol, cloud and layer optics recomputed
     // at one call site for both modes (identical rounding)
     if (K == 0)
       taug = active ? sp(j, 3) : 1., taur = active ? sp(j, 4) : 0.1;
     else
-      gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-    in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
-    ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
+      gas_rec(j, taug, taur);
+    const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
+    const LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     // C_j, E_j and U_j from the layer
     Sweep tmp = sj;
@@ -613,7 +621,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 #pragma unroll
     for (int k = 0; k < AC_N; k++) la[k] = 0.0;
     if (active) {
-      gas_optics_adj(gp, T.tab, v, lay, lay < laytrop, xb[0], xb[1], la);
+      if (j >= jsplit)
+        gas_adj_r<true>(gas_lane<true>(gp, L), T.tab, v, lay, xb[0], xb[1], la);
+      else
+        gas_adj_r<false>(gas_lane<false>(gp, L), T.tab, v, lay, xb[0], xb[1], la);
       if (in.cf > 0.0) {
         const double *o = CJ + ((size_t)lay * NBND + gp.ibnd) * CJ_N;
         la[AC_CLWP] = xb[2] * o[3] + xb[3] * o[4] + xb[4] * o[5];

@@ -616,6 +616,7 @@ struct RTx {
   double r, rd, t, td, e2;
 };
 __device__ inline RTx reftra_x(double zto1, double zw, double zg, double prmuz, double rmu) {
+#pragma clang fp contract(off)
   const double zwcrit = 0.9999995, eps = 1.e-08;
   RTx o;
   double zg3 = 3. * zg;
@@ -796,6 +797,7 @@ struct Sweep {
 };
 __device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A, double &B,
                                    double &Cc, double &E) {
+#pragma clang fp contract(off)
   A = s.ztdn;
   B = s.prdnd;
   double a = s.ztdn - s.tdbt;
@@ -908,6 +910,7 @@ __device__ inline void sweep_r(Sweep &s, const RT4 &l, double dbt, double &Cc, d
 struct GasLane {
   uint32_t kb, kjs, kt, kp, ks;   // key table: base, species step, temperature, pressure, row
   int sp1, sp2, spm;              // LDS column offsets of the key pair and the minor absorber
+  int s1, s2, sm;                 // their species indices
   double strrat, kmult, mcoef, rayl;
   bool bin, rayl_js, cont;        // binary key; Rayleigh interpolated in js; continuum present
   uint32_t selfb, selfs, forb, fors, rab, ras;
@@ -917,6 +920,7 @@ struct GasLane {
 // where the interpolation is continuous across the bin edge.
 template <typename R>
 __device__ inline SpecT<R> key_spec_r(R c1, R c2, R strrat, R mult, bool bin) {
+#pragma clang fp contract(off)
   SpecT<R> s;
   R speccomb = c1 + strrat * c2;
   R specparm = c1 * (sizeof(R) == 8 ? R(frcp(double(speccomb))) : R(__builtin_amdgcn_rcpf(float(speccomb))));
@@ -937,9 +941,10 @@ __device__ inline GasLane gas_lane(const GPoint &g, int L) {
   q.kjs = q.bin ? st : 0;
   q.kt = nsp * st;
   q.kp = 5 * nsp * st;
-  q.sp1 = (LOWER ? g.lo_s1 : g.hi_s1) * L;
-  q.sp2 = (LOWER ? g.lo_s2 : g.hi_s2) * L;
-  q.spm = (LOWER ? g.lo_minor : g.hi_minor) * L;
+  q.s1 = LOWER ? g.lo_s1 : g.hi_s1;
+  q.s2 = LOWER ? g.lo_s2 : g.hi_s2;
+  q.sm = LOWER ? g.lo_minor : g.hi_minor;
+  q.sp1 = q.s1 * L, q.sp2 = q.s2 * L, q.spm = q.sm * L;
   q.strrat = LOWER ? g.lo_strrat : g.hi_strrat;
   q.kmult = LOWER ? g.lo_mult : g.hi_mult;
   q.mcoef = LOWER ? g.lo_minor_coef : g.hi_minor_coef;
@@ -971,6 +976,7 @@ struct GasFetch {
 };
 template <bool LOWER, typename R>
 __device__ inline GasFetch<R> gas_fetch(const GasLane &q, const R *__restrict__ tab, ColView &v, int lay) {
+#pragma clang fp contract(off)
   GasFetch<R> f;
   const int jp = uni(LOWER ? min(v.jp[lay], 12) : max(v.jp[lay], 13));
   const int jt = uni(v.jt[lay]), jt1 = uni(v.jt1[lay]);
@@ -996,11 +1002,19 @@ __device__ inline GasFetch<R> gas_fetch(const GasLane &q, const R *__restrict__ 
 template <bool LOWER, typename R>
 __device__ inline void gas_finish(const GasLane &q, const GasFetch<R> &f, ColView &v, int lay, R &taug,
                                   R &taur) {
+  // no implicit contraction: every caller (forward sweep, checkpoint recomputation) rounds alike
+#pragma clang fp contract(off)
   const R fac00 = uni(R(v.f(0, lay))), fac01 = uni(R(v.f(1, lay))), fac10 = uni(R(v.f(2, lay))), fac11 = uni(R(v.f(3, lay)));
   const R colh2o = uni(R(v.c(SP_H2O, lay))), colmol = uni(R(v.c(5, lay)));
   const R fs = f.fs, f0 = R(1) - fs;
-  const R k = f0 * fac00 * f.a0 + fs * fac00 * f.a1 + f0 * fac10 * f.a2 + fs * fac10 * f.a3 +
-              f0 * fac01 * f.b0 + fs * fac01 * f.b1 + f0 * fac11 * f.b2 + fs * fac11 * f.b3;
+  R k = fs * fac11 * f.b3;
+  k = fma(f0 * fac11, f.b2, k);
+  k = fma(fs * fac01, f.b1, k);
+  k = fma(f0 * fac01, f.b0, k);
+  k = fma(fs * fac10, f.a3, k);
+  k = fma(f0 * fac10, f.a2, k);
+  k = fma(fs * fac00, f.a1, k);
+  k = fma(f0 * fac00, f.a0, k);
   R cont = uni(R(v.f(6, lay))) * (f.fo0 + uni(R(v.f(7, lay))) * (f.fo1 - f.fo0));
   if (LOWER) cont = uni(R(v.f(4, lay))) * (f.s0 + uni(R(v.f(5, lay))) * (f.s1 - f.s0)) + cont;
   cont = q.cont ? cont : R(0);
