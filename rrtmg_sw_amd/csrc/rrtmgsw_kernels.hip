@@ -24,13 +24,6 @@
 
 namespace rrtmgsw {
 
-// Layer reflectance/transmittance and direct-beam transmittance with the reference's table.
-__device__ inline RT4 reftra_m(double tau, double w, double g, double mu0, double rmu,
-                               const DeviceTables &T, double &dbt) {
-  dbt = tbl_exp2(tau * rmu, T.exp_tbl, T.rexp_tbl).e;
-  return reftra1(tau, w, g, mu0, rmu, T.exp_tbl, T.rexp_tbl);
-}
-
 // Broadband fluxes at the nlay+1 levels and heating rates (src/rrtmg_sw_rad.f90:805-836,
 // top layer 0) from the per-level sums over g of both waves in LDS.
 __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
@@ -65,333 +58,6 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 }
 
 // ------------------------------------------------------------------------------------------
-// The radiative-transfer kernel.
-// ------------------------------------------------------------------------------------------
-// The mixed FP32/FP64 form (MODE 1). Its FP64 branches are those of the earlier all-FP64
-// kernel; the FP64 forms run in rt_kernel below.
-template <int MODE>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
-rt_kernel_m(RtArgs a, DeviceTables T) {
-  constexpr bool MIXED = MODE == 1;
-  extern __shared__ __attribute__((aligned(16))) char smem[];
-  const int L = a.nlay, C = a.ncol;
-  const int nmw = (L + 31) / 32;
-  ColLayout lo(L, a.fused ? nmw : 0);
-  ColView v(smem, lo);
-  // XCD-aware column order: consecutive columns run on the same XCD (its own L2), so the
-  // strided (ncol, nlay) reads and writes of neighbouring columns share cache lines.
-  const int per = (a.ncol_chunk + 7) / 8;
-  const int cl = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (cl >= a.ncol_chunk) return;
-  const int c = a.col0 + cl;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int g = wave * WAVE_G + lane;
-  const bool active = lane < WAVE_G;
-
-  // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band ----
-  if (tid == 0) v.misc[M_LAYTROP] = 0;
-  if (MIXED)
-    for (int w = tid; w < (L + 31) / 32; w += blockDim.x) v.thin[w] = 0u;
-  __syncthreads();
-  if (MIXED)   // layers (top-based j) thin enough in pressure to need FP64 flux divergences
-    for (int j = tid; j < L; j += blockDim.x) {
-      int lay = L - 1 - j;
-      double pdp = a.plev[c + (size_t)C * lay] - a.plev[c + (size_t)C * (lay + 1)];
-      if (pdp < a.thin_pdp) atomicOr(&v.thin[j >> 5], 1u << (j & 31));
-    }
-  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, c, lay, v, &v.misc[M_LAYTROP]);
-  __syncthreads();
-  const int laytrop = v.misc[M_LAYTROP];
-  if (tid < NBND) {
-    // first g-point of each band: NGS offsets
-    const int first[NBND] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100};
-    GPoint gb = T.gp[first[tid]];
-    v.misc[M_LAYSOL + tid] = find_laysolfr(gb, v, laytrop, L);
-  }
-  int err = 0;
-  if (a.fused && a.sc.icld >= 1 && active) {
-    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
-                       a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
-  }
-  __syncthreads();
-
-  // The lane's g-point record: the FP64 form keeps a register copy (fastest measured); the
-  // other forms read it field by field where used, which saves ~40 VGPRs per lane.
-  GPoint gp_copy;
-  if (MODE == 0) gp_copy = T.gp[active ? g : 0];
-  const GPoint &gp = MODE == 0 ? gp_copy : T.gp[active ? g : 0];
-  const double mu0 = fmax(a.coszen[c], 1.e-10);
-  const bool vis = gp.ibnd >= 9 && gp.ibnd <= 12;
-  const double albdir = vis ? a.asdir[c] : a.aldir[c];
-  const double albdif = vis ? a.asdif[c] : a.aldif[c];
-  const double zinc = active ? incident_flux(gp, T.tab, v, a.sc, mu0) : 0.0;
-
-  // ---- pass 1: top to bottom ----
-  // Per layer and sky the lane stores B = prdnd, C and E of the elimination (slots 0-2 clear,
-  // 3-5 total sky); the A = ztdn term of the downward flux is summed over g here. A sub-column
-  // stays identical to the clear sky until its first cloudy layer (jfirst): above it the total-sky
-  // sweep is not computed and nothing is stored for it.
-  const double rmu = 1.0 / mu0;
-  Sweep sc_ = {1.0, 1.0, 0.0}, st_ = {1.0, 1.0, 0.0};
-  int jfirst = L;                   // first layer (from the top) whose total sky differs
-  double *scr = a.scratch + (size_t)cl * scr_col_doubles(L);
-  for (int j = 0; j < L; j++) {
-    const int lay = L - 1 - j;
-    const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);   // uniform over the workgroup
-    double taug, taur;
-    float taugf = 0.f, taurf = 0.f;
-    if (MIXED && !thin) {
-      gas_optics<float>(gp, T.tab32, v, lay, lay < laytrop, taugf, taurf);
-      taug = taugf, taur = taurf;
-    } else {
-      gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-    }
-    // aerosol (src/rrtmg_sw_rad.f90:709-765)
-    double ta = 0.0, wa = 1.0, ga = 0.0;
-    if (a.sc.iaer == 10) {
-      size_t q = c + (size_t)C * (lay + (size_t)L * gp.ibnd);
-      ta = a.tauaer[q], wa = a.ssaaer[q], ga = a.asmaer[q];
-    } else if (a.sc.iaer == 6) {
-      double t6 = 0.0, w6 = 0.0, g6 = 0.0;
-      for (int ia = 0; ia < NAER; ia++) {
-        double ec = a.ecaer[c + (size_t)C * (lay + (size_t)L * ia)];
-        double rt = T.tab[T.rsrtaua + gp.ibnd + NBND * ia], rp = T.tab[T.rsrpiza + gp.ibnd + NBND * ia];
-        double rg = T.tab[T.rsrasya + gp.ibnd + NBND * ia];
-        t6 = t6 + rt * ec;
-        w6 = w6 + rt * ec * rp;
-        g6 = g6 + rt * ec * rp * rg;
-      }
-      if (t6 == 0.) {
-        ta = 0., ga = 0., wa = 1.;
-      } else {
-        ga = w6 != 0. ? g6 / w6 : g6;
-        wa = w6 / t6;
-        ta = t6;
-      }
-    }
-    // cloud optics of this sub-column (cldprmc_sw) and total-sky optics (:500-505)
-    double cf = 0.0, ct = 0.0, cw = 1.0, cg = 0.0;
-    if (a.sc.icld >= 1 && active) {
-      double ciwp, clwp, tin, win, gin, fin;
-      size_t qg = (size_t)g + NGPT * (c + (size_t)C * lay);
-      if (a.fused) {
-        bool cloudy = (v.mask[(lay >> 5) * NGPT + g] >> (lay & 31)) & 1u;
-        size_t q = c + (size_t)C * lay, qb = gp.ibnd + NBND * q;
-        cf = cloudy ? 1.0 : 0.0;
-        ciwp = cloudy ? a.ciwp[q] : 0.0;
-        clwp = cloudy ? a.clwp[q] : 0.0;
-        tin = cloudy ? a.tauc[qb] : 0.0;
-        win = cloudy ? a.ssac[qb] : 1.0;
-        gin = cloudy ? a.asmc[qb] : 0.0;
-        fin = cloudy ? a.fsfc[qb] : 0.0;
-      } else {
-        cf = a.cldfmcl[qg], ciwp = a.ciwpmcl[qg], clwp = a.clwpmcl[qg];
-        tin = a.taucmcl[qg], win = a.ssacmcl[qg], gin = a.asmcmcl[qg], fin = a.fsfcmcl[qg];
-      }
-      if (cf >= CLDMIN) {
-        double rei = a.fused ? a.rei[c + (size_t)C * lay] : a.reicmcl[c + (size_t)C * lay];
-        double rel = a.fused ? a.rel[c + (size_t)C * lay] : a.relqmcl[c + (size_t)C * lay];
-        CloudOpt co = cloud_optics(T, a.sc, gp.band, cf, ciwp, clwp, rei, rel, tin, win, gin, fin, err);
-        ct = co.tau, cw = co.ssa, cg = co.asm_;
-      } else {
-        ct = tin, cw = win, cg = gin;
-      }
-    }
-    // Mixed form: FP32 layer optics with the co-albedo carried explicitly. Where k mu0 is close to
-    // one the two-stream solution divides by 1 - (k mu0)^2: the reference's removable singularity
-    // turns the exponential-table error into a large reflectance, so the result there depends on
-    // its inputs to the last bits. Such lanes redo the layer in FP64 from the FP64 gas optics.
-    float tauc32 = 0.f, omc32 = 0.f, ccl32 = 0.f, gcc32 = 0.f;
-    bool use64 = !MIXED || thin;
-    if (MIXED && !thin) {
-      float taf = float(ta), waf = float(wa), gaf = float(ga);
-      float tauc = taurf + taugf + taf;
-      float scat = taurf + taf * waf;
-      float coal = taugf + taf * (1.f - waf);
-      float gcc = gaf * waf * taf * rcpf(scat);
-      float rtau = rcpf(tauc);
-      float omc = scat * rtau, ccl = coal * rtau;
-      float zf = gcc * gcc, zwf = omc * zf;
-      float r1 = rcpf(1.f - zwf);
-      tauc = (1.f - zwf) * tauc;
-      omc = (omc - zwf) * r1;
-      ccl = ccl * r1;
-      gcc = gcc * rcpf(1.f + gcc);   // (g - g^2) / (1 - g^2)
-      tauc32 = tauc, omc32 = omc, ccl32 = ccl, gcc32 = gcc;
-      bool sing = near_singular(omc, ccl, gcc, float(mu0));
-      if (cf > REPCLC && !sing) {
-        float ctf = float(ct), cwf = float(cw), cgf = float(cg), ccw = float(1.0 - cw);
-        float scato = tauc * omc + ctf * cwf;
-        float coalo = tauc * ccl + ctf * ccw;
-        float go = (ctf * cwf * cgf + tauc * omc * gcc) * rcpf(scato);
-        float rto = rcpf(tauc + ctf);
-        sing = near_singular(scato * rto, coalo * rto, go, float(mu0));
-      }
-      if (sing) {
-        use64 = true;
-        gas_optics<double>(gp, T.tab, v, lay, lay < laytrop, taug, taur);
-      }
-    }
-    double Ac, At;
-    if (use64) {
-      // clear-sky optics and delta scaling (src/rrtmg_sw_spcvmc.f90:442-496)
-      double tauc = taur + taug + ta;
-      double omc = taur * 1.0 + ta * wa;
-      double gcc = ga * wa * ta / omc;
-      omc = omc / tauc;
-      double zf = gcc * gcc, zwf = omc * zf;
-      tauc = (1.0 - zwf) * tauc;
-      omc = (omc - zwf) / (1.0 - zwf);
-      gcc = (gcc - zf) / (1.0 - zf);
-      double dbc;
-      RT4 rc = reftra_m(tauc, omc, gcc, mu0, rmu, T, dbc);
-      Sweep pre = sc_;
-      double Bc, Cc, Ec;
-      sweep_layer(sc_, rc, dbc, Ac, Bc, Cc, Ec);
-      At = Ac;
-      if (cf > 0.0 && jfirst == L) {
-        jfirst = j;
-        st_ = pre;
-      }
-      if (jfirst <= j) {
-        double tauo = tauc + ct;
-        double omo = tauc * omc + ct * cw;
-        double go = (ct * cw * cg + tauc * omc * gcc) / omo;
-        omo = omo / tauo;
-        RT4 rt = rc;
-        double dbt = dbc;
-        if (cf > 0.0) {
-          double dbo;
-          RT4 ro = reftra_m(tauo, omo, go, mu0, rmu, T, dbo);
-          if (!(cf > REPCLC)) ro = {0.0, 0.0, 1.0, 1.0};
-          double zcl = 1.0 - cf;
-          rt.r = zcl * rc.r + cf * ro.r;
-          rt.rd = zcl * rc.rd + cf * ro.rd;
-          rt.t = zcl * rc.t + cf * ro.t;
-          rt.td = zcl * rc.td + cf * ro.td;
-          dbt = zcl * dbc + cf * dbo;
-        }
-        double Bt, Ct, Et;
-        sweep_layer(st_, rt, dbt, At, Bt, Ct, Et);
-        if (active && MIXED && !thin) {   // FP32 storage of the mixed form's thick layers
-          float *pf = reinterpret_cast<float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
-          pf[3 * NGPT] = float(Bt), pf[4 * NGPT] = float(Ct), pf[5 * NGPT] = float(Et);
-        } else if (active) {
-          double *p = scr + ((size_t)j * SCR_COEFS + 3) * NGPT + g;
-          p[0] = Bt, p[NGPT] = Ct, p[2 * NGPT] = Et;
-        }
-      }
-      if (active && MIXED && !thin) {
-        float *pf = reinterpret_cast<float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
-        pf[0] = float(Bc), pf[NGPT] = float(Cc), pf[2 * NGPT] = float(Ec);
-      } else if (active) {
-        double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
-        p[0] = Bc, p[NGPT] = Cc, p[2 * NGPT] = Ec;
-      }
-    } else {
-      const float tauc = tauc32, omc = omc32, ccl = ccl32, gcc = gcc32;
-      const float mu0f = float(mu0), rmuf = float(rmu);
-      RT4f rc = reftra_f(tauc, omc, ccl, gcc, mu0f, rmuf, T.exp32, T.rexp32);
-      float dbc = tbl_expf(tauc * rmuf, T.exp32, T.rexp32).e;
-      Sweep pre = sc_;
-      SweepF sf = {float(sc_.tdbt), float(sc_.ztdn), float(sc_.prdnd)};
-      float Af, Bc, Cc, Ec;
-      sweep_layer_f(sf, rc, dbc, Af, Bc, Cc, Ec);
-      sc_ = {double(sf.tdbt), double(sf.ztdn), double(sf.prdnd)};
-      Ac = Af, At = Af;
-      if (cf > 0.0 && jfirst == L) {
-        jfirst = j;
-        st_ = pre;
-      }
-      float *pf = reinterpret_cast<float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
-      if (jfirst <= j) {
-        RT4f rt = rc;
-        float dbt = dbc;
-        if (cf > 0.0) {
-          float ctf = float(ct), cwf = float(cw), cgf = float(cg), ccw = float(1.0 - cw);
-          float tauo = tauc + ctf;
-          float scato = tauc * omc + ctf * cwf;
-          float coalo = tauc * ccl + ctf * ccw;
-          float go = (ctf * cwf * cgf + tauc * omc * gcc) * rcpf(scato);
-          float rto = rcpf(tauo);
-          float omo = scato * rto, cco = coalo * rto;
-          float dbo = tbl_expf(tauo * rmuf, T.exp32, T.rexp32).e;
-          RT4f ro = {0.f, 0.f, 1.f, 1.f};
-          if (cf > REPCLC) ro = reftra_f(tauo, omo, cco, go, mu0f, rmuf, T.exp32, T.rexp32);
-          float zcl = float(1.0 - cf), cff = float(cf);
-          rt.r = zcl * rc.r + cff * ro.r;
-          rt.rd = zcl * rc.rd + cff * ro.rd;
-          rt.t = zcl * rc.t + cff * ro.t;
-          rt.td = zcl * rc.td + cff * ro.td;
-          dbt = zcl * dbc + cff * dbo;
-        }
-        SweepF tf = {float(st_.tdbt), float(st_.ztdn), float(st_.prdnd)};
-        float Atf, Bt, Ct, Et;
-        sweep_layer_f(tf, rt, dbt, Atf, Bt, Ct, Et);
-        st_ = {double(tf.tdbt), double(tf.ztdn), double(tf.prdnd)};
-        At = Atf;
-        if (active) pf[3 * NGPT] = Bt, pf[4 * NGPT] = Ct, pf[5 * NGPT] = Et;
-      }
-      if (active) pf[0] = Bc, pf[NGPT] = Cc, pf[2 * NGPT] = Ec;
-    }
-    // sum over g of the incident-flux-weighted ztdn of level j, both skies
-    double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
-    if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
-  }
-  if (jfirst == L) st_ = sc_;
-  if (err) atomicCAS(a.status, 0, err);
-
-  // ---- pass 2: bottom to top ----
-  // surface (src/rrtmg_sw_vrtqdr.f90:163-169 with prup = albdir, prupd = albdif)
-  double uc = (albdir * sc_.tdbt + albdif * (sc_.ztdn - sc_.tdbt)) / (1. - albdif * sc_.prdnd);
-  double dc = sc_.ztdn + sc_.prdnd * uc;
-  double ut = (albdir * st_.tdbt + albdif * (st_.ztdn - st_.tdbt)) / (1. - albdif * st_.prdnd);
-  double dt = st_.ztdn + st_.prdnd * ut;
-  {
-    double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
-    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 2)] = s;
-  }
-  for (int j = L - 1; j >= 0; j--) {
-    double bu_c = 0.0, bu_t = 0.0;
-    const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);
-    if (MIXED && !thin && active) {
-      const float *p = reinterpret_cast<const float *>(scr + (size_t)j * SCR_COEFS * NGPT) + g;
-      float Bc = p[0], Cc = p[NGPT], Ec = p[2 * NGPT];
-      float ucf = Cc + Ec * float(uc);
-      uc = ucf;
-      bu_c = double(Bc * ucf);
-      float utf;
-      if (j >= jfirst) {
-        utf = p[4 * NGPT] + p[5 * NGPT] * float(ut);
-        bu_t = double(p[3 * NGPT] * utf);
-      } else {
-        utf = Cc + Ec * float(ut);
-        bu_t = double(Bc * utf);
-      }
-      ut = utf;
-    } else if (active) {
-      const double *p = scr + (size_t)j * SCR_COEFS * NGPT + g;
-      double Bc = p[0], Cc = p[NGPT], Ec = p[2 * NGPT];
-      uc = Cc + Ec * uc;
-      bu_c = Bc * uc;
-      if (j >= jfirst) {
-        const double *q = p + 3 * NGPT;
-        ut = q[NGPT] + q[2 * NGPT] * ut;
-        bu_t = q[0] * ut;
-      } else {
-        ut = Cc + Ec * ut;
-        bu_t = Bc * ut;
-      }
-    }
-    // downward flux = sum(zinc ztdn) (pass 1) + sum(zinc prdnd U)
-    double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
-    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 2)] = s;
-  }
-  __syncthreads();
-  rt_epilogue(a, v, c);
-}
-
-// ------------------------------------------------------------------------------------------
 // FP64 forms: MODE 0 = the reference's exponential table, MODE 2 = exact exponentials (the
 // smooth forward the adjoint differentiates). FUSED: McICA masks generated here from the
 // physical cloud inputs; otherwise the materialised (112, ncol, nlay) sub-column arrays are read.
@@ -409,6 +75,7 @@ template <int MODE, bool FUSED>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && FUSED ? 4 : 3)))
 rt_kernel(RtArgs a, DeviceTables T) {
   constexpr bool EXACT = MODE == 2;
+  constexpr bool MIXED = MODE == 1;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
   ColLayout lo(L, FUSED ? (L + 31) / 32 : 0);
@@ -422,11 +89,21 @@ rt_kernel(RtArgs a, DeviceTables T) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = wave * WAVE_G + lane;
   const bool active = lane < WAVE_G;
-  const int gi = active ? g : 0;
+  // Idle lanes (56-63) mirror their own wave's first g-point: every scratch location a lane
+  // reads in pass 2 is written by its own wave in pass 1 (there is no barrier between the passes).
+  const int gi = active ? g : wave * WAVE_G;
 
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
   if (tid == 0) v.misc[M_LAYTROP] = 0;
+  if (MIXED)
+    for (int w = tid; w < (L + 31) / 32; w += blockDim.x) v.thin[w] = 0u;
   __syncthreads();
+  if (MIXED)   // layers (top-based j) thin enough in pressure to need FP64 flux divergences
+    for (int j = tid; j < L; j += blockDim.x) {
+      const int lay = L - 1 - j;
+      const double pdp = a.plev[c + (size_t)C * lay] - a.plev[c + (size_t)C * (lay + 1)];
+      if (pdp < a.thin_pdp) atomicOr(&v.thin[j >> 5], 1u << (j & 31));
+    }
   for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, c, lay, v, &v.misc[M_LAYTROP]);
   __syncthreads();
   const int laytrop = v.misc[M_LAYTROP];
@@ -477,27 +154,33 @@ rt_kernel(RtArgs a, DeviceTables T) {
   double *scr = a.scratch + (size_t)cl * scr_col_doubles(L) + gi;
   // A layer's coefficients are stored once the next layer's table reads have been issued, so that
   // the waits for those reads do not include the stores (the vector memory counter retires in
-  // issue order).
+  // issue order). The mixed form stores its thick layers' coefficients as FP32.
   double pend[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int pend_j = -1;
-  bool pend_t = false;
+  bool pend_t = false, pend_f = false;
   auto flush = [&]() {
     if (active && pend_j >= 0) {
-      double *p = scr + (size_t)pend_j * SCR_COEFS * NGPT;
-      p[0] = pend[0], p[NGPT] = pend[1], p[2 * NGPT] = pend[2];
-      if (pend_t) p[3 * NGPT] = pend[3], p[4 * NGPT] = pend[4], p[5 * NGPT] = pend[5];
+      if (MIXED && pend_f) {
+        float *p = reinterpret_cast<float *>(scr - gi + (size_t)pend_j * SCR_COEFS * NGPT) + gi;
+        p[0] = float(pend[0]), p[NGPT] = float(pend[1]), p[2 * NGPT] = float(pend[2]);
+        if (pend_t) p[3 * NGPT] = float(pend[3]), p[4 * NGPT] = float(pend[4]), p[5 * NGPT] = float(pend[5]);
+      } else {
+        double *p = scr + (size_t)pend_j * SCR_COEFS * NGPT;
+        p[0] = pend[0], p[NGPT] = pend[1], p[2 * NGPT] = pend[2];
+        if (pend_t) p[3 * NGPT] = pend[3], p[4 * NGPT] = pend[4], p[5 * NGPT] = pend[5];
+      }
     }
   };
-  auto layer = [&](auto lower_tag, const GasLane &q, const GasFetch<double> &gf, int j) {
-    constexpr bool LOWER = decltype(lower_tag)::value;
-    const int lay = L - 1 - j;
-    double taug, taur;
-    gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
-    // aerosol (src/rrtmg_sw_rad.f90:709-765)
-    double ta = 0.0, wa = 1.0, ga = 0.0;
+  // aerosol (src/rrtmg_sw_rad.f90:709-765) and this sub-column's cloud optics (cldprmc_sw) and
+  // total-sky inputs (src/rrtmg_sw_spcvmc.f90:500-505)
+  struct LayIn {
+    double ta, wa, ga, cf, ct, cw, cg;
+  };
+  auto inputs = [&](int lay) {
+    LayIn in = {0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0};
     if (a.sc.iaer == 10) {
       size_t qa = c + (size_t)C * (lay + (size_t)L * ibnd);
-      ta = a.tauaer[qa], wa = a.ssaaer[qa], ga = a.asmaer[qa];
+      in.ta = a.tauaer[qa], in.wa = a.ssaaer[qa], in.ga = a.asmaer[qa];
     } else if (a.sc.iaer == 6) {
       double t6 = 0.0, w6 = 0.0, g6 = 0.0;
       for (int ia = 0; ia < NAER; ia++) {
@@ -509,36 +192,40 @@ rt_kernel(RtArgs a, DeviceTables T) {
         g6 = g6 + rt * ec * rp * rg;
       }
       if (t6 == 0.) {
-        ta = 0., ga = 0., wa = 1.;
+        in.ta = 0., in.ga = 0., in.wa = 1.;
       } else {
-        ga = w6 != 0. ? g6 / w6 : g6;
-        wa = w6 / t6;
-        ta = t6;
+        in.ga = w6 != 0. ? g6 / w6 : g6;
+        in.wa = w6 / t6;
+        in.ta = t6;
       }
     }
-    // cloud optics of this sub-column (cldprmc_sw) and total-sky inputs (:500-505)
-    double cf = 0.0, ct = 0.0, cw = 1.0, cg = 0.0;
     if (FUSED && a.sc.icld >= 1 && active) {
       if ((v.mask[(lay >> 5) * NGPT + g] >> (lay & 31)) & 1u) {
         const double *o = csc + ((size_t)lay * NBND + ibnd) * 3;
-        cf = 1.0, ct = o[0], cw = o[1], cg = o[2];
+        in.cf = 1.0, in.ct = o[0], in.cw = o[1], in.cg = o[2];
       }
     } else if (!FUSED && a.sc.icld >= 1 && active) {
       const size_t q1 = c + (size_t)C * lay, qg = (size_t)g + NGPT * q1;
-      cf = a.cldfmcl[qg];
+      in.cf = a.cldfmcl[qg];
       const double ciwp = a.ciwpmcl[qg], clwp = a.clwpmcl[qg];
       const double tin = a.taucmcl[qg], win = a.ssacmcl[qg], gin = a.asmcmcl[qg], fin = a.fsfcmcl[qg];
-      if (cf >= CLDMIN) {
-        CloudOpt co = cloud_optics(T, a.sc, band, cf, ciwp, clwp, a.reicmcl[q1], a.relqmcl[q1], tin, win,
-                                   gin, fin, err);
-        ct = co.tau, cw = co.ssa, cg = co.asm_;
+      if (in.cf >= CLDMIN) {
+        CloudOpt co = cloud_optics(T, a.sc, band, in.cf, ciwp, clwp, a.reicmcl[q1], a.relqmcl[q1], tin,
+                                   win, gin, fin, err);
+        in.ct = co.tau, in.cw = co.ssa, in.cg = co.asm_;
       } else {
-        ct = tin, cw = win, cg = gin;
+        in.ct = tin, in.cw = win, in.cg = gin;
       }
     }
-    // clear-sky optics and delta scaling (src/rrtmg_sw_spcvmc.f90:442-496):
-    // g0 = ga wa ta / s and w0 = s / tau0 share 1 / (s tau0); omc = (w0 - w0 f)/(1 - w0 f) and
-    // gcc = (g0 - f)/(1 - f) = g0 / (1 + g0) share 1 / ((1 - w0 f)(1 + g0)), f = g0^2.
+    return in;
+  };
+  // The FP64 layer: clear-sky optics and delta scaling (src/rrtmg_sw_spcvmc.f90:442-496):
+  // g0 = ga wa ta / s and w0 = s / tau0 share 1 / (s tau0); omc = (w0 - w0 f)/(1 - w0 f) and
+  // gcc = (g0 - f)/(1 - f) = g0 / (1 + g0) share 1 / ((1 - w0 f)(1 + g0)), f = g0^2; reftra;
+  // total sky; the elimination steps. Leaves the coefficients in pend and returns the ztdn
+  // terms (total, clear) of the level's downward flux.
+  auto body64 = [&](int j, double taug, double taur, const LayIn &in, double &At, double &Ac) {
+    const double ta = in.ta, wa = in.wa, ga = in.ga, cf = in.cf, ct = in.ct, cw = in.cw, cg = in.cg;
     const double tau0 = taur + taug + ta;
     const double sct = taur * 1.0 + ta * wa;
     const double r1 = frcp(sct * tau0);
@@ -560,14 +247,16 @@ rt_kernel(RtArgs a, DeviceTables T) {
       rc = reftra_e(tauc, omc, gcc, mu0, e2, T.exp_tbl);
       dbc = e2;
     }
-    const double Ac = sc_.ztdn, Bc = sc_.prdnd;
+    Ac = sc_.ztdn;
+    const double Bc = sc_.prdnd;
     if (cf > 0.0 && jfirst == L) {
       jfirst = j;
       st_ = sc_;
     }
     double Cc, Ec;
     sweep_r(sc_, rc, dbc, Cc, Ec);
-    double At = Ac, Bt = Bc, Ct = Cc, Et = Ec;
+    At = Ac;
+    double Bt = Bc, Ct = Cc, Et = Ec;
     if (jfirst <= j) {
       At = st_.ztdn, Bt = st_.prdnd;
       RT4 rt = rc;
@@ -601,7 +290,105 @@ rt_kernel(RtArgs a, DeviceTables T) {
       sweep_r(st_, rt, dbt, Ct, Et);
     }
     pend[0] = Bc, pend[1] = Cc, pend[2] = Ec, pend[3] = Bt, pend[4] = Ct, pend[5] = Et;
-    pend_j = j, pend_t = jfirst <= j;
+  };
+  // The mixed form's FP32 layer (thick layers): optics with the co-albedo carried explicitly,
+  // reftra_f and the FP32 elimination step on the FP64 sweep state. Where k mu0 is close to one
+  // the two-stream solution divides by 1 - (k mu0)^2, a removable singularity that turns the
+  // exponential table's steps into a large reflectance: such lanes redo the layer in FP64 from
+  // FP64 gas optics.
+  auto body32 = [&](auto lower_tag, const GasLane &q, int j, float taugf, float taurf, const LayIn &in,
+                    double &At, double &Ac) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
+    const int lay = L - 1 - j;
+    const float taf = float(in.ta), waf = float(in.wa), gaf = float(in.ga);
+    float tauc = taurf + taugf + taf;
+    const float scat = taurf + taf * waf;
+    const float coal = taugf + taf * (1.f - waf);
+    float gcc = gaf * waf * taf * rcpf(scat);
+    const float rtau = rcpf(tauc);
+    float omc = scat * rtau, ccl = coal * rtau;
+    const float zf = gcc * gcc, zwf = omc * zf;
+    const float r1 = rcpf(1.f - zwf);
+    tauc = (1.f - zwf) * tauc;
+    omc = (omc - zwf) * r1;
+    ccl = ccl * r1;
+    gcc = gcc * rcpf(1.f + gcc);   // (g - g^2) / (1 - g^2)
+    const float mu0f = float(mu0);
+    bool sing = near_singular(omc, ccl, gcc, mu0f);
+    const float ctf = float(in.ct), cwf = float(in.cw), cgf = float(in.cg), ccw = float(1.0 - in.cw);
+    float tauo = 0.f, omo = 0.f, cco = 0.f, go = 0.f;
+    if (in.cf > REPCLC) {
+      tauo = tauc + ctf;
+      const float scato = tauc * omc + ctf * cwf;
+      const float coalo = tauc * ccl + ctf * ccw;
+      go = (ctf * cwf * cgf + tauc * omc * gcc) * rcpf(scato);
+      const float rto = rcpf(tauo);
+      omo = scato * rto, cco = coalo * rto;
+      sing = sing || near_singular(omo, cco, go, mu0f);
+    }
+    if (sing) {
+      double taug, taur;
+      gas_finish<LOWER, double>(q, gas_fetch<LOWER, double>(q, T.tab, v, lay), v, lay, taug, taur);
+      body64(j, taug, taur, in, At, Ac);
+      return;
+    }
+    const float rmuf = float(rmu);
+    const RT4f rc = reftra_f(tauc, omc, ccl, gcc, mu0f, rmuf, T.exp32, T.rexp32);
+    const float dbc = tbl_expf(tauc * rmuf, T.exp32, T.rexp32).e;
+    Ac = sc_.ztdn;
+    const double Bc = sc_.prdnd;
+    if (in.cf > 0.0 && jfirst == L) {
+      jfirst = j;
+      st_ = sc_;
+    }
+    SweepF sf = {float(sc_.tdbt), float(sc_.ztdn), float(sc_.prdnd)};
+    float Af, Bf, Cc, Ec;
+    sweep_layer_f(sf, rc, dbc, Af, Bf, Cc, Ec);
+    sc_ = {double(sf.tdbt), double(sf.ztdn), double(sf.prdnd)};
+    At = Ac;
+    double Bt = Bc;
+    float Ct = Cc, Et = Ec;
+    if (jfirst <= j) {
+      At = st_.ztdn, Bt = st_.prdnd;
+      RT4f rt = rc;
+      float dbt = dbc;
+      if (in.cf > 0.0) {
+        const float dbo = tbl_expf(tauo * rmuf, T.exp32, T.rexp32).e;
+        RT4f ro = {0.f, 0.f, 1.f, 1.f};
+        if (in.cf > REPCLC) ro = reftra_f(tauo, omo, cco, go, mu0f, rmuf, T.exp32, T.rexp32);
+        const float zcl = float(1.0 - in.cf), cff = float(in.cf);
+        rt.r = zcl * rc.r + cff * ro.r;
+        rt.rd = zcl * rc.rd + cff * ro.rd;
+        rt.t = zcl * rc.t + cff * ro.t;
+        rt.td = zcl * rc.td + cff * ro.td;
+        dbt = zcl * dbc + cff * dbo;
+      }
+      SweepF tf = {float(st_.tdbt), float(st_.ztdn), float(st_.prdnd)};
+      float Atf, Btf;
+      sweep_layer_f(tf, rt, dbt, Atf, Btf, Ct, Et);
+      st_ = {double(tf.tdbt), double(tf.ztdn), double(tf.prdnd)};
+    }
+    pend[0] = Bc, pend[1] = Cc, pend[2] = Ec, pend[3] = Bt, pend[4] = Ct, pend[5] = Et;
+  };
+  auto layer = [&](auto lower_tag, const GasLane &q, int j) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
+    const int lay = L - 1 - j;
+    const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);   // uniform over the workgroup
+    double At, Ac;
+    if (MIXED && !thin) {
+      const GasFetch<float> gf = gas_fetch<LOWER, float>(q, T.tab32, v, lay);
+      flush();
+      float taugf, taurf;
+      gas_finish<LOWER, float>(q, gf, v, lay, taugf, taurf);
+      body32(lower_tag, q, j, taugf, taurf, inputs(lay), At, Ac);
+    } else {
+      const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
+      flush();
+      double taug, taur;
+      gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
+      body64(j, taug, taur, inputs(lay), At, Ac);
+    }
+    pend_j = j, pend_t = jfirst <= j, pend_f = MIXED && !thin;
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
     const double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
     if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
@@ -612,11 +399,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
   auto regime = [&](auto lower_tag, int j0, int j1) {
     constexpr bool LOWER = decltype(lower_tag)::value;
     const GasLane q = gas_lane<LOWER>(gp, L);
-    for (int j = j0; j < j1; j++) {
-      const GasFetch<double> cur = gas_fetch<LOWER, double>(q, T.tab, v, L - 1 - j);
-      flush();
-      layer(lower_tag, q, cur, j);
-    }
+    for (int j = j0; j < j1; j++) layer(lower_tag, q, j);
   };
   regime(std::integral_constant<bool, false>(), 0, jsplit);
   regime(std::integral_constant<bool, true>(), jsplit, L);
@@ -646,10 +429,16 @@ rt_kernel(RtArgs a, DeviceTables T) {
   auto fetch = [&](int j) {
     Co o;
     const int jj = j < 0 ? 0 : j;
-    const double *p = scr + (size_t)jj * SCR_COEFS * NGPT;
-    o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
     o.tB = 0.0, o.tC = 0.0, o.tE = 0.0;
-    if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
+    if (MIXED && !((v.thin[jj >> 5] >> (jj & 31)) & 1u)) {   // FP32 coefficients (uniform branch)
+      const float *p = reinterpret_cast<const float *>(scr - gi + (size_t)jj * SCR_COEFS * NGPT) + gi;
+      o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
+      if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
+    } else {
+      const double *p = scr + (size_t)jj * SCR_COEFS * NGPT;
+      o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
+      if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
+    }
     return o;
   };
   auto step = [&](int j, const Co &o) {
@@ -683,13 +472,6 @@ rt_kernel(RtArgs a, DeviceTables T) {
   rt_epilogue(a, v, c);
 }
 
-template <int MODE>
-static hipError_t launch_rt_m(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  ColLayout lo(a.nlay, a.fused ? (a.nlay + 31) / 32 : 0);
-  int per = (a.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL(rt_kernel_m<MODE>, dim3(per * 8), dim3(128), lo.total, s, a, t);
-  return hipGetLastError();
-}
 template <int MODE, bool FUSED>
 static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   ColLayout lo(a.nlay, FUSED ? (a.nlay + 31) / 32 : 0);
@@ -701,7 +483,8 @@ hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s)
   return a.fused ? launch_rt<0, true>(a, t, s) : launch_rt<0, false>(a, t, s);
 }
 hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  return launch_rt_m<1>(a, t, s);
+  // the mixed form is offered by the fused entry point only
+  return a.fused ? launch_rt<1, true>(a, t, s) : hipErrorInvalidValue;
 }
 hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   // the exact-exponential form is offered by the fused entry point only
