@@ -35,8 +35,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the timed kernel on mcica8192x72, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over this script, each scaled by what the counter reads for a 2 GiB device copy
 # (x2.0 and x1.0); profiles/pmc_rt_kernel_adj_kernel_mcica8192x72_expran4096x60.md.
-MEASURED_TRAFFIC = {"fp64": {"read": 1.993e9, "write": 1.905e9},
-                    "mixed": {"read": 1.645e9, "write": 1.548e9}}
+MEASURED_TRAFFIC = {"fp64": {"read": 2.064e9, "write": 1.911e9},
+                    "mixed": {"read": 1.623e9, "write": 1.521e9}}
 
 
 def cpu_baseline(cols_full, budget_s=12.0, chunk=16):
@@ -101,6 +101,31 @@ def adjoint_ratio(args, name="expran4096x60"):
     res["adjoint_over_forward_fp64"] = res["adjoint_ckpt0_ms"] / res["forward_fp64_ms"]
     res["adjoint_over_forward_exact"] = res["adjoint_ckpt0_ms"] / res["forward_exact_ms"]
     return res
+
+
+def nomcica_form(args, name="overcast8192x72"):
+    """The non-McICA driver (rrtmgsw_sw_nomcica_dev: layer cloud fractions, cldprop_sw optics) on
+    the same columns with every cloudy layer overcast, FP64, device-resident inputs."""
+    from rrtmg_sw_amd import api
+    from rrtmg_sw_amd.workloads import workload
+    cols = workload(name)
+    d = api.device_inputs(cols)
+    kw = dict(ncol=cols.ncol, nlay=cols.nlay, icld=cols.icld, iaer=cols.iaer, play=d["play"],
+              plev=d["plev"], tlay=d["tlay"], tlev=None, tsfc=None, h2ovmr=d["h2ovmr"],
+              o3vmr=d["o3vmr"], co2vmr=d["co2vmr"], ch4vmr=d["ch4vmr"], n2ovmr=None,
+              o2vmr=d["o2vmr"], asdir=d["asdir"], asdif=d["asdif"], aldir=d["aldir"],
+              aldif=d["aldif"], coszen=d["coszen"], adjes=cols.adjes, dyofyr=cols.dyofyr,
+              scon=cols.scon, isolvar=cols.isolvar, inflgsw=cols.inflgsw,
+              iceflgsw=cols.iceflgsw, liqflgsw=cols.liqflgsw, cldfr=d["cldfrac"],
+              taucld=d["tauc"], ssacld=d["ssac"], asmcld=d["asmc"], fsfcld=d["fsfc"],
+              cicewp=d["ciwp"], cliqwp=d["clwp"], reice=d["rei"], reliq=d["rel"], tauaer=None,
+              ssaaer=None, asmaer=None, ecaer=None)
+    ms = _time(lambda: api.rrtmg_sw_nomcica(**kw), args.steps, args.warmup)
+    # per call: six output tensors allocated and zeroed, the kernel, the status read-back
+    return {"workload": f"{name}: rrtmg_sw of the non-McICA driver (cldprop_sw, spcvrt_sw), "
+                        f"{cols.ncol} columns x {cols.nlay} layers, every cloudy layer overcast",
+            "ms_per_step": ms, "columns_per_s": cols.ncol / (ms * 1e-3),
+            "kernel": "rt_kernel<0, 2>"}
 
 
 def bench_adjoint(args):
@@ -217,7 +242,7 @@ def main():
                                  if args.workload == "mcica8192x72" else None),
                      "traffic_read_write": MEASURED_TRAFFIC[args.precision]
                                            if args.workload == "mcica8192x72" else None,
-                     "kernel": f"rt_kernel<{0 if args.precision == 'fp64' else 1}>",
+                     "kernel": f"rt_kernel<{0 if args.precision == 'fp64' else 1}, 1>",
                      "note": "FP64 VALU work (no dense contraction): algorithmic FLOPs of "
                              "rrtmg_sw_amd/flops.py over the per-step event time on the launch "
                              "stream; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s",
@@ -233,6 +258,7 @@ def main():
         },
     }
     line["adjoint"] = adjoint_ratio(args)
+    line["nomcica"] = nomcica_form(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cols)
     print(json.dumps(line), flush=True)

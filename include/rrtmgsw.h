@@ -32,6 +32,7 @@ extern "C" {
 #define RRTMGSW_ERR_INFLAG 3       /* "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA" */
 #define RRTMGSW_ERR_RADIUS 4       /* ice / liquid particle size out of the table bounds */
 #define RRTMGSW_ERR_OPTICS 5       /* cloud optical property outside its physical limits */
+#define RRTMGSW_ERR_PARTIAL_CLOUD 6 /* "PARTIAL CLOUD NOT ALLOWED" (non-McICA driver) */
 #define RRTMGSW_ERR_ICLD 10        /* "MCICA_SUBCOL: INVALID ICLD" */
 #define RRTMGSW_ERR_KISS_SEED 11   /* "... REQUIRES PMID FROM BOTTOM FOUR LAYERS." */
 #define RRTMGSW_ERR_HIP 20         /* a HIP runtime call failed */
@@ -87,6 +88,47 @@ int rrtmgsw_sw_dev(int ncol, int nlay, int *icld, int *iaer,
                    const double *ecaer, double *swuflx, double *swdflx, double *swhr,
                    double *swuflxc, double *swdflxc, double *swhrc, const double *bndsolvar,
                    const double *indsolvar, const double *solcycfrac, void *stream);
+
+/* Replaces rrtmg_sw(...) of the non-McICA driver, src/rrtmg_sw_rad.nomcica.f90:97-111
+ * (dummies :209-376; cldprop_sw, spcvrt_sw). Layer clouds: cldfr, cicewp, cliqwp, reice,
+ * reliq (ncol, nlay); taucld, ssacld, asmcld, fsfcld (nbndsw = 14, ncol, nlay). A layer cloud
+ * fraction strictly between 1e-6 and 1 - 1e-6 returns RRTMGSW_ERR_PARTIAL_CLOUD (the
+ * reference's STOP 'PARTIAL CLOUD NOT ALLOWED', :617-622). icld outside 0..3 becomes 2 and iaer
+ * outside {0, 6, 10} becomes 0, written back (:560, :568). Host arrays, as rrtmgsw_sw. */
+int rrtmgsw_sw_nomcica(int ncol, int nlay, int *icld, int *iaer,
+                       const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc, const double *h2ovmr,
+                       const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                       const double *n2ovmr, const double *o2vmr, const double *asdir,
+                       const double *asdif, const double *aldir, const double *aldif,
+                       const double *coszen, double adjes, int dyofyr, double scon, int isolvar,
+                       int inflgsw, int iceflgsw, int liqflgsw, const double *cldfr,
+                       const double *taucld, const double *ssacld, const double *asmcld,
+                       const double *fsfcld, const double *cicewp, const double *cliqwp,
+                       const double *reice, const double *reliq, const double *tauaer,
+                       const double *ssaaer, const double *asmaer, const double *ecaer,
+                       double *swuflx, double *swdflx, double *swhr, double *swuflxc,
+                       double *swdflxc, double *swhrc, const double *bndsolvar,
+                       const double *indsolvar, const double *solcycfrac);
+
+/* Same as rrtmgsw_sw_nomcica with device pointers (inputs and outputs resident in HBM); the
+ * optional solar-variability inputs stay host pointers (NULL = absent). */
+int rrtmgsw_sw_nomcica_dev(int ncol, int nlay, int *icld, int *iaer,
+                           const double *play, const double *plev, const double *tlay,
+                           const double *tlev, const double *tsfc, const double *h2ovmr,
+                           const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                           const double *n2ovmr, const double *o2vmr, const double *asdir,
+                           const double *asdif, const double *aldir, const double *aldif,
+                           const double *coszen, double adjes, int dyofyr, double scon,
+                           int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                           const double *cldfr, const double *taucld, const double *ssacld,
+                           const double *asmcld, const double *fsfcld, const double *cicewp,
+                           const double *cliqwp, const double *reice, const double *reliq,
+                           const double *tauaer, const double *ssaaer, const double *asmaer,
+                           const double *ecaer, double *swuflx, double *swdflx, double *swhr,
+                           double *swuflxc, double *swdflxc, double *swhrc,
+                           const double *bndsolvar, const double *indsolvar,
+                           const double *solcycfrac, void *stream);
 
 /* Replaces mcica_subcol_sw(...), src/mcica_subcol_gen_sw.f90:181-185 (dummies :200-246).
  * Writes the seven (112, ncol, nlay) sub-column arrays and reicmcl / relqmcl (ncol, nlay).

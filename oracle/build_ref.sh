@@ -83,3 +83,31 @@ echo "  LD librrtmg_ref.so"
 $FC -shared -o "$OUT/librrtmg_ref.so.tmp" "${OBJS[@]}"
 mv "$OUT/librrtmg_ref.so.tmp" "$OUT/librrtmg_ref.so"
 echo "build_ref: $OUT/librrtmg_ref.so"
+
+# Non-McICA driver: src/rrtmg_sw_rad.nomcica.f90 declares module rrtmg_sw_rad like the McICA
+# driver, so it and its wrapper are compiled with their module files in obj/nomcica and linked
+# into a library of their own with the shared modules, tables and routines.
+NM="$OBJ/nomcica"
+mkdir -p "$NM"
+compile "$REF/src/rrtmg_sw_cldprop.f90" "$OBJ/rrtmg_sw_cldprop.o"
+compile "$REF/src/rrtmg_sw_spcvrt.f90" "$OBJ/rrtmg_sw_spcvrt.o"
+compile_nm() { # src obj
+  if [ ! -f "$2" ] || [ "$1" -nt "$2" ]; then
+    echo "  FC $(basename "$1") (non-McICA)"
+    (cd "$NM" && $FC $FFLAGS -module-dir "$NM" -I "$NM" -I "$OBJ" -c "$1" -o "$2")
+  fi
+}
+compile_nm "$REF/src/rrtmg_sw_rad.nomcica.f90" "$NM/rrtmg_sw_rad_nomcica.o"
+compile_nm "$HERE/ref_capi_nomcica.F90" "$NM/ref_capi_nomcica.o"
+NMOBJS=()
+for o in "${OBJS[@]}"; do
+  case "$(basename "$o")" in
+    rrtmg_sw_rad.o|ref_capi.o|mcica_subcol_gen_sw.o|rrtmg_sw_spcvmc.o|rrtmg_sw_cldprmc.o) ;;
+    *) NMOBJS+=("$o") ;;
+  esac
+done
+NMOBJS+=("$OBJ/rrtmg_sw_cldprop.o" "$OBJ/rrtmg_sw_spcvrt.o" "$NM/rrtmg_sw_rad_nomcica.o" "$NM/ref_capi_nomcica.o")
+echo "  LD librrtmg_ref_nomcica.so"
+$FC -shared -Wl,-Bsymbolic -o "$OUT/librrtmg_ref_nomcica.so.tmp" "${NMOBJS[@]}"
+mv "$OUT/librrtmg_ref_nomcica.so.tmp" "$OUT/librrtmg_ref_nomcica.so"
+echo "build_ref: $OUT/librrtmg_ref_nomcica.so"

@@ -1405,8 +1405,8 @@ int orc_rrtmg_sw_ext(int ncol, int nlay, int *icld, int *iaer, const double *pla
                      double *bandd, double *dirdflx, int idelm) {
   (void)tsfc; (void)tlev; (void)n2ovmr; /* feed only unused setcoef locals / never reach taumol */
   if (!g_ready) return fail(9, "orc_init not called");
-  if (*icld < 0 || *icld > 3) *icld = 2;                   /* :589 */
-  if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0; /* :597 */
+  if (*icld < 0 || *icld > 3) *icld = 2;                   /* :560 */
+  if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0; /* :568 */
   const int N = ncol, L = nlay, n1 = nlay + 1;
   const double amd = 28.9660, amw = 18.0160;
   solvar_t sv;
@@ -1533,6 +1533,322 @@ int orc_rrtmg_sw_ext(int ncol, int nlay, int *icld, int *iaer, const double *pla
   free(zcldfmc), free(ztaucmc), free(ztaormc), free(zasycmc), free(zomgcmc), free(ztaua), free(zasya), free(zomga);
   for (int k = 0; k < 5; k++) free(pbb[k]);
   free(bu), free(bd);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Non-McICA GCM path: cldprop_sw (src/rrtmg_sw_cldprop.f90:53-365) and the driver
+ * rrtmg_sw of src/rrtmg_sw_rad.nomcica.f90:97-816 (inatm_sw :846-1615).
+ * ---------------------------------------------------------------------------------- */
+
+/* cldprop_sw for one column: cldfrac, ciwp, clwp, rei, rel (nlay); tauc, ssac, asmc, fsfc
+ * (14, nlay); outputs (nlay, 14) for bands 16..29 (column b = band - 16). */
+static int cldprop(int nlay, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                   const double *tauc, const double *ssac, const double *asmc, const double *fsfc,
+                   const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                   double *taucldorig, double *taucloud, double *ssacloud, double *asmcloud) {
+  const double eps = 1.e-06;
+  double extcoice[NBND], ssacoice[NBND], gice[NBND], forwice[NBND];
+  double extcoliq[NBND], ssacoliq[NBND], gliq[NBND], forwliq[NBND];
+  for (int b = 0; b < NBND; b++)
+    extcoice[b] = ssacoice[b] = gice[b] = forwice[b] = extcoliq[b] = ssacoliq[b] = gliq[b] = forwliq[b] = 0.;
+  for (int lay = 0; lay < nlay; lay++) { /* :141-148 */
+    double tauctot = 0.;
+    for (int b = 0; b < NBND; b++) {
+      taucldorig[lay + nlay * b] = tauc[b + NBND * lay];
+      taucloud[lay + nlay * b] = 0.0;
+      ssacloud[lay + nlay * b] = 1.0;
+      asmcloud[lay + nlay * b] = 0.0;
+      tauctot = tauctot + tauc[b + NBND * lay];
+    }
+    double cwp = ciwp[lay] + clwp[lay]; /* :153-155 */
+    if (!(cldfrac[lay] >= CLDMIN && (cwp >= CLDMIN || tauctot >= CLDMIN))) continue;
+    if (inflag == 0) { /* :161-174 */
+      for (int b = 0; b < NBND; b++) {
+        int q = b + NBND * lay, t = lay + nlay * b;
+        double taucldorig_a = tauc[q], ffp = fsfc[q], ffp1 = 1.0 - ffp;
+        double ffpssa = 1.0 - ffp * ssac[q];
+        double ssacloud_a = ffp1 * ssac[q] / ffpssa;
+        double taucloud_a = ffpssa * taucldorig_a;
+        taucldorig[t] = taucldorig_a;
+        ssacloud[t] = ssacloud_a;
+        taucloud[t] = taucloud_a;
+        asmcloud[t] = (asmc[q] - ffp) / (ffp1);
+      }
+      continue;
+    }
+    if (inflag != 2) continue; /* no other option: the layer keeps the clear defaults */
+    double radice = rei[lay];
+    if (ciwp[lay] == 0.0) { /* :182-189 */
+      for (int b = 0; b < NBND; b++) extcoice[b] = ssacoice[b] = gice[b] = forwice[b] = 0.;
+    } else if (iceflag == 1) { /* :193-221 */
+      if (radice < 13.0 || radice > 130.) return fail(4, "ICE RADIUS OUT OF BOUNDS");
+      for (int b = 0; b < NBND; b++) {
+        double w2 = WAVENUM2[b];
+        int icx = 0;
+        if (w2 > 1.43e04) icx = 1;
+        else if (w2 > 7.7e03) icx = 2;
+        else if (w2 > 5.3e03) icx = 3;
+        else if (w2 > 4.0e03) icx = 4;
+        else if (w2 >= 2.5e03) icx = 5;
+        extcoice[b] = ABARI[icx - 1] + BBARI[icx - 1] / radice;
+        ssacoice[b] = 1. - CBARI[icx - 1] - DBARI[icx - 1] * radice;
+        gice[b] = EBARI[icx - 1] + FBARI[icx - 1] * radice;
+        if (gice[b] >= 1.0) gice[b] = 1.0 - eps;
+        forwice[b] = gice[b] * gice[b];
+        if (extcoice[b] < 0.0) return fail(5, "ICE EXTINCTION LESS THAN 0.0");
+        if (ssacoice[b] > 1.0) return fail(5, "ICE SSA GRTR THAN 1.0");
+        if (ssacoice[b] < 0.0) return fail(5, "ICE SSA LESS THAN 0.0");
+        if (gice[b] > 1.0) return fail(5, "ICE ASYM GRTR THAN 1.0");
+        if (gice[b] < 0.0) return fail(5, "ICE ASYM LESS THAN 0.0");
+      }
+    } else if (iceflag == 2) { /* :225-245 */
+      if (radice < 5.0 || radice > 131.0) return fail(4, "ICE RADIUS OUT OF BOUNDS");
+      double factor = (radice - 2.) / 3.;
+      int index = (int)factor;
+      if (index == 43) index = 42;
+      double fint = factor - (double)index;
+      for (int b = 0; b < NBND; b++) {
+        const int o = 43 * b - 1;
+        extcoice[b] = EXTICE2[o + index] + fint * (EXTICE2[o + index + 1] - EXTICE2[o + index]);
+        ssacoice[b] = SSAICE2[o + index] + fint * (SSAICE2[o + index + 1] - SSAICE2[o + index]);
+        gice[b] = ASYICE2[o + index] + fint * (ASYICE2[o + index + 1] - ASYICE2[o + index]);
+        forwice[b] = gice[b] * gice[b];
+        if (extcoice[b] < 0.0) return fail(5, "ICE EXTINCTION LESS THAN 0.0");
+        if (ssacoice[b] > 1.0) return fail(5, "ICE SSA GRTR THAN 1.0");
+        if (ssacoice[b] < 0.0) return fail(5, "ICE SSA LESS THAN 0.0");
+        if (gice[b] > 1.0) return fail(5, "ICE ASYM GRTR THAN 1.0");
+        if (gice[b] < 0.0) return fail(5, "ICE ASYM LESS THAN 0.0");
+      }
+    } else if (iceflag == 3) { /* :249-275 */
+      if (radice < 5.0 || radice > 140.0) return fail(4, "ICE GENERALIZED EFFECTIVE SIZE OUT OF BOUNDS");
+      double factor = (radice - 2.) / 3.;
+      int index = (int)factor;
+      if (index == 46) index = 45;
+      double fint = factor - (double)index;
+      for (int b = 0; b < NBND; b++) {
+        const int o = 46 * b - 1;
+        extcoice[b] = EXTICE3[o + index] + fint * (EXTICE3[o + index + 1] - EXTICE3[o + index]);
+        ssacoice[b] = SSAICE3[o + index] + fint * (SSAICE3[o + index + 1] - SSAICE3[o + index]);
+        gice[b] = ASYICE3[o + index] + fint * (ASYICE3[o + index + 1] - ASYICE3[o + index]);
+        double fdelta = FDLICE3[o + index] + fint * (FDLICE3[o + index + 1] - FDLICE3[o + index]);
+        if (fdelta < 0.0) return fail(5, "FDELTA LESS THAN 0.0");
+        if (fdelta > 1.0) return fail(5, "FDELTA GT THAN 1.0");
+        forwice[b] = fdelta + 0.5 / ssacoice[b];
+        if (forwice[b] > gice[b]) forwice[b] = gice[b];
+        if (extcoice[b] < 0.0) return fail(5, "ICE EXTINCTION LESS THAN 0.0");
+        if (ssacoice[b] > 1.0) return fail(5, "ICE SSA GRTR THAN 1.0");
+        if (ssacoice[b] < 0.0) return fail(5, "ICE SSA LESS THAN 0.0");
+        if (gice[b] > 1.0) return fail(5, "ICE ASYM GRTR THAN 1.0");
+        if (gice[b] < 0.0) return fail(5, "ICE ASYM LESS THAN 0.0");
+      }
+    }
+    if (clwp[lay] == 0.0) { /* :280-287 */
+      for (int b = 0; b < NBND; b++) extcoliq[b] = ssacoliq[b] = gliq[b] = forwliq[b] = 0.;
+    } else if (liqflag == 1) { /* :288-313 */
+      double radliq = rel[lay];
+      if (radliq < 2.5 || radliq > 60.) return fail(4, "LIQUID EFFECTIVE RADIUS OUT OF BOUNDS");
+      int index = (int)(radliq - 1.5);
+      if (index == 0) index = 1;
+      if (index == 58) index = 57;
+      double fint = radliq - 1.5 - (double)index;
+      for (int b = 0; b < NBND; b++) {
+        const int o = 58 * b - 1;
+        extcoliq[b] = EXTLIQ1[o + index] + fint * (EXTLIQ1[o + index + 1] - EXTLIQ1[o + index]);
+        ssacoliq[b] = SSALIQ1[o + index] + fint * (SSALIQ1[o + index + 1] - SSALIQ1[o + index]);
+        if (fint < 0. && ssacoliq[b] > 1.) ssacoliq[b] = SSALIQ1[o + index];
+        gliq[b] = ASYLIQ1[o + index] + fint * (ASYLIQ1[o + index + 1] - ASYLIQ1[o + index]);
+        forwliq[b] = gliq[b] * gliq[b];
+        if (extcoliq[b] < 0.0) return fail(5, "LIQUID EXTINCTION LESS THAN 0.0");
+        if (ssacoliq[b] > 1.0) return fail(5, "LIQUID SSA GRTR THAN 1.0");
+        if (ssacoliq[b] < 0.0) return fail(5, "LIQUID SSA LESS THAN 0.0");
+        if (gliq[b] > 1.0) return fail(5, "LIQUID ASYM GRTR THAN 1.0");
+        if (gliq[b] < 0.0) return fail(5, "LIQUID ASYM LESS THAN 0.0");
+      }
+    }
+    for (int b = 0; b < NBND; b++) { /* :315-357 */
+      int t = lay + nlay * b;
+      double tauliqorig = clwp[lay] * extcoliq[b];
+      double tauiceorig = ciwp[lay] * extcoice[b];
+      taucldorig[t] = tauliqorig + tauiceorig;
+      double ssaliq = ssacoliq[b] * (1.0 - forwliq[b]) / (1.0 - forwliq[b] * ssacoliq[b]);
+      double tauliq = (1.0 - forwliq[b] * ssacoliq[b]) * tauliqorig;
+      double ssaice = ssacoice[b] * (1.0 - forwice[b]) / (1.0 - forwice[b] * ssacoice[b]);
+      double tauice = (1.0 - forwice[b] * ssacoice[b]) * tauiceorig;
+      double scatliq = ssaliq * tauliq;
+      double scatice = ssaice * tauice;
+      taucloud[t] = tauliq + tauice;
+      if (taucloud[t] == 0.0) taucloud[t] = CLDMIN;
+      if (scatice == 0.0) scatice = CLDMIN;
+      ssacloud[t] = (scatliq + scatice) / taucloud[t];
+      if (iceflag == 3)
+        asmcloud[t] = (1.0 / (scatliq + scatice)) *
+                      (scatliq * (gliq[b] - forwliq[b]) / (1.0 - forwliq[b]) +
+                       scatice * ((gice[b] - forwice[b]) / (1.0 - forwice[b])));
+      else
+        asmcloud[t] = (scatliq * (gliq[b] - forwliq[b]) / (1.0 - forwliq[b]) +
+                       scatice * (gice[b] - forwice[b]) / (1.0 - forwice[b])) / (scatliq + scatice);
+    }
+  }
+  return 0;
+}
+
+int orc_cldprop(int nlay, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                const double *tauc, const double *ssac, const double *asmc, const double *fsfc,
+                const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                double *taucldorig, double *taucloud, double *ssacloud, double *asmcloud) {
+  return cldprop(nlay, inflag, iceflag, liqflag, cldfrac, tauc, ssac, asmc, fsfc, ciwp, clwp, rei,
+                 rel, taucldorig, taucloud, ssacloud, asmcloud);
+}
+
+/* rrtmg_sw of src/rrtmg_sw_rad.nomcica.f90:97-816. spcvrt_sw (src/rrtmg_sw_spcvrt.f90:53-667)
+ * is spcvmc_sw with the cloud fraction of the layer (pclfr(ikl)) in place of the sub-column's
+ * and the band's cloud optics (ptauc(ikl, ibm) etc.) in place of the g-point's (their only
+ * differences, :429, :453-454, :473, :503-515, :547-548): spcvmc is evaluated on those values
+ * spread over the band's g-points. */
+int orc_rrtmg_sw_nomcica(int ncol, int nlay, int *icld, int *iaer, const double *play,
+                         const double *plev, const double *tlay, const double *tlev, const double *tsfc,
+                         const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                         const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *asdir, const double *asdif, const double *aldir,
+                         const double *aldif, const double *coszen, double adjes, int dyofyr,
+                         double scon, int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                         const double *cldfr, const double *taucld, const double *ssacld,
+                         const double *asmcld, const double *fsfcld, const double *cicewp,
+                         const double *cliqwp, const double *reice, const double *reliq,
+                         const double *tauaer, const double *ssaaer, const double *asmaer,
+                         const double *ecaer, double *swuflx, double *swdflx, double *swhr,
+                         double *swuflxc, double *swdflxc, double *swhrc, const double *bndsolvar,
+                         const double *indsolvar, const double *solcycfrac) {
+  (void)tsfc; (void)tlev; (void)n2ovmr;
+  if (!g_ready) return fail(9, "orc_init not called");
+  if (*icld < 0 || *icld > 3) *icld = 2;                   /* :560 */
+  if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0; /* :568 */
+  const int N = ncol, L = nlay, n1 = nlay + 1;
+  const double amd = 28.9660, amw = 18.0160, zepsec = 1.e-06, oneminus = 1.0 - zepsec;
+  solvar_t sv;
+  solar_setup(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, &sv);
+  solar_t sol = {isolvar, sv.svar_f, sv.svar_s, sv.svar_i, sv.svf_bnd + 15, sv.svs_bnd + 15, sv.svi_bnd + 15};
+  coef_t c = coef_alloc(L);
+  double *pavel = malloc(8 * L), *tavel = malloc(8 * L), *pz = malloc(8 * n1), *pdp = malloc(8 * L);
+  double *coldry = malloc(8 * L), *wkl = calloc(MXMOL * L, 8);
+  double *cldfrac = malloc(8 * L), *ciwp = malloc(8 * L), *clwp = malloc(8 * L), *rei = malloc(8 * L), *rel = malloc(8 * L);
+  double *tauc = malloc(8 * L * NBND), *ssac = malloc(8 * L * NBND), *asmc = malloc(8 * L * NBND), *fsfc = malloc(8 * L * NBND);
+  double *tco = malloc(8 * L * NBND), *tcl = malloc(8 * L * NBND), *scl = malloc(8 * L * NBND), *acl = malloc(8 * L * NBND);
+  double *zcldfmc = malloc(8 * L * NGPT), *ztaucmc = malloc(8 * L * NGPT), *ztaormc = malloc(8 * L * NGPT);
+  double *zasycmc = malloc(8 * L * NGPT), *zomgcmc = malloc(8 * L * NGPT);
+  double *ztaua = malloc(8 * L * NBND), *zasya = malloc(8 * L * NBND), *zomga = malloc(8 * L * NBND);
+  double *pbb[5];
+  for (int i = 0; i < 5; i++) pbb[i] = malloc(8 * n1);
+  int rc = 0;
+  for (int i = 0; i < N && !rc; i++) {
+    /* ---- inatm_sw (:846-1615): layer amounts as the McICA driver; clouds :1199-1207, :1579-1599 ---- */
+    for (int k = 0; k < MXMOL * L; k++) wkl[k] = 0.;
+    for (int l = 0; l < L; l++) cldfrac[l] = 0., ciwp[l] = 0., clwp[l] = 0., rei[l] = 0., rel[l] = 0.;
+    for (int k = 0; k < L * NBND; k++) tauc[k] = 0., ssac[k] = 1., asmc[k] = 0., fsfc[k] = 0.;
+    pz[0] = plev[i];
+    for (int l = 0; l < L; l++) {
+      pavel[l] = play[i + N * l];
+      tavel[l] = tlay[i + N * l];
+      pz[l + 1] = plev[i + N * (l + 1)];
+      pdp[l] = pz[l] - pz[l + 1];
+      double *w = wkl + MXMOL * l;
+      w[0] = h2ovmr[i + N * l], w[1] = co2vmr[i + N * l], w[2] = o3vmr[i + N * l];
+      w[3] = n2ovmr[i + N * l], w[5] = ch4vmr[i + N * l], w[6] = o2vmr[i + N * l];
+      double amm = (1. - w[0]) * amd + w[0] * amw;
+      coldry[l] = (pz[l] - pz[l + 1]) * 1.e3 * AVOGAD / (1.e2 * GRAV * amm * (1. + w[0]));
+    }
+    for (int l = 0; l < L; l++)
+      for (int m = 0; m < 7; m++) wkl[m + MXMOL * l] = coldry[l] * wkl[m + MXMOL * l];
+    int inflag = 0, iceflag = 0, liqflag = 0;
+    if (*icld >= 1) {
+      inflag = inflgsw, iceflag = iceflgsw, liqflag = liqflgsw;
+      for (int l = 0; l < L; l++) {
+        cldfrac[l] = cldfr[i + N * l];
+        ciwp[l] = cicewp[i + N * l];
+        clwp[l] = cliqwp[i + N * l];
+        rei[l] = reice[i + N * l];
+        rel[l] = reliq[i + N * l];
+        for (int b = 0; b < NBND; b++) {
+          size_t src = b + (size_t)NBND * (i + (size_t)N * l);
+          tauc[b + NBND * l] = taucld[src], ssac[b + NBND * l] = ssacld[src];
+          asmc[b + NBND * l] = asmcld[src], fsfc[b + NBND * l] = fsfcld[src];
+        }
+      }
+    }
+    /* ---- partial cloud stop (:617-622), cldprop_sw (:623), setcoef_sw ---- */
+    for (int l = 0; l < L; l++)
+      if (cldfrac[l] > zepsec && cldfrac[l] < oneminus) { rc = fail(6, "PARTIAL CLOUD NOT ALLOWED"); break; }
+    if (rc) break;
+    rc = cldprop(L, inflag, iceflag, liqflag, cldfrac, tauc, ssac, asmc, fsfc, ciwp, clwp, rei, rel,
+                 tco, tcl, scl, acl);
+    if (rc) break;
+    setcoef(L, pavel, tavel, coldry, wkl, &c);
+    double cossza = coszen[i];
+    if (cossza < 1.e-10) cossza = 1.e-10; /* :643-644 */
+    double albdir[NBND], albdif[NBND];     /* :651-662 */
+    for (int ib = 0; ib < 9; ib++) albdir[ib] = aldir[i], albdif[ib] = aldif[i];
+    albdir[13] = aldir[i], albdif[13] = aldif[i];
+    for (int ib = 9; ib < 13; ib++) albdir[ib] = asdir[i], albdif[ib] = asdif[i];
+    for (int l = 0; l < L; l++) /* clouds (:665-682) spread over each band's g-points */
+      for (int g = 0; g < NGPT; g++) {
+        int t = l + L * g, tb = l + L * (NGB[g] - 16);
+        zcldfmc[t] = cldfrac[l];
+        if (*icld == 0) {
+          ztaucmc[t] = 0., ztaormc[t] = 0., zasycmc[t] = 0., zomgcmc[t] = 1.;
+        } else {
+          ztaucmc[t] = tcl[tb], ztaormc[t] = tco[tb], zasycmc[t] = acl[tb], zomgcmc[t] = scl[tb];
+        }
+      }
+    for (int l = 0; l < L; l++) /* aerosol, :686-738 */
+      for (int ib = 0; ib < NBND; ib++) {
+        int t = l + L * ib;
+        if (*iaer == 0) {
+          ztaua[t] = 0., zasya[t] = 0., zomga[t] = 1.;
+        } else if (*iaer == 6) {
+          ztaua[t] = 0., zasya[t] = 0., zomga[t] = 0.;
+          for (int ia = 0; ia < 6; ia++) {
+            double ec = ecaer[i + N * (l + L * ia)];
+            ztaua[t] = ztaua[t] + RSRTAUA[ib + 14 * ia] * ec;
+            zomga[t] = zomga[t] + RSRTAUA[ib + 14 * ia] * ec * RSRPIZA[ib + 14 * ia];
+            zasya[t] = zasya[t] + RSRTAUA[ib + 14 * ia] * ec * RSRPIZA[ib + 14 * ia] * RSRASYA[ib + 14 * ia];
+          }
+          if (ztaua[t] == 0.) {
+            ztaua[t] = 0., zasya[t] = 0., zomga[t] = 1.;
+          } else {
+            if (zomga[t] != 0.) zasya[t] = zasya[t] / zomga[t];
+            if (ztaua[t] != 0.) zomga[t] = zomga[t] / ztaua[t];
+          }
+        } else {
+          size_t sidx = i + (size_t)N * (l + (size_t)L * ib);
+          ztaua[t] = tauaer[sidx], zasya[t] = asmaer[sidx], zomga[t] = ssaaer[sidx];
+        }
+      }
+    spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], NULL, NULL, NULL};
+    spcvmc(L, 1, albdif, albdir, zcldfmc, ztaucmc, zasycmc, zomgcmc, ztaormc, ztaua, zasya, zomga,
+           cossza, sv.adjflux, &sol, &c, &o);
+    for (int k = 0; k < n1; k++) { /* :780-794 */
+      swuflxc[i + N * k] = pbb[2][k];
+      swdflxc[i + N * k] = pbb[3][k];
+      swuflx[i + N * k] = pbb[0][k];
+      swdflx[i + N * k] = pbb[1][k];
+    }
+    for (int k = 0; k < L; k++) { /* :800-812 */
+      double zdpgcp = g_heatfac / pdp[k];
+      double nc1 = swdflxc[i + N * (k + 1)] - swuflxc[i + N * (k + 1)], nc0 = swdflxc[i + N * k] - swuflxc[i + N * k];
+      double nf1 = swdflx[i + N * (k + 1)] - swuflx[i + N * (k + 1)], nf0 = swdflx[i + N * k] - swuflx[i + N * k];
+      swhrc[i + N * k] = (nc1 - nc0) * zdpgcp;
+      swhr[i + N * k] = (nf1 - nf0) * zdpgcp;
+    }
+    swhrc[i + N * (L - 1)] = 0.;
+    swhr[i + N * (L - 1)] = 0.;
+  }
+  coef_free(&c);
+  free(pavel), free(tavel), free(pz), free(pdp), free(coldry), free(wkl);
+  free(cldfrac), free(ciwp), free(clwp), free(rei), free(rel), free(tauc), free(ssac), free(asmc), free(fsfc);
+  free(tco), free(tcl), free(scl), free(acl);
+  free(zcldfmc), free(ztaucmc), free(ztaormc), free(zasycmc), free(zomgcmc), free(ztaua), free(zasya), free(zomga);
+  for (int k = 0; k < 5; k++) free(pbb[k]);
   return rc;
 }
 
@@ -1806,12 +2122,20 @@ int orc_cldprmc(int nlay, int inflag, int iceflag, int liqflag, const double *cl
  * blocks 1..14 bands 16..29 (the IOUT=98 band loop runs spcvmc_sw on one band at a time,
  * which is the band's share of the broadband sums).
  * ---------------------------------------------------------------------------------- */
-int orc_column_sw(int nlay, const double *pavel, const double *tavel, const double *pz,
-                  const double *coldry, const double *wkl7, const double *semiss, double zenith,
-                  int juldat, double scon, int isolvar, const double *bndsolvar,
-                  const double *indsolvar, const double *solcycfrac, int iaer,
-                  const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
-                  double cpdair, double *out) {
+/* Cloud inputs of the column program for imca = 0 (readcld, src/rrtmg_sw.1col.f90:1884-1958,
+ * and the conversion :1807-1834): cldfrac, ciwp, clwp, rei, rel (nlay, from the surface);
+ * tauc, ssac, asmc, fsfc (14, nlay). */
+typedef struct {
+  int inflag, iceflag, liqflag;
+  const double *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc, *ssac, *asmc, *fsfc;
+} colcld_t;
+
+static int column_run(int nlay, const double *pavel, const double *tavel, const double *pz,
+                      const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                      int juldat, double scon, int isolvar, const double *bndsolvar,
+                      const double *indsolvar, const double *solcycfrac, int iaer,
+                      const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                      double cpdair, const colcld_t *cld, double *out) {
   if (!g_ready) return fail(9, "orc_init not called");
   const int L = nlay, n1 = nlay + 1;
   solvar_t sv;
@@ -1827,6 +2151,36 @@ int orc_column_sw(int nlay, const double *pavel, const double *tavel, const doub
   for (int ib = 0; ib < NBND; ib++) alb[ib] = 1. - semiss[ib];
   double *zc0 = calloc((size_t)L * NGPT, 8), *zw1 = malloc(8 * (size_t)L * NGPT);
   for (int k = 0; k < L * NGPT; k++) zw1[k] = 1.;
+  /* clouds (imca = 0, :612-621, :669-680): cldprop_sw, then each band's optics on its g-points
+   * for layers with cldfrac >= 1e-6 (spcvrt_sw reads them only where cldfrac > 1e-12) */
+  double *zcf = calloc((size_t)L * NGPT, 8), *ztc = calloc((size_t)L * NGPT, 8);
+  double *zto = calloc((size_t)L * NGPT, 8), *zgc = calloc((size_t)L * NGPT, 8);
+  double *zwc = malloc(8 * (size_t)L * NGPT);
+  for (int k = 0; k < L * NGPT; k++) zwc[k] = 1.;
+  if (cld) {
+    for (int l = 0; l < L; l++)
+      if (cld->cldfrac[l] != 0. && cld->cldfrac[l] != 1.) {
+        free(zc0), free(zw1), free(zcf), free(ztc), free(zto), free(zgc), free(zwc), free(wkl);
+        coef_free(&c);
+        return fail(6, "PARTIAL CLOUD NOT ALLOWED FOR IMCA=0");
+      }
+    double *tco = malloc(8 * (size_t)L * NBND), *tcl = malloc(8 * (size_t)L * NBND);
+    double *scl = malloc(8 * (size_t)L * NBND), *acl = malloc(8 * (size_t)L * NBND);
+    int rc = cldprop(L, cld->inflag, cld->iceflag, cld->liqflag, cld->cldfrac, cld->tauc, cld->ssac,
+                     cld->asmc, cld->fsfc, cld->ciwp, cld->clwp, cld->rei, cld->rel, tco, tcl, scl, acl);
+    for (int l = 0; l < L && !rc; l++)
+      for (int g = 0; g < NGPT; g++) {
+        int t = l + L * g, tb = l + L * (NGB[g] - 16);
+        zcf[t] = cld->cldfrac[l];
+        if (cld->cldfrac[l] >= 1.e-06) ztc[t] = tcl[tb], zto[t] = tco[tb], zgc[t] = acl[tb], zwc[t] = scl[tb];
+      }
+    free(tco), free(tcl), free(scl), free(acl);
+    if (rc) {
+      free(zc0), free(zw1), free(zcf), free(ztc), free(zto), free(zgc), free(zwc), free(wkl);
+      coef_free(&c);
+      return rc;
+    }
+  }
   double *ta = calloc((size_t)L * NBND, 8), *ga = calloc((size_t)L * NBND, 8), *wa = malloc(8 * (size_t)L * NBND);
   for (int k = 0; k < L * NBND; k++) wa[k] = 1.;
   if (iaer == 10)
@@ -1835,7 +2189,7 @@ int orc_column_sw(int nlay, const double *pavel, const double *tavel, const doub
   for (int i = 0; i < 5; i++) pbb[i] = calloc(n1, 8);
   double *bu = calloc((size_t)n1 * NBND, 8), *bd = calloc((size_t)n1 * NBND, 8), *bdir = calloc((size_t)n1 * NBND, 8);
   spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], bu, bd, bdir};
-  spcvmc(L, idelm, alb, alb, zc0, zc0, zc0, zw1, zc0, ta, ga, wa, cossza, sv.adjflux, &sol, &c, &o);
+  spcvmc(L, idelm, alb, alb, zcf, ztc, zgc, zwc, zto, ta, ga, wa, cossza, sv.adjflux, &sol, &c, &o);
   const double heatfac = GRAV * SECDY / (cpdair * 1.e2);
   for (int b = 0; b <= NBND; b++) {
     double *up = out + (size_t)n1 * (0 + 6 * b), *dif = out + (size_t)n1 * (1 + 6 * b);
@@ -1853,6 +2207,33 @@ int orc_column_sw(int nlay, const double *pavel, const double *tavel, const doub
   }
   coef_free(&c);
   free(wkl), free(zc0), free(zw1), free(ta), free(ga), free(wa), free(bu), free(bd), free(bdir);
+  free(zcf), free(ztc), free(zto), free(zgc), free(zwc);
   for (int i = 0; i < 5; i++) free(pbb[i]);
   return 0;
+}
+
+int orc_column_sw(int nlay, const double *pavel, const double *tavel, const double *pz,
+                  const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                  int juldat, double scon, int isolvar, const double *bndsolvar,
+                  const double *indsolvar, const double *solcycfrac, int iaer,
+                  const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                  double cpdair, double *out) {
+  return column_run(nlay, pavel, tavel, pz, coldry, wkl7, semiss, zenith, juldat, scon, isolvar,
+                    bndsolvar, indsolvar, solcycfrac, iaer, tauaer, ssaaer, asmaer, idelm, cpdair,
+                    NULL, out);
+}
+
+int orc_column_sw_cld(int nlay, const double *pavel, const double *tavel, const double *pz,
+                      const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                      int juldat, double scon, int isolvar, const double *bndsolvar,
+                      const double *indsolvar, const double *solcycfrac, int iaer,
+                      const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                      double cpdair, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                      const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                      const double *tauc, const double *ssac, const double *asmc,
+                      const double *fsfc, double *out) {
+  colcld_t cl = {inflag, iceflag, liqflag, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc};
+  return column_run(nlay, pavel, tavel, pz, coldry, wkl7, semiss, zenith, juldat, scon, isolvar,
+                    bndsolvar, indsolvar, solcycfrac, iaer, tauaer, ssaaer, asmaer, idelm, cpdair,
+                    &cl, out);
 }

@@ -70,6 +70,19 @@ int orc_column_sw(int nlay, const double *pavel, const double *tavel, const doub
                   const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
                   double cpdair, double *out);
 
+/* orc_column_sw with the column program's imca = 0 clouds (cldprop_sw + spcvrt_sw,
+ * src/rrtmg_sw.1col.f90:612-621, :669-680): cldfrac, ciwp, clwp, rei, rel (nlay, from the
+ * surface); tauc, ssac, asmc, fsfc (14, nlay). Returns 6 for a fraction other than 0 or 1. */
+int orc_column_sw_cld(int nlay, const double *pavel, const double *tavel, const double *pz,
+                      const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                      int juldat, double scon, int isolvar, const double *bndsolvar,
+                      const double *indsolvar, const double *solcycfrac, int iaer,
+                      const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                      double cpdair, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                      const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                      const double *tauc, const double *ssac, const double *asmc,
+                      const double *fsfc, double *out);
+
 /* mcica_subcol_sw(...) -- src/mcica_subcol_gen_sw.f90:181-296 (+ :300-753). */
 int orc_mcica_subcol_sw(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
                         const double *play, const double *cldfrac, const double *ciwp,
@@ -114,6 +127,32 @@ int orc_cldprmc(int nlay, int inflag, int iceflag, int liqflag, const double *cl
                 const double *ciwpmc, const double *clwpmc, const double *reicmc,
                 const double *relqmc, double *taormc, double *taucmc, double *ssacmc,
                 double *asmcmc, const double *fsfcmc);
+
+/* cldprop_sw -- src/rrtmg_sw_cldprop.f90:53-365 for one column. cldfrac, ciwp, clwp, rei, rel
+ * (nlay); tauc, ssac, asmc, fsfc (14, nlay); outputs (nlay, 14), column b = band 16 + b. */
+int orc_cldprop(int nlay, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                const double *tauc, const double *ssac, const double *asmc, const double *fsfc,
+                const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                double *taucldorig, double *taucloud, double *ssacloud, double *asmcloud);
+
+/* rrtmg_sw of the non-McICA driver, src/rrtmg_sw_rad.nomcica.f90:97-816 (spcvrt_sw,
+ * cldprop_sw). Cloud inputs: cldfr, cicewp, cliqwp, reice, reliq (ncol, nlay); taucld, ssacld,
+ * asmcld, fsfcld (14, ncol, nlay). Returns 6 ("PARTIAL CLOUD NOT ALLOWED") for a layer
+ * cloud fraction strictly between 1e-6 and 1 - 1e-6. */
+int orc_rrtmg_sw_nomcica(int ncol, int nlay, int *icld, int *iaer, const double *play,
+                         const double *plev, const double *tlay, const double *tlev, const double *tsfc,
+                         const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                         const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *asdir, const double *asdif, const double *aldir,
+                         const double *aldif, const double *coszen, double adjes, int dyofyr,
+                         double scon, int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                         const double *cldfr, const double *taucld, const double *ssacld,
+                         const double *asmcld, const double *fsfcld, const double *cicewp,
+                         const double *cliqwp, const double *reice, const double *reliq,
+                         const double *tauaer, const double *ssaaer, const double *asmaer,
+                         const double *ecaer, double *swuflx, double *swdflx, double *swhr,
+                         double *swuflxc, double *swdflxc, double *swhrc, const double *bndsolvar,
+                         const double *indsolvar, const double *solcycfrac);
 
 /* exp lookup table value and the reduced-table accessor (for table-level tests). */
 double orc_exp_tbl(int i);

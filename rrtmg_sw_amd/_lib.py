@@ -42,6 +42,8 @@ def lib():
           C.c_int, C.c_int, C.c_int] + [_dp] * 9 + [_dp] * 4 + [_dp] * 6 + [_dp] * 3)
     L.rrtmgsw_sw.argtypes = sw
     L.rrtmgsw_sw_dev.argtypes = sw + [_vp]
+    L.rrtmgsw_sw_nomcica.argtypes = sw          # same dummy kinds, layer clouds in place of McICA
+    L.rrtmgsw_sw_nomcica_dev.argtypes = sw + [_vp]
     mc = [C.c_int] * 5 + [_ip] + [_dp] * 11 + [_dp] * 9
     L.rrtmgsw_mcica_subcol.argtypes = mc
     L.rrtmgsw_mcica_subcol_dev.argtypes = mc + [_vp]
@@ -61,7 +63,8 @@ def lib():
 
 
 EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_sw",
-            "rrtmgsw_sw_dev", "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
+            "rrtmgsw_sw_dev", "rrtmgsw_sw_nomcica", "rrtmgsw_sw_nomcica_dev",
+            "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
             "rrtmgsw_get_alpha", "rrtmgsw_mcica_sw_dev", "rrtmgsw_mcica_sw_adjoint_dev", "rrtmgsw_adjoint_work_bytes"]
 
 

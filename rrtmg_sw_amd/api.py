@@ -3,6 +3,8 @@
 Names, argument order and meaning follow the reference routines:
   rrtmg_sw_ini(cpdair)                          src/rrtmg_sw_init.f90:47
   rrtmg_sw(ncol, nlay, icld, iaer, play, ...)   src/rrtmg_sw_rad.f90:99-111
+  rrtmg_sw_nomcica(ncol, nlay, icld, ...)       src/rrtmg_sw_rad.nomcica.f90:97-111 (the
+                                                non-McICA driver's rrtmg_sw)
   mcica_subcol_sw(iplon, ncol, nlay, icld, ...) src/mcica_subcol_gen_sw.f90:181-185
   get_alpha(iplon, ncol, nlayers, icld, ...)    src/mcica_subcol_gen_sw.f90:66-67
 plus mcica_rrtmg_sw(...), both calls fused on the GPU without the (112, ncol, nlay) arrays.
@@ -110,6 +112,45 @@ def rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr
         check(_lib.lib().rrtmgsw_sw_dev(*args, _stream()))
     else:
         check(_lib.lib().rrtmgsw_sw(*args))
+    out["icld"], out["iaer"] = ic.value, ia.value
+    return out
+
+
+def rrtmg_sw_nomcica(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr,
+                     ch4vmr, n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon,
+                     isolvar, inflgsw, iceflgsw, liqflgsw, cldfr, taucld, ssacld, asmcld, fsfcld,
+                     cicewp, cliqwp, reice, reliq, tauaer, ssaaer, asmaer, ecaer,
+                     bndsolvar=None, indsolvar=None, solcycfrac=None):
+    """rrtmg_sw of the non-McICA driver (src/rrtmg_sw_rad.nomcica.f90:97): layer cloud fractions
+    (clear or overcast: 'PARTIAL CLOUD NOT ALLOWED' otherwise) with cldprop_sw optics. Returns
+    dict(swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc, icld, iaer), icld / iaer as clamped."""
+    dev = _is_dev(play)
+    N, L = ncol, nlay
+    out = {k: _new(dev, (N, L + 1)) for k in ("swuflx", "swdflx", "swuflxc", "swdflxc")}
+    out["swhr"] = _new(dev, (N, L))
+    out["swhrc"] = _new(dev, (N, L))
+    ic, ia = C.c_int(icld), C.c_int(iaer)
+    opt = [None if bndsolvar is None else np.ascontiguousarray(bndsolvar, dtype=np.float64),
+           None if indsolvar is None else np.ascontiguousarray(indsolvar, dtype=np.float64),
+           None if solcycfrac is None else np.array([solcycfrac], dtype=np.float64)]
+    cl = icld != 0          # icld outside 0..3 is clamped to 2: clouds are read unless icld = 0
+    nl = N * L if cl else None
+    nb = NBND * N * L if cl else None
+    args = [C.c_int(N), C.c_int(L), C.byref(ic), C.byref(ia),
+            _ptr(play, N * L), _ptr(plev, N * (L + 1)), _ptr(tlay, N * L), _ptr(tlev), _ptr(tsfc),
+            _ptr(h2ovmr, N * L), _ptr(o3vmr, N * L), _ptr(co2vmr, N * L), _ptr(ch4vmr, N * L),
+            _ptr(n2ovmr), _ptr(o2vmr, N * L), _ptr(asdir, N), _ptr(asdif, N), _ptr(aldir, N),
+            _ptr(aldif, N), _ptr(coszen, N), C.c_double(adjes), C.c_int(dyofyr), C.c_double(scon),
+            C.c_int(isolvar), C.c_int(inflgsw), C.c_int(iceflgsw), C.c_int(liqflgsw),
+            _ptr(cldfr, nl), *[_ptr(x, nb) for x in (taucld, ssacld, asmcld, fsfcld)],
+            *[_ptr(x, nl) for x in (cicewp, cliqwp, reice, reliq)],
+            _ptr(tauaer), _ptr(ssaaer), _ptr(asmaer), _ptr(ecaer),
+            _ptr(out["swuflx"]), _ptr(out["swdflx"]), _ptr(out["swhr"]), _ptr(out["swuflxc"]),
+            _ptr(out["swdflxc"]), _ptr(out["swhrc"]), *[_ptr(o) for o in opt]]
+    if dev:
+        check(_lib.lib().rrtmgsw_sw_nomcica_dev(*args, _stream()))
+    else:
+        check(_lib.lib().rrtmgsw_sw_nomcica(*args))
     out["icld"], out["iaer"] = ic.value, ia.value
     return out
 

@@ -45,7 +45,7 @@ struct Context {
   double heatfac = 0.0;
   HostTables host;
   DeviceTables dev{};
-  double *d_tab = nullptr, *d_exp = nullptr, *d_rexp = nullptr;
+  double *d_tab = nullptr, *d_exp = nullptr;
   float *d_tab32 = nullptr, *d_exp32 = nullptr, *d_rexp32 = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
@@ -294,6 +294,7 @@ int finish(hipStream_t s, int *d_status) {
     case RRTMGSW_ERR_INFLAG: return fail(st, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
     case RRTMGSW_ERR_RADIUS: return fail(st, "CLOUD PARTICLE RADIUS OUT OF BOUNDS");
     case RRTMGSW_ERR_OPTICS: return fail(st, "CLOUD OPTICAL PROPERTY OUT OF PHYSICAL LIMITS");
+    case RRTMGSW_ERR_PARTIAL_CLOUD: return fail(st, "PARTIAL CLOUD NOT ALLOWED");
     case RRTMGSW_ERR_KISS_SEED:
       return fail(st, "MCICA_SUBCOL: KISSVEC SEED GENERATOR REQUIRES PMID FROM BOTTOM FOUR LAYERS.");
     default: return fail(st, "kernel error");
@@ -374,12 +375,6 @@ int rrtmgsw_init(double cpdair) {
   HIPCHK(hipMalloc(&ctx.d_exp, H.exp_tbl.size() * sizeof(double)));
   HIPCHK(hipMemcpy(ctx.d_exp, H.exp_tbl.data(), H.exp_tbl.size() * sizeof(double), hipMemcpyHostToDevice));
   {
-    std::vector<double> r(H.exp_tbl.size());
-    for (size_t i = 0; i < r.size(); i++) r[i] = 1.0 / H.exp_tbl[i];
-    HIPCHK(hipMalloc(&ctx.d_rexp, r.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(ctx.d_rexp, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
-  {
     // FP32 copies for the mixed form
     auto up32 = [](const std::vector<double> &src, float **dst, bool recip) -> hipError_t {
       std::vector<float> f(src.size());
@@ -395,7 +390,7 @@ int rrtmgsw_init(double cpdair) {
   HIPCHK(hipMalloc(&ctx.d_gp, H.gp.size() * sizeof(GPoint)));
   HIPCHK(hipMemcpy(ctx.d_gp, H.gp.data(), H.gp.size() * sizeof(GPoint), hipMemcpyHostToDevice));
   DeviceTables &D = ctx.dev;
-  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.rexp_tbl = ctx.d_rexp, D.gp = ctx.d_gp;
+  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.gp = ctx.d_gp;
   D.tab32 = ctx.d_tab32, D.exp32 = ctx.d_exp32, D.rexp32 = ctx.d_rexp32;
   D.extliq1 = H.extliq1, D.ssaliq1 = H.ssaliq1, D.asyliq1 = H.asyliq1;
   D.extice2 = H.extice2, D.ssaice2 = H.ssaice2, D.asyice2 = H.asyice2;
@@ -413,7 +408,6 @@ int rrtmgsw_finalize(void) {
   if (!ctx.ready) return 0;
   hipFree(ctx.d_tab);
   hipFree(ctx.d_exp);
-  hipFree(ctx.d_rexp);
   hipFree(ctx.d_tab32);
   hipFree(ctx.d_exp32);
   hipFree(ctx.d_rexp32);
@@ -551,6 +545,144 @@ int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer, const double *play, con
                           dyofyr, scon, isolvar, inflgsw, iceflgsw, liqflgsw, dp[13], dp[14], dp[15],
                           dp[16], dp[17], dp[18], dp[19], dp[20], dp[21], dp[22], dp[23], dp[24], dp[25],
                           ou, od, oh, ouc, odc, ohc, bndsolvar, indsolvar, solcycfrac, nullptr);
+  if (rc == 0) {
+    hipError_t e = hipMemcpy(swuflx, ou, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swdflx, od, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swuflxc, ouc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swdflxc, odc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swhr, oh, C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(swhrc, ohc, C * L * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+  }
+  hipFree(d);
+  return rc;
+}
+
+// Non-McICA driver (src/rrtmg_sw_rad.nomcica.f90:97-816): layer cloud fractions and cldprop_sw
+// optics; inputs already on the device.
+int rrtmgsw_sw_nomcica_dev(int ncol, int nlay, int *icld, int *iaer, const double *play,
+                           const double *plev, const double *tlay, const double *tlev,
+                           const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                           const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                           const double *o2vmr, const double *asdir, const double *asdif,
+                           const double *aldir, const double *aldif, const double *coszen,
+                           double adjes, int dyofyr, double scon, int isolvar, int inflgsw,
+                           int iceflgsw, int liqflgsw, const double *cldfr, const double *taucld,
+                           const double *ssacld, const double *asmcld, const double *fsfcld,
+                           const double *cicewp, const double *cliqwp, const double *reice,
+                           const double *reliq, const double *tauaer, const double *ssaaer,
+                           const double *asmaer, const double *ecaer, double *swuflx, double *swdflx,
+                           double *swhr, double *swuflxc, double *swdflxc, double *swhrc,
+                           const double *bndsolvar, const double *indsolvar,
+                           const double *solcycfrac, void *stream) {
+  (void)tlev, (void)tsfc, (void)n2ovmr;  // reach only unused setcoef locals / no taumol term
+  if (int rc = check_init()) return rc;
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  if (!icld || !iaer) return fail(RRTMGSW_ERR_ARGUMENT, "icld and iaer are required");
+  if (*icld < 0 || *icld > 3) *icld = 2;                    // src/rrtmg_sw_rad.nomcica.f90:560
+  if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;  // :568
+  if (ncol == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  RtArgs a{};
+  a.ncol = ncol, a.nlay = nlay;
+  a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
+  a.co2vmr = co2vmr, a.ch4vmr = ch4vmr, a.o2vmr = o2vmr;
+  a.asdir = asdir, a.asdif = asdif, a.aldir = aldir, a.aldif = aldif, a.coszen = coszen;
+  a.tauaer = tauaer, a.ssaaer = ssaaer, a.asmaer = asmaer, a.ecaer = ecaer;
+  a.swuflx = swuflx, a.swdflx = swdflx, a.swhr = swhr, a.swuflxc = swuflxc, a.swdflxc = swdflxc;
+  a.swhrc = swhrc;
+  a.fused = 0, a.layer_clouds = 1;
+  a.cldfrac = cldfr, a.ciwp = cicewp, a.clwp = cliqwp, a.rei = reice, a.rel = reliq;
+  a.tauc = taucld, a.ssac = ssacld, a.asmc = asmcld, a.fsfc = fsfcld;
+  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
+  a.sc.heatfac = ctx.heatfac;
+  a.sc.icld = *icld, a.sc.iaer = *iaer;
+  a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
+  const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
+                        aldir, aldif, coszen, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (*icld >= 1 && (!cldfr || !taucld || !ssacld || !asmcld || !fsfcld || !cicewp || !cliqwp ||
+                     !reice || !reliq))
+    return fail(RRTMGSW_ERR_ARGUMENT, "cloud arrays are required when icld >= 1");
+  if (*iaer == 10 && (!tauaer || !ssaaer || !asmaer))
+    return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer are required when iaer = 10");
+  if (*iaer == 6 && !ecaer) return fail(RRTMGSW_ERR_ARGUMENT, "ecaer is required when iaer = 6");
+  return run_rt(a, ctx.dev, s);
+}
+
+// Host arrays in and out of the non-McICA driver: one device allocation for the inputs the
+// flags select and the outputs, synchronous copies, the device entry on the null stream.
+int rrtmgsw_sw_nomcica(int ncol, int nlay, int *icld, int *iaer, const double *play,
+                       const double *plev, const double *tlay, const double *tlev,
+                       const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                       const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                       const double *o2vmr, const double *asdir, const double *asdif,
+                       const double *aldir, const double *aldif, const double *coszen,
+                       double adjes, int dyofyr, double scon, int isolvar, int inflgsw,
+                       int iceflgsw, int liqflgsw, const double *cldfr, const double *taucld,
+                       const double *ssacld, const double *asmcld, const double *fsfcld,
+                       const double *cicewp, const double *cliqwp, const double *reice,
+                       const double *reliq, const double *tauaer, const double *ssaaer,
+                       const double *asmaer, const double *ecaer, double *swuflx, double *swdflx,
+                       double *swhr, double *swuflxc, double *swdflxc, double *swhrc,
+                       const double *bndsolvar, const double *indsolvar, const double *solcycfrac) {
+  if (int rc = check_init()) return rc;
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  if (!icld || !iaer) return fail(RRTMGSW_ERR_ARGUMENT, "icld and iaer are required");
+  if (ncol == 0) {
+    if (*icld < 0 || *icld > 3) *icld = 2;
+    if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;
+    return 0;
+  }
+  const size_t C = ncol, L = nlay;
+  struct In {
+    const double *h;
+    size_t n;
+  };
+  int ic = (*icld < 0 || *icld > 3) ? 2 : *icld;
+  int ia = (*iaer != 0 && *iaer != 6 && *iaer != 10) ? 0 : *iaer;
+  const size_t cl = ic >= 1 ? C * L : 0, cb = ic >= 1 ? NBND * C * L : 0;
+  In in[] = {{play, C * L}, {plev, C * (L + 1)}, {tlay, C * L}, {h2ovmr, C * L}, {o3vmr, C * L},
+             {co2vmr, C * L}, {ch4vmr, C * L}, {o2vmr, C * L}, {asdir, C}, {asdif, C}, {aldir, C},
+             {aldif, C}, {coszen, C}, {cldfr, cl}, {taucld, cb}, {ssacld, cb}, {asmcld, cb},
+             {fsfcld, cb}, {cicewp, cl}, {cliqwp, cl}, {reice, cl}, {reliq, cl},
+             {tauaer, ia == 10 ? C * L * NBND : 0}, {ssaaer, ia == 10 ? C * L * NBND : 0},
+             {asmaer, ia == 10 ? C * L * NBND : 0}, {ecaer, ia == 6 ? C * L * NAER : 0}};
+  const int NIN = sizeof in / sizeof *in;
+  size_t total = 0;
+  for (auto &x : in) total += x.n;
+  size_t nout = 4 * C * (L + 1) + 2 * C * L;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
+  const double *dp[NIN];
+  size_t off = 0;
+  for (int i = 0; i < NIN; i++) {
+    dp[i] = in[i].n ? d + off : nullptr;
+    if (in[i].n) {
+      if (!in[i].h) {
+        hipFree(d);
+        return fail(RRTMGSW_ERR_ARGUMENT, "a required input array is NULL");
+      }
+      hipError_t e = hipMemcpy(d + off, in[i].h, in[i].n * sizeof(double), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        hipFree(d);
+        return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+      }
+    }
+    off += in[i].n;
+  }
+  double *o = d + total;
+  double *ou = o, *od = o + C * (L + 1), *ouc = o + 2 * C * (L + 1), *odc = o + 3 * C * (L + 1);
+  double *oh = o + 4 * C * (L + 1), *ohc = oh + C * L;
+  int rc = rrtmgsw_sw_nomcica_dev(ncol, nlay, icld, iaer, dp[0], dp[1], dp[2], nullptr, nullptr, dp[3],
+                                  dp[4], dp[5], dp[6], nullptr, dp[7], dp[8], dp[9], dp[10], dp[11],
+                                  dp[12], adjes, dyofyr, scon, isolvar, inflgsw, iceflgsw, liqflgsw,
+                                  dp[13], dp[14], dp[15], dp[16], dp[17], dp[18], dp[19], dp[20], dp[21],
+                                  dp[22], dp[23], dp[24], dp[25], ou, od, oh, ouc, odc, ohc, bndsolvar,
+                                  indsolvar, solcycfrac, nullptr);
   if (rc == 0) {
     hipError_t e = hipMemcpy(swuflx, ou, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(swdflx, od, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);

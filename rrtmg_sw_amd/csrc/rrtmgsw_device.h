@@ -68,6 +68,7 @@ struct ColView {
 };
 // misc slots
 constexpr int M_LAYTROP = 0, M_LAYSOL = 1;   // laysolfr of band b at misc[M_LAYSOL + b]
+constexpr int M_PARTIAL = 15;                 // non-McICA: a layer has a partial cloud fraction
 
 // ------------------------------------------------------------------------------------------
 // inatm_sw + setcoef_sw for one layer (src/rrtmg_sw_rad.f90:1531-1586,
@@ -206,73 +207,6 @@ __device__ inline Spec key_spec(double c1, double c2, double strrat, double mult
   return key_spec_t<double>(c1, c2, strrat, mult, bin);
 }
 
-// 8-corner interpolation of a key-species table. Corner strides: cjs (species step: 1 for
-// binary species, 0 otherwise) and nsp (temperature step); st = table row stride (0: unused).
-template <typename R>
-__device__ inline R kinterp(const R *__restrict__ A, int st, int ind0, int ind1, int cjs, int nsp,
-                            R fs, R fac00, R fac10, R fac01, R fac11) {
-  R f0 = R(1) - fs;
-  R a0 = A[(ind0 - 1) * st], a1 = A[(ind0 - 1 + cjs) * st];
-  R a2 = A[(ind0 - 1 + nsp) * st], a3 = A[(ind0 - 1 + nsp + cjs) * st];
-  R b0 = A[(ind1 - 1) * st], b1 = A[(ind1 - 1 + cjs) * st];
-  R b2 = A[(ind1 - 1 + nsp) * st], b3 = A[(ind1 - 1 + nsp + cjs) * st];
-  return f0 * fac00 * a0 + fs * fac00 * a1 + f0 * fac10 * a2 + fs * fac10 * a3 +
-         f0 * fac01 * b0 + fs * fac01 * b1 + f0 * fac11 * b2 + fs * fac11 * b3;
-}
-
-// taug and taur of one g-point at one layer. R = double reproduces the reference's FP64
-// arithmetic; R = float is the gas optics of the mixed form (FP32 copies of the tables).
-template <typename R>
-__device__ inline void gas_optics(const GPoint &g, const R *__restrict__ tab, ColView &v, int lay,
-                                  bool lower, R &taug, R &taur) {
-  // A layer below laytrop has jp <= 12 and one above has jp >= 13 whenever pressure decreases
-  // with height; the clamps keep table reads in bounds for profiles that break that order.
-  const int jp = lower ? min(v.jp[lay], 12) : max(v.jp[lay], 13);
-  const int jt = v.jt[lay], jt1 = v.jt1[lay];
-  const R fac00 = R(v.f(0, lay)), fac01 = R(v.f(1, lay)), fac10 = R(v.f(2, lay)), fac11 = R(v.f(3, lay));
-  const R colh2o = R(v.c(SP_H2O, lay)), colmol = R(v.c(5, lay));
-  if (lower) {
-    bool bin = g.lo_mode == KEY_BIN;
-    SpecT<R> s = key_spec_t<R>(R(v.c(g.lo_s1, lay)), R(v.c(g.lo_s2, lay)), R(g.lo_strrat), R(8), bin);
-    R kmul = bin ? s.speccomb : R(v.c(g.lo_s1, lay)) * R(g.lo_mult);
-    int nsp = g.lo_nsp;
-    int ind0 = ((jp - 1) * 5 + (jt - 1)) * nsp + s.js;
-    int ind1 = (jp * 5 + (jt1 - 1)) * nsp + s.js;
-    R k = kinterp<R>(tab + g.absa, g.sa, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
-    const R *S = tab + g.selfref;
-    const R *F = tab + g.forref;
-    int is = max(v.inds[lay], 1), jf = min(v.indf[lay], 2);
-    R s0 = S[(is - 1) * g.sself], s1 = S[is * g.sself];
-    R f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
-    R cont = R(v.f(4, lay)) * (s0 + R(v.f(5, lay)) * (s1 - s0)) + R(v.f(6, lay)) * (f0 + R(v.f(7, lay)) * (f1 - f0));
-    taug = kmul * k + colh2o * cont + R(v.c(g.lo_minor, lay)) * R(g.lo_minor_coef);
-    R ray = R(g.rayl_lo);
-    if (g.rayl_js) {
-      const R *RA = tab + g.rayla;
-      R r0 = RA[(s.js - 1) * g.sra], r1 = RA[s.js * g.sra];
-      ray = r0 + s.fs * (r1 - r0);
-    }
-    taur = colmol * ray;
-  } else {
-    bool bin = g.hi_mode == KEY_BIN;
-    SpecT<R> s = key_spec_t<R>(R(v.c(g.hi_s1, lay)), R(v.c(g.hi_s2, lay)), R(g.hi_strrat), R(4), bin);
-    R kmul = bin ? s.speccomb : R(v.c(g.hi_s1, lay)) * R(g.hi_mult);
-    int nsp = g.hi_nsp;
-    int ind0 = ((jp - 13) * 5 + (jt - 1)) * nsp + s.js;
-    int ind1 = ((jp - 12) * 5 + (jt1 - 1)) * nsp + s.js;
-    R k = kinterp<R>(tab + g.absb, g.sb, ind0, ind1, bin ? 1 : 0, nsp, s.fs, fac00, fac10, fac01, fac11);
-    R fo = R(0);
-    if (g.hi_for) {
-      const R *F = tab + g.forref;
-      int jf = min(max(v.indf[lay], 1), 3);
-      R f0 = F[(jf - 1) * g.sfor], f1 = F[jf * g.sfor];
-      fo = colh2o * R(v.f(6, lay)) * (f0 + R(v.f(7, lay)) * (f1 - f0));
-    }
-    taug = kmul * k + fo + R(v.c(g.hi_minor, lay)) * R(g.hi_minor_coef);
-    taur = colmol * R(g.rayl_hi);
-  }
-}
-
 // Solar source of the lane's g-point at its band's laysolfr (src/rrtmg_sw_taumol.f90, e.g.
 // :366-375 plain, :495-520 interpolated in the binary species parameter), times
 // adjflux * cos(sza): zincflx of src/rrtmg_sw_spcvmc.f90:349-356.
@@ -394,22 +328,20 @@ __device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars 
   return p;
 }
 
-__device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &sc, int band,
-                                        double cldf, double ciwp, double clwp, double rei,
-                                        double rel, double tauc, double ssac, double asmc,
-                                        double fsfc, int &err) {
-  CloudOpt o = {tauc, ssac, asmc};
-  double cwp = ciwp + clwp;
-  if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauc >= CLDMIN))) return o;
-  if (sc.inflag == 0) {  // :150-163
+// The optics of a cloudy cell once the cloud test has passed, inflag 0 (optical properties
+// given, delta-M scaled here) or 2 (from water paths and particle sizes). cldprmc_sw and
+// cldprop_sw evaluate the same expressions here (src/rrtmg_sw_cldprmc.f90:150-163, :297-337;
+// src/rrtmg_sw_cldprop.f90:161-174, :316-357).
+__device__ static CloudOpt cloud_optics_in(const DeviceTables &T, const SwScalars &sc, int band,
+                                           double ciwp, double clwp, double rei, double rel,
+                                           double tauc, double ssac, double asmc, double fsfc,
+                                           int &err) {
+  CloudOpt o;
+  if (sc.inflag == 0) {
     double ffp = fsfc, ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssac;
     o.ssa = ffp1 * ssac / ffpssa;
     o.tau = ffpssa * tauc;
     o.asm_ = (asmc - ffp) / ffp1;
-    return o;
-  }
-  if (sc.inflag != 2) {
-    err = RRTMGSW_ERR_INFLAG;
     return o;
   }
   CloudProps p = cloud_props(T, sc, band, ciwp, clwp, rei, rel, err);
@@ -434,6 +366,36 @@ __device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &
     o.asm_ = (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * (gice - forwice) / (1.0 - forwice)) /
              (scatliq + scatice);
   return o;
+}
+
+// cldprmc_sw for one (g, layer) (src/rrtmg_sw_cldprmc.f90:141-347): a cell failing the cloud test
+// keeps its McICA inputs; inflag = 1 is the reference's STOP.
+__device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &sc, int band,
+                                        double cldf, double ciwp, double clwp, double rei,
+                                        double rel, double tauc, double ssac, double asmc,
+                                        double fsfc, int &err) {
+  const CloudOpt keep = {tauc, ssac, asmc};
+  double cwp = ciwp + clwp;
+  if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauc >= CLDMIN))) return keep;
+  if (sc.inflag != 0 && sc.inflag != 2) {
+    err = RRTMGSW_ERR_INFLAG;
+    return keep;
+  }
+  return cloud_optics_in(T, sc, band, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, err);
+}
+
+// cldprop_sw for one (layer, band) of the non-McICA path (src/rrtmg_sw_cldprop.f90:141-362): the
+// cloud test uses the layer's band-summed tauc (tauctot); a layer failing it, or an inflag other
+// than 0 and 2, gives the clear defaults tau = 0, ssa = 1, g = 0.
+__device__ static CloudOpt cloud_optics_layer(const DeviceTables &T, const SwScalars &sc, int band,
+                                              double cldf, double ciwp, double clwp, double rei,
+                                              double rel, double tauc, double ssac, double asmc,
+                                              double fsfc, double tauctot, int &err) {
+  const CloudOpt none = {0.0, 1.0, 0.0};
+  double cwp = ciwp + clwp;
+  if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauctot >= CLDMIN))) return none;
+  if (sc.inflag != 0 && sc.inflag != 2) return none;
+  return cloud_optics_in(T, sc, band, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, err);
 }
 
 // Adjoint of cloud_optics (inflag = 2) with respect to the in-cloud water paths, given the
@@ -480,101 +442,10 @@ __device__ inline double frcp(double x) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Exponential-table transmittance (e.g. src/rrtmg_sw_spcvmc.f90:558-566, reftra :203-209).
-// ------------------------------------------------------------------------------------------
-__device__ inline double tbl_exp(double ze, const double *__restrict__ exp_tbl) {
-#pragma clang fp contract(off)
-  if (ze <= OD_LO) return 1. - ze + 0.5 * ze * ze;
-  double tblind = ze / (BPADE + ze);
-  int itind = (int)(TBLINT * tblind + 0.5);
-  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);   // NaN or negative input stays in the table
-  return exp_tbl[itind];
-}
-
-// Transmittance and its reciprocal: the reciprocal comes from a table of 1/exp_tbl (built on
-// the host with IEEE division, so it equals the division the reference performs).
-struct Ex {
-  double e, r;
-};
-__device__ inline Ex tbl_exp2(double ze, const double *__restrict__ exp_tbl,
-                              const double *__restrict__ rexp_tbl) {
-#pragma clang fp contract(off)
-  if (ze <= OD_LO) {
-    double e = 1. - ze + 0.5 * ze * ze;
-    return {e, 1. / e};
-  }
-  double tblind = ze / (BPADE + ze);
-  int itind = (int)(TBLINT * tblind + 0.5);
-  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
-  return {exp_tbl[itind], rexp_tbl[itind]};
-}
-
-// reftra_sw for one layer (src/rrtmg_sw_reftra.f90:151-327, PIFM kmodts = 2). Algebraically the
-// reference's expressions with fewer divisions: 1/mu0 is passed in (rmu), exponentials come
-// with their reciprocals, R and T share one reciprocal of the common denominator, and
-// 1/((1 - beta e^-2k tau)(k + g1)) is formed as 1/((k + g1) - (g1 - k) e^-2k tau).
+// Layer reflectance, diffuse reflectance, transmittance and diffuse transmittance.
 struct RT4 {
   double r, rd, t, td;
 };
-__device__ inline RT4 reftra1(double zto1, double zw, double zg, double prmuz, double rmu,
-                              const double *__restrict__ exp_tbl,
-                              const double *__restrict__ rexp_tbl) {
-  const double zwcrit = 0.9999995, eps = 1.e-08;
-  RT4 o;
-  double zg3 = 3. * zg;
-  double zgamma1 = (8. - zw * (5. + zg3)) * 0.25;
-  double zgamma2 = 3. * (zw * (1. - zg)) * 0.25;
-  double zgamma3 = (2. - zg3 * prmuz) * 0.25;
-  double zgamma4 = 1. - zgamma3;
-  // zwo = zw / (1 - (1-zw) (zg/(1-zg))^2) >= zwcrit, compared without dividing
-  double omg = 1. - zg;
-  double den2 = omg * omg - (1. - zw) * zg * zg;   // denom * (1-zg)^2
-  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
-  if (cons) {  // conservative scattering, :188-228
-    double za1 = zgamma1 * prmuz - zgamma3;
-    double zgt = zgamma1 * zto1;
-    double ze2 = tbl_exp2(fmin(zto1 * rmu, 500.), exp_tbl, rexp_tbl).e;
-    double inv = 1. / (1. + zgt);
-    o.r = (zgt - za1 * (1. - ze2)) * inv;
-    o.t = 1. - o.r;
-    o.rd = zgt * inv;
-    o.td = 1. - o.rd;
-    if (ze2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
-  } else {     // general two-stream solution, :233-322
-    double za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
-    double za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
-    double zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
-    double zrp = zrk * prmuz;
-    double zrp1 = 1. + zrp, zrm1 = 1. - zrp, zrk2 = 2. * zrk;
-    double zrpp = 1. - zrp * zrp;
-    double zrkg = zrk + zgamma1;
-    double zr1 = zrm1 * (za2 + zrk * zgamma3);
-    double zr2 = zrp1 * (za2 - zrk * zgamma3);
-    double zr3 = zrk2 * (zgamma3 - za2 * prmuz);
-    double zr4 = zrpp * zrkg;
-    double zr5 = zrpp * (zrk - zgamma1);
-    double zt1 = zrp1 * (za1 + zrk * zgamma4);
-    double zt2 = zrm1 * (za1 - zrk * zgamma4);
-    double zt3 = zrk2 * (zgamma4 + za1 * prmuz);
-    Ex e1 = tbl_exp2(fmin(zrk * zto1, 500.), exp_tbl, rexp_tbl);
-    Ex e2 = tbl_exp2(fmin(zto1 * rmu, 500.), exp_tbl, rexp_tbl);
-    double zem1 = e1.e, zep1 = e1.r, zem2 = e2.e, zep2 = e2.r;
-    double zdenr = zr4 * zep1 + zr5 * zem1;
-    if (zdenr >= -eps && zdenr <= eps) {
-      o.r = eps;
-      o.t = zem2;
-    } else {
-      double rden = 1. / zdenr;
-      o.r = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) * rden;
-      o.t = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) * rden;
-    }
-    double zemm = zem1 * zem1;
-    double zdend = 1. / (zrkg - (zgamma1 - zrk) * zemm);
-    o.rd = zgamma2 * (1. - zemm) * zdend;
-    o.td = zrk2 * zem1 * zdend;
-  }
-  return o;
-}
 
 // ------------------------------------------------------------------------------------------
 // Exact-exponential reftra_sw: the smooth function of (tau, w, g) that the adjoint
@@ -825,7 +696,9 @@ __device__ inline R ldt(const R *__restrict__ base, uint32_t i) {
                                       (size_t)(i * (uint32_t)sizeof(R)));
 }
 
-// Exponential-table transmittance alone (tbl_exp with the index quotient from frcp).
+// Exponential-table transmittance (e.g. src/rrtmg_sw_spcvmc.f90:558-566, reftra :203-209): the
+// table entry at int(1e4 ze / (bpade + ze) + 0.5) (the quotient from frcp), or 1 - ze + ze^2/2
+// below od_lo.
 __device__ inline double tbl_exp1(double ze, const double *__restrict__ exp_tbl) {
   double tblind = ze * frcp(BPADE + ze);
   int itind = (int)(TBLINT * tblind + 0.5);
@@ -967,8 +840,8 @@ __device__ inline float uni(float x) {
 }
 __device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// The gas optics in two halves so that a layer's table reads can be issued one layer ahead:
-// gas_fetch computes the key-species parameter and issues every read, gas_finish combines them.
+// The gas optics in two halves: gas_fetch computes the key-species parameter and issues every
+// table read, gas_finish combines them (callers place their pending stores in between).
 template <typename R>
 struct GasFetch {
   R a0, a1, a2, a3, b0, b1, b2, b3, fo0, fo1, s0, s1, r0, r1;

@@ -55,8 +55,7 @@ struct GPoint {
 struct DeviceTables {
   const double *tab;        // k-tables, solar tables, cloud optics
   const double *exp_tbl;    // NTBL+1 entries
-  const double *rexp_tbl;   // 1 / exp_tbl, NTBL+1 entries
-  const float *tab32, *exp32, *rexp32;   // FP32 copies of tab, exp_tbl, rexp_tbl (mixed form)
+  const float *tab32, *exp32, *rexp32;   // FP32 copies of tab, exp_tbl and 1 / exp_tbl (mixed form)
   const GPoint *gp;         // NGPT records
   // cloud optics offsets (tables (n, 16:29) column-major: value(i, band) at off + (i-1) + n*(band-16))
   int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;
@@ -99,6 +98,8 @@ struct RtArgs {
   // fused path: physical cloud inputs of mcica_subcol_sw
   int fused, mc_icld, mc_irng, permuteseed;
   const double *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc, *ssac, *asmc, *fsfc, *alpha;
+  // non-McICA path: the physical inputs above (no alpha) are the layer clouds of cldprop_sw
+  int layer_clouds;
   const uint32_t *mt_stream;     // Mersenne-Twister draws for irng = 1 (fused path)
   // aerosol
   const double *tauaer, *ssaaer, *asmaer, *ecaer;

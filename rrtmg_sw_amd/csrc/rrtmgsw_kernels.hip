@@ -8,7 +8,8 @@
 //   prologue  inatm_sw + setcoef_sw for every layer of the column (lanes = layers), laytrop,
 //             laysolfr of each band (src/rrtmg_sw_rad.f90:1531-1586, src/rrtmg_sw_setcoef.f90)
 //   masks     (fused entry) McICA sub-column cloud masks for the lane's g-point, bottom to top
-//             (src/mcica_subcol_gen_sw.f90:468-659)
+//             (src/mcica_subcol_gen_sw.f90:468-659); the non-McICA entry instead takes each
+//             layer's cloud fraction for every g-point (src/rrtmg_sw_spcvrt.f90:429, :453-454)
 //   pass 1    top to bottom: gas optics (one data-driven template for the fourteen taumolNN
 //             routines), cloud optics (cldprmc_sw), delta scaling, two-stream layer
 //             reflectance / transmittance (reftra_sw), direct beam, and the downward half of
@@ -59,8 +60,13 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 
 // ------------------------------------------------------------------------------------------
 // FP64 forms: MODE 0 = the reference's exponential table, MODE 2 = exact exponentials (the
-// smooth forward the adjoint differentiates). FUSED: McICA masks generated here from the
-// physical cloud inputs; otherwise the materialised (112, ncol, nlay) sub-column arrays are read.
+// smooth forward the adjoint differentiates). CLD, where the clouds come from:
+//   CLD_SUBCOL  the materialised (112, ncol, nlay) McICA sub-column arrays (rrtmg_sw), cloud
+//               optics per cell (cldprmc_sw);
+//   CLD_FUSED   McICA masks generated here from the physical cloud inputs (mcica_subcol_sw +
+//               rrtmg_sw), cloud optics per (layer, band);
+//   CLD_LAYER   the non-McICA driver (src/rrtmg_sw_rad.nomcica.f90): every g-point sees the
+//               layer's cloud fraction and its band's cldprop_sw optics (spcvrt_sw).
 //
 // Pass 1 runs the layers above laytrop and those below it as two loops, each with only its
 // regime's gas-optics constants (GasLane) live. Per layer and g: gas optics, aerosol, cloud
@@ -69,13 +75,16 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 // over g by DPP.
 // Pass 2 requests the coefficients of layer j-1 while layer j is processed.
 // ------------------------------------------------------------------------------------------
-// Occupancy target: four waves per SIMD for the fused table form (measured fastest, a few VGPRs
-// spilled outside the layer loop), three for the others.
-template <int MODE, bool FUSED>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && FUSED ? 4 : 3)))
+// Occupancy target: four waves per SIMD for the table form with per-(layer, band) cloud optics
+// (measured fastest, a few VGPRs spilled outside the layer loop), three for the others.
+enum : int { CLD_SUBCOL = 0, CLD_FUSED = 1, CLD_LAYER = 2 };
+template <int MODE, int CLD>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && CLD != CLD_SUBCOL ? 4 : 3)))
 rt_kernel(RtArgs a, DeviceTables T) {
   constexpr bool EXACT = MODE == 2;
   constexpr bool MIXED = MODE == 1;
+  constexpr bool FUSED = CLD == CLD_FUSED;
+  constexpr bool LAYERCLD = CLD == CLD_LAYER;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
   ColLayout lo(L, FUSED ? (L + 31) / 32 : 0);
@@ -94,7 +103,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
   const int gi = active ? g : wave * WAVE_G;
 
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
-  if (tid == 0) v.misc[M_LAYTROP] = 0;
+  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0;
   if (MIXED)
     for (int w = tid; w < (L + 31) / 32; w += blockDim.x) v.thin[w] = 0u;
   __syncthreads();
@@ -115,12 +124,36 @@ rt_kernel(RtArgs a, DeviceTables T) {
   if (FUSED && a.sc.icld >= 1 && active)
     err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
                        a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
+  if (LAYERCLD && a.sc.icld >= 1)   // src/rrtmg_sw_rad.nomcica.f90:617-622
+    for (int lay = tid; lay < L; lay += blockDim.x) {
+      const double f = a.cldfrac[c + (size_t)C * lay];
+      if (f > 1.e-06 && f < 1.0 - 1.e-06) v.misc[M_PARTIAL] = 1;
+    }
   __syncthreads();
   // Fused entry: every cloudy sub-column cell of a band sees the same cloud inputs (the layer's
   // water paths and radii, the band's tauc/ssac/asmc/fsfc), so cldprmc_sw's result depends on
   // (layer, band) only. It is evaluated here for each (layer, band) with at least one cloudy
   // sub-column (the cells cldprmc_sw visits, so its STOP conditions fire as in the reference).
   double *csc = a.scratch + (size_t)cl * scr_col_doubles(L) + (size_t)L * SCR_COEFS * NGPT;
+  if (LAYERCLD && a.sc.icld >= 1) {
+    // cldprop_sw (src/rrtmg_sw_cldprop.f90:141-362) for every (layer, band): its cloud test
+    // reads the layer's tauc summed over the bands in band order (tauctot). The reference stops
+    // on a partial cloud before calling it, so that error takes precedence in the column.
+    const bool partial = v.misc[M_PARTIAL] != 0;
+    if (partial && tid == 0) err = RRTMGSW_ERR_PARTIAL_CLOUD;
+    for (int idx = partial ? L * NBND : tid; idx < L * NBND; idx += blockDim.x) {
+      const int lay = idx / NBND, b = idx - lay * NBND;
+      const size_t q1 = c + (size_t)C * lay;
+      double tauctot = 0.0;
+      for (int bb = 0; bb < NBND; bb++) tauctot = tauctot + a.tauc[bb + NBND * q1];
+      const size_t qb = b + NBND * q1;
+      CloudOpt co = cloud_optics_layer(T, a.sc, b + 16, a.cldfrac[q1], a.ciwp[q1], a.clwp[q1], a.rei[q1],
+                                       a.rel[q1], a.tauc[qb], a.ssac[qb], a.asmc[qb], a.fsfc[qb], tauctot, err);
+      double *o = csc + (size_t)idx * 3;
+      o[0] = co.tau, o[1] = co.ssa, o[2] = co.asm_;
+    }
+    __syncthreads();
+  }
   if (FUSED && a.sc.icld >= 1) {
     const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};
     for (int idx = tid; idx < L * NBND; idx += blockDim.x) {
@@ -204,7 +237,13 @@ rt_kernel(RtArgs a, DeviceTables T) {
         const double *o = csc + ((size_t)lay * NBND + ibnd) * 3;
         in.cf = 1.0, in.ct = o[0], in.cw = o[1], in.cg = o[2];
       }
-    } else if (!FUSED && a.sc.icld >= 1 && active) {
+    } else if (LAYERCLD && a.sc.icld >= 1 && active) {
+      const double f = a.cldfrac[c + (size_t)C * lay];
+      if (f != 0.0) {
+        const double *o = csc + ((size_t)lay * NBND + ibnd) * 3;
+        in.cf = f, in.ct = o[0], in.cw = o[1], in.cg = o[2];
+      }
+    } else if (CLD == CLD_SUBCOL && a.sc.icld >= 1 && active) {
       const size_t q1 = c + (size_t)C * lay, qg = (size_t)g + NGPT * q1;
       in.cf = a.cldfmcl[qg];
       const double ciwp = a.ciwpmcl[qg], clwp = a.clwpmcl[qg];
@@ -249,7 +288,8 @@ rt_kernel(RtArgs a, DeviceTables T) {
     }
     Ac = sc_.ztdn;
     const double Bc = sc_.prdnd;
-    if (cf > 0.0 && jfirst == L) {
+    // a cloud fraction other than zero (negative ones included) enters the clear / cloudy blend
+    if (cf != 0.0 && jfirst == L) {
       jfirst = j;
       st_ = sc_;
     }
@@ -261,7 +301,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
       At = st_.ztdn, Bt = st_.prdnd;
       RT4 rt = rc;
       double dbt = dbc;
-      if (cf > 0.0) {
+      if (cf != 0.0) {
         // total-sky optics (:500-505): go = (ct cw cg + tauc omc gcc) / so, omo = so / tauo
         const double tauo = tauc + ct;
         const double so = tauc * omc + ct * cw;
@@ -337,7 +377,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
     const float dbc = tbl_expf(tauc * rmuf, T.exp32, T.rexp32).e;
     Ac = sc_.ztdn;
     const double Bc = sc_.prdnd;
-    if (in.cf > 0.0 && jfirst == L) {
+    if (in.cf != 0.0 && jfirst == L) {
       jfirst = j;
       st_ = sc_;
     }
@@ -352,7 +392,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
       At = st_.ztdn, Bt = st_.prdnd;
       RT4f rt = rc;
       float dbt = dbc;
-      if (in.cf > 0.0) {
+      if (in.cf != 0.0) {
         const float dbo = tbl_expf(tauo * rmuf, T.exp32, T.rexp32).e;
         RT4f ro = {0.f, 0.f, 1.f, 1.f};
         if (in.cf > REPCLC) ro = reftra_f(tauo, omo, cco, go, mu0f, rmuf, T.exp32, T.rexp32);
@@ -472,23 +512,24 @@ rt_kernel(RtArgs a, DeviceTables T) {
   rt_epilogue(a, v, c);
 }
 
-template <int MODE, bool FUSED>
+template <int MODE, int CLD>
 static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  ColLayout lo(a.nlay, FUSED ? (a.nlay + 31) / 32 : 0);
+  ColLayout lo(a.nlay, CLD == CLD_FUSED ? (a.nlay + 31) / 32 : 0);
   int per = (a.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL((rt_kernel<MODE, FUSED>), dim3(per * 8), dim3(128), lo.total, s, a, t);
+  hipLaunchKernelGGL((rt_kernel<MODE, CLD>), dim3(per * 8), dim3(128), lo.total, s, a, t);
   return hipGetLastError();
 }
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  return a.fused ? launch_rt<0, true>(a, t, s) : launch_rt<0, false>(a, t, s);
+  if (a.layer_clouds) return launch_rt<0, CLD_LAYER>(a, t, s);
+  return a.fused ? launch_rt<0, CLD_FUSED>(a, t, s) : launch_rt<0, CLD_SUBCOL>(a, t, s);
 }
 hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   // the mixed form is offered by the fused entry point only
-  return a.fused ? launch_rt<1, true>(a, t, s) : hipErrorInvalidValue;
+  return a.fused && !a.layer_clouds ? launch_rt<1, CLD_FUSED>(a, t, s) : hipErrorInvalidValue;
 }
 hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   // the exact-exponential form is offered by the fused entry point only
-  return a.fused ? launch_rt<2, true>(a, t, s) : hipErrorInvalidValue;
+  return a.fused && !a.layer_clouds ? launch_rt<2, CLD_FUSED>(a, t, s) : hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------------------------------------

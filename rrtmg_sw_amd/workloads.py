@@ -15,6 +15,8 @@ Named workloads:
   expran4096x60  4096 columns x 60 layers, exponential-random overlap (icld=5, decorrelation
                  length U(1,4) km per column) plus aerosols (iaer=10) below 3 km.
   expran65536x137  as expran4096x60 at 65536 columns x 137 layers.
+  overcast8192x72  the columns of mcica8192x72 for the non-McICA driver, which takes clear and
+                 overcast layers only: every cloudy layer has cloud fraction 1.
 
 Choices the workload descriptions leave open, applied identically to every implementation:
 level pressures log-spaced from the surface value to 0.01 hPa, layer pressure the geometric
@@ -41,6 +43,7 @@ WORKLOADS = {
     "mcica8192x72": dict(ncol=8192, nlay=72, icld=2, iaer=0, seed=1),
     "expran4096x60": dict(ncol=4096, nlay=60, icld=5, iaer=10, seed=3),
     "expran65536x137": dict(ncol=65536, nlay=137, icld=5, iaer=10, seed=4),
+    "overcast8192x72": dict(ncol=8192, nlay=72, icld=2, iaer=0, seed=1, overcast=True),
 }
 
 
@@ -221,10 +224,21 @@ def alpha_for(cols):
     return a
 
 
+def overcast(cols):
+    """The non-McICA driver's clouds from these columns: src/rrtmg_sw_rad.nomcica.f90:617-622
+    stops on a cloud fraction between 1e-6 and 1 - 1e-6, so every layer with a cloud fraction
+    above zero becomes overcast (fraction 1); water paths and radii are kept."""
+    out = Columns(**{k: v for k, v in cols.__dict__.items()})
+    out.cldfrac = _f(np.where(cols.cldfrac > 0.0, 1.0, 0.0))
+    return out
+
+
 def workload(name, ncol=None):
     """Columns of a named workload (optionally only its first `ncol` columns)."""
     w = dict(WORKLOADS[name])
     cols = make_columns(w["ncol"], w["nlay"], icld=w["icld"], iaer=w["iaer"], seed=w["seed"])
+    if w.get("overcast"):
+        cols = overcast(cols)
     if ncol is not None and ncol < cols.ncol:
         cols = cols.subset(slice(0, ncol))
     return cols

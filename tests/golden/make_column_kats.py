@@ -23,8 +23,13 @@ import numpy as np
 # The five ...-sza45-isolvar* cases describe their atmosphere with IATM = 1 (layers built by
 # the LBLRTM-style generator src/rrtatm.f from a model atmosphere); they are not restated
 # here, so only the six IATM = 0 cases (layer amounts given directly) become fixtures.
-CASES = ["MLS-clr", "MLS-clr-aer12", "MLS-clr-sza65", "MLW-clr", "SAW-clr", "TROP-clr"]
+# The one cloudy case without McICA (imca = 0: cldprop_sw + spcvrt_sw, overcast layers only)
+# is kept too; its input file and cloud file are named apart from its output file.
+CASES = ["MLS-clr", "MLS-clr-aer12", "MLS-clr-sza65", "MLW-clr", "SAW-clr", "TROP-clr",
+         "MLS-cld6-imca0-icld2"]
 AER = {"MLS-clr-aer12": "in_aer_rrtm-aer12"}
+INPUT = {"MLS-cld6-imca0-icld2": "input_rrtm_MLS-cld-imca0-icld2"}
+CLD = {"MLS-cld6-imca0-icld2": "in_cld_rrtm-cld6"}
 IB1, IB2 = 16, 29
 
 
@@ -56,7 +61,7 @@ def read_input(path):
         lines[i + 1], [("x", 18), ("i", 2), ("x", 29), ("i", 1), ("x", 32), ("i", 1), ("x", 1),
                        ("i", 1), ("x", 2), ("i", 3), ("x", 3), ("i", 1), ("i", 1), ("x", 3),
                        ("i", 1), ("i", 1)])
-    assert iatm == 0 and icld == 0, path
+    assert iatm == 0 and (icld == 0 or imca == 0), path
     # 9020 (12x, i3, 3x, f7.4, 3x, i2, f10.4, f10.5, 14f10.5)
     v = fields(lines[i + 2], [("x", 12), ("i", 3), ("x", 3), ("f", 7), ("x", 3), ("i", 2),
                               ("f", 10), ("f", 10)] + [("f", 10)] * 14)
@@ -111,7 +116,7 @@ def read_input(path):
     if isolvar in (-1, 3):
         bndsolvar = solvar.copy()
     zenith = math.cos(sza * (2.0 * math.asin(1.0)) / 180.0)
-    return dict(iaer=iaer, iout=iout, imca=imca, idelm=idelm, juldat=juldat, sza=sza,
+    return dict(iaer=iaer, iout=iout, imca=imca, icld=icld, idelm=idelm, juldat=juldat, sza=sza,
                 isolvar=isolvar, scon=scon, solcycfrac=solcycfrac, indsolvar=indsolvar,
                 bndsolvar=bndsolvar, semiss=semiss, nlay=nlay, pavel=pavel, tavel=tavel,
                 pz=pz, altz=altz, coldry=coldry, wkl=wkl, zenith=zenith)
@@ -181,6 +186,25 @@ def read_aer(path, nlay, zenith):
     return tau, ssa_l, asm_l
 
 
+def read_cld(path, nlay):
+    """readcld (src/rrtmg_sw.1col.f90:1884-1958) for inflag 2, with the conversion of
+    :1807-1834 to ice and liquid water paths; arrays from the surface up."""
+    lines = open(path).read().split("\n")
+    inflag, iceflag, liqflag = fields(lines[0], [("x", 3), ("i", 2), ("x", 4), ("i", 1), ("x", 4), ("i", 1)])
+    assert inflag == 2, path
+    cldfrac, cwp, fice, rei, rel = (np.zeros(nlay) for _ in range(5))
+    for line in lines[1:]:
+        ctest, lay, frac, d1, d2, d3, d4 = fields(line, [("a", 1), ("x", 1), ("i", 3)] + [("f", 10)] * 5)
+        if ctest == "%" or not line.strip():
+            break
+        cldfrac[lay - 1], cwp[lay - 1], fice[lay - 1], rei[lay - 1], rel[lay - 1] = frac, d1, d2, d3, d4
+    ciwp = cwp * fice
+    clwp = cwp * (1.0 - fice)
+    return dict(inflag=inflag, iceflag=iceflag, liqflag=liqflag, cldfrac=cldfrac, ciwp=ciwp,
+                clwp=clwp, rei=rei, rel=rel, tauc=np.zeros((14, nlay)), ssac=np.ones((14, nlay)),
+                asmc=np.zeros((14, nlay)), fsfc=np.zeros((14, nlay)))
+
+
 def read_output(path, nlay):
     """All flux blocks: dict wavenumber range -> (nlay+1, 6) from the surface up."""
     lines = open(path).read().split("\n")
@@ -206,7 +230,7 @@ def main(ref):
     d = os.path.join(ref, "run_examples_std_atm")
     out = {}
     for name in CASES:
-        c = read_input(os.path.join(d, "input_rrtm_" + name))
+        c = read_input(os.path.join(d, INPUT.get(name, "input_rrtm_" + name)))
         L = c["nlay"]
         if c["iaer"] == 10:
             tau, ssa, asm = read_aer(os.path.join(d, AER[name]), L, c["zenith"])
@@ -221,6 +245,8 @@ def main(ref):
             else:
                 b = 1 + WAVENUM2.index(w2)
             exp[b], have[b] = rows, True
+        if name in CLD:
+            c.update(read_cld(os.path.join(d, CLD[name]), L))
         for k, v in c.items():
             out[f"{name}/{k}"] = np.asarray(v)
         out[f"{name}/tauaer"] = tau

@@ -4,7 +4,7 @@ reference's own sources by oracle/build_ref.sh). Each fixture holds the inputs (
 column count and seed recipe are enough to regenerate them, so only outputs are stored) and the
 reference's outputs. Run in a container where the reference tree and oracle/_ref exist:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 import os
 import sys
@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
 import oracle_bind as ob  # noqa: E402
-from rrtmg_sw_amd.workloads import make_columns, workload  # noqa: E402
+from rrtmg_sw_amd.workloads import make_columns, overcast, workload  # noqa: E402
 
 CASES = {
     # name: (builder, kwargs)
@@ -24,17 +24,37 @@ CASES = {
     "mcica8x60_icld5_kiss_aer": ("make", dict(ncol=8, nlay=60, icld=5, iaer=10, seed=3)),
     "mcica4x40_icld2_mt": ("make", dict(ncol=4, nlay=40, icld=2, iaer=0, seed=9, irng=1)),
 }
+# rrtmg_sw of the non-McICA driver (oracle/_ref/librrtmg_ref_nomcica.so), clear and overcast
+# layers with cldprop_sw optics
+NOMCICA_CASES = {
+    "nomcica12x72_overcast": ("workload", dict(name="overcast8192x72", ncol=12)),
+    "nomcica8x60_overcast_aer": ("make", dict(ncol=8, nlay=60, icld=5, iaer=10, seed=3, overcast=True)),
+}
 
 
 def build(kind, kw):
     if kind == "workload":
         return workload(kw["name"], ncol=kw.get("ncol"))
-    return make_columns(**kw)
+    kw = dict(kw)
+    oc = kw.pop("overcast", False)
+    cols = make_columns(**kw)
+    return overcast(cols) if oc else cols
 
 
-def main():
-    assert ob.ref_available(), "oracle/_ref not built"
+def main(only=None):
+    assert ob.ref_available() and ob.ref_nomcica_available(), "oracle/_ref not built"
+    for name, (kind, kw) in NOMCICA_CASES.items():
+        if only and name not in only:
+            continue
+        cols = build(kind, kw)
+        out, _, _ = ob.run_nomcica(cols, "ref")
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **{k: out[k] for k in ("swuflx", "swdflx", "swuflxc", "swdflxc",
+                                                         "swhr", "swhrc")})
+        print(path, os.path.getsize(path))
     for name, (kind, kw) in CASES.items():
+        if only and name not in only:
+            continue
         cols = build(kind, kw)
         rec = {}
         if cols.icld:
@@ -51,4 +71,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

@@ -9,6 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BLOB = os.path.join(ROOT, "rrtmg_sw_amd", "data", "rrtmg_sw_tables.bin")
 ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "librrtmg_ref.so")
+REF_NM_PATH = os.path.join(ROOT, "oracle", "_ref", "librrtmg_ref_nomcica.so")
 CPDAIR = 1004.64
 NGPT, NBND = 112, 14
 
@@ -30,6 +31,7 @@ def _ip_of(a):
 
 _orc = None
 _ref = None
+_ref_nm = None
 
 
 def orc():
@@ -66,6 +68,61 @@ def ref():
         lib.ref_init(CPDAIR)
         _ref = lib
     return _ref
+
+
+def ref_nomcica_available():
+    return os.path.exists(REF_NM_PATH)
+
+
+def ref_nomcica():
+    """The compiled reference's non-McICA driver (its own library: the module names clash)."""
+    global _ref_nm
+    if _ref_nm is None:
+        lib = C.CDLL(REF_NM_PATH)
+        lib.ref_nm_init.argtypes = [C.c_double]
+        lib.ref_nm_init(CPDAIR)
+        _ref_nm = lib
+    return _ref_nm
+
+
+def run_nomcica(cols, which="orc", icld=None, iaer=None, solvar=None):
+    """rrtmg_sw of the non-McICA driver on `cols` (layer clouds cols.cldfrac, cols.tauc, ...);
+    solvar = (bndsolvar, indsolvar, solcycfrac) or None. Returns outputs and written-back flags;
+    raises RuntimeError with the STOP message for the oracle's non-zero returns."""
+    N, L = cols.ncol, cols.nlay
+    out = {k: _out((N, L + 1)) for k in ("swuflx", "swdflx", "swuflxc", "swdflxc")}
+    out["swhr"] = _out((N, L))
+    out["swhrc"] = _out((N, L))
+    ic = C.c_int(cols.icld if icld is None else icld)
+    ia = C.c_int(cols.iaer if iaer is None else iaer)
+    common = [C.c_int(N), C.c_int(L), C.byref(ic), C.byref(ia), _p(cols.play), _p(cols.plev),
+              _p(cols.tlay), _p(cols.tlev), _p(cols.tsfc), _p(cols.h2ovmr), _p(cols.o3vmr),
+              _p(cols.co2vmr), _p(cols.ch4vmr), _p(cols.n2ovmr), _p(cols.o2vmr), _p(cols.asdir),
+              _p(cols.asdif), _p(cols.aldir), _p(cols.aldif), _p(cols.coszen),
+              C.c_double(cols.adjes), C.c_int(cols.dyofyr), C.c_double(cols.scon),
+              C.c_int(cols.isolvar), C.c_int(cols.inflgsw), C.c_int(cols.iceflgsw),
+              C.c_int(cols.liqflgsw), _p(cols.cldfrac), _p(cols.tauc), _p(cols.ssac),
+              _p(cols.asmc), _p(cols.fsfc), _p(cols.ciwp), _p(cols.clwp), _p(cols.rei),
+              _p(cols.rel), _p(cols.tauaer), _p(cols.ssaaer), _p(cols.asmaer), _p(cols.ecaer),
+              _p(out["swuflx"]), _p(out["swdflx"]), _p(out["swhr"]), _p(out["swuflxc"]),
+              _p(out["swdflxc"]), _p(out["swhrc"])]
+    sv = None
+    if solvar is not None:
+        sv = [np.ascontiguousarray(solvar[0], dtype=np.float64),
+              np.ascontiguousarray(solvar[1], dtype=np.float64),
+              np.array([solvar[2]], dtype=np.float64)]
+    if which == "orc":
+        rc = orc().orc_rrtmg_sw_nomcica(*common, *([None] * 3 if sv is None else [_p(x) for x in sv]))
+        if rc:
+            raise RuntimeError(orc().orc_last_error().decode())
+    else:
+        if sv is None:
+            sv = [np.ones(NBND), np.ones(2), np.zeros(1)]
+            has = 0
+        else:
+            has = 1
+        ref_nomcica().ref_rrtmg_sw_nomcica(*common, C.c_int(has), *[_p(x) for x in sv])
+    return out, ic.value, ia.value
 
 
 def _out(shape):

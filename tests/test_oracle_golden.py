@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from golden.make_golden import CASES, build
+from golden.make_golden import CASES, NOMCICA_CASES, build
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -24,5 +24,15 @@ def test_oracle_matches_reference_golden(name):
     else:
         mc = ob.empty_mcica(cols)
     out = ob.run_rrtmg(cols, mc, which="orc", icld=min(cols.icld, 3))
+    for k in ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc"):
+        assert np.array_equal(out[k], g[k]), k
+
+
+@pytest.mark.parametrize("name", sorted(NOMCICA_CASES))
+def test_oracle_nomcica_matches_reference_golden(name):
+    kind, kw = NOMCICA_CASES[name]
+    cols = build(kind, kw)
+    g = np.load(os.path.join(HERE, "golden", f"{name}.npz"))
+    out, _, _ = ob.run_nomcica(cols, "orc")
     for k in ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc"):
         assert np.array_equal(out[k], g[k]), k

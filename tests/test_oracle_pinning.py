@@ -1,7 +1,9 @@
 """Pins of the C oracle that do not go through the compiled reference:
 
 * the reference's own column-mode example suite (run_examples_std_atm, six clear-sky IATM = 0
-  cases, committed as tests/golden/column_kats.npz by tests/golden/make_column_kats.py):
+  cases and the overcast non-McICA case MLS-cld6-imca0-icld2 (cldprop_sw with ice and liquid
+  clouds, spcvrt_sw), committed as tests/golden/column_kats.npz by
+  tests/golden/make_column_kats.py):
   broadband and per-band up / diffuse-down / direct-down / down / net flux and heating rate at
   every level, to the printed precision (fluxes 1e-4 W/m2, heating rates 1e-5 K/day);
 * MT19937's published known answer (the C++ standard's std::mt19937 check: the 10000th output
@@ -38,13 +40,25 @@ def _column(name):
             g("indsolvar").copy(), np.array([float(g("solcycfrac"))]),
             np.asfortranarray(g("tauaer")), np.asfortranarray(g("ssaaer")),
             np.asfortranarray(g("asmaer"))]
-    rc = lib.orc_column_sw(
+    common = [
         C.c_int(L), _dp(keep[0]), _dp(keep[1]), _dp(keep[2]), _dp(keep[3]), _dp(keep[4]),
         _dp(keep[5]), C.c_double(float(g("zenith"))), C.c_int(int(g("juldat"))),
         C.c_double(float(g("scon"))), C.c_int(int(g("isolvar"))), _dp(keep[6]), _dp(keep[7]),
         _dp(keep[8]), C.c_int(int(g("iaer"))), _dp(keep[9]), _dp(keep[10]), _dp(keep[11]),
-        C.c_int(int(g("idelm"))), C.c_double(1004.0), out.ctypes.data_as(ob._dp))
-    assert rc == 0
+        C.c_int(int(g("idelm"))), C.c_double(1004.0)]
+    if f"{name}/inflag" in KATS:
+        # imca = 0 cloudy case: cldprop_sw + spcvrt_sw on overcast layers
+        cl = [np.ascontiguousarray(g(k), dtype=np.float64)
+              for k in ("cldfrac", "ciwp", "clwp", "rei", "rel")]
+        cl += [np.asfortranarray(g(k), dtype=np.float64) for k in ("tauc", "ssac", "asmc", "fsfc")]
+        keep += cl
+        lib.orc_column_sw_cld.restype = C.c_int
+        rc = lib.orc_column_sw_cld(*common, C.c_int(int(g("inflag"))), C.c_int(int(g("iceflag"))),
+                                   C.c_int(int(g("liqflag"))), *[_dp(a) for a in cl],
+                                   out.ctypes.data_as(ob._dp))
+    else:
+        rc = lib.orc_column_sw(*common, out.ctypes.data_as(ob._dp))
+    assert rc == 0, ob.orc().orc_last_error()
     return out, g("expected"), g("have")
 
 
