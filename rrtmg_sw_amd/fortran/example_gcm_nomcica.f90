@@ -1,0 +1,50 @@
+! example_gcm_nomcica.f90 -- a host program that calls the non-McICA drop-in modules as a GCM
+! calls the reference's non-McICA driver: rrtmg_sw_ini once, then rrtmg_sw with layer clouds.
+! Same stream files as example_gcm (the McICA-only header fields permuteseed and irng and the
+! alpha array are read and ignored):
+!   example_gcm_nomcica <input.bin> <output.bin>
+program example_gcm_nomcica
+  use rrtmg_sw_init, only : rrtmg_sw_ini
+  use rrtmg_sw_rad, only : rrtmg_sw
+  implicit none
+  integer, parameter :: rb = selected_real_kind(12), nbndsw = 14, naerec = 6
+  integer :: hdr(11), ncol, nlay, icld, iaer, dyofyr, isolvar, inflgsw, iceflgsw, liqflgsw, u
+  real(rb) :: sc(2)
+  character(len=512) :: fin, fout
+  real(rb), allocatable :: play(:,:), plev(:,:), tlay(:,:), tlev(:,:), tsfc(:), h2o(:,:), o3(:,:)
+  real(rb), allocatable :: co2(:,:), ch4(:,:), n2o(:,:), o2(:,:), asdir(:), asdif(:), aldir(:)
+  real(rb), allocatable :: aldif(:), coszen(:), cldfr(:,:), ciwp(:,:), clwp(:,:), rei(:,:)
+  real(rb), allocatable :: rel(:,:), tauc(:,:,:), ssac(:,:,:), asmc(:,:,:), fsfc(:,:,:), alpha(:,:)
+  real(rb), allocatable :: tauaer(:,:,:), ssaaer(:,:,:), asmaer(:,:,:), ecaer(:,:,:)
+  real(rb), allocatable :: swuflx(:,:), swdflx(:,:), swhr(:,:), swuflxc(:,:), swdflxc(:,:), swhrc(:,:)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) hdr, sc
+  ncol = hdr(1); nlay = hdr(2); icld = hdr(3); iaer = hdr(4); dyofyr = hdr(7); isolvar = hdr(8)
+  inflgsw = hdr(9); iceflgsw = hdr(10); liqflgsw = hdr(11)
+  allocate(play(ncol,nlay), plev(ncol,nlay+1), tlay(ncol,nlay), tlev(ncol,nlay+1), tsfc(ncol))
+  allocate(h2o(ncol,nlay), o3(ncol,nlay), co2(ncol,nlay), ch4(ncol,nlay), n2o(ncol,nlay))
+  allocate(o2(ncol,nlay), asdir(ncol), asdif(ncol), aldir(ncol), aldif(ncol), coszen(ncol))
+  allocate(cldfr(ncol,nlay), ciwp(ncol,nlay), clwp(ncol,nlay), rei(ncol,nlay), rel(ncol,nlay))
+  allocate(tauc(nbndsw,ncol,nlay), ssac(nbndsw,ncol,nlay), asmc(nbndsw,ncol,nlay))
+  allocate(fsfc(nbndsw,ncol,nlay), alpha(ncol,nlay))
+  allocate(tauaer(ncol,nlay,nbndsw), ssaaer(ncol,nlay,nbndsw), asmaer(ncol,nlay,nbndsw))
+  allocate(ecaer(ncol,nlay,naerec))
+  read(u) play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, asdir, asdif, aldir, aldif, &
+          coszen, cldfr, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, alpha, tauaer, ssaaer, &
+          asmaer, ecaer
+  close(u)
+  allocate(swuflx(ncol,nlay+1), swdflx(ncol,nlay+1), swhr(ncol,nlay))
+  allocate(swuflxc(ncol,nlay+1), swdflxc(ncol,nlay+1), swhrc(ncol,nlay))
+
+  call rrtmg_sw_ini(1004.64_rb)
+  call rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, &
+                o2, asdir, asdif, aldir, aldif, coszen, sc(1), dyofyr, sc(2), isolvar, &
+                inflgsw, iceflgsw, liqflgsw, cldfr, tauc, ssac, asmc, fsfc, ciwp, clwp, rei, rel, &
+                tauaer, ssaaer, asmaer, ecaer, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) icld, iaer, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc
+  close(u)
+end program example_gcm_nomcica

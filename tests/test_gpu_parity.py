@@ -102,3 +102,54 @@ def test_mcica8192x72_slice_fused_device(gpu_lib):
     ref = ob.run_rrtmg(cols, mc)
     print("fused", {k: float(np.abs(got[k] - ref[k]).max()) for k in FLUX + HR})
     assert_close(got, ref, "fused")
+
+
+@pytest.mark.parametrize("juldat", [40, 200, 330])
+def test_get_alpha_bitexact(gpu_lib, juldat):
+    """get_alpha of the library (host code) against the oracle, icld 4/5 x idcor 0/1: the
+    overlap parameters decide McICA draws by comparison, so they must agree bit for bit."""
+    cols = make_columns(16, 50, icld=5, seed=11)
+    lat = np.linspace(-85.0, 85.0, 16)
+    for icld in (4, 5):
+        for idcor in (0, 1):
+            got = gpu_lib.get_alpha(1, 16, 50, icld, idcor, 2500.0, cols.dz, lat, juldat, cols.cldfrac)
+            ref = np.zeros((16, 50), order="F")
+            rc = ob.orc().orc_get_alpha(1, 16, 50, icld, idcor, ob.C.c_double(2500.0), ob._p(cols.dz),
+                                        ob._p(lat), juldat, ob._p(cols.cldfrac), ob._p(ref))
+            assert rc == 0
+            assert np.array_equal(got, ref), (icld, idcor, np.abs(got - ref).max())
+
+
+def _ecmwf(cols, seed):
+    cols.ecaer = np.asfortranarray(np.random.default_rng(seed).uniform(0.0, 0.3, cols.ecaer.shape))
+    cols.ecaer[:, ::7, :] = 0.0          # layers without aerosol take the zero-depth branch
+    cols.iaer = 6
+    return cols
+
+
+def test_ecmwf_aerosols_iaer6_host_abi(gpu_lib):
+    """iaer = 6: the six ECMWF aerosol types mixed with the rsrtaua/rsrpiza/rsrasya tables
+    (src/rrtmg_sw_rad.f90:719-752) on McICA sub-columns, through the host-array entry."""
+    cols = _ecmwf(make_columns(24, 60, icld=2, iaer=0, seed=91), 91)
+    mc, _ = ob.run_mcica(cols)
+    got = gpu_lib.rrtmg_sw(**sw_args(cols, mc))
+    ref = ob.run_rrtmg(cols, mc)
+    assert got["iaer"] == ref["iaer"] == 6
+    noaer = ob.run_rrtmg(cols, mc, iaer=0)
+    assert np.abs(ref["swdflx"] - noaer["swdflx"]).max() > 1.0       # the aerosol matters
+    print("iaer6", {k: float(np.abs(got[k] - ref[k]).max()) for k in FLUX + HR})
+    assert_close(got, ref, "iaer=6")
+
+
+def test_ecmwf_aerosols_iaer6_fused_device(gpu_lib):
+    import torch
+    from rrtmg_sw_amd import api
+    cols = _ecmwf(make_columns(64, 72, icld=5, iaer=0, seed=92), 92)
+    d = api.device_inputs(cols)
+    out = api.mcica_rrtmg_sw(cols, d)
+    torch.cuda.synchronize()
+    got = {k: api.to_host(v) for k, v in out.items()}
+    mc, _ = ob.run_mcica(cols)
+    ref = ob.run_rrtmg(cols, mc, icld=2)
+    print("iaer6 fused", {k: float(np.abs(got[k] - ref[k]).max()) for k in FLUX + HR})
+    assert_close(got, ref, "iaer=6 fused")
