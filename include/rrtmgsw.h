@@ -130,6 +130,29 @@ int rrtmgsw_sw_nomcica_dev(int ncol, int nlay, int *icld, int *iaer,
                            const double *bndsolvar, const double *indsolvar,
                            const double *solcycfrac, void *stream);
 
+/* The column program's radiative transfer without McICA (src/rrtmg_sw.1col.f90:540-1000 with
+ * imca = 0: cldprop_sw, setcoef_sw, spcvrt_sw), for the column-mode I/O driver
+ * (rrtmg_sw_amd/column/rrtmgsw_column.cpp). Inputs are the layer quantities that program hands to
+ * setcoef_sw / spcvrt_sw: pavel, tavel, coldry (ncol, nlay) and pz (ncol, nlay+1), surface first;
+ * wkl (7, ncol, nlay) molecules/cm2 (H2O, CO2, O3, N2O, CO, CH4, O2); albedo (14, ncol) =
+ * 1 - semiss; coszen (ncol) = cos(SZA) (0 becomes 1e-10, :644-645); the solar inputs as for
+ * rrtmgsw_sw (juldat as dyofyr, adjes = 1); clouds as cldprop_sw takes them, (ncol, nlay) and
+ * (14, ncol, nlay), read when icld >= 1, fractions 0 or 1 only (:612-616); aerosol optics
+ * (ncol, nlay, 14) or NULL. idelm 0/1: direct beam without / with delta scaling. Outputs fup,
+ * fdn, fdir (ncol, nlay+1, 15): total-sky upward, downward and direct downward flux, block 0
+ * the broadband and blocks 1..14 bands 16..29 (the program's iout = 98 blocks, :824-997). Host
+ * arrays. */
+int rrtmgsw_column_sw(int ncol, int nlay, int icld, int idelm, const double *pavel,
+                      const double *tavel, const double *pz, const double *coldry,
+                      const double *wkl, const double *albedo, const double *coszen, double adjes,
+                      int dyofyr, double scon, int isolvar, const double *bndsolvar,
+                      const double *indsolvar, const double *solcycfrac, int inflag, int iceflag,
+                      int liqflag, const double *cldfrac, const double *ciwp, const double *clwp,
+                      const double *rei, const double *rel, const double *tauc, const double *ssac,
+                      const double *asmc, const double *fsfc, const double *tauaer,
+                      const double *ssaaer, const double *asmaer, double *fup, double *fdn,
+                      double *fdir);
+
 /* Replaces mcica_subcol_sw(...), src/mcica_subcol_gen_sw.f90:181-185 (dummies :200-246).
  * Writes the seven (112, ncol, nlay) sub-column arrays and reicmcl / relqmcl (ncol, nlay).
  * irng is clamped (non-zero -> 1) and written back, as src/mcica_subcol_gen_sw.f90:450 does.

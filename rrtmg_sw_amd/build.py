@@ -2,8 +2,9 @@
 
     python -m rrtmg_sw_amd.build        # or rrtmg_sw_amd.build.build()
 
-Objects go to rrtmg_sw_amd/_build/; the library to rrtmg_sw_amd/librrtmgsw_hip.so. The Fortran
-iso_c_binding module (rrtmg_sw_amd/fortran/) is built as well when amdflang is present.
+Objects go to rrtmg_sw_amd/_build/; the library to rrtmg_sw_amd/librrtmgsw_hip.so; the
+column-mode program to rrtmg_sw_amd/column/rrtmgsw_column. The Fortran iso_c_binding modules
+(rrtmg_sw_amd/fortran/) are built as well when amdflang is present.
 """
 import os
 import shutil
@@ -56,6 +57,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    # the column-mode program (C++ host code over the C-ABI)
+    col = os.path.join(HERE, "column")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o",
+          os.path.join(col, "rrtmgsw_column"), os.path.join(col, "rrtmgsw_column.cpp"), f"-L{HERE}",
+          "-lrrtmgsw_hip", "-Wl,-rpath,$ORIGIN/.."], verbose)
     fortran = os.path.join(HERE, "fortran", "build_fortran.sh")
     if os.path.exists(fortran) and shutil.which("amdflang"):
         _run(["bash", fortran], verbose)

@@ -327,7 +327,10 @@ struct SwCall {
 // 0.003 K/day at 3 hPa).
 constexpr double THIN_PDP_HPA = 3.0;
 
-int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int precision = RRTMGSW_FP64) {
+// spectral: the column program's per-band outputs (a.bandu/bandd/banddir); the per-g level terms
+// of a chunk go to a buffer allocated for the call.
+int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int precision = RRTMGSW_FP64,
+           bool spectral = false) {
   int chunk = chunk_columns(base.ncol, base.nlay);
   void *scr = nullptr, *st = nullptr;
   {
@@ -335,19 +338,31 @@ int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int pr
     if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * scr_col_doubles(base.nlay) * sizeof(double), &scr)) return rc;
     if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
   }
-  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
   RtArgs a = base;
+  if (spectral) {
+    size_t per = (size_t)(base.nlay + 1) * 4 * NGPT;
+    chunk = (int)std::min<size_t>((size_t)chunk, std::max<size_t>(1, ((size_t)1 << 30) / (per * sizeof(double))));
+    HIPCHK(hipMalloc(&a.gspec, (size_t)chunk * per * sizeof(double)));
+  }
+  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
   a.scratch = (double *)scr;
   a.status = (int *)st;
   a.thin_pdp = THIN_PDP_HPA;
-  for (int c0 = 0; c0 < base.ncol; c0 += chunk) {
+  hipError_t e = hipSuccess;
+  for (int c0 = 0; c0 < base.ncol && e == hipSuccess; c0 += chunk) {
     a.col0 = c0;
     a.ncol_chunk = std::min(chunk, base.ncol - c0);
-    HIPCHK(precision == RRTMGSW_MIXED     ? launch_rt_mixed(a, tables, s)
-           : precision == RRTMGSW_FP64_EXP ? launch_rt_exact(a, tables, s)
-                                           : launch_rt_fp64(a, tables, s));
+    e = spectral                        ? launch_rt_spectral(a, tables, s)
+        : precision == RRTMGSW_MIXED     ? launch_rt_mixed(a, tables, s)
+        : precision == RRTMGSW_FP64_EXP ? launch_rt_exact(a, tables, s)
+                                        : launch_rt_fp64(a, tables, s);
   }
-  return finish(s, (int *)st);
+  int rc = e != hipSuccess ? fail(RRTMGSW_ERR_HIP, hipGetErrorString(e)) : finish(s, (int *)st);
+  if (spectral) {
+    hipStreamSynchronize(s);
+    hipFree(a.gspec);
+  }
+  return rc;
 }
 
 int check_init() {
@@ -691,6 +706,108 @@ int rrtmgsw_sw_nomcica(int ncol, int nlay, int *icld, int *iaer, const double *p
     if (e == hipSuccess) e = hipMemcpy(swhr, oh, C * L * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(swhrc, ohc, C * L * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+  }
+  hipFree(d);
+  return rc;
+}
+
+// Column program without McICA (src/rrtmg_sw.1col.f90:540-1000 for imca = 0): layer amounts in,
+// broadband and per-band up / down / direct-down fluxes out. Host arrays.
+int rrtmgsw_column_sw(int ncol, int nlay, int icld, int idelm, const double *pavel,
+                      const double *tavel, const double *pz, const double *coldry,
+                      const double *wkl, const double *albedo, const double *coszen, double adjes,
+                      int dyofyr, double scon, int isolvar, const double *bndsolvar,
+                      const double *indsolvar, const double *solcycfrac, int inflag, int iceflag,
+                      int liqflag, const double *cldfrac, const double *ciwp, const double *clwp,
+                      const double *rei, const double *rel, const double *tauc, const double *ssac,
+                      const double *asmc, const double *fsfc, const double *tauaer,
+                      const double *ssaaer, const double *asmaer, double *fup, double *fdn,
+                      double *fdir) {
+  if (int rc = check_init()) return rc;
+  if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  if (idelm != 0 && idelm != 1) return fail(RRTMGSW_ERR_ARGUMENT, "idelm must be 0 or 1");
+  if (ncol == 0) return 0;
+  const void *need[] = {pavel, tavel, pz, coldry, wkl, albedo, coszen, fup, fdn, fdir};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  const bool clouds = icld >= 1, aer = tauaer != nullptr;
+  if (clouds && (!cldfrac || !ciwp || !clwp || !rei || !rel || !tauc || !ssac || !asmc || !fsfc))
+    return fail(RRTMGSW_ERR_ARGUMENT, "cloud arrays are required when icld >= 1");
+  if (aer && (!ssaaer || !asmaer))
+    return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer go together");
+  const size_t C = ncol, L = nlay;
+  // the column program stops on any fraction other than 0 and 1 (:612-616)
+  if (clouds)
+    for (size_t q = 0; q < C * L; q++)
+      if (cldfrac[q] != 0.0 && cldfrac[q] != 1.0)
+        return fail(RRTMGSW_ERR_PARTIAL_CLOUD, "PARTIAL CLOUD NOT ALLOWED FOR IMCA=0");
+  struct In {
+    const double *h;
+    size_t n;
+  };
+  In in[] = {{pavel, C * L}, {tavel, C * L}, {pz, C * (L + 1)}, {coldry, C * L}, {wkl, 7 * C * L},
+             {albedo, NBND * C}, {coszen, C}, {cldfrac, clouds ? C * L : 0},
+             {ciwp, clouds ? C * L : 0}, {clwp, clouds ? C * L : 0}, {rei, clouds ? C * L : 0},
+             {rel, clouds ? C * L : 0}, {tauc, clouds ? NBND * C * L : 0},
+             {ssac, clouds ? NBND * C * L : 0}, {asmc, clouds ? NBND * C * L : 0},
+             {fsfc, clouds ? NBND * C * L : 0}, {tauaer, aer ? C * L * NBND : 0},
+             {ssaaer, aer ? C * L * NBND : 0}, {asmaer, aer ? C * L * NBND : 0}};
+  const int NIN = sizeof in / sizeof *in;
+  size_t total = 0;
+  for (auto &x : in) total += x.n;
+  const size_t nlev = C * (L + 1), nband = nlev * NBND;
+  const size_t nout = 4 * nlev + 2 * C * L + 3 * nband;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
+  const double *dp[NIN];
+  size_t off = 0;
+  for (int i = 0; i < NIN; i++) {
+    dp[i] = in[i].n ? d + off : nullptr;
+    if (in[i].n) {
+      hipError_t e = hipMemcpy(d + off, in[i].h, in[i].n * sizeof(double), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        hipFree(d);
+        return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+      }
+    }
+    off += in[i].n;
+  }
+  double *o = d + total;
+  RtArgs a{};
+  a.ncol = ncol, a.nlay = nlay;
+  a.play = dp[0], a.tlay = dp[1], a.plev = dp[2], a.coldry_in = dp[3], a.wkl_in = dp[4];
+  a.albbnd = dp[5], a.coszen = dp[6];
+  a.cldfrac = dp[7], a.ciwp = dp[8], a.clwp = dp[9], a.rei = dp[10], a.rel = dp[11];
+  a.tauc = dp[12], a.ssac = dp[13], a.asmc = dp[14], a.fsfc = dp[15];
+  a.tauaer = dp[16], a.ssaaer = dp[17], a.asmaer = dp[18];
+  a.swuflx = o, a.swdflx = o + nlev, a.swuflxc = o + 2 * nlev, a.swdflxc = o + 3 * nlev;
+  a.swhr = o + 4 * nlev, a.swhrc = a.swhr + C * L;
+  a.bandu = a.swhrc + C * L, a.bandd = a.bandu + nband, a.banddir = a.bandd + nband;
+  a.fused = 0, a.layer_clouds = 1, a.idelm = idelm;
+  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
+  a.sc.heatfac = ctx.heatfac;
+  a.sc.icld = clouds ? 1 : 0, a.sc.iaer = aer ? 10 : 0;
+  a.sc.inflag = inflag, a.sc.iceflag = iceflag, a.sc.liqflag = liqflag;
+  int rc = run_rt(a, ctx.dev, nullptr, RRTMGSW_FP64, true);
+  if (rc == 0) {
+    std::vector<double> h(nout);
+    hipError_t e = hipMemcpy(h.data(), o, nout * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+    } else {
+      const double *bu = h.data() + 4 * nlev + 2 * C * L, *bd = bu + nband, *bdir = bd + nband;
+      for (size_t q = 0; q < nlev; q++) {
+        double dsum = 0.0;
+        for (int b = 0; b < NBND; b++) {
+          fup[q + nlev * (b + 1)] = bu[q + nlev * b];
+          fdn[q + nlev * (b + 1)] = bd[q + nlev * b];
+          fdir[q + nlev * (b + 1)] = bdir[q + nlev * b];
+          dsum += bdir[q + nlev * b];
+        }
+        fup[q] = h[q], fdn[q] = h[nlev + q], fdir[q] = dsum;
+      }
+    }
   }
   hipFree(d);
   return rc;

@@ -102,12 +102,22 @@ __device__ static void setcoef_layer(const RtArgs &a, int c, int lay, ColView &v
   const double amd = 28.9660, amw = 18.0160, avogad = 6.02214199e+23, grav = 9.8066;
   double p = a.play[c + (size_t)C * lay], t = a.tlay[c + (size_t)C * lay];
   double pz0 = a.plev[c + (size_t)C * lay], pz1 = a.plev[c + (size_t)C * (lay + 1)];
-  double q = a.h2ovmr[c + (size_t)C * lay];
-  double amm = (1. - q) * amd + q * amw;
-  double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1. + q));
-  double w_h2o = coldry * q, w_co2 = coldry * a.co2vmr[c + (size_t)C * lay];
-  double w_o3 = coldry * a.o3vmr[c + (size_t)C * lay], w_ch4 = coldry * a.ch4vmr[c + (size_t)C * lay];
-  double w_o2 = coldry * a.o2vmr[c + (size_t)C * lay];
+  double coldry, w_h2o, w_co2, w_o3, w_ch4, w_o2;
+  if (a.coldry_in) {
+    // column program: dry-air and molecular amounts (molecules/cm2) given per layer
+    // (src/rrtmg_sw.1col.f90:1733-1752); wkl rows H2O, CO2, O3, N2O, CO, CH4, O2
+    const size_t q7 = (size_t)7 * (c + (size_t)C * lay);
+    coldry = a.coldry_in[c + (size_t)C * lay];
+    w_h2o = a.wkl_in[q7 + 0], w_co2 = a.wkl_in[q7 + 1], w_o3 = a.wkl_in[q7 + 2];
+    w_ch4 = a.wkl_in[q7 + 5], w_o2 = a.wkl_in[q7 + 6];
+  } else {
+    double q = a.h2ovmr[c + (size_t)C * lay];
+    double amm = (1. - q) * amd + q * amw;
+    coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1. + q));
+    w_h2o = coldry * q, w_co2 = coldry * a.co2vmr[c + (size_t)C * lay];
+    w_o3 = coldry * a.o3vmr[c + (size_t)C * lay], w_ch4 = coldry * a.ch4vmr[c + (size_t)C * lay];
+    w_o2 = coldry * a.o2vmr[c + (size_t)C * lay];
+  }
 
   double plog = log(p);
   int jp = (int)(36. - 5 * (plog + 0.04));
@@ -243,6 +253,7 @@ __device__ inline double incident_flux(const GPoint &g, const double *__restrict
 // ------------------------------------------------------------------------------------------
 struct CloudOpt {
   double tau, ssa, asm_;
+  double orig;   // optical depth before delta scaling (taormc / taucldorig)
 };
 __device__ inline double tab2(const double *tab, int off, int n, int i, int band) {
   return tab[off + (i - 1) + n * (band - 16)];
@@ -339,6 +350,7 @@ __device__ static CloudOpt cloud_optics_in(const DeviceTables &T, const SwScalar
   CloudOpt o;
   if (sc.inflag == 0) {
     double ffp = fsfc, ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssac;
+    o.orig = tauc;
     o.ssa = ffp1 * ssac / ffpssa;
     o.tau = ffpssa * tauc;
     o.asm_ = (asmc - ffp) / ffp1;
@@ -349,6 +361,7 @@ __device__ static CloudOpt cloud_optics_in(const DeviceTables &T, const SwScalar
   const double extcoliq = p.extcoliq, ssacoliq = p.ssacoliq, gliq = p.gliq, forwliq = p.forwliq;
   // :297-337
   double tauliqorig = clwp * extcoliq, tauiceorig = ciwp * extcoice;
+  o.orig = tauliqorig + tauiceorig;
   double ssaliq = ssacoliq * (1. - forwliq) / (1. - forwliq * ssacoliq);
   double tauliq = (1. - forwliq * ssacoliq) * tauliqorig;
   double ssaice = ssacoice * (1. - forwice) / (1. - forwice * ssacoice);
@@ -374,7 +387,7 @@ __device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &
                                         double cldf, double ciwp, double clwp, double rei,
                                         double rel, double tauc, double ssac, double asmc,
                                         double fsfc, int &err) {
-  const CloudOpt keep = {tauc, ssac, asmc};
+  const CloudOpt keep = {tauc, ssac, asmc, tauc};
   double cwp = ciwp + clwp;
   if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauc >= CLDMIN))) return keep;
   if (sc.inflag != 0 && sc.inflag != 2) {
@@ -391,7 +404,7 @@ __device__ static CloudOpt cloud_optics_layer(const DeviceTables &T, const SwSca
                                               double cldf, double ciwp, double clwp, double rei,
                                               double rel, double tauc, double ssac, double asmc,
                                               double fsfc, double tauctot, int &err) {
-  const CloudOpt none = {0.0, 1.0, 0.0};
+  const CloudOpt none = {0.0, 1.0, 0.0, tauc};
   double cwp = ciwp + clwp;
   if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauctot >= CLDMIN))) return none;
   if (sc.inflag != 0 && sc.inflag != 2) return none;

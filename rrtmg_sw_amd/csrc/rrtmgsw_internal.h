@@ -15,7 +15,7 @@ constexpr int SCR_COEFS = 6;   // elimination coefficients kept per (layer, g-po
 // Per-column scratch of the forward kernels, in doubles: the elimination coefficients
 // (nlay, SCR_COEFS, NGPT), then the cloud optics (tau, ssa, asymmetry) of each (layer, band) of
 // the fused entry (nlay, NBND, 3).
-constexpr size_t scr_col_doubles(int nlay) { return (size_t)nlay * (SCR_COEFS * NGPT + 3 * NBND); }
+constexpr size_t scr_col_doubles(int nlay) { return (size_t)nlay * (SCR_COEFS * NGPT + 4 * NBND); }
 constexpr double PADE = 0.278;
 constexpr double OD_LO = 0.06;
 
@@ -100,6 +100,14 @@ struct RtArgs {
   const double *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc, *ssac, *asmc, *fsfc, *alpha;
   // non-McICA path: the physical inputs above (no alpha) are the layer clouds of cldprop_sw
   int layer_clouds;
+  // column program (non-McICA kernel with spectral output): layer amounts given directly,
+  // coldry (ncol, nlay) and wkl (7, ncol, nlay); per-band surface albedo albbnd (14, ncol);
+  // per-g level terms gspec (per chunk column (nlay+1) x 4 x 112) reduced into the per-band
+  // total-sky fluxes bandu, bandd, banddir (ncol, nlay+1, 14); idelm 0/1 selects the direct
+  // beam without / with delta scaling (src/rrtmg_sw_spcvmc.f90:449-470, :623-628)
+  const double *coldry_in, *wkl_in, *albbnd;
+  double *gspec, *bandu, *bandd, *banddir;
+  int idelm;
   const uint32_t *mt_stream;     // Mersenne-Twister draws for irng = 1 (fused path)
   // aerosol
   const double *tauaer, *ssaaer, *asmaer, *ecaer;
@@ -136,6 +144,7 @@ size_t adj_work_bytes(int nlay, int ckpt);   // work area per column
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_rt_spectral(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, hipStream_t s);

@@ -77,10 +77,13 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 // ------------------------------------------------------------------------------------------
 // Occupancy target: four waves per SIMD for the table form with per-(layer, band) cloud optics
 // (measured fastest, a few VGPRs spilled outside the layer loop), three for the others.
+// SPEC (rt_column_kernel: the column program, with CLD_LAYER): per-g level terms written to
+// a.gspec for per-band fluxes, the direct beam with or without delta scaling, per-band surface
+// albedo, layer amounts given directly.
 enum : int { CLD_SUBCOL = 0, CLD_FUSED = 1, CLD_LAYER = 2 };
-template <int MODE, int CLD>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && CLD != CLD_SUBCOL ? 4 : 3)))
-rt_kernel(RtArgs a, DeviceTables T) {
+template <int MODE, int CLD, bool SPEC>
+__device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, const DeviceTables &T) {
+  static_assert(!SPEC || (MODE == 0 && CLD == CLD_LAYER), "spectral output: FP64 layer-cloud form only");
   constexpr bool EXACT = MODE == 2;
   constexpr bool MIXED = MODE == 1;
   constexpr bool FUSED = CLD == CLD_FUSED;
@@ -151,6 +154,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
                                        a.rel[q1], a.tauc[qb], a.ssac[qb], a.asmc[qb], a.fsfc[qb], tauctot, err);
       double *o = csc + (size_t)idx * 3;
       o[0] = co.tau, o[1] = co.ssa, o[2] = co.asm_;
+      if (SPEC) csc[(size_t)L * NBND * 3 + idx] = co.orig;   // taucldorig for the idelm = 0 beam
     }
     __syncthreads();
   }
@@ -207,10 +211,10 @@ rt_kernel(RtArgs a, DeviceTables T) {
   // aerosol (src/rrtmg_sw_rad.f90:709-765) and this sub-column's cloud optics (cldprmc_sw) and
   // total-sky inputs (src/rrtmg_sw_spcvmc.f90:500-505)
   struct LayIn {
-    double ta, wa, ga, cf, ct, cw, cg;
+    double ta, wa, ga, cf, ct, cw, cg, co;
   };
   auto inputs = [&](int lay) {
-    LayIn in = {0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+    LayIn in = {0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
     if (a.sc.iaer == 10) {
       size_t qa = c + (size_t)C * (lay + (size_t)L * ibnd);
       in.ta = a.tauaer[qa], in.wa = a.ssaaer[qa], in.ga = a.asmaer[qa];
@@ -242,6 +246,7 @@ rt_kernel(RtArgs a, DeviceTables T) {
       if (f != 0.0) {
         const double *o = csc + ((size_t)lay * NBND + ibnd) * 3;
         in.cf = f, in.ct = o[0], in.cw = o[1], in.cg = o[2];
+        if (SPEC) in.co = csc[(size_t)L * NBND * 3 + (size_t)lay * NBND + ibnd];
       }
     } else if (CLD == CLD_SUBCOL && a.sc.icld >= 1 && active) {
       const size_t q1 = c + (size_t)C * lay, qg = (size_t)g + NGPT * q1;
@@ -410,12 +415,36 @@ rt_kernel(RtArgs a, DeviceTables T) {
     }
     pend[0] = Bc, pend[1] = Cc, pend[2] = Ec, pend[3] = Bt, pend[4] = Ct, pend[5] = Et;
   };
+  // column program: the per-g level terms of the per-band fluxes, and the direct beam without
+  // delta scaling (src/rrtmg_sw_spcvmc.f90:449-470: the table transmittance of the unscaled
+  // clear and cloudy optical depths, blended by the cloud fraction)
+  double *gs = SPEC ? a.gspec + (size_t)cl * (L + 1) * 4 * NGPT + g : nullptr;
+  double tnd = 1.0;
+  auto spec_level = [&](int j, double A, double Tdir) {
+    if (SPEC && active) {
+      gs[(size_t)(j * 4 + 0) * NGPT] = zinc * A;
+      gs[(size_t)(j * 4 + 1) * NGPT] = zinc * (a.idelm == 0 ? tnd : Tdir);
+    }
+  };
   auto layer = [&](auto lower_tag, const GasLane &q, int j) {
     constexpr bool LOWER = decltype(lower_tag)::value;
     const int lay = L - 1 - j;
     const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);   // uniform over the workgroup
     double At, Ac;
-    if (MIXED && !thin) {
+    if (SPEC) {
+      const double Tdir = jfirst <= j ? st_.tdbt : sc_.tdbt;   // total-sky beam at level j
+      const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
+      flush();
+      double taug, taur;
+      gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
+      const LayIn in = inputs(lay);
+      body64(j, taug, taur, in, At, Ac);
+      spec_level(j, At, Tdir);
+      const double tau0 = taur + taug + in.ta;
+      const double dmc = tbl_exp1(tau0 / mu0, T.exp_tbl);
+      const double dmo = tbl_exp1((tau0 + in.co) / mu0, T.exp_tbl);
+      tnd = (in.cf != 0.0 ? (1.0 - in.cf) * dmc + in.cf * dmo : dmc) * tnd;
+    } else if (MIXED && !thin) {
       const GasFetch<float> gf = gas_fetch<LOWER, float>(q, T.tab32, v, lay);
       flush();
       float taugf, taurf;
@@ -445,13 +474,19 @@ rt_kernel(RtArgs a, DeviceTables T) {
   regime(std::integral_constant<bool, true>(), jsplit, L);
   flush();
   if (jfirst == L) st_ = sc_;
+  spec_level(L, st_.ztdn, st_.tdbt);
   if (err) atomicCAS(a.status, 0, err);
 
   // ---- pass 2: bottom to top ----
   // surface (src/rrtmg_sw_vrtqdr.f90:163-169 with prup = albdir, prupd = albdif)
   const bool vis = ibnd >= 9 && ibnd <= 12;
-  const double albdir = vis ? a.asdir[c] : a.aldir[c];
-  const double albdif = vis ? a.asdif[c] : a.aldif[c];
+  double albdir, albdif;
+  if (SPEC) {
+    albdir = albdif = a.albbnd[ibnd + NBND * (size_t)c];   // column program: 1 - semiss per band
+  } else {
+    albdir = vis ? a.asdir[c] : a.aldir[c];
+    albdif = vis ? a.asdif[c] : a.aldif[c];
+  }
   double uc = (albdir * sc_.tdbt + albdif * (sc_.ztdn - sc_.tdbt)) / (1. - albdif * sc_.prdnd);
   double dc = sc_.ztdn + sc_.prdnd * uc;
   double ut = (albdir * st_.tdbt + albdif * (st_.ztdn - st_.tdbt)) / (1. - albdif * st_.prdnd);
@@ -459,6 +494,10 @@ rt_kernel(RtArgs a, DeviceTables T) {
   {
     const double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
     if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 2)] = s;
+  }
+  if (SPEC && active) {
+    gs[(size_t)(L * 4 + 2) * NGPT] = zinc * ut;
+    gs[(size_t)(L * 4 + 3) * NGPT] = zinc * (st_.prdnd * ut);
   }
   // Coefficients are requested four layers ahead (a ring of four register sets); the total-sky
   // slots only by lanes whose sub-column is cloudy at or above that layer (the others were
@@ -489,6 +528,10 @@ rt_kernel(RtArgs a, DeviceTables T) {
     uc = o.cC + o.cE * uc;
     const double bu_c = o.cB * uc;
     // downward flux = sum(zinc ztdn) (pass 1) + sum(zinc prdnd U)
+    if (SPEC && active) {
+      gs[(size_t)(j * 4 + 2) * NGPT] = zinc * ut;
+      gs[(size_t)(j * 4 + 3) * NGPT] = zinc * bu_t;
+    }
     const double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
     if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 2)] = s;
   };
@@ -513,12 +556,59 @@ rt_kernel(RtArgs a, DeviceTables T) {
 }
 
 template <int MODE, int CLD>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && CLD != CLD_SUBCOL ? 4 : 3)))
+rt_kernel(RtArgs a, DeviceTables T) {
+  rt_body<MODE, CLD, false>(a, T);
+}
+
+// The column program's form: non-McICA clouds, per-band and direct-beam outputs.
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
+rt_column_kernel(RtArgs a, DeviceTables T) {
+  rt_body<0, CLD_LAYER, true>(a, T);
+}
+
+template <int MODE, int CLD>
 static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   ColLayout lo(a.nlay, CLD == CLD_FUSED ? (a.nlay + 31) / 32 : 0);
   int per = (a.ncol_chunk + 7) / 8;
   hipLaunchKernelGGL((rt_kernel<MODE, CLD>), dim3(per * 8), dim3(128), lo.total, s, a, t);
   return hipGetLastError();
 }
+
+// Per-band total-sky fluxes of one chunk from the per-g level terms (column program output):
+// one thread per (column, level, band), g summed in order within the band.
+__global__ void __launch_bounds__(256) spec_reduce_kernel(RtArgs a, DeviceTables T) {
+  const int L = a.nlay, C = a.ncol;
+  const long long n = (long long)a.ncol_chunk * (L + 1) * NBND;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int b = (int)(t % NBND);
+  const long long r = t / NBND;
+  const int j = (int)(r % (L + 1)), cl = (int)(r / (L + 1));
+  const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};
+  const double *gs = a.gspec + ((size_t)cl * (L + 1) + j) * 4 * NGPT;
+  double up = 0.0, dn = 0.0, dir = 0.0;
+  for (int g = first[b]; g < first[b + 1]; g++) {
+    up += gs[2 * NGPT + g];
+    dn += gs[0 * NGPT + g] + gs[3 * NGPT + g];
+    dir += gs[1 * NGPT + g];
+  }
+  (void)T;
+  const size_t q = (size_t)(a.col0 + cl) + (size_t)C * ((L - j) + (size_t)(L + 1) * b);
+  a.bandu[q] = up, a.bandd[q] = dn, a.banddir[q] = dir;
+}
+
+hipError_t launch_rt_spectral(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
+  ColLayout lo(a.nlay, 0);
+  int per = (a.ncol_chunk + 7) / 8;
+  hipLaunchKernelGGL(rt_column_kernel, dim3(per * 8), dim3(128), lo.total, s, a, t);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long n = (long long)a.ncol_chunk * (a.nlay + 1) * NBND;
+  hipLaunchKernelGGL(spec_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, t);
+  return hipGetLastError();
+}
+
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   if (a.layer_clouds) return launch_rt<0, CLD_LAYER>(a, t, s);
   return a.fused ? launch_rt<0, CLD_FUSED>(a, t, s) : launch_rt<0, CLD_SUBCOL>(a, t, s);

@@ -6,7 +6,8 @@ run_examples_std_atm/script.run_std_atm); the six whose layers are given directl
 are kept. This script reads those files as data: the
 INPUT_RRTM records the column program reads (src/rrtmg_sw.1col.f90:1364-1766, formats
 :1870-1877 and :1321-1326), the IN_AER_RRTM records for iaer = 10 (readaer, :1985-2127), and
-every flux block of each OUTPUT_RRTM. It stores, per case, the layer quantities the column
+every flux block of each OUTPUT_RRTM, and copies each case's input files (data) to
+tests/golden/column_inputs/<case>/ under the names the column program reads. It stores, per case, the layer quantities the column
 program hands to setcoef_sw / spcvmc_sw (pavel, tavel, pz, coldry, wkl after the
 mixing-ratio conversion of :1733-1752, semiss, zenith, solar-variability inputs, aerosol
 optics) and the expected outputs (broadband and per-band up, diffuse down, direct down, down,
@@ -258,6 +259,16 @@ def main(ref):
               f"sza {c['sza']} juldat {c['juldat']} blocks {int(have.sum())}")
     np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "column_kats.npz"),
                         cases=np.array(CASES), **out)
+    # the example suite's own input files (data), under the names the column program reads
+    import shutil
+    for name in CASES:
+        dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "column_inputs", name)
+        os.makedirs(dst, exist_ok=True)
+        shutil.copyfile(os.path.join(d, INPUT.get(name, "input_rrtm_" + name)), os.path.join(dst, "INPUT_RRTM"))
+        if name in CLD:
+            shutil.copyfile(os.path.join(d, CLD[name]), os.path.join(dst, "IN_CLD_RRTM"))
+        if name in AER:
+            shutil.copyfile(os.path.join(d, AER[name]), os.path.join(dst, "IN_AER_RRTM"))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,50 @@
+"""The column-mode program (rrtmg_sw_amd/column/rrtmgsw_column: INPUT_RRTM / IN_CLD_RRTM /
+IN_AER_RRTM in, OUTPUT_RRTM out, radiative transfer on the GPU through rrtmgsw_column_sw) on the
+reference's own example suite: every printed block of the expected outputs (broadband and, where
+the case prints them, bands 16-29; upward, diffuse and direct downward, downward and net flux,
+heating rate) within 0.1 W/m2 and 0.01 K/day. The expected outputs are printed to 1e-4 W/m2 and
+1e-5 K/day; the largest differences are printed."""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+EXE = os.path.join(ROOT, "rrtmg_sw_amd", "column", "rrtmgsw_column")
+INPUTS = os.path.join(HERE, "golden", "column_inputs")
+KATS = np.load(os.path.join(HERE, "golden", "column_kats.npz"))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from make_column_kats import WAVENUM2, read_output  # noqa: E402
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(EXE), reason="column program not built")]
+
+
+@pytest.mark.parametrize("name", [str(n) for n in KATS["cases"]])
+def test_column_program_example_suite(tmp_path, name):
+    src = os.path.join(INPUTS, name)
+    for f in os.listdir(src):
+        shutil.copyfile(os.path.join(src, f), tmp_path / f)
+    r = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    L = int(KATS[f"{name}/nlay"])
+    exp, have = KATS[f"{name}/expected"], KATS[f"{name}/have"]
+    blocks = read_output(str(tmp_path / "OUTPUT_RRTM"), L)
+    got = {}
+    for w1, w2, rows in blocks:
+        b = 0 if (w1, w2) == (820.0, 50000.0) else 1 + WAVENUM2.index(w2)
+        got[b] = rows
+    assert sorted(got) == [b for b in range(15) if have[b]]
+    worst = (0.0, 0.0)
+    for b, rows in got.items():
+        flux = np.abs(rows[:, :5] - exp[b][:, :5]).max()
+        hr = np.abs(rows[:, 5] - exp[b][:, 5]).max()
+        worst = (max(worst[0], flux), max(worst[1], hr))
+        assert flux <= 0.1, (name, b, flux)
+        assert hr <= 0.01, (name, b, hr)
+    print(f"{name}: {len(got)} blocks, max flux diff {worst[0]:.2e} W/m2, max HR diff {worst[1]:.2e} K/day",
+          flush=True)
