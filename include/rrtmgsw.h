@@ -153,6 +153,25 @@ int rrtmgsw_column_sw(int ncol, int nlay, int icld, int idelm, const double *pav
                       const double *ssaaer, const double *asmaer, double *fup, double *fdn,
                       double *fdir);
 
+/* The column program's McICA loop (src/rrtmg_sw.1col.f90:566-610 with imca = 1): nsamples draws
+ * of one column, sample s (ims = s + 1) with permuteseed (ims + 1) * 112 as
+ * src/mcica_subcol_gen_sw.1col.f90 sets it, irng 0 (KISS) or 1 (Mersenne Twister, what the
+ * program uses), overlap icld 1-5 with alpha (nlay) for icld 4/5, then cldprmc_sw and spcvmc_sw;
+ * all samples run in one launch. Column inputs as rrtmgsw_column_sw for one column (coszen a
+ * scalar); fup, fdn, fdir (nsamples, nlay+1, 15) per sample, which the caller averages as the
+ * program does (:874-894). Host arrays. */
+int rrtmgsw_column_mcica_sw(int nlay, int nsamples, int icld, int irng, int idelm,
+                            const double *pavel, const double *tavel, const double *pz,
+                            const double *coldry, const double *wkl, const double *albedo,
+                            double coszen, double adjes, int dyofyr, double scon, int isolvar,
+                            const double *bndsolvar, const double *indsolvar,
+                            const double *solcycfrac, int inflag, int iceflag, int liqflag,
+                            const double *cldfrac, const double *ciwp, const double *clwp,
+                            const double *rei, const double *rel, const double *tauc,
+                            const double *ssac, const double *asmc, const double *fsfc,
+                            const double *alpha, const double *tauaer, const double *ssaaer,
+                            const double *asmaer, double *fup, double *fdn, double *fdir);
+
 /* Replaces mcica_subcol_sw(...), src/mcica_subcol_gen_sw.f90:181-185 (dummies :200-246).
  * Writes the seven (112, ncol, nlay) sub-column arrays and reicmcl / relqmcl (ncol, nlay).
  * irng is clamped (non-zero -> 1) and written back, as src/mcica_subcol_gen_sw.f90:450 does.

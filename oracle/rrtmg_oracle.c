@@ -2128,7 +2128,40 @@ int orc_cldprmc(int nlay, int inflag, int iceflag, int liqflag, const double *cl
 typedef struct {
   int inflag, iceflag, liqflag;
   const double *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc, *ssac, *asmc, *fsfc;
+  int icld, nmca;          /* imca = 1: nmca McICA samples with overlap icld */
+  const double *alpha;     /* (nlay) overlap parameters for icld 4/5 */
 } colcld_t;
+
+/* spcvmc for one column on (nlay, 112) cloud arrays, then the six output quantities of every
+ * block (src/rrtmg_sw.1col.f90:784-800) into out (6, nlay+1, 15). */
+static void column_block(int L, int idelm, const double *alb, const double *zcf, const double *ztc,
+                         const double *zgc, const double *zwc, const double *zto, const double *ta,
+                         const double *ga, const double *wa, double cossza, const double *adjflux,
+                         const solar_t *sol, const coef_t *c, const double *pz, double heatfac,
+                         double *out) {
+  const int n1 = L + 1;
+  double *pbb[5];
+  for (int i = 0; i < 5; i++) pbb[i] = calloc(n1, 8);
+  double *bu = calloc((size_t)n1 * NBND, 8), *bd = calloc((size_t)n1 * NBND, 8), *bdir = calloc((size_t)n1 * NBND, 8);
+  spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], bu, bd, bdir};
+  spcvmc(L, idelm, alb, alb, zcf, ztc, zgc, zwc, zto, ta, ga, wa, cossza, adjflux, sol, c, &o);
+  for (int b = 0; b <= NBND; b++) {
+    double *up = out + (size_t)n1 * (0 + 6 * b), *dif = out + (size_t)n1 * (1 + 6 * b);
+    double *dir = out + (size_t)n1 * (2 + 6 * b), *dn = out + (size_t)n1 * (3 + 6 * b);
+    double *net = out + (size_t)n1 * (4 + 6 * b), *htr = out + (size_t)n1 * (5 + 6 * b);
+    for (int k = 0; k < n1; k++) {
+      up[k] = b == 0 ? pbb[0][k] : bu[k + n1 * (b - 1)];
+      dn[k] = b == 0 ? pbb[1][k] : bd[k + n1 * (b - 1)];
+      dir[k] = b == 0 ? pbb[4][k] : bdir[k + n1 * (b - 1)];
+      dif[k] = dn[k] - dir[k];
+      net[k] = dn[k] - up[k];
+    }
+    for (int k = 0; k < L; k++) htr[k] = (net[k + 1] - net[k]) * (heatfac / (pz[k] - pz[k + 1]));
+    htr[L] = 0.;
+  }
+  free(bu), free(bd), free(bdir);
+  for (int i = 0; i < 5; i++) free(pbb[i]);
+}
 
 static int column_run(int nlay, const double *pavel, const double *tavel, const double *pz,
                       const double *coldry, const double *wkl7, const double *semiss, double zenith,
@@ -2157,10 +2190,52 @@ static int column_run(int nlay, const double *pavel, const double *tavel, const 
   double *zto = calloc((size_t)L * NGPT, 8), *zgc = calloc((size_t)L * NGPT, 8);
   double *zwc = malloc(8 * (size_t)L * NGPT);
   for (int k = 0; k < L * NGPT; k++) zwc[k] = 1.;
+  const double heatfac = GRAV * SECDY / (cpdair * 1.e2);
+  double *ta = calloc((size_t)L * NBND, 8), *ga = calloc((size_t)L * NBND, 8), *wa = malloc(8 * (size_t)L * NBND);
+  for (int k = 0; k < L * NBND; k++) wa[k] = 1.;
+  if (iaer == 10)
+    for (int k = 0; k < L * NBND; k++) ta[k] = tauaer[k], wa[k] = ssaaer[k], ga[k] = asmaer[k];
+  if (cld && cld->nmca > 0) {
+    /* imca = 1 (:577-610, :660-668, :874-894): sample ims draws sub-columns with permuteseed
+     * (ims + 1) * 112 (src/mcica_subcol_gen_sw.1col.f90, the single-column generator: the GCM
+     * generator restated here on one column), cldprmc_sw, spcvmc_sw; the six outputs of every
+     * block are summed over the samples in order and divided by their number. */
+    const size_t G = NGPT, tot = (size_t)6 * n1 * (NBND + 1);
+    double *acc = calloc(tot, 8), *one = malloc(8 * tot);
+    double *cf = malloc(8 * G * L), *ci = malloc(8 * G * L), *cw = malloc(8 * G * L);
+    double *tcm = malloc(8 * G * L), *scm = malloc(8 * G * L), *acm = malloc(8 * G * L);
+    double *fcm = malloc(8 * G * L), *tom = malloc(8 * G * L), *rim = malloc(8 * L), *rlm = malloc(8 * L);
+    int rc = 0;
+    for (int ims = 1; ims <= cld->nmca && !rc; ims++) {
+      int irng = 1;
+      rc = orc_mcica_subcol_sw(1, 1, L, cld->icld, (ims + 1) * NGPT, &irng, pavel, cld->cldfrac,
+                               cld->ciwp, cld->clwp, cld->rei, cld->rel, cld->tauc, cld->ssac,
+                               cld->asmc, cld->fsfc, cld->alpha, cf, ci, cw, rim, rlm, tcm, scm, acm, fcm);
+      if (rc) break;
+      rc = cldprmc(L, cld->inflag, cld->iceflag, cld->liqflag, cf, ci, cw, rim, rlm, tom, tcm, scm, acm, fcm);
+      if (rc) break;
+      for (int l = 0; l < L; l++)   /* (112, nlay) -> (nlay, 112), :684-692 */
+        for (int g = 0; g < NGPT; g++) {
+          size_t t = l + (size_t)L * g, q = g + G * l;
+          zcf[t] = cf[q], ztc[t] = tcm[q], zto[t] = tom[q], zgc[t] = acm[q], zwc[t] = scm[q];
+        }
+      column_block(L, idelm, alb, zcf, ztc, zgc, zwc, zto, ta, ga, wa, cossza, sv.adjflux, &sol, &c, pz,
+                   heatfac, one);
+      for (size_t k = 0; k < tot; k++) acc[k] = acc[k] + one[k];
+    }
+    for (size_t k = 0; k < tot; k++) out[k] = acc[k] / cld->nmca;
+    free(acc), free(one), free(cf), free(ci), free(cw), free(tcm), free(scm), free(acm), free(fcm);
+    free(tom), free(rim), free(rlm);
+    free(zc0), free(zw1), free(zcf), free(ztc), free(zto), free(zgc), free(zwc), free(wkl);
+    free(ta), free(ga), free(wa);
+    coef_free(&c);
+    return rc;
+  }
   if (cld) {
     for (int l = 0; l < L; l++)
       if (cld->cldfrac[l] != 0. && cld->cldfrac[l] != 1.) {
         free(zc0), free(zw1), free(zcf), free(ztc), free(zto), free(zgc), free(zwc), free(wkl);
+        free(ta), free(ga), free(wa);
         coef_free(&c);
         return fail(6, "PARTIAL CLOUD NOT ALLOWED FOR IMCA=0");
       }
@@ -2177,38 +2252,16 @@ static int column_run(int nlay, const double *pavel, const double *tavel, const 
     free(tco), free(tcl), free(scl), free(acl);
     if (rc) {
       free(zc0), free(zw1), free(zcf), free(ztc), free(zto), free(zgc), free(zwc), free(wkl);
+      free(ta), free(ga), free(wa);
       coef_free(&c);
       return rc;
     }
   }
-  double *ta = calloc((size_t)L * NBND, 8), *ga = calloc((size_t)L * NBND, 8), *wa = malloc(8 * (size_t)L * NBND);
-  for (int k = 0; k < L * NBND; k++) wa[k] = 1.;
-  if (iaer == 10)
-    for (int k = 0; k < L * NBND; k++) ta[k] = tauaer[k], wa[k] = ssaaer[k], ga[k] = asmaer[k];
-  double *pbb[5];
-  for (int i = 0; i < 5; i++) pbb[i] = calloc(n1, 8);
-  double *bu = calloc((size_t)n1 * NBND, 8), *bd = calloc((size_t)n1 * NBND, 8), *bdir = calloc((size_t)n1 * NBND, 8);
-  spc_out_t o = {pbb[0], pbb[1], pbb[2], pbb[3], pbb[4], bu, bd, bdir};
-  spcvmc(L, idelm, alb, alb, zcf, ztc, zgc, zwc, zto, ta, ga, wa, cossza, sv.adjflux, &sol, &c, &o);
-  const double heatfac = GRAV * SECDY / (cpdair * 1.e2);
-  for (int b = 0; b <= NBND; b++) {
-    double *up = out + (size_t)n1 * (0 + 6 * b), *dif = out + (size_t)n1 * (1 + 6 * b);
-    double *dir = out + (size_t)n1 * (2 + 6 * b), *dn = out + (size_t)n1 * (3 + 6 * b);
-    double *net = out + (size_t)n1 * (4 + 6 * b), *htr = out + (size_t)n1 * (5 + 6 * b);
-    for (int k = 0; k < n1; k++) {
-      up[k] = b == 0 ? pbb[0][k] : bu[k + n1 * (b - 1)];
-      dn[k] = b == 0 ? pbb[1][k] : bd[k + n1 * (b - 1)];
-      dir[k] = b == 0 ? pbb[4][k] : bdir[k + n1 * (b - 1)];
-      dif[k] = dn[k] - dir[k];
-      net[k] = dn[k] - up[k];
-    }
-    for (int k = 0; k < L; k++) htr[k] = (net[k + 1] - net[k]) * (heatfac / (pz[k] - pz[k + 1]));
-    htr[L] = 0.;
-  }
+  column_block(L, idelm, alb, zcf, ztc, zgc, zwc, zto, ta, ga, wa, cossza, sv.adjflux, &sol, &c, pz,
+               heatfac, out);
   coef_free(&c);
-  free(wkl), free(zc0), free(zw1), free(ta), free(ga), free(wa), free(bu), free(bd), free(bdir);
+  free(wkl), free(zc0), free(zw1), free(ta), free(ga), free(wa);
   free(zcf), free(ztc), free(zto), free(zgc), free(zwc);
-  for (int i = 0; i < 5; i++) free(pbb[i]);
   return 0;
 }
 
@@ -2232,7 +2285,23 @@ int orc_column_sw_cld(int nlay, const double *pavel, const double *tavel, const 
                       const double *ciwp, const double *clwp, const double *rei, const double *rel,
                       const double *tauc, const double *ssac, const double *asmc,
                       const double *fsfc, double *out) {
-  colcld_t cl = {inflag, iceflag, liqflag, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc};
+  colcld_t cl = {inflag, iceflag, liqflag, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, 0, 0, NULL};
+  return column_run(nlay, pavel, tavel, pz, coldry, wkl7, semiss, zenith, juldat, scon, isolvar,
+                    bndsolvar, indsolvar, solcycfrac, iaer, tauaer, ssaaer, asmaer, idelm, cpdair,
+                    &cl, out);
+}
+
+int orc_column_sw_mcica(int nlay, const double *pavel, const double *tavel, const double *pz,
+                        const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                        int juldat, double scon, int isolvar, const double *bndsolvar,
+                        const double *indsolvar, const double *solcycfrac, int iaer,
+                        const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                        double cpdair, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                        const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                        const double *tauc, const double *ssac, const double *asmc,
+                        const double *fsfc, int icld, int nmca, const double *alpha, double *out) {
+  colcld_t cl = {inflag, iceflag, liqflag, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc,
+                 icld, nmca, alpha};
   return column_run(nlay, pavel, tavel, pz, coldry, wkl7, semiss, zenith, juldat, scon, isolvar,
                     bndsolvar, indsolvar, solcycfrac, iaer, tauaer, ssaaer, asmaer, idelm, cpdair,
                     &cl, out);

@@ -83,6 +83,20 @@ int orc_column_sw_cld(int nlay, const double *pavel, const double *tavel, const 
                       const double *tauc, const double *ssac, const double *asmc,
                       const double *fsfc, double *out);
 
+/* orc_column_sw with the column program's McICA loop (imca = 1, src/rrtmg_sw.1col.f90:577-610,
+ * :874-894): nmca samples, sample ims drawn with permuteseed (ims + 1) * 112 and the Mersenne
+ * Twister (irng = 1, as the program sets it), overlap icld with alpha (nlay) for icld 4/5;
+ * out holds the outputs averaged over the samples as the program prints them. */
+int orc_column_sw_mcica(int nlay, const double *pavel, const double *tavel, const double *pz,
+                        const double *coldry, const double *wkl7, const double *semiss, double zenith,
+                        int juldat, double scon, int isolvar, const double *bndsolvar,
+                        const double *indsolvar, const double *solcycfrac, int iaer,
+                        const double *tauaer, const double *ssaaer, const double *asmaer, int idelm,
+                        double cpdair, int inflag, int iceflag, int liqflag, const double *cldfrac,
+                        const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                        const double *tauc, const double *ssac, const double *asmc,
+                        const double *fsfc, int icld, int nmca, const double *alpha, double *out);
+
 /* mcica_subcol_sw(...) -- src/mcica_subcol_gen_sw.f90:181-296 (+ :300-753). */
 int orc_mcica_subcol_sw(int iplon, int ncol, int nlay, int icld, int permuteseed, int *irng,
                         const double *play, const double *cldfrac, const double *ciwp,

@@ -64,6 +64,7 @@ def lib():
 
 EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_sw",
             "rrtmgsw_sw_dev", "rrtmgsw_sw_nomcica", "rrtmgsw_sw_nomcica_dev", "rrtmgsw_column_sw",
+            "rrtmgsw_column_mcica_sw",
             "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
             "rrtmgsw_get_alpha", "rrtmgsw_mcica_sw_dev", "rrtmgsw_mcica_sw_adjoint_dev", "rrtmgsw_adjoint_work_bytes"]
 

@@ -5,10 +5,12 @@
 //   rrtmgsw_column [directory]          (default: the current directory)
 //
 // Covered: layer input given directly (IATM = 0; src/rrtmg_sw.1col.f90:1693-1766), clear or
-// overcast clouds without McICA (imca = 0; readcld :1884-1958), the aerosol file for iaer = 10
-// (readaer :1961-2127, nstr = 2), output blocks for iout = 0, 16..29 and 98 (:824-997).
-// Not covered, with an error message: IATM = 1 (the atmosphere generator src/rrtatm.f),
-// McICA sampling (imca = 1), iaer = 6, cross-section molecules.
+// overcast clouds without McICA (imca = 0; readcld :1884-1958), McICA sampling (imca = 1: 200
+// samples with the Mersenne Twister, irng = 1 as the program sets it, averaged; :566-600,
+// :874-946), the overlap parameters of icld 4/5 (get_alpha, idcor 0/1), the aerosol file for
+// iaer = 10 (readaer :1961-2127, nstr = 2), output blocks for iout = 0, 16..29 and 98
+// (:824-997). Not covered, with an error message: IATM = 1 (the atmosphere generator
+// src/rrtatm.f), iaer = 6, cross-section molecules.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -72,11 +74,11 @@ std::vector<std::string> read_lines(const std::string &path) {
 
 struct Profile {
   int iaer = 0, iatm = 0, iscat = 0, istrm = 0, iout = 0, imca = 0, icld = 0, idelm = 0, icos = 0;
-  int juldat = 0, isolvar = 0, nlay = 0;
-  double sza = 0, scon = 0, solcycfrac = 0, zenith = 0;
+  int juldat = 0, isolvar = 0, nlay = 0, idcor = 0, juldat_alpha = 0;
+  double sza = 0, scon = 0, solcycfrac = 0, zenith = 0, decorr_con = 0, lat = 0;
   double solvar[NB] = {}, semiss[NB] = {};
   double bndsolvar[NB], indsolvar[2];
-  std::vector<double> pavel, tavel, pz, coldry, wkl;   // wkl (7, nlay)
+  std::vector<double> pavel, tavel, pz, coldry, wkl, altz;   // wkl (7, nlay), altz (nlay+1) km
 };
 
 // readprof for IATM = 0 (src/rrtmg_sw.1col.f90:1364-1766, formats :1331-1336, :1870-1877)
@@ -119,7 +121,17 @@ Profile read_input(const std::string &path) {
     for (double &e : p.semiss) e = p.semiss[0];
   else if (iemis != 2)
     die("IEMIS = " + std::to_string(iemis) + " NOT A VALID INPUT VALUE");
-  if (p.imca != 0) die("McICA sampling (imca = 1) is not available in this program; use imca = 0");
+  if (p.icld == 4 || p.icld == 5) {   // :1664-1690
+    pos = 0;
+    const std::string &rc = lines.at(++k);
+    skip(pos, 8), p.idcor = ifield(rc, pos, 2);
+    pos = 0;
+    if (p.idcor == 0) p.decorr_con = ffield(lines.at(++k), pos, 10, 3);
+    if (p.idcor == 1) p.lat = ffield(lines.at(++k), pos, 10, 3);
+    // the decorrelation length takes day 94 for juldat = 0 (:1692); the Earth-Sun factor was
+    // already taken from juldat itself (:1431-1435)
+    p.juldat_alpha = (p.juldat == 0 && p.idcor == 1) ? 94 : p.juldat;
+  }
   if (p.iatm != 0) die("IATM = 1 (the rrtatm atmosphere generator) is not available; give the layers (IATM = 0)");
   if (p.iaer == 6) die("iaer = 6 is not available in this program");
   pos = 0;
@@ -135,6 +147,7 @@ Profile read_input(const std::string &path) {
   if (L < 4 || L > 256) die("the library takes 4..256 layers");
   p.pavel.assign(L, 0.), p.tavel.assign(L, 0.), p.pz.assign(L + 1, 0.), p.coldry.assign(L, 0.);
   p.wkl.assign(7 * L, 0.);
+  p.altz.assign(L + 1, 0.);
   std::vector<double> wbrodl(L);
   // iform 0: (3f10.4,a3,i2,1x|23x,...(f7.2,f8.3,f7.2)), (8e10.3); iform 1: g15.7 / g10.4 fields
   const int w1 = iform ? 15 : 10, d1 = iform ? 7 : 4, w2 = 10, d2 = iform ? 4 : 4;
@@ -148,13 +161,13 @@ Profile read_input(const std::string &path) {
     field(ra, pos, 3), ifield(ra, pos, 2);   // cinp, ipthak
     if (l == 0) {
       skip(pos, 1);
-      ffield(ra, pos, 7, 2);
+      p.altz[0] = ffield(ra, pos, 7, 2);
       p.pz[0] = ffield(ra, pos, 8, 3);
       ffield(ra, pos, 7, 2);
     } else {
       skip(pos, 23);
     }
-    ffield(ra, pos, 7, 2);
+    p.altz[l + 1] = ffield(ra, pos, 7, 2);
     p.pz[l + 1] = ffield(ra, pos, 8, 3);
     ffield(ra, pos, 7, 2);
     pos = 0;
@@ -333,8 +346,8 @@ std::string fmt_f(double v, int w, int d) {
 
 // One output block (src/rrtmg_sw.1col.f90:835-873): levels from the top, pressure in the
 // format its magnitude selects (outform, :498-505).
-void write_block(FILE *f, const Profile &p, double w1, double w2, const double *up, const double *dn,
-                 const double *dir, double heatfac) {
+void write_rows(FILE *f, const Profile &p, double w1, double w2, const double *up, const double *dif,
+                const double *dir, const double *dn, const double *net, const double *htr) {
   const int L = p.nlay;
   std::fprintf(f, "\n");
   if (p.idelm == 0)
@@ -344,9 +357,6 @@ void write_block(FILE *f, const Profile &p, double w1, double w2, const double *
   std::fprintf(f, " Wavenumbers: %s - %s cm-1, ATM %6d\n", fmt_f(w1, 6, 0).c_str(), fmt_f(w2, 6, 0).c_str(), 1);
   std::fprintf(f, " LEVEL PRESSURE   UPWARD FLUX   DIFDOWN FLUX  DIRDOWN FLUX  DOWNWARD FLUX   NET FLUX    HEATING RATE\n");
   std::fprintf(f, "          mb          W/m2          W/m2          W/m2        W/m2          W/m2       degree/day\n");
-  std::vector<double> net(L + 1), htr(L + 1, 0.0);
-  for (int i = 0; i <= L; i++) net[i] = dn[i] - up[i];
-  for (int i = 0; i < L; i++) htr[i] = (net[i + 1] - net[i]) * (heatfac / (p.pz[i] - p.pz[i + 1]));
   for (int i = L; i >= 0; i--) {
     const double pzi = p.pz[i];
     std::string ps;
@@ -355,10 +365,21 @@ void write_block(FILE *f, const Profile &p, double w1, double w2, const double *
     else
       ps = "    " + fmt_f(pzi, 6, pzi < 1.e-1 ? 5 : pzi < 1. ? 4 : pzi < 10. ? 3 : pzi < 100. ? 2 : 1);
     std::fprintf(f, " %3d%s    %s    %s    %s    %s    %s    %s\n", i, ps.c_str(), fmt_f(up[i], 10, 4).c_str(),
-                 fmt_f(dn[i] - dir[i], 10, 4).c_str(), fmt_f(dir[i], 10, 4).c_str(), fmt_f(dn[i], 10, 4).c_str(),
+                 fmt_f(dif[i], 10, 4).c_str(), fmt_f(dir[i], 10, 4).c_str(), fmt_f(dn[i], 10, 4).c_str(),
                  fmt_f(net[i], 11, 6).c_str(), fmt_f(htr[i], 10, 5).c_str());
   }
   std::fprintf(f, "\f\n");
+}
+
+// One sample's block: difdown = down - direct, net = down - up, heating rate from the net flux
+// (:784-800), as the program forms them.
+void write_block(FILE *f, const Profile &p, double w1, double w2, const double *up, const double *dn,
+                 const double *dir, double heatfac) {
+  const int L = p.nlay;
+  std::vector<double> dif(L + 1), net(L + 1), htr(L + 1, 0.0);
+  for (int i = 0; i <= L; i++) dif[i] = dn[i] - dir[i], net[i] = dn[i] - up[i];
+  for (int i = 0; i < L; i++) htr[i] = (net[i + 1] - net[i]) * (heatfac / (p.pz[i] - p.pz[i + 1]));
+  write_rows(f, p, w1, w2, up, dif.data(), dir, dn, net.data(), htr.data());
 }
 
 }  // namespace
@@ -388,6 +409,8 @@ int main(int argc, char **argv) {
     dump("pz", p.pz.data(), L + 1), dump("coldry", p.coldry.data(), L);
     dump("wkl", p.wkl.data(), 7 * (size_t)L), dump("semiss", p.semiss, NB);
     dump("bndsolvar", p.bndsolvar, NB), dump("indsolvar", p.indsolvar, 2);
+    const double ov[] = {(double)p.imca, (double)p.idcor, p.decorr_con, p.lat, (double)p.juldat_alpha};
+    dump("overlap", ov, 5), dump("altz", p.altz.data(), L + 1);
     if (p.icld >= 1) {
       const double fl[] = {(double)cl.inflag, (double)cl.iceflag, (double)cl.liqflag};
       dump("cldflags", fl, 3);
@@ -407,6 +430,65 @@ int main(int argc, char **argv) {
   const size_t nlev = (size_t)L + 1;
   std::vector<double> fup(nlev * 15), fdn(nlev * 15), fdir(nlev * 15);
   const bool c = p.icld >= 1;
+  const double heatfac = 9.8066 * 8.6400e4 / (CPDAIR * 1.e2);   // src/rrtmg_sw_init.f90:258
+  if (p.imca == 1 && c) {
+    // McICA: 200 samples (nmca, :571) averaged level by level in sample order (:874-894)
+    const int nmca = 200, irng = 1;
+    std::vector<double> alpha(L, 0.0);
+    if (p.icld == 4 || p.icld == 5) {
+      std::vector<double> dz(L);
+      for (int l = 0; l < L; l++) dz[l] = (p.altz[l + 1] - p.altz[l]) * 1000.0;   // km -> m (:1796)
+      if (rrtmgsw_get_alpha(1, 1, L, p.icld, p.idcor, p.decorr_con, dz.data(), &p.lat, p.juldat_alpha,
+                            cl.cldfrac.data(), alpha.data()))
+        die(rrtmgsw_last_error());
+    }
+    std::vector<double> su(nmca * nlev * 15), sd(nmca * nlev * 15), sdir(nmca * nlev * 15);
+    int rc = rrtmgsw_column_mcica_sw(L, nmca, p.icld, irng, p.idelm, p.pavel.data(), p.tavel.data(),
+                                     p.pz.data(), p.coldry.data(), p.wkl.data(), alb.data(), cosz, 1.0,
+                                     p.juldat, p.scon, p.isolvar, p.bndsolvar, p.indsolvar,
+                                     &p.solcycfrac, cl.inflag, cl.iceflag, cl.liqflag, cl.cldfrac.data(),
+                                     cl.ciwp.data(), cl.clwp.data(), cl.rei.data(), cl.rel.data(),
+                                     cl.tauc.data(), cl.ssac.data(), cl.asmc.data(), cl.fsfc.data(),
+                                     alpha.data(), p.iaer == 10 ? tauaer.data() : nullptr,
+                                     p.iaer == 10 ? ssaaer.data() : nullptr,
+                                     p.iaer == 10 ? asmaer.data() : nullptr, su.data(), sd.data(), sdir.data());
+    if (rc) die(rrtmgsw_last_error());
+    FILE *f = std::fopen((dir + "OUTPUT_RRTM").c_str(), "w");
+    if (!f) die("cannot write OUTPUT_RRTM");
+    std::vector<int> blocks;
+    if (p.iout == 98) {
+      blocks.push_back(0);
+      for (int b = 1; b <= NB; b++) blocks.push_back(b);
+    } else if (p.iout >= 16 && p.iout <= 29) {
+      blocks.push_back(p.iout - 15);
+    } else if (p.iout == 0) {
+      blocks.push_back(0);
+    }
+    for (int b : blocks) {
+      std::vector<double> up(nlev, 0.), dn(nlev, 0.), di(nlev, 0.), df(nlev, 0.), nt(nlev, 0.), hr(nlev, 0.);
+      for (int s = 0; s < nmca; s++) {
+        // sample s, block b: (sample, level) fastest in the library's (nsamples, nlay+1, 15)
+        auto at = [&](const std::vector<double> &x, int i) { return x[s + (size_t)nmca * (i + nlev * b)]; };
+        for (int i = 0; i <= L; i++) {
+          double u = at(su, i), d = at(sd, i), r = at(sdir, i);
+          up[i] += u, dn[i] += d, di[i] += r, df[i] += d - r, nt[i] += d - u;
+        }
+        for (int i = 0; i < L; i++) {
+          double n1 = at(sd, i + 1) - at(su, i + 1), n0 = at(sd, i) - at(su, i);
+          hr[i] += (n1 - n0) * (heatfac / (p.pz[i] - p.pz[i + 1]));
+        }
+      }
+      for (size_t i = 0; i < nlev; i++) {
+        up[i] /= nmca, dn[i] /= nmca, di[i] /= nmca, df[i] /= nmca, nt[i] /= nmca, hr[i] /= nmca;
+      }
+      const double w1 = b == 0 ? WAVENUM1[NB - 1] : WAVENUM1[b - 1];
+      const double w2 = b == 0 ? WAVENUM2[NB - 2] : WAVENUM2[b - 1];
+      write_rows(f, p, w1, w2, up.data(), df.data(), di.data(), dn.data(), nt.data(), hr.data());
+    }
+    std::fprintf(f, "  Computed by rrtmgsw_column (librrtmgsw_hip.so, MI355X): column program, McICA, %d samples.\n", nmca);
+    std::fclose(f);
+    return 0;
+  }
   int rc = rrtmgsw_column_sw(1, L, p.icld, p.idelm, p.pavel.data(), p.tavel.data(), p.pz.data(),
                              p.coldry.data(), p.wkl.data(), alb.data(), &cosz, 1.0, p.juldat, p.scon,
                              p.isolvar, p.bndsolvar, p.indsolvar, &p.solcycfrac, cl.inflag,
@@ -420,7 +502,6 @@ int main(int argc, char **argv) {
   if (rc) die(rrtmgsw_last_error());
   FILE *f = std::fopen((dir + "OUTPUT_RRTM").c_str(), "w");
   if (!f) die("cannot write OUTPUT_RRTM");
-  const double heatfac = 9.8066 * 8.6400e4 / (CPDAIR * 1.e2);   // src/rrtmg_sw_init.f90:258
   // blocks: iout = 0 broadband; 16..29 that band; 98 broadband then every band (:560-575, :950-960)
   std::vector<int> blocks;
   if (p.iout == 98) {

@@ -121,13 +121,10 @@ uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
   return (uint32_t)r;
 }
 
-int kiss_jumps(int permuteseed, int nlay, int per_sub, hipStream_t s, const uint32_t **out) {
-  auto key = std::make_tuple(permuteseed, nlay, per_sub);
-  auto it = ctx.jumps.find(key);
-  if (it != ctx.jumps.end()) {
-    *out = it->second;
-    return 0;
-  }
+// The state map of n kissvec draws (src/mcica_subcol_gen_sw.f90:721-753): an affine map for the
+// congruential part, a GF(2) matrix power for the xorshift, modular powers for the two
+// multiply-with-carry parts.
+KissJump kiss_jump_of(uint64_t n) {
   Gf2 xs;
   for (int i = 0; i < 32; i++) {
     uint32_t k = 1u << i;
@@ -137,17 +134,26 @@ int kiss_jumps(int permuteseed, int nlay, int per_sub, hipStream_t s, const uint
     xs.col[i] = k;
   }
   const Affine lcg = {69069u, 1327217885u};
+  KissJump J;
+  Affine a = affine_pow(lcg, n);
+  Gf2 m = gf2_pow(xs, n);
+  J.a1 = a.a, J.c1 = a.c;
+  memcpy(J.m2, m.col, sizeof m.col);
+  J.p3 = powmod(18000, n, 1179647999ull);
+  J.p4 = powmod(30903, n, 2025259007ull);
+  return J;
+}
+
+int kiss_jumps(int permuteseed, int nlay, int per_sub, hipStream_t s, const uint32_t **out) {
+  auto key = std::make_tuple(permuteseed, nlay, per_sub);
+  auto it = ctx.jumps.find(key);
+  if (it != ctx.jumps.end()) {
+    *out = it->second;
+    return 0;
+  }
   std::vector<KissJump> J(NGPT);
   uint64_t W = permuteseed > 0 ? (uint64_t)permuteseed : 0;
-  for (int g = 0; g < NGPT; g++) {
-    uint64_t n = W + (uint64_t)g * (uint64_t)per_sub;
-    Affine a = affine_pow(lcg, n);
-    Gf2 m = gf2_pow(xs, n);
-    J[g].a1 = a.a, J[g].c1 = a.c;
-    memcpy(J[g].m2, m.col, sizeof m.col);
-    J[g].p3 = powmod(18000, n, 1179647999ull);
-    J[g].p4 = powmod(30903, n, 2025259007ull);
-  }
+  for (int g = 0; g < NGPT; g++) J[g] = kiss_jump_of(W + (uint64_t)g * (uint64_t)per_sub);
   uint32_t *d = nullptr;
   HIPCHK(hipMalloc(&d, sizeof(KissJump) * NGPT));
   HIPCHK(hipMemcpyAsync(d, J.data(), sizeof(KissJump) * NGPT, hipMemcpyHostToDevice, s));
@@ -806,6 +812,138 @@ int rrtmgsw_column_sw(int ncol, int nlay, int icld, int idelm, const double *pav
           dsum += bdir[q + nlev * b];
         }
         fup[q] = h[q], fdn[q] = h[nlev + q], fdir[q] = dsum;
+      }
+    }
+  }
+  hipFree(d);
+  return rc;
+}
+
+// The column program with McICA (src/rrtmg_sw.1col.f90:577-600 with imca = 1): nsamples draws
+// of one column, sample s (ims = s + 1) with permuteseed (ims + 1) * 112 as
+// src/mcica_subcol_gen_sw.1col.f90 sets it, all samples in one launch (a workgroup each).
+// Outputs per sample, (nsamples, nlay+1, 15) as rrtmgsw_column_sw's.
+int rrtmgsw_column_mcica_sw(int nlay, int nsamples, int icld, int irng, int idelm,
+                            const double *pavel, const double *tavel, const double *pz,
+                            const double *coldry, const double *wkl, const double *albedo,
+                            double coszen, double adjes, int dyofyr, double scon, int isolvar,
+                            const double *bndsolvar, const double *indsolvar,
+                            const double *solcycfrac, int inflag, int iceflag, int liqflag,
+                            const double *cldfrac, const double *ciwp, const double *clwp,
+                            const double *rei, const double *rel, const double *tauc,
+                            const double *ssac, const double *asmc, const double *fsfc,
+                            const double *alpha, const double *tauaer, const double *ssaaer,
+                            const double *asmaer, double *fup, double *fdn, double *fdir) {
+  if (int rc = check_init()) return rc;
+  if (nsamples < 0 || nlay < 4 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and nsamples >= 0");
+  if (icld < 1 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "RRTMG_SW: INVALID ICLD");
+  if (idelm != 0 && idelm != 1) return fail(RRTMGSW_ERR_ARGUMENT, "idelm must be 0 or 1");
+  if (inflag == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
+  if (nsamples == 0) return 0;
+  const void *need[] = {pavel, tavel, pz, coldry, wkl, albedo, cldfrac, ciwp, clwp, rei, rel,
+                        tauc, ssac, asmc, fsfc, fup, fdn, fdir};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (icld >= 4 && !alpha) return fail(RRTMGSW_ERR_ARGUMENT, "alpha is required when icld = 4 or 5");
+  const bool aer = tauaer != nullptr;
+  if (aer && (!ssaaer || !asmaer)) return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer go together");
+  const int ir = irng != 0 ? 1 : 0;
+  const size_t C = nsamples, L = nlay;
+  // every sample is the same column: (ncol, ...) layouts with the column repeated
+  std::vector<double> h;
+  auto rep2 = [&](const double *x, size_t inner, size_t outer) {   // x (inner, outer) -> (inner, C, outer)
+    size_t off = h.size();
+    h.resize(off + inner * C * outer);
+    for (size_t o = 0; o < outer; o++)
+      for (size_t c = 0; c < C; c++)
+        for (size_t i = 0; i < inner; i++) h[off + i + inner * (c + C * o)] = x[i + inner * o];
+    return off;
+  };
+  size_t o_pavel = rep2(pavel, 1, L), o_tavel = rep2(tavel, 1, L), o_pz = rep2(pz, 1, L + 1);
+  size_t o_coldry = rep2(coldry, 1, L), o_wkl = rep2(wkl, 7, L), o_alb = rep2(albedo, NBND, 1);
+  std::vector<double> mu(C, coszen);
+  size_t o_mu = h.size();
+  h.insert(h.end(), mu.begin(), mu.end());
+  size_t o_cf = rep2(cldfrac, 1, L), o_ci = rep2(ciwp, 1, L), o_cl = rep2(clwp, 1, L);
+  size_t o_rei = rep2(rei, 1, L), o_rel = rep2(rel, 1, L), o_tc = rep2(tauc, NBND, L);
+  size_t o_sc = rep2(ssac, NBND, L), o_ac = rep2(asmc, NBND, L), o_fc = rep2(fsfc, NBND, L);
+  size_t o_al = icld >= 4 ? rep2(alpha, 1, L) : 0;
+  size_t o_ta = 0, o_sa = 0, o_ga = 0;
+  if (aer) {   // (nlay, 14) -> (C, nlay, 14)
+    o_ta = rep2(tauaer, 1, L * NBND), o_sa = rep2(ssaaer, 1, L * NBND), o_ga = rep2(asmaer, 1, L * NBND);
+  }
+  const size_t nlev = C * (L + 1), nband = nlev * NBND;
+  const size_t nout = 4 * nlev + 2 * C * L + 3 * nband;
+  // KISS: one jump per sample (its permuteseed) ahead of the per-sub-column jumps;
+  // MT19937: one stream per sample, seeded with its permuteseed
+  const int per_sub = icld == 3 ? 1 : (icld >= 4 ? 2 * nlay : nlay);
+  const size_t ndraw = (size_t)NGPT * (icld == 3 ? 1 : (size_t)per_sub);
+  std::vector<KissJump> cj;
+  if (ir == 0)
+    for (size_t c = 0; c < C; c++) cj.push_back(kiss_jump_of((uint64_t)(c + 2) * NGPT));
+  double *d = nullptr;
+  const size_t bytes = (h.size() + nout) * sizeof(double) + cj.size() * sizeof(KissJump) +
+                       (ir ? C * ndraw * 4 : 0) + 256;
+  HIPCHK(hipMalloc(&d, bytes));
+  hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+  double *o = d + h.size();
+  char *tail = reinterpret_cast<char *>(o + nout);
+  uint32_t *d_cj = reinterpret_cast<uint32_t *>(tail);
+  uint32_t *d_mt = reinterpret_cast<uint32_t *>(tail + cj.size() * sizeof(KissJump));
+  if (e == hipSuccess && !cj.empty())
+    e = hipMemcpy(d_cj, cj.data(), cj.size() * sizeof(KissJump), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ir)
+    for (size_t c = 0; c < C && e == hipSuccess; c++)
+      e = launch_mt_stream((uint32_t)((c + 2) * NGPT), (long long)ndraw, d_mt + c * ndraw, nullptr);
+  if (e != hipSuccess) {
+    hipFree(d);
+    return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+  }
+  const uint32_t *jumps = nullptr;
+  if (ir == 0)
+    if (int rc = kiss_jumps(0, nlay, per_sub, nullptr, &jumps)) {
+      hipFree(d);
+      return rc;
+    }
+  RtArgs a{};
+  a.ncol = (int)C, a.nlay = nlay;
+  a.play = d + o_pavel, a.tlay = d + o_tavel, a.plev = d + o_pz, a.coldry_in = d + o_coldry;
+  a.wkl_in = d + o_wkl, a.albbnd = d + o_alb, a.coszen = d + o_mu;
+  a.cldfrac = d + o_cf, a.ciwp = d + o_ci, a.clwp = d + o_cl, a.rei = d + o_rei, a.rel = d + o_rel;
+  a.tauc = d + o_tc, a.ssac = d + o_sc, a.asmc = d + o_ac, a.fsfc = d + o_fc;
+  a.alpha = icld >= 4 ? d + o_al : nullptr;
+  if (aer) a.tauaer = d + o_ta, a.ssaaer = d + o_sa, a.asmaer = d + o_ga;
+  a.swuflx = o, a.swdflx = o + nlev, a.swuflxc = o + 2 * nlev, a.swdflxc = o + 3 * nlev;
+  a.swhr = o + 4 * nlev, a.swhrc = a.swhr + C * L;
+  a.bandu = a.swhrc + C * L, a.bandd = a.bandu + nband, a.banddir = a.bandd + nband;
+  a.fused = 1, a.mc_icld = icld, a.mc_irng = ir, a.permuteseed = 0, a.idelm = idelm;
+  a.col_jump = ir == 0 ? d_cj : nullptr;
+  a.mt_stream = ir ? d_mt : nullptr;
+  a.mt_stride = ir ? ndraw : 0;
+  DeviceTables t = ctx.dev;
+  t.kiss_jump = jumps;
+  solar_scalars(adjes, dyofyr, scon, isolvar, bndsolvar, indsolvar, solcycfrac, a.sc);
+  a.sc.heatfac = ctx.heatfac;
+  a.sc.icld = 1, a.sc.iaer = aer ? 10 : 0;
+  a.sc.inflag = inflag, a.sc.iceflag = iceflag, a.sc.liqflag = liqflag;
+  int rc = run_rt(a, t, nullptr, RRTMGSW_FP64, true);
+  if (rc == 0) {
+    std::vector<double> r(nout);
+    e = hipMemcpy(r.data(), o, nout * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+    } else {
+      const double *bu = r.data() + 4 * nlev + 2 * C * L, *bd = bu + nband, *bdir = bd + nband;
+      for (size_t q = 0; q < nlev; q++) {
+        double dsum = 0.0;
+        for (int b = 0; b < NBND; b++) {
+          fup[q + nlev * (b + 1)] = bu[q + nlev * b];
+          fdn[q + nlev * (b + 1)] = bd[q + nlev * b];
+          fdir[q + nlev * (b + 1)] = bdir[q + nlev * b];
+          dsum += bdir[q + nlev * b];
+        }
+        fup[q] = r[q], fdn[q] = r[nlev + q], fdir[q] = dsum;
       }
     }
   }

@@ -1109,9 +1109,13 @@ __device__ inline double mt_real(uint32_t u) { return (double)u / (4294967296.0 
 
 // Sub-column cloudiness of g-point g in every layer of column c (bottom to top), written as
 // bits into mask[w * NGPT + g] (bit l%32 of word l/32). Returns an error code.
+// Sample mode (the column program's McICA loop, src/rrtmg_sw.1col.f90:577-600, with every sample
+// a workgroup): col_jump[c] advances column c's KISS streams by its own permuteseed, and each
+// column reads an MT19937 stream of its own, mt + c * mt_stride, indexed as a single column.
 __device__ static int column_masks(int icld, int irng, int permuteseed, int C, int L, int c, int g,
                                    const double *play, const double *cldfrac, const double *alpha,
-                                   const uint32_t *mt, const KissJump *jumps, uint32_t *mask) {
+                                   const uint32_t *mt, const KissJump *jumps, uint32_t *mask,
+                                   const KissJump *col_jump = nullptr, size_t mt_stride = 0) {
 #pragma clang fp contract(off)
   Kiss k = {0, 0, 0, 0};
   if (irng == 0) {
@@ -1120,10 +1124,14 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
     k.s2 = kiss_seed(play[c + (size_t)C]);
     k.s3 = kiss_seed(play[c + 2 * (size_t)C]);
     k.s4 = kiss_seed(play[c + 3 * (size_t)C]);
+    if (col_jump) k = kiss_jump(k, col_jump[c]);
     k = kiss_jump(k, jumps[g]);
   }
+  // MT draw index: ((g C + c) L + l) for one stream per call, (g L + l) in column c's own stream
+  const int mC = mt_stride ? 1 : C, mc = mt_stride ? 0 : c;
+  if (mt_stride) mt += (size_t)c * mt_stride;
   double cdf_prev = 0.0, r3 = 0.0;
-  if (icld == 3) r3 = irng == 0 ? kiss_draw(k) : mt_real(mt[(size_t)g * C + c]);
+  if (icld == 3) r3 = irng == 0 ? kiss_draw(k) : mt_real(mt[(size_t)g * mC + mc]);
   uint32_t word = 0;
   for (int l = 0; l < L; l++) {
     double cldf = cldfrac[c + (size_t)C * l];
@@ -1135,7 +1143,7 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
       cdf = kiss_draw(k);
       if (icld >= 4) cdf2 = kiss_draw(k);
     } else {
-      size_t base = ((size_t)g * C + c) * L + l;
+      size_t base = ((size_t)g * mC + mc) * L + l;
       if (icld >= 4) {
         cdf = mt_real(mt[2 * base]);
         cdf2 = mt_real(mt[2 * base + 1]);

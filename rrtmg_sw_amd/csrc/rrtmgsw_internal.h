@@ -108,6 +108,10 @@ struct RtArgs {
   const double *coldry_in, *wkl_in, *albbnd;
   double *gspec, *bandu, *bandd, *banddir;
   int idelm;
+  // column program with McICA: one sample per column, each with its own KISS jump (col_jump,
+  // KissJump per column) or MT19937 stream (mt_stream + c * mt_stride)
+  const uint32_t *col_jump;
+  size_t mt_stride;
   const uint32_t *mt_stream;     // Mersenne-Twister draws for irng = 1 (fused path)
   // aerosol
   const double *tauaer, *ssaaer, *asmaer, *ecaer;

@@ -83,7 +83,7 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 enum : int { CLD_SUBCOL = 0, CLD_FUSED = 1, CLD_LAYER = 2 };
 template <int MODE, int CLD, bool SPEC>
 __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, const DeviceTables &T) {
-  static_assert(!SPEC || (MODE == 0 && CLD == CLD_LAYER), "spectral output: FP64 layer-cloud form only");
+  static_assert(!SPEC || (MODE == 0 && CLD != CLD_SUBCOL), "spectral output: FP64 layer-cloud or fused form");
   constexpr bool EXACT = MODE == 2;
   constexpr bool MIXED = MODE == 1;
   constexpr bool FUSED = CLD == CLD_FUSED;
@@ -124,9 +124,15 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     v.misc[M_LAYSOL + tid] = find_laysolfr(T.gp[first[tid]], v, laytrop, L);
   }
   int err = 0;
-  if (FUSED && a.sc.icld >= 1 && active)
-    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
-                       a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
+  if (FUSED && a.sc.icld >= 1 && active) {
+    if (SPEC)   // the column program's McICA samples
+      err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
+                         a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask,
+                         reinterpret_cast<const KissJump *>(a.col_jump), a.mt_stride);
+    else
+      err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
+                         a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
+  }
   if (LAYERCLD && a.sc.icld >= 1)   // src/rrtmg_sw_rad.nomcica.f90:617-622
     for (int lay = tid; lay < L; lay += blockDim.x) {
       const double f = a.cldfrac[c + (size_t)C * lay];
@@ -170,6 +176,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
                                    a.tauc[qb], a.ssac[qb], a.asmc[qb], a.fsfc[qb], err);
         double *o = csc + (size_t)idx * 3;
         o[0] = co.tau, o[1] = co.ssa, o[2] = co.asm_;
+        if (SPEC) csc[(size_t)L * NBND * 3 + idx] = co.orig;   // taormc for the idelm = 0 beam
       }
     }
     __syncthreads();
@@ -240,6 +247,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       if ((v.mask[(lay >> 5) * NGPT + g] >> (lay & 31)) & 1u) {
         const double *o = csc + ((size_t)lay * NBND + ibnd) * 3;
         in.cf = 1.0, in.ct = o[0], in.cw = o[1], in.cg = o[2];
+        if (SPEC) in.co = csc[(size_t)L * NBND * 3 + (size_t)lay * NBND + ibnd];
       }
     } else if (LAYERCLD && a.sc.icld >= 1 && active) {
       const double f = a.cldfrac[c + (size_t)C * lay];
@@ -561,10 +569,15 @@ rt_kernel(RtArgs a, DeviceTables T) {
   rt_body<MODE, CLD, false>(a, T);
 }
 
-// The column program's form: non-McICA clouds, per-band and direct-beam outputs.
+// The column program's forms: per-band and direct-beam outputs, with the non-McICA clouds
+// (imca = 0) or McICA samples (imca = 1, one sample per workgroup).
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
 rt_column_kernel(RtArgs a, DeviceTables T) {
   rt_body<0, CLD_LAYER, true>(a, T);
+}
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
+rt_column_mcica_kernel(RtArgs a, DeviceTables T) {
+  rt_body<0, CLD_FUSED, true>(a, T);
 }
 
 template <int MODE, int CLD>
@@ -599,9 +612,13 @@ __global__ void __launch_bounds__(256) spec_reduce_kernel(RtArgs a, DeviceTables
 }
 
 hipError_t launch_rt_spectral(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  ColLayout lo(a.nlay, 0);
+  const bool mcica = a.fused != 0;
+  ColLayout lo(a.nlay, mcica ? (a.nlay + 31) / 32 : 0);
   int per = (a.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL(rt_column_kernel, dim3(per * 8), dim3(128), lo.total, s, a, t);
+  if (mcica)
+    hipLaunchKernelGGL(rt_column_mcica_kernel, dim3(per * 8), dim3(128), lo.total, s, a, t);
+  else
+    hipLaunchKernelGGL(rt_column_kernel, dim3(per * 8), dim3(128), lo.total, s, a, t);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const long long n = (long long)a.ncol_chunk * (a.nlay + 1) * NBND;

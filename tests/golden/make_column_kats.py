@@ -26,11 +26,20 @@ import numpy as np
 # here, so only the six IATM = 0 cases (layer amounts given directly) become fixtures.
 # The one cloudy case without McICA (imca = 0: cldprop_sw + spcvrt_sw, overcast layers only)
 # is kept too; its input file and cloud file are named apart from its output file.
+# The four McICA cases (imca = 1: 200 samples averaged) are kept as well; the input and cloud
+# files of each output follow run_examples_std_atm/script.run_std_atm.
 CASES = ["MLS-clr", "MLS-clr-aer12", "MLS-clr-sza65", "MLW-clr", "SAW-clr", "TROP-clr",
-         "MLS-cld6-imca0-icld2"]
+         "MLS-cld6-imca0-icld2", "MLS-cld5-imca1-icld2", "MLS-cld7-imca1-icld2",
+         "MLS-cld7-imca1-icld5-idcor0", "MLS-cld7-imca1-icld5-idcor1"]
 AER = {"MLS-clr-aer12": "in_aer_rrtm-aer12"}
-INPUT = {"MLS-cld6-imca0-icld2": "input_rrtm_MLS-cld-imca0-icld2"}
-CLD = {"MLS-cld6-imca0-icld2": "in_cld_rrtm-cld6"}
+INPUT = {"MLS-cld6-imca0-icld2": "input_rrtm_MLS-cld-imca0-icld2",
+         "MLS-cld5-imca1-icld2": "input_rrtm_MLS-cld-imca1-icld2",
+         "MLS-cld7-imca1-icld2": "input_rrtm_MLS-cld-imca1-icld2",
+         "MLS-cld7-imca1-icld5-idcor0": "input_rrtm_MLS-cld-imca1-icld5-idcor0",
+         "MLS-cld7-imca1-icld5-idcor1": "input_rrtm_MLS-cld-imca1-icld5-idcor1"}
+CLD = {"MLS-cld6-imca0-icld2": "in_cld_rrtm-cld6", "MLS-cld5-imca1-icld2": "in_cld_rrtm-cld5",
+       "MLS-cld7-imca1-icld2": "in_cld_rrtm-cld7", "MLS-cld7-imca1-icld5-idcor0": "in_cld_rrtm-cld7",
+       "MLS-cld7-imca1-icld5-idcor1": "in_cld_rrtm-cld7"}
 IB1, IB2 = 16, 29
 
 
@@ -62,7 +71,7 @@ def read_input(path):
         lines[i + 1], [("x", 18), ("i", 2), ("x", 29), ("i", 1), ("x", 32), ("i", 1), ("x", 1),
                        ("i", 1), ("x", 2), ("i", 3), ("x", 3), ("i", 1), ("i", 1), ("x", 3),
                        ("i", 1), ("i", 1)])
-    assert iatm == 0 and (icld == 0 or imca == 0), path
+    assert iatm == 0, path
     # 9020 (12x, i3, 3x, f7.4, 3x, i2, f10.4, f10.5, 14f10.5)
     v = fields(lines[i + 2], [("x", 12), ("i", 3), ("x", 3), ("f", 7), ("x", 3), ("i", 2),
                               ("f", 10), ("f", 10)] + [("f", 10)] * 14)
@@ -75,8 +84,18 @@ def read_input(path):
         semiss[:] = 1.0
     elif iemis == 1:
         semiss[:] = semiss[0]
+    k = i + 4
+    idcor, decorr_con, lat = 0, 0.0, 0.0
+    if icld in (4, 5):      # 9014 (8x,i2), then 9015 (f10.3) decorrelation length or latitude
+        idcor = fields(lines[k], [("x", 8), ("i", 2)])[0]
+        val = fields(lines[k + 1], [("f", 10)])[0]
+        decorr_con, lat = (val, 0.0) if idcor == 0 else (0.0, val)
+        k += 2
+    # the decorrelation length of idcor = 1 takes day 94 for juldat = 0 (:1692), after the
+    # Earth-Sun factor was computed from juldat itself
+    juldat_alpha = 94 if (juldat == 0 and idcor == 1) else juldat
     # 9013 (1x,i1,i3,i5)
-    iform, nlay, nmol = fields(lines[i + 4], [("x", 1), ("i", 1), ("i", 3), ("i", 5)])
+    iform, nlay, nmol = fields(lines[k], [("x", 1), ("i", 1), ("i", 3), ("i", 5)])
     nmol = nmol or 7
     assert iform == 1 and nmol == 7, path
     f1 = [("f", 15), ("f", 10), ("f", 10), ("a", 3), ("i", 2), ("x", 1),
@@ -85,7 +104,7 @@ def read_input(path):
     f3 = [("f", 15)] * 8
     pavel, tavel, pz, altz = np.zeros(nlay), np.zeros(nlay), np.zeros(nlay + 1), np.zeros(nlay + 1)
     wkl, wbrodl = np.zeros((7, nlay)), np.zeros(nlay)
-    k = i + 5
+    k = k + 1
     for l in range(nlay):
         if l == 0:
             v = fields(lines[k], f1)
@@ -117,7 +136,8 @@ def read_input(path):
     if isolvar in (-1, 3):
         bndsolvar = solvar.copy()
     zenith = math.cos(sza * (2.0 * math.asin(1.0)) / 180.0)
-    return dict(iaer=iaer, iout=iout, imca=imca, icld=icld, idelm=idelm, juldat=juldat, sza=sza,
+    return dict(iaer=iaer, iout=iout, imca=imca, icld=icld, idelm=idelm, idcor=idcor,
+                decorr_con=decorr_con, lat=lat, juldat=juldat, juldat_alpha=juldat_alpha, sza=sza,
                 isolvar=isolvar, scon=scon, solcycfrac=solcycfrac, indsolvar=indsolvar,
                 bndsolvar=bndsolvar, semiss=semiss, nlay=nlay, pavel=pavel, tavel=tavel,
                 pz=pz, altz=altz, coldry=coldry, wkl=wkl, zenith=zenith)

@@ -40,6 +40,8 @@ def test_readers_match(name):
     for k in ("pavel", "tavel", "pz", "coldry", "semiss", "bndsolvar", "indsolvar"):
         assert np.array_equal(d[k], np.ravel(g(k))), k
     assert np.array_equal(d["wkl"], np.ravel(g("wkl"), order="F"))
+    assert np.array_equal(d["overlap"], [g("imca"), g("idcor"), g("decorr_con"), g("lat"), g("juldat_alpha")])
+    assert np.array_equal(d["altz"], g("altz"))
     if f"{name}/inflag" in KATS:
         assert icld >= 1
         assert np.array_equal(d["cldflags"], [g("inflag"), g("iceflag"), g("liqflag")])
@@ -51,17 +53,17 @@ def test_readers_match(name):
         assert L == 51 and d["tauaer"].max() > 0
 
 
-def test_refuses_mcica_sampling(tmp_path):
-    """imca = 1 (200 McICA samples averaged) is not covered: a clear error, no output."""
-    src = os.path.join(INPUTS, "MLS-cld6-imca0-icld2")
+def test_refuses_generated_atmosphere(tmp_path):
+    """IATM = 1 (layers built by src/rrtatm.f) is not covered: a clear error, no output."""
+    src = os.path.join(INPUTS, "MLS-clr")
     for f in os.listdir(src):
         shutil.copyfile(os.path.join(src, f), tmp_path / f)
     lines = open(tmp_path / "INPUT_RRTM").read().split("\n")
     i = next(k for k, l in enumerate(lines) if l.startswith("$")) + 1
     rec = lines[i].ljust(100)
-    assert rec[93] == "0"                 # record 1.2, column 94: imca
-    lines[i] = rec[:93] + "1" + rec[94:]
+    assert rec[49] == "0"                 # record 1.2, column 50: iatm
+    lines[i] = rec[:49] + "1" + rec[50:]
     open(tmp_path / "INPUT_RRTM", "w").write("\n".join(lines))
     r = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0 and "imca = 1" in r.stderr
+    assert r.returncode != 0 and "IATM = 1" in r.stderr
     assert not (tmp_path / "OUTPUT_RRTM").exists()

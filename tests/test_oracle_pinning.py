@@ -1,8 +1,10 @@
 """Pins of the C oracle that do not go through the compiled reference:
 
 * the reference's own column-mode example suite (run_examples_std_atm, six clear-sky IATM = 0
-  cases and the overcast non-McICA case MLS-cld6-imca0-icld2 (cldprop_sw with ice and liquid
-  clouds, spcvrt_sw), committed as tests/golden/column_kats.npz by
+  cases, the overcast non-McICA case MLS-cld6-imca0-icld2 (cldprop_sw with ice and liquid
+  clouds, spcvrt_sw) and the four McICA cases (200 samples of mcica_subcol_sw with the Mersenne
+  Twister averaged; maximum-random and exponential-random overlap, decorrelation length given
+  or from latitude), committed as tests/golden/column_kats.npz by
   tests/golden/make_column_kats.py):
   broadband and per-band up / diffuse-down / direct-down / down / net flux and heating rate at
   every level, to the printed precision (fluxes 1e-4 W/m2, heating rates 1e-5 K/day);
@@ -46,7 +48,25 @@ def _column(name):
         C.c_double(float(g("scon"))), C.c_int(int(g("isolvar"))), _dp(keep[6]), _dp(keep[7]),
         _dp(keep[8]), C.c_int(int(g("iaer"))), _dp(keep[9]), _dp(keep[10]), _dp(keep[11]),
         C.c_int(int(g("idelm"))), C.c_double(1004.0)]
-    if f"{name}/inflag" in KATS:
+    if f"{name}/inflag" in KATS and int(g("imca")) == 1:
+        # imca = 1: 200 McICA samples (Mersenne Twister), alpha from get_alpha for icld 4/5
+        icld = int(g("icld"))
+        cl = [np.ascontiguousarray(g(k), dtype=np.float64)
+              for k in ("cldfrac", "ciwp", "clwp", "rei", "rel")]
+        cl += [np.asfortranarray(g(k), dtype=np.float64) for k in ("tauc", "ssac", "asmc", "fsfc")]
+        alpha = np.zeros(L)
+        if icld in (4, 5):
+            dz = np.ascontiguousarray(np.diff(g("altz")) * 1000.0)     # km -> m
+            lat = np.array([float(g("lat"))])
+            assert lib.orc_get_alpha(1, 1, L, icld, int(g("idcor")), C.c_double(float(g("decorr_con"))),
+                                     _dp(dz), _dp(lat), int(g("juldat_alpha")), _dp(cl[0]), _dp(alpha)) == 0
+            keep += [dz, lat]
+        keep += cl + [alpha]
+        lib.orc_column_sw_mcica.restype = C.c_int
+        rc = lib.orc_column_sw_mcica(*common, C.c_int(int(g("inflag"))), C.c_int(int(g("iceflag"))),
+                                     C.c_int(int(g("liqflag"))), *[_dp(a) for a in cl], C.c_int(icld),
+                                     C.c_int(200), _dp(alpha), out.ctypes.data_as(ob._dp))
+    elif f"{name}/inflag" in KATS:
         # imca = 0 cloudy case: cldprop_sw + spcvrt_sw on overcast layers
         cl = [np.ascontiguousarray(g(k), dtype=np.float64)
               for k in ("cldfrac", "ciwp", "clwp", "rei", "rel")]
