@@ -144,7 +144,10 @@ KissJump kiss_jump_of(uint64_t n) {
   return J;
 }
 
+// Device table of the per-sub-column KISS jumps for (permuteseed, nlay, per_sub), built once and
+// kept for the process (calls from several threads share the cache under ctx.mu).
 int kiss_jumps(int permuteseed, int nlay, int per_sub, hipStream_t s, const uint32_t **out) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
   auto key = std::make_tuple(permuteseed, nlay, per_sub);
   auto it = ctx.jumps.find(key);
   if (it != ctx.jumps.end()) {
