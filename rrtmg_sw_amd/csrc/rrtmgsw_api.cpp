@@ -7,6 +7,7 @@
 #include <cmath>
 #include <tuple>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -311,12 +312,22 @@ int finish(hipStream_t s, int *d_status) {
 }
 
 // Columns per launch so that the per-column elimination coefficients stay within ~4 GiB.
+// RRTMGSW_MAX_LAUNCH_COLUMNS (environment, read per call) lowers the columns per launch further,
+// so that the multi-launch path can be exercised at small sizes.
+size_t launch_column_limit(size_t n) {
+  if (const char *e = std::getenv("RRTMGSW_MAX_LAUNCH_COLUMNS")) {
+    long v = std::atol(e);
+    if (v > 0) n = std::min<size_t>(n, (size_t)v);
+  }
+  return n;
+}
+
 int chunk_columns(int ncol, int nlay) {
   size_t per = scr_col_doubles(nlay) * sizeof(double);
   size_t cap = (size_t)4 << 30;
   size_t n = cap / per;
   if (n < 64) n = 64;
-  return (int)std::min<size_t>((size_t)ncol, n);
+  return (int)std::min<size_t>((size_t)ncol, launch_column_limit(n));
 }
 
 struct SwCall {
@@ -1232,7 +1243,7 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
   if (ckpt < 0 || ckpt > nlay) return fail(RRTMGSW_ERR_ARGUMENT, "ckpt must lie in 0..nlay");
   A.ckpt = ckpt;
   const size_t per_col = adj_work_bytes(nlay, ckpt);
-  int chunk = (int)std::min<size_t>((size_t)ncol, std::max<size_t>(64, ((size_t)4 << 30) / per_col));
+  int chunk = (int)std::min<size_t>((size_t)ncol, launch_column_limit(std::max<size_t>(64, ((size_t)4 << 30) / per_col)));
   void *work = nullptr, *st = nullptr;
   {
     std::lock_guard<std::mutex> lk(ctx.mu);
