@@ -396,6 +396,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   double *acc = reinterpret_cast<double *>(smem + ((lo.total + 15) & ~15));   // (L, AC_N)
   double *wU = acc + (size_t)L * AC_N, *wD = wU + (L + 1);     // level weights, top-based
   double *albacc = wD + (L + 1);                                // asdir asdif aldir aldif
+  double *solb = albacc + 4;                                    // (NGPT, 2) incident-flux terms
+  int *solt = reinterpret_cast<int *>(solb + 2 * NGPT);         // (NGPT, 2) their acc entries
   const int per = (a.ncol_chunk + 7) / 8;
   const int cl = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (cl >= a.ncol_chunk) return;
@@ -634,12 +636,17 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 #pragma unroll
     for (int q4 = 0; q4 < AC_N / 4; q4++) {
       double s = wave_sum4d(la[4 * q4], la[4 * q4 + 1], la[4 * q4 + 2], la[4 * q4 + 3], lane);
+      // one term per wave and entry, onto zero: a + b rounds as b + a, whichever wave is first
       if (lane < 16 && (lane & 3) == 0) atomicAdd(&acc[(size_t)lay * AC_N + 4 * q4 + (lane >> 2)], s);
     }
   }
 
   // ---- incident flux: zinc = adjflux src mu0, src interpolated in the key-species parameter
-  // at laysolfr for some bands (src/rrtmg_sw_taumol.f90, e.g. :495-520) ----
+  // at laysolfr for some bands (src/rrtmg_sw_taumol.f90, e.g. :495-520). Several g-points add
+  // to one accumulator: their terms are staged and added in g-point order by one thread once
+  // both waves' layer terms are in, so the rounding does not depend on wave timing. ----
+  double c1_b = 0.0, c2_b = 0.0;
+  int t1 = -1, t2 = -1;
   if (active && gp.sol_interp) {
     const int ls = v.misc[M_LAYSOL + gp.ibnd] - 1;
     const int s1 = gp.sol_upper ? gp.hi_s1 : gp.lo_s1, s2 = gp.sol_upper ? gp.hi_s2 : gp.lo_s2;
@@ -656,11 +663,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
       dsrc = a.sc.svar_f * dif(gp.facb) + a.sc.svar_s * dif(gp.snsp) + a.sc.svar_i * dif(gp.irrad);
     }
     double fs_b = phi * a.sc.adjflux[gp.ibnd] * mu0 * dsrc;
-    double c1_b = 0.0, c2_b = 0.0;
     key_spec_adj(v.c(s1, ls), v.c(s2, ls), strrat, mult, true, fs_b, 0.0, c1_b, c2_b);
-    if (c1_b != 0.0) atomicAdd(&acc[(size_t)ls * AC_N + AC_COL + s1], c1_b);
-    if (c2_b != 0.0) atomicAdd(&acc[(size_t)ls * AC_N + AC_COL + s2], c2_b);
+    t1 = ls * AC_N + AC_COL + s1, t2 = ls * AC_N + AC_COL + s2;
   }
+  if (active) solb[2 * g] = c1_b, solb[2 * g + 1] = c2_b, solt[2 * g] = t1, solt[2 * g + 1] = t2;
+  __syncthreads();
+  if (tid == 0)
+    for (int k = 0; k < 2 * NGPT; k++)
+      if (solb[k] != 0.0) acc[solt[k]] += solb[k];
   __syncthreads();
 
   // ---- per layer: adjoint of inatm_sw + setcoef_sw; outputs ----
@@ -679,7 +689,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 
 size_t adj_lds_bytes(int nlay) {
   ColLayout lo(nlay, (nlay + 31) / 32, false);
-  return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4);
+  return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4) +
+         (sizeof(double) + sizeof(int)) * 2 * NGPT;
 }
 
 size_t adj_work_bytes(int nlay, int ckpt) { return adj_work_doubles(nlay, ckpt) * sizeof(double); }

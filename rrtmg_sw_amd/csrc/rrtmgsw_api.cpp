@@ -299,11 +299,24 @@ int finish(hipStream_t s, int *d_status) {
   int st = 0;
   HIPCHK(hipMemcpyAsync(&st, d_status, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  // the reference's STOP messages of the cloud optics, by CloudStop
+  static const char *const cloud_msg[CS_COUNT] = {
+      nullptr, "ICE RADIUS OUT OF BOUNDS", "ICE GENERALIZED EFFECTIVE SIZE OUT OF BOUNDS",
+      "LIQUID EFFECTIVE RADIUS OUT OF BOUNDS", "FDELTA LESS THAN 0.0", "FDELTA GT THAN 1.0",
+      "ICE EXTINCTION LESS THAN 0.0", "ICE SSA GRTR THAN 1.0", "ICE SSA LESS THAN 0.0",
+      "ICE ASYM GRTR THAN 1.0", "ICE ASYM LESS THAN 0.0", "LIQUID EXTINCTION LESS THAN 0.0",
+      "LIQUID SSA GRTR THAN 1.0", "LIQUID SSA LESS THAN 0.0", "LIQUID ASYM GRTR THAN 1.0",
+      "LIQUID ASYM LESS THAN 0.0"};
+  const int which = st >> 8;
+  st &= 0xff;
   switch (st) {
     case 0: return 0;
     case RRTMGSW_ERR_INFLAG: return fail(st, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
-    case RRTMGSW_ERR_RADIUS: return fail(st, "CLOUD PARTICLE RADIUS OUT OF BOUNDS");
-    case RRTMGSW_ERR_OPTICS: return fail(st, "CLOUD OPTICAL PROPERTY OUT OF PHYSICAL LIMITS");
+    case RRTMGSW_ERR_RADIUS:
+    case RRTMGSW_ERR_OPTICS:
+      if (which > 0 && which < CS_COUNT) return fail(st, cloud_msg[which]);
+      return fail(st, st == RRTMGSW_ERR_RADIUS ? "CLOUD PARTICLE RADIUS OUT OF BOUNDS"
+                                               : "CLOUD OPTICAL PROPERTY OUT OF PHYSICAL LIMITS");
     case RRTMGSW_ERR_PARTIAL_CLOUD: return fail(st, "PARTIAL CLOUD NOT ALLOWED");
     case RRTMGSW_ERR_KISS_SEED:
       return fail(st, "MCICA_SUBCOL: KISSVEC SEED GENERATOR REQUIRES PMID FROM BOTTOM FOUR LAYERS.");
@@ -525,6 +538,7 @@ int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer, const double *play, con
   if (int rc = check_init()) return rc;
   if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
     return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
+  if (!icld || !iaer) return fail(RRTMGSW_ERR_ARGUMENT, "icld and iaer are required");
   if (ncol == 0) {
     if (*icld < 0 || *icld > 3) *icld = 2;
     if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;
@@ -1182,9 +1196,11 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
   hipStream_t s = (hipStream_t)stream;
   RtArgs a{};
   DeviceTables t = ctx.dev;
-  const void *outs[] = {swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
-  for (const void *p : outs)
-    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (ncol > 0) {
+    const void *outs[] = {swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+    for (const void *p : outs)
+      if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  }
   if (int rc = fused_setup(ncol, nlay, icld, iaer, permuteseed, irng, play, plev, tlay, h2ovmr, o3vmr,
                            co2vmr, ch4vmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr,
                            scon, isolvar, inflgsw, iceflgsw, liqflgsw, cldfrac, ciwp, clwp, rei, rel,
@@ -1225,9 +1241,11 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
   hipStream_t s = (hipStream_t)stream;
   RtArgs a{};
   DeviceTables t = ctx.dev;
-  const void *need[] = {seed_toa, seed_hr, d_tlay, d_h2ovmr, d_o3vmr, d_clwp, d_ciwp, d_alb};
-  for (const void *p : need)
-    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  if (ncol > 0) {
+    const void *need[] = {seed_toa, seed_hr, d_tlay, d_h2ovmr, d_o3vmr, d_clwp, d_ciwp, d_alb};
+    for (const void *p : need)
+      if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  }
   if (int rc = fused_setup(ncol, nlay, icld, iaer, permuteseed, irng, play, plev, tlay, h2ovmr, o3vmr,
                            co2vmr, ch4vmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr,
                            scon, isolvar, inflgsw, iceflgsw, liqflgsw, cldfrac, ciwp, clwp, rei, rel,
@@ -1242,6 +1260,14 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
   A.d_clwp = d_clwp, A.d_ciwp = d_ciwp, A.d_alb = d_alb;
   if (ckpt < 0 || ckpt > nlay) return fail(RRTMGSW_ERR_ARGUMENT, "ckpt must lie in 0..nlay");
   A.ckpt = ckpt;
+  {
+    // an adjoint workgroup needs more than 64 KB of LDS from 216 layers on (77 KB at 256)
+    int dev = 0, lds_max = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (adj_lds_bytes(nlay) > (size_t)lds_max)
+      return fail(RRTMGSW_ERR_ARGUMENT, "nlay needs more LDS per workgroup than this device has");
+  }
   const size_t per_col = adj_work_bytes(nlay, ckpt);
   int chunk = (int)std::min<size_t>((size_t)ncol, launch_column_limit(std::max<size_t>(64, ((size_t)4 << 30) / per_col)));
   void *work = nullptr, *st = nullptr;

@@ -260,7 +260,9 @@ __device__ inline double tab2(const double *tab, int off, int n, int i, int band
 }
 // Ice and liquid single-particle optics of one band at the layer's particle sizes
 // (src/rrtmg_sw_cldprmc.f90:176-295): extinction per unit water path, single-scattering albedo,
-// asymmetry and forward-scattering fraction; raises *err on the reference's STOP conditions.
+// asymmetry and forward-scattering fraction; sets err on the reference's STOP conditions, the
+// first one in the reference's order of tests (radius, then the derived properties; ice before
+// liquid) unless err is already set.
 struct CloudProps {
   double extcoice, ssacoice, gice, forwice, extcoliq, ssacoliq, gliq, forwliq;
 };
@@ -268,10 +270,13 @@ __device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars 
                                          double ciwp, double clwp, double rei, double rel, int &err) {
   CloudProps p = {0, 0, 0, 0, 0, 0, 0, 0};
   const double *tab = T.tab;
+  auto stop = [&](int code, int which) {
+    if (!err) err = cloud_stop(code, which);
+  };
   if (ciwp != 0.0) {
     double radice = rei;
     if (sc.iceflag == 1) {  // :182-208
-      if (radice < 13.0 || radice > 130.) err = RRTMGSW_ERR_RADIUS;
+      if (radice < 13.0 || radice > 130.) stop(RRTMGSW_ERR_RADIUS, CS_ICE_RADIUS);
       const double w2[14] = {3250., 4000., 4650., 5150., 6150., 7700., 8050.,
                              12850., 16000., 22650., 29000., 38000., 50000., 2600.};
       double wn = w2[band - 16];
@@ -282,7 +287,7 @@ __device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars 
       if (p.gice >= 1.) p.gice = 1. - 1.e-06;
       p.forwice = p.gice * p.gice;
     } else if (sc.iceflag == 2) {  // :212-231
-      if (radice < 5.0 || radice > 131.0) err = RRTMGSW_ERR_RADIUS;
+      if (radice < 5.0 || radice > 131.0) stop(RRTMGSW_ERR_RADIUS, CS_ICE_RADIUS);
       double factor = (radice - 2.) / 3.;
       int index = (int)factor;
       if (index == 43) index = 42;
@@ -296,7 +301,7 @@ __device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars 
       p.gice = g0 + fint * (g1 - g0);
       p.forwice = p.gice * p.gice;
     } else if (sc.iceflag == 3) {  // :235-260
-      if (radice < 5.0 || radice > 140.0) err = RRTMGSW_ERR_RADIUS;
+      if (radice < 5.0 || radice > 140.0) stop(RRTMGSW_ERR_RADIUS, CS_ICE_GEN_SIZE);
       double factor = (radice - 2.) / 3.;
       int index = (int)factor;
       if (index == 46) index = 45;
@@ -310,16 +315,20 @@ __device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars 
       p.ssacoice = s0 + fint * (s1 - s0);
       p.gice = g0 + fint * (g1 - g0);
       double fdelta = d0 + fint * (d1 - d0);
-      if (fdelta < 0.0 || fdelta > 1.0) err = RRTMGSW_ERR_OPTICS;
+      if (fdelta < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_FDELTA_LO);
+      if (fdelta > 1.0) stop(RRTMGSW_ERR_OPTICS, CS_FDELTA_HI);
       p.forwice = fdelta + 0.5 / p.ssacoice;
       if (p.forwice > p.gice) p.forwice = p.gice;
     }
-    if (p.extcoice < 0.0 || p.ssacoice > 1.0 || p.ssacoice < 0.0 || p.gice > 1.0 || p.gice < 0.0)
-      err = err ? err : RRTMGSW_ERR_OPTICS;
+    if (p.extcoice < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_ICE_EXT_LO);
+    if (p.ssacoice > 1.0) stop(RRTMGSW_ERR_OPTICS, CS_ICE_SSA_HI);
+    if (p.ssacoice < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_ICE_SSA_LO);
+    if (p.gice > 1.0) stop(RRTMGSW_ERR_OPTICS, CS_ICE_ASY_HI);
+    if (p.gice < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_ICE_ASY_LO);
   }
   if (clwp != 0.0 && sc.liqflag == 1) {  // :265-295
     double radliq = rel;
-    if (radliq < 2.5 || radliq > 60.) err = RRTMGSW_ERR_RADIUS;
+    if (radliq < 2.5 || radliq > 60.) stop(RRTMGSW_ERR_RADIUS, CS_LIQ_RADIUS);
     int index = (int)(radliq - 1.5);
     if (index == 0) index = 1;
     if (index == 58) index = 57;
@@ -333,8 +342,11 @@ __device__ static CloudProps cloud_props(const DeviceTables &T, const SwScalars 
     if (fint < 0. && p.ssacoliq > 1.) p.ssacoliq = s0;
     p.gliq = g0 + fint * (g1 - g0);
     p.forwliq = p.gliq * p.gliq;
-    if (p.extcoliq < 0.0 || p.ssacoliq > 1.0 || p.ssacoliq < 0.0 || p.gliq > 1.0 || p.gliq < 0.0)
-      err = err ? err : RRTMGSW_ERR_OPTICS;
+    if (p.extcoliq < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_LIQ_EXT_LO);
+    if (p.ssacoliq > 1.0) stop(RRTMGSW_ERR_OPTICS, CS_LIQ_SSA_HI);
+    if (p.ssacoliq < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_LIQ_SSA_LO);
+    if (p.gliq > 1.0) stop(RRTMGSW_ERR_OPTICS, CS_LIQ_ASY_HI);
+    if (p.gliq < 0.0) stop(RRTMGSW_ERR_OPTICS, CS_LIQ_ASY_LO);
   }
   return p;
 }

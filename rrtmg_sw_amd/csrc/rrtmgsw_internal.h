@@ -19,6 +19,16 @@ constexpr size_t scr_col_doubles(int nlay) { return (size_t)nlay * (SCR_COEFS * 
 constexpr double PADE = 0.278;
 constexpr double OD_LO = 0.06;
 
+// Which of the reference's cloud-optics STOP messages an error is (src/rrtmg_sw_cldprmc.f90:
+// 183-294, src/rrtmg_sw_cldprop.f90:194-311): kernels report RRTMGSW_ERR_RADIUS or
+// RRTMGSW_ERR_OPTICS in the low byte of their status word and one of these above it.
+enum CloudStop : int {
+  CS_ICE_RADIUS = 1, CS_ICE_GEN_SIZE, CS_LIQ_RADIUS, CS_FDELTA_LO, CS_FDELTA_HI,
+  CS_ICE_EXT_LO, CS_ICE_SSA_HI, CS_ICE_SSA_LO, CS_ICE_ASY_HI, CS_ICE_ASY_LO,
+  CS_LIQ_EXT_LO, CS_LIQ_SSA_HI, CS_LIQ_SSA_LO, CS_LIQ_ASY_HI, CS_LIQ_ASY_LO, CS_COUNT
+};
+constexpr int cloud_stop(int code, int which) { return code | (which << 8); }
+
 // Key-species interpolation modes of the unified gas-optics template (one record per g-point
 // replaces the fourteen taumolNN routines of src/rrtmg_sw_taumol.f90).
 enum KeyMode : int8_t { KEY_NONE = 0, KEY_BIN = 1, KEY_MONO = 2 };
@@ -143,6 +153,7 @@ struct AdjArgs {
   int ckpt;                       // 0: store every sweep state; K > 0: checkpoint every K layers
 };
 size_t adj_work_bytes(int nlay, int ckpt);   // work area per column
+size_t adj_lds_bytes(int nlay);               // dynamic LDS of one adjoint workgroup
 
 // Kernel launchers (rrtmgsw_kernels.hip)
 hipError_t launch_rt_fp64(const RtArgs &a, const DeviceTables &t, hipStream_t s);

@@ -23,12 +23,17 @@ CASES = {
     "mcica16x72_icld2_kiss": ("workload", dict(name="mcica8192x72", ncol=16)),
     "mcica8x60_icld5_kiss_aer": ("make", dict(ncol=8, nlay=60, icld=5, iaer=10, seed=3)),
     "mcica4x40_icld2_mt": ("make", dict(ncol=4, nlay=40, icld=2, iaer=0, seed=9, irng=1)),
+    # the layer-count limits of the library: 4 and 256 layers
+    "mcica6x4_icld5_kiss_aer": ("make", dict(ncol=6, nlay=4, icld=5, iaer=10, seed=44)),
+    "mcica3x256_icld2_mt": ("make", dict(ncol=3, nlay=256, icld=2, iaer=0, seed=47, irng=1)),
 }
 # rrtmg_sw of the non-McICA driver (oracle/_ref/librrtmg_ref_nomcica.so), clear and overcast
 # layers with cldprop_sw optics
 NOMCICA_CASES = {
     "nomcica12x72_overcast": ("workload", dict(name="overcast8192x72", ncol=12)),
     "nomcica8x60_overcast_aer": ("make", dict(ncol=8, nlay=60, icld=5, iaer=10, seed=3, overcast=True)),
+    "nomcica6x4_overcast": ("make", dict(ncol=6, nlay=4, icld=2, iaer=0, seed=52, overcast=True)),
+    "nomcica3x256_overcast_aer": ("make", dict(ncol=3, nlay=256, icld=2, iaer=10, seed=53, overcast=True)),
 }
 
 

@@ -95,6 +95,9 @@ def _cases():
     f[f > 0] = np.where(np.arange((f > 0).sum()) % 2 == 0, 1.0 - 5e-7, 5e-7)
     k.cldfrac = np.asfortranarray(f)
     c["near_edges"] = k
+    # the layer-count limits of the library: 4 and 256 layers
+    c["layers4_aer10"] = overcast(make_columns(8, 4, icld=2, iaer=10, seed=11))
+    c["layers256"] = overcast(make_columns(4, 256, icld=2, iaer=0, seed=12))
     return c
 
 

@@ -51,6 +51,24 @@ def test_mcica_and_cloudy_fluxes_bitexact(icld, irng):
         assert np.array_equal(r[k], o[k]), k
 
 
+@pytest.mark.parametrize("nlay,icld,irng,seed", [(4, 2, 0, 11), (4, 5, 1, 14), (256, 2, 1, 47),
+                                                 (256, 5, 0, 48)])
+def test_layer_count_limits_bitexact(nlay, icld, irng, seed):
+    """The fewest (4) and most (256) layers the library takes; 256 fills eight 32-layer mask
+    words exactly."""
+    cols = make_columns(8, nlay, icld=icld, iaer=10, seed=seed)
+    cols.irng = irng
+    mo, _ = ob.run_mcica(cols, "orc")
+    mr, _ = ob.run_mcica(cols, "ref")
+    for k in mo:
+        assert np.array_equal(mo[k], mr[k]), k
+    assert mo["cldfmcl"].any()
+    r = ob.run_rrtmg(cols, mr, which="ref", icld=min(icld, 3))
+    o = ob.run_rrtmg(cols, mr, which="orc", icld=min(icld, 3))
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
 @pytest.mark.parametrize("isolvar,scon,opts", [
     (-1, 0.0, {}), (-1, 1361.0, {"bndsolvar": np.linspace(0.9, 1.1, 14)}),
     (0, 0.0, {}), (1, 1361.0, {"indsolvar": [1.2, 0.8], "solcycfrac": 0.3}),
