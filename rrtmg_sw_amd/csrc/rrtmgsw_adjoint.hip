@@ -430,8 +430,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   }
   __syncthreads();
   const int laytrop = v.misc[M_LAYTROP];
+  const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};   // g of each band
   if (tid < NBND) {
-    const int first[NBND] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100};
     GPoint gb = T.gp[first[tid]];
     v.misc[M_LAYSOL + tid] = find_laysolfr(gb, v, laytrop, L);
   }
@@ -445,7 +445,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
   double *CJ = UB + adj_work_doubles(L, K) - (size_t)L * NBND * CJ_N;
   if (a.sc.icld >= 1) {
-    const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};
     for (int idx = tid; idx < L * NBND; idx += blockDim.x) {
       const int lay = idx / NBND, b = idx - lay * NBND;
       uint32_t any = 0;
@@ -642,9 +641,11 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   }
 
   // ---- incident flux: zinc = adjflux src mu0, src interpolated in the key-species parameter
-  // at laysolfr for some bands (src/rrtmg_sw_taumol.f90, e.g. :495-520). Several g-points add
-  // to one accumulator: their terms are staged and added in g-point order by one thread once
-  // both waves' layer terms are in, so the rounding does not depend on wave timing. ----
+  // at laysolfr for some bands (src/rrtmg_sw_taumol.f90, e.g. :495-520). The g-points of a band
+  // add to the same two accumulators (laysolfr and the key species are band constants), and
+  // bands can share one: the terms are staged, summed per band in g-point order and the band
+  // sums added in band order once both waves' layer terms are in, so the rounding does not
+  // depend on wave timing. ----
   double c1_b = 0.0, c2_b = 0.0;
   int t1 = -1, t2 = -1;
   if (active && gp.sol_interp) {
@@ -668,9 +669,23 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   }
   if (active) solb[2 * g] = c1_b, solb[2 * g + 1] = c2_b, solt[2 * g] = t1, solt[2 * g + 1] = t2;
   __syncthreads();
-  if (tid == 0)
-    for (int k = 0; k < 2 * NGPT; k++)
-      if (solb[k] != 0.0) acc[solt[k]] += solb[k];
+  if (wave == 0) {
+    // lane 2b + k sums slot k of band b; lane 0 then adds the 28 sums
+    double sum = 0.0;
+    int at = -1;
+    if (lane < 2 * NBND) {
+      const int b = lane >> 1, k = lane & 1;
+      for (int gg = first[b]; gg < first[b + 1]; gg++) {
+        sum += solb[2 * gg + k];
+        at = max(at, solt[2 * gg + k]);
+      }
+    }
+    for (int t = 0; t < 2 * NBND; t++) {
+      const double v = __shfl(sum, t);
+      const int to = __shfl(at, t);
+      if (lane == 0 && v != 0.0) acc[to] += v;
+    }
+  }
   __syncthreads();
 
   // ---- per layer: adjoint of inatm_sw + setcoef_sw; outputs ----
