@@ -12,7 +12,10 @@ device entry point (rrtmgsw_mcica_sw_dev, FP64). Prints one JSON line.
 Both precision forms are timed in the same run; `value` is the one --precision names (FP64 by
 default), and `forms` carries both, their speed ratio and the mixed form's largest error
 against the FP64 form (fluxes W/m2, heating rates K/day). `adjoint` carries the adjoint's cost
-against the forward on the 4096 x 60 exponential-random-overlap + aerosol workload.
+against the forward on the 4096 x 60 exponential-random-overlap + aerosol workload; `nomcica`
+the non-McICA driver on the same columns made overcast; `host_abi` the reference's own two
+calls on host arrays (mcica_subcol_sw, then rrtmg_sw on the materialised sub-column arrays),
+PCIe copies included.
 
 --adjoint prints instead one line for the adjoint on a large workload (default 65536 x 137):
 columns/s of the forward and of forward + adjoint, with full storage of every layer's terms
@@ -126,6 +129,56 @@ def nomcica_form(args, name="overcast8192x72"):
                         f"{cols.ncol} columns x {cols.nlay} layers, every cloudy layer overcast",
             "ms_per_step": ms, "columns_per_s": cols.ncol / (ms * 1e-3),
             "kernel": "rt_kernel<0, 2>"}
+
+
+def host_abi_form(cols, steps=3, warmup=1):
+    """The reference's two calls on host arrays, as a GCM makes them through the drop-in:
+    rrtmgsw_mcica_subcol (profiles copied in; the seven (112, ncol, nlay) sub-column arrays and
+    two (ncol, nlay) arrays copied back) then rrtmgsw_sw (those arrays and the profiles copied
+    in, fluxes and heating rates copied back), each synchronous, on pageable host arrays
+    allocated once, as a caller's arrays are. Wall time per call; PCIe-inclusive, so never the
+    bench value."""
+    from rrtmg_sw_amd import api
+    N, L, G = cols.ncol, cols.nlay, api.NGPT
+    mc_kw = dict(iplon=1, ncol=N, nlay=L, icld=cols.icld, permuteseed=cols.permuteseed,
+                 irng=cols.irng, play=cols.play, cldfrac=cols.cldfrac, ciwp=cols.ciwp,
+                 clwp=cols.clwp, rei=cols.rei, rel=cols.rel, tauc=cols.tauc, ssac=cols.ssac,
+                 asmc=cols.asmc, fsfc=cols.fsfc, alpha=cols.alpha)
+    mc = api.mcica_subcol_sw(**mc_kw)
+
+    def sw_kw(mc):
+        return dict(ncol=N, nlay=L, icld=cols.icld, iaer=cols.iaer, play=cols.play,
+                    plev=cols.plev, tlay=cols.tlay, tlev=cols.tlev, tsfc=cols.tsfc,
+                    h2ovmr=cols.h2ovmr, o3vmr=cols.o3vmr, co2vmr=cols.co2vmr,
+                    ch4vmr=cols.ch4vmr, n2ovmr=cols.n2ovmr, o2vmr=cols.o2vmr, asdir=cols.asdir,
+                    asdif=cols.asdif, aldir=cols.aldir, aldif=cols.aldif, coszen=cols.coszen,
+                    adjes=cols.adjes, dyofyr=cols.dyofyr, scon=cols.scon, isolvar=cols.isolvar,
+                    inflgsw=cols.inflgsw, iceflgsw=cols.iceflgsw, liqflgsw=cols.liqflgsw,
+                    cldfmcl=mc["cldfmcl"], taucmcl=mc["taucmcl"], ssacmcl=mc["ssacmcl"],
+                    asmcmcl=mc["asmcmcl"], fsfcmcl=mc["fsfcmcl"], ciwpmcl=mc["ciwpmcl"],
+                    clwpmcl=mc["clwpmcl"], reicmcl=mc["reicmcl"], relqmcl=mc["relqmcl"],
+                    tauaer=cols.tauaer, ssaaer=cols.ssaaer, asmaer=cols.asmaer, ecaer=cols.ecaer)
+
+    out = api.rrtmg_sw(**sw_kw(mc))
+    t_sub = t_sw = 0.0
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        api.mcica_subcol_sw(**mc_kw, out=mc)
+        t1 = time.perf_counter()
+        api.rrtmg_sw(**sw_kw(mc), out=out)
+        t2 = time.perf_counter()
+        if i >= warmup:
+            t_sub += t1 - t0
+            t_sw += t2 - t1
+    t_sub, t_sw = t_sub / steps * 1e3, t_sw / steps * 1e3
+    big = 8.0 * (7 * G * N * L + 2 * N * L)
+    return {"workload": f"mcica8192x72 on host arrays: rrtmgsw_mcica_subcol then rrtmgsw_sw, "
+                        f"{steps} timed pairs after {warmup}",
+            "ms_mcica_subcol": t_sub, "ms_rrtmg_sw": t_sw, "ms_per_step": t_sub + t_sw,
+            "columns_per_s": N / ((t_sub + t_sw) * 1e-3),
+            "subcolumn_bytes_each_way": big,
+            "note": "PCIe-inclusive: the sub-column arrays go device -> host and back; the fused "
+                    "device entry avoids both copies"}
 
 
 def bench_adjoint(args):
@@ -259,6 +312,10 @@ def main():
     }
     line["adjoint"] = adjoint_ratio(args)
     line["nomcica"] = nomcica_form(args)
+    if args.workload == "mcica8192x72":
+        del d
+        torch.cuda.empty_cache()
+        line["host_abi"] = host_abi_form(cols)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cols)
     print(json.dumps(line), flush=True)

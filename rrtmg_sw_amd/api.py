@@ -84,14 +84,16 @@ def rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr
              ch4vmr, n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon,
              isolvar, inflgsw, iceflgsw, liqflgsw, cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl,
              ciwpmcl, clwpmcl, reicmcl, relqmcl, tauaer, ssaaer, asmaer, ecaer,
-             bndsolvar=None, indsolvar=None, solcycfrac=None):
+             bndsolvar=None, indsolvar=None, solcycfrac=None, out=None):
     """rrtmg_sw (src/rrtmg_sw_rad.f90:99). Returns dict(swuflx, swdflx, swhr, swuflxc, swdflxc,
-    swhrc, icld, iaer) -- icld / iaer as clamped by the reference (intent(inout))."""
+    swhrc, icld, iaer) -- icld / iaer as clamped by the reference (intent(inout)). `out` may
+    hold the six output arrays of a previous call to write into."""
     dev = _is_dev(play)
     N, L, G = ncol, nlay, NGPT
-    out = {k: _new(dev, (N, L + 1)) for k in ("swuflx", "swdflx", "swuflxc", "swdflxc")}
-    out["swhr"] = _new(dev, (N, L))
-    out["swhrc"] = _new(dev, (N, L))
+    if out is None:
+        out = {k: _new(dev, (N, L + 1)) for k in ("swuflx", "swdflx", "swuflxc", "swdflxc")}
+        out["swhr"] = _new(dev, (N, L))
+        out["swhrc"] = _new(dev, (N, L))
     ic, ia = C.c_int(icld), C.c_int(iaer)
     opt = [None if bndsolvar is None else np.ascontiguousarray(bndsolvar, dtype=np.float64),
            None if indsolvar is None else np.ascontiguousarray(indsolvar, dtype=np.float64),
@@ -156,16 +158,17 @@ def rrtmg_sw_nomcica(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovm
 
 
 def mcica_subcol_sw(iplon, ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei,
-                    rel, tauc, ssac, asmc, fsfc, alpha):
+                    rel, tauc, ssac, asmc, fsfc, alpha, out=None):
     """mcica_subcol_sw (src/mcica_subcol_gen_sw.f90:181). Returns dict of the nine outputs
     (cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl) and
-    irng as written back."""
+    irng as written back. `out` may hold the nine arrays of a previous call to write into."""
     dev = _is_dev(play)
     N, L, G = ncol, nlay, NGPT
-    out = {k: _new(dev, (G, N, L)) for k in ("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl",
-                                              "ssacmcl", "asmcmcl", "fsfcmcl")}
-    out["reicmcl"] = _new(dev, (N, L))
-    out["relqmcl"] = _new(dev, (N, L))
+    if out is None:
+        out = {k: _new(dev, (G, N, L)) for k in ("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl",
+                                                  "ssacmcl", "asmcmcl", "fsfcmcl")}
+        out["reicmcl"] = _new(dev, (N, L))
+        out["relqmcl"] = _new(dev, (N, L))
     ir = C.c_int(irng)
     args = [C.c_int(iplon), C.c_int(N), C.c_int(L), C.c_int(icld), C.c_int(permuteseed),
             C.byref(ir), _ptr(play, N * L), _ptr(cldfrac, N * L), _ptr(ciwp, N * L),

@@ -396,8 +396,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   double *acc = reinterpret_cast<double *>(smem + ((lo.total + 15) & ~15));   // (L, AC_N)
   double *wU = acc + (size_t)L * AC_N, *wD = wU + (L + 1);     // level weights, top-based
   double *albacc = wD + (L + 1);                                // asdir asdif aldir aldif
-  double *solb = albacc + 4;                                    // (NGPT, 2) incident-flux terms
-  int *solt = reinterpret_cast<int *>(solb + 2 * NGPT);         // (NGPT, 2) their acc entries
   const int per = (a.ncol_chunk + 7) / 8;
   const int cl = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (cl >= a.ncol_chunk) return;
@@ -643,9 +641,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   // ---- incident flux: zinc = adjflux src mu0, src interpolated in the key-species parameter
   // at laysolfr for some bands (src/rrtmg_sw_taumol.f90, e.g. :495-520). The g-points of a band
   // add to the same two accumulators (laysolfr and the key species are band constants), and
-  // bands can share one: the terms are staged, summed per band in g-point order and the band
-  // sums added in band order once both waves' layer terms are in, so the rounding does not
-  // depend on wave timing. ----
+  // bands can share one. Each wave sums its bands over lanes in lane order (shuffles), and once
+  // both waves' layer terms are in, wave 0 then wave 1 adds its band sums in band order: the
+  // rounding does not depend on wave timing, and no LDS beyond the accumulators is needed. ----
   double c1_b = 0.0, c2_b = 0.0;
   int t1 = -1, t2 = -1;
   if (active && gp.sol_interp) {
@@ -667,26 +665,20 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     key_spec_adj(v.c(s1, ls), v.c(s2, ls), strrat, mult, true, fs_b, 0.0, c1_b, c2_b);
     t1 = ls * AC_N + AC_COL + s1, t2 = ls * AC_N + AC_COL + s2;
   }
-  if (active) solb[2 * g] = c1_b, solb[2 * g + 1] = c2_b, solt[2 * g] = t1, solt[2 * g + 1] = t2;
   __syncthreads();
-  if (wave == 0) {
-    // lane 2b + k sums slot k of band b; lane 0 then adds the 28 sums
-    double sum = 0.0;
-    int at = -1;
-    if (lane < 2 * NBND) {
-      const int b = lane >> 1, k = lane & 1;
-      for (int gg = first[b]; gg < first[b + 1]; gg++) {
-        sum += solb[2 * gg + k];
-        at = max(at, solt[2 * gg + k]);
+  for (int w = 0; w < 2; w++) {
+    if (wave == w) {
+      for (int b = w * (NBND / 2); b < (w + 1) * (NBND / 2); b++) {   // bands of this wave
+        const int l0 = first[b] - w * WAVE_G, l1 = first[b + 1] - w * WAVE_G;
+        double s1 = 0.0, s2 = 0.0;
+        for (int l = l0; l < l1; l++) s1 += __shfl(c1_b, l), s2 += __shfl(c2_b, l);
+        const int to1 = __shfl(t1, l0), to2 = __shfl(t2, l0);
+        if (lane == 0 && s1 != 0.0) acc[to1] += s1;
+        if (lane == 0 && s2 != 0.0) acc[to2] += s2;
       }
     }
-    for (int t = 0; t < 2 * NBND; t++) {
-      const double v = __shfl(sum, t);
-      const int to = __shfl(at, t);
-      if (lane == 0 && v != 0.0) acc[to] += v;
-    }
+    __syncthreads();
   }
-  __syncthreads();
 
   // ---- per layer: adjoint of inatm_sw + setcoef_sw; outputs ----
   for (int lay = tid; lay < L; lay += blockDim.x) {
@@ -704,8 +696,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 
 size_t adj_lds_bytes(int nlay) {
   ColLayout lo(nlay, (nlay + 31) / 32, false);
-  return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4) +
-         (sizeof(double) + sizeof(int)) * 2 * NGPT;
+  return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4);
 }
 
 size_t adj_work_bytes(int nlay, int ckpt) { return adj_work_doubles(nlay, ckpt) * sizeof(double); }

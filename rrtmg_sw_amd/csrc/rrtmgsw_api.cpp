@@ -1261,7 +1261,7 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
   if (ckpt < 0 || ckpt > nlay) return fail(RRTMGSW_ERR_ARGUMENT, "ckpt must lie in 0..nlay");
   A.ckpt = ckpt;
   {
-    // an adjoint workgroup needs more than 64 KB of LDS from 216 layers on (77 KB at 256)
+    // an adjoint workgroup needs more than 64 KB of LDS from 225 layers on (74 KB at 256)
     int dev = 0, lds_max = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
