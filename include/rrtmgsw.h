@@ -14,7 +14,11 @@
  * Host-array entry points copy inputs to the GPU and outputs back (synchronous). The *_dev
  * entry points take device pointers and a hipStream_t (passed as void*, NULL = default stream)
  * and are asynchronous except for the final status check, which synchronizes the stream.
- * All entry points are reentrant after rrtmgsw_init; calls on different streams may overlap.
+ * Threads: after rrtmgsw_init every entry point may be called from several host threads at
+ * once. Calls on different streams overlap; calls on one stream (the host-array and column
+ * entry points use the default stream) run one at a time, since a call's scratch, status and
+ * generator buffers belong to its stream until it returns. rrtmgsw_init and rrtmgsw_finalize
+ * must not overlap other calls.
  */
 #ifndef RRTMGSW_H
 #define RRTMGSW_H

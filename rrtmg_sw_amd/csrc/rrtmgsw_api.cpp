@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -51,8 +52,23 @@ struct Context {
   GPoint *d_gp = nullptr;
   std::mutex mu;
   std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork;
+  std::map<hipStream_t, std::unique_ptr<std::mutex>> calls;   // one call at a time per stream
   std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
 } ctx;
+
+// A call's scratch, status word and generator buffers are its stream's from their acquisition
+// to the call's last read-back (every entry ends by synchronising its stream), so calls on one
+// stream from different host threads run one at a time; calls on different streams overlap.
+std::unique_lock<std::mutex> stream_call(hipStream_t s) {
+  std::mutex *m;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    auto &p = ctx.calls[s];
+    if (!p) p.reset(new std::mutex);
+    m = p.get();
+  }
+  return std::unique_lock<std::mutex>(*m);
+}
 
 // Grow-only per-stream device buffer.
 int stream_buffer(std::map<hipStream_t, Scratch> &m, hipStream_t s, size_t bytes, void **out) {
@@ -493,6 +509,7 @@ int rrtmgsw_sw_dev(int ncol, int nlay, int *icld, int *iaer, const double *play,
   if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;  // :597
   if (ncol == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
+  auto call = stream_call(s);
   RtArgs a{};
   a.ncol = ncol, a.nlay = nlay;
   a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
@@ -633,6 +650,7 @@ int rrtmgsw_sw_nomcica_dev(int ncol, int nlay, int *icld, int *iaer, const doubl
   if (*iaer != 0 && *iaer != 6 && *iaer != 10) *iaer = 0;  // :568
   if (ncol == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
+  auto call = stream_call(s);
   RtArgs a{};
   a.ncol = ncol, a.nlay = nlay;
   a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
@@ -758,6 +776,7 @@ int rrtmgsw_column_sw(int ncol, int nlay, int icld, int idelm, const double *pav
                       const double *ssaaer, const double *asmaer, double *fup, double *fdn,
                       double *fdir) {
   if (int rc = check_init()) return rc;
+  auto call = stream_call(nullptr);
   if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
     return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
   if (idelm != 0 && idelm != 1) return fail(RRTMGSW_ERR_ARGUMENT, "idelm must be 0 or 1");
@@ -863,6 +882,7 @@ int rrtmgsw_column_mcica_sw(int nlay, int nsamples, int icld, int irng, int idel
                             const double *alpha, const double *tauaer, const double *ssaaer,
                             const double *asmaer, double *fup, double *fdn, double *fdir) {
   if (int rc = check_init()) return rc;
+  auto call = stream_call(nullptr);
   if (nsamples < 0 || nlay < 4 || nlay > MAX_LAY)
     return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and nsamples >= 0");
   if (icld < 1 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "RRTMG_SW: INVALID ICLD");
@@ -1019,6 +1039,7 @@ int rrtmgsw_mcica_subcol_dev(int iplon, int ncol, int nlay, int icld, int permut
     if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
   if (icld >= 4 && !alpha) return fail(RRTMGSW_ERR_ARGUMENT, "alpha is required when icld = 4 or 5");
   hipStream_t s = (hipStream_t)stream;
+  auto call = stream_call(s);
   const uint32_t *mt, *jumps;
   if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, *irng, s, &mt, &jumps)) return rc;
   void *st = nullptr;
@@ -1194,6 +1215,7 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
   if (precision != RRTMGSW_FP64 && precision != RRTMGSW_MIXED && precision != RRTMGSW_FP64_EXP)
     return fail(RRTMGSW_ERR_ARGUMENT, "precision must be RRTMGSW_FP64, RRTMGSW_MIXED or RRTMGSW_FP64_EXP");
   hipStream_t s = (hipStream_t)stream;
+  auto call = stream_call(s);
   RtArgs a{};
   DeviceTables t = ctx.dev;
   if (ncol > 0) {
@@ -1239,6 +1261,7 @@ int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int per
                                  int ckpt, void *stream) {
   if (int rc = check_init()) return rc;
   hipStream_t s = (hipStream_t)stream;
+  auto call = stream_call(s);
   RtArgs a{};
   DeviceTables t = ctx.dev;
   if (ncol > 0) {
