@@ -2,7 +2,8 @@
 """Write golden fixtures from the compiled reference (oracle/_ref/librrtmg_ref.so, built from the
 reference's own sources by oracle/build_ref.sh). Each fixture holds the inputs (a workload name,
 column count and seed recipe are enough to regenerate them, so only outputs are stored) and the
-reference's outputs. Run in a container where the reference tree and oracle/_ref exist:
+reference's outputs, and reduced_tables.json the SHA-256 of each reduced k-table the reference
+builds. Run in a container where the reference tree and oracle/_ref exist:
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -46,8 +47,25 @@ def build(kind, kw):
     return overcast(cols) if oc else cols
 
 
+def table_digests():
+    """SHA-256 of each reduced (112-point) table of the compiled reference, by band and kind."""
+    import hashlib
+    out = {}
+    for b in range(16, 30):
+        for k in ob.TABLE_KINDS:
+            r = ob.ref_table(b, k)
+            if r is not None:
+                out[f"{k}{b}"] = {"n": int(r.size), "sha256": hashlib.sha256(r.tobytes()).hexdigest()}
+    return out
+
+
 def main(only=None):
     assert ob.ref_available() and ob.ref_nomcica_available(), "oracle/_ref not built"
+    if not only or "reduced_tables" in only:
+        import json
+        path = os.path.join(HERE, "reduced_tables.json")
+        json.dump(table_digests(), open(path, "w"), indent=0, sort_keys=True)
+        print(path, os.path.getsize(path))
     for name, (kind, kw) in NOMCICA_CASES.items():
         if only and name not in only:
             continue

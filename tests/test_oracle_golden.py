@@ -1,6 +1,7 @@
 """The C oracle against golden fixtures written by the compiled reference
-(tests/golden/make_golden.py): bit-for-bit equal fluxes, heating rates and McICA masks.
-These pin the oracle where oracle/_ref is not built."""
+(tests/golden/make_golden.py): bit-for-bit equal fluxes, heating rates and McICA masks, and the
+224 -> 112 reduced k-tables equal to the reference's by SHA-256. These pin the oracle where
+oracle/_ref is not built."""
 import os
 
 import numpy as np
@@ -36,3 +37,14 @@ def test_oracle_nomcica_matches_reference_golden(name):
     out, _, _ = ob.run_nomcica(cols, "orc")
     for k in ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc"):
         assert np.array_equal(out[k], g[k]), k
+
+
+def test_oracle_reduced_tables_match_reference_digests():
+    import hashlib
+    import json
+    want = json.load(open(os.path.join(HERE, "golden", "reduced_tables.json")))
+    assert len(want) > 100
+    for name, d in want.items():
+        o = ob.orc_table(name)
+        assert o is not None and o.size == d["n"], name
+        assert hashlib.sha256(o.tobytes()).hexdigest() == d["sha256"], name
