@@ -15,7 +15,8 @@ against the FP64 form (fluxes W/m2, heating rates K/day). `adjoint` carries the 
 against the forward on the 4096 x 60 exponential-random-overlap + aerosol workload; `nomcica`
 the non-McICA driver on the same columns made overcast; `host_abi` the reference's own two
 calls on host arrays (mcica_subcol_sw, then rrtmg_sw on the materialised sub-column arrays),
-PCIe copies included.
+PCIe copies included; `cpu_baseline_all_cores` the CPU leg on 16 worker processes over the
+whole workload.
 
 --adjoint prints instead one line for the adjoint on a large workload (default 65536 x 137):
 columns/s of the forward and of forward + adjoint, with full storage of every layer's terms
@@ -66,6 +67,65 @@ def cpu_baseline(cols_full, budget_s=12.0, chunk=16):
     return {"value": done / t, "unit": "columns/s", "cores": 1, "kind": kind,
             "sample": f"{done} columns of the 8192x72 McICA workload (mcica_subcol_sw + rrtmg_sw, "
                       f"chunks of {chunk}), {t:.1f} s on one host core"}
+
+
+_CPU = {}
+
+
+def _cpu_worker_init(name, barrier):
+    _CPU["barrier"] = barrier
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from rrtmg_sw_amd.workloads import workload
+    _CPU["ob"] = ob
+    _CPU["which"] = "ref" if ob.ref_available() else "orc"
+    ob.ref() if _CPU["which"] == "ref" else ob.orc()
+    _CPU["cols"] = workload(name)
+
+
+def _cpu_worker_ready(_):
+    _CPU["barrier"].wait(timeout=300)   # one task per worker: all of them initialised
+    return 0
+
+
+def _cpu_worker_block(lo_hi, chunk=16):
+    ob, which, full = _CPU["ob"], _CPU["which"], _CPU["cols"]
+    lo, hi = lo_hi
+    for c0 in range(lo, hi, chunk):
+        cols = full.subset(slice(c0, min(c0 + chunk, hi)))
+        mc, _ = ob.run_mcica(cols, which)
+        ob.run_rrtmg(cols, mc, which=which)
+    return hi - lo
+
+
+def cpu_baseline_all_cores(name="mcica8192x72", procs=16, reps=3):
+    """The same CPU leg on `procs` worker processes (the reference has no threading), each on a
+    contiguous block of the workload's columns in 16-column chunks; KISS streams are per column,
+    so the blocks compute exactly the columns of a single run. Workers are forked before this
+    process touches the GPU; the time is the wall clock of the whole workload once every worker
+    has loaded the library and built the inputs."""
+    import multiprocessing as mp
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from rrtmg_sw_amd.workloads import WORKLOADS
+    n = WORKLOADS[name]["ncol"]
+    procs = max(1, min(procs, os.cpu_count() or 1))
+    edges = [n * i // procs for i in range(procs + 1)]
+    ctx = mp.get_context("fork")
+    barrier = ctx.Barrier(procs)
+    with ctx.Pool(procs, initializer=_cpu_worker_init, initargs=(name, barrier)) as pool:
+        pool.map(_cpu_worker_ready, range(procs), chunksize=1)
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            done = sum(pool.map(_cpu_worker_block, list(zip(edges[:-1], edges[1:])), chunksize=1))
+            walls.append(time.perf_counter() - t0)
+    t = sorted(walls)[len(walls) // 2]
+    return {"value": done / t, "unit": "columns/s", "cores": procs,
+            "kind": "reference" if ob.ref_available() else "port",
+            "sample": f"all {done} columns of {name} (mcica_subcol_sw + rrtmg_sw, chunks of 16) "
+                      f"on {procs} worker processes, one contiguous block each; median of {reps} "
+                      f"runs, {t:.2f} s wall"}
 
 
 def _time(fn, steps, warmup):
@@ -233,6 +293,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # the all-core CPU leg forks its workers before this process touches the GPU
+    cpu_all = None
+    if not args.no_cpu and not args.adjoint and args.workload == "mcica8192x72":
+        cpu_all = cpu_baseline_all_cores(args.workload)
+
     import torch
     from rrtmg_sw_amd import api, flops
     from rrtmg_sw_amd.workloads import workload
@@ -318,6 +383,8 @@ def main():
         line["host_abi"] = host_abi_form(cols)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cols)
+    if cpu_all is not None:
+        line["cpu_baseline_all_cores"] = cpu_all
     print(json.dumps(line), flush=True)
 
 
