@@ -8,7 +8,7 @@
   exactly and take the adjoint workgroup past 64 KB of LDS. (The non-McICA driver at 4 and 256
   layers runs among the oracle cases of test_gpu_nomcica.)
 * One column (a grid of eight workgroups, seven without a column) and no columns (every entry
-  returns at once).
+  returns at once); rrtmgsw_finalize then rrtmgsw_init (same results after).
 * Layer counts outside 4..256 and the reference's STOP conditions, each with its error code
   and message: invalid icld (src/mcica_subcol_gen_sw.f90:262-263), inflag = 1 with McICA
   (src/rrtmg_sw_cldprmc.f90:166), pressures not ordered bottom-up for the KISS seed
@@ -278,3 +278,22 @@ def test_reference_stops(gpu_lib):
     got = _fused(api, cols)
     mc, _ = ob.run_mcica(cols)
     assert_close(got, ob.run_rrtmg(cols, mc), "after errors")
+
+
+def test_finalize_and_reinit(gpu_lib):
+    """rrtmgsw_finalize frees the tables and buffers; calls then return RRTMGSW_ERR_NOT_INITIALIZED
+    until rrtmgsw_init, after which results are bit-for-bit those before."""
+    api = gpu_lib
+    cols = make_columns(16, 30, icld=2, iaer=10, seed=81)
+    before = _fused(api, cols)
+    from rrtmg_sw_amd import _lib
+    try:
+        assert _lib.lib().rrtmgsw_finalize() == 0
+        with pytest.raises(api.RRTMGSWError) as e:
+            api.mcica_rrtmg_sw(cols, api.device_inputs(cols))
+        assert e.value.code == 1
+    finally:
+        api.rrtmg_sw_ini(1004.64)
+    after = _fused(api, cols)
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
