@@ -9,6 +9,8 @@
   layers runs among the oracle cases of test_gpu_nomcica.)
 * One column (a grid of eight workgroups, seven without a column) and no columns (every entry
   returns at once); rrtmgsw_finalize then rrtmgsw_init (same results after).
+* Night and grazing sun: coszen from -0.5 to 1e-12 against the oracle on every forward driver,
+  the adjoint finite.
 * Layer counts outside 4..256 and the reference's STOP conditions, each with its error code
   and message: invalid icld (src/mcica_subcol_gen_sw.f90:262-263), inflag = 1 with McICA
   (src/rrtmg_sw_cldprmc.f90:166), pressures not ordered bottom-up for the KISS seed
@@ -297,3 +299,29 @@ def test_finalize_and_reinit(gpu_lib):
     after = _fused(api, cols)
     for k in before:
         assert np.array_equal(before[k], after[k]), k
+
+
+def test_night_and_grazing_sun(gpu_lib):
+    """coszen at and below zero and down to 1e-12 (the driver clamps it to 1e-10,
+    src/rrtmg_sw_rad.f90:665-666): the fused and host-array McICA entries and the non-McICA driver
+    against the oracle, and the adjoint finite everywhere (exp(-tau/mu0) underflows to zero)."""
+    import torch
+    api = gpu_lib
+    cols = make_columns(8, 60, icld=2, iaer=10, seed=91)
+    cols.coszen = np.asfortranarray(np.array([-0.5, 0.0, 1e-12, 1e-9, 1e-6, 1e-3, 0.02, 1.0]))
+    mc, _ = ob.run_mcica(cols)
+    ref = ob.run_rrtmg(cols, mc)
+    got = _fused(api, cols)
+    print("fused", _err(got, ref))
+    assert_close(got, ref, "night fused")
+    assert_close(api.rrtmg_sw(**sw_args(cols, mc)), ref, "night host")
+    oc = overcast(cols)
+    nref, _, _ = ob.run_nomcica(oc)
+    assert_close(api.rrtmg_sw_nomcica(**nm_args(oc)), nref, "night nomcica")
+    d = api.device_inputs(cols)
+    st = torch.ones(cols.ncol, dtype=torch.float64, device="cuda")
+    sh = torch.ones(cols.nlay, cols.ncol, dtype=torch.float64, device="cuda")
+    for ck in (0, 16):
+        g = _adjoint(api, cols, d, st, sh, ck)
+        for k, v in g.items():
+            assert torch.isfinite(v).all(), (ck, k)

@@ -10,6 +10,7 @@ from rrtmg_sw_amd.workloads import make_columns, workload
 pytestmark = pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built")
 
 OUTS = ("swuflx", "swdflx", "swuflxc", "swdflxc", "swhr", "swhrc")
+NIGHT_COSZEN = np.array([-0.5, 0.0, 1e-12, 1e-9, 1e-6, 1e-3, 0.02, 1.0])
 
 
 def test_reduced_tables_bitexact():
@@ -65,6 +66,18 @@ def test_layer_count_limits_bitexact(nlay, icld, irng, seed):
     assert mo["cldfmcl"].any()
     r = ob.run_rrtmg(cols, mr, which="ref", icld=min(icld, 3))
     o = ob.run_rrtmg(cols, mr, which="orc", icld=min(icld, 3))
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
+def test_night_and_grazing_sun_bitexact():
+    """coszen at and below zero and down to 1e-12: the driver's clamp to 1e-10
+    (src/rrtmg_sw_rad.f90:665-666) and the saturated exponential-table paths."""
+    cols = make_columns(8, 60, icld=2, iaer=10, seed=91)
+    cols.coszen = np.asfortranarray(NIGHT_COSZEN)
+    mr, _ = ob.run_mcica(cols, "ref")
+    r = ob.run_rrtmg(cols, mr, which="ref")
+    o = ob.run_rrtmg(cols, mr, which="orc")
     for k in OUTS:
         assert np.array_equal(r[k], o[k]), k
 
