@@ -54,11 +54,16 @@ enum : int {
 __device__ inline void key_spec_adj(double c1, double c2, double strrat, double mult, bool bin,
                                     double fs_b, double speccomb_b, double &c1_b, double &c2_b) {
   double speccomb = c1 + strrat * c2;
-  const double rsc = frcp(speccomb);
-  double specparm = c1 * rsc;
-  double specparm_b = (bin && specparm < ONEMINUS) ? fs_b * mult : 0.0;
-  speccomb_b -= specparm_b * c1 * rsc * rsc;
-  c1_b += specparm_b * rsc + speccomb_b;
+  if (bin) {
+    // only binned bands use the ratio; the others can have speccomb = 0 (no water vapour in a
+    // band keyed on it alone), where its reciprocal is infinite
+    const double rsc = frcp(speccomb);
+    const double specparm = c1 * rsc;
+    const double specparm_b = specparm < ONEMINUS ? fs_b * mult : 0.0;
+    speccomb_b -= specparm_b * c1 * rsc * rsc;
+    c1_b += specparm_b * rsc;
+  }
+  c1_b += speccomb_b;
   c2_b += speccomb_b * strrat;
 }
 

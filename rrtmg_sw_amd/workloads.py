@@ -233,6 +233,24 @@ def overcast(cols):
     return out
 
 
+def extreme(seed=95):
+    """Eight 60-layer McICA + aerosol columns up to 0.001 hPa at the edges of the tables:
+    temperatures below and above the reference grid (150 K, 350 K, a 140-360 K ramp, 330-350 K),
+    water vapour at 1e-12, 0.06 and exactly 0, ozone at 0 and 1e-4."""
+    cols = make_columns(8, 60, icld=2, iaer=10, seed=seed, ptop=0.001)
+    r = np.random.default_rng(seed)
+    t = cols.tlay.copy()
+    t[0], t[1] = 150.0, 350.0
+    t[2] = np.linspace(140.0, 360.0, 60)
+    t[3] = 330.0 + 20.0 * r.uniform(size=60)
+    q = cols.h2ovmr.copy()
+    q[4], q[5], q[6] = 1e-12, 0.06, 0.0
+    o = cols.o3vmr.copy()
+    o[6], o[7] = 0.0, 1e-4
+    cols.tlay, cols.h2ovmr, cols.o3vmr = _f(t), _f(q), _f(o)
+    return cols
+
+
 def workload(name, ncol=None):
     """Columns of a named workload (optionally only its first `ncol` columns)."""
     w = dict(WORKLOADS[name])

@@ -9,8 +9,9 @@
   layers runs among the oracle cases of test_gpu_nomcica.)
 * One column (a grid of eight workgroups, seven without a column) and no columns (every entry
   returns at once); rrtmgsw_finalize then rrtmgsw_init (same results after).
-* Night and grazing sun: coszen from -0.5 to 1e-12 against the oracle on every forward driver,
-  the adjoint finite.
+* Night and grazing sun: coszen from -0.5 to 1e-12, and extreme atmospheres (temperatures off
+  the reference grid, no water vapour or ozone, a 0.001 hPa top), against the oracle on every
+  forward driver, the adjoint finite.
 * Layer counts outside 4..256 and the reference's STOP conditions, each with its error code
   and message: invalid icld (src/mcica_subcol_gen_sw.f90:262-263), inflag = 1 with McICA
   (src/rrtmg_sw_cldprmc.f90:166), pressures not ordered bottom-up for the KISS seed
@@ -318,6 +319,32 @@ def test_night_and_grazing_sun(gpu_lib):
     oc = overcast(cols)
     nref, _, _ = ob.run_nomcica(oc)
     assert_close(api.rrtmg_sw_nomcica(**nm_args(oc)), nref, "night nomcica")
+    d = api.device_inputs(cols)
+    st = torch.ones(cols.ncol, dtype=torch.float64, device="cuda")
+    sh = torch.ones(cols.nlay, cols.ncol, dtype=torch.float64, device="cuda")
+    for ck in (0, 16):
+        g = _adjoint(api, cols, d, st, sh, ck)
+        for k, v in g.items():
+            assert torch.isfinite(v).all(), (ck, k)
+
+
+def test_extreme_atmospheres(gpu_lib):
+    """Temperatures off the reference grid (140-360 K), water vapour 0 to 0.06, ozone 0 to 1e-4,
+    a 0.001 hPa top (heating rates up to 2000 K/day in the thinnest layers): the fused and
+    host-array McICA entries and the non-McICA driver against the oracle, the adjoint finite."""
+    import torch
+    from rrtmg_sw_amd.workloads import extreme
+    api = gpu_lib
+    cols = extreme()
+    mc, _ = ob.run_mcica(cols)
+    ref = ob.run_rrtmg(cols, mc)
+    got = _fused(api, cols)
+    print("fused", _err(got, ref))
+    assert_close(got, ref, "extreme fused")
+    assert_close(api.rrtmg_sw(**sw_args(cols, mc)), ref, "extreme host")
+    oc = overcast(cols)
+    nref, _, _ = ob.run_nomcica(oc)
+    assert_close(api.rrtmg_sw_nomcica(**nm_args(oc)), nref, "extreme nomcica")
     d = api.device_inputs(cols)
     st = torch.ones(cols.ncol, dtype=torch.float64, device="cuda")
     sh = torch.ones(cols.nlay, cols.ncol, dtype=torch.float64, device="cuda")

@@ -82,6 +82,18 @@ def test_night_and_grazing_sun_bitexact():
         assert np.array_equal(r[k], o[k]), k
 
 
+def test_extreme_atmospheres_bitexact():
+    """Temperatures off the reference grid, no water vapour, no ozone, a 0.001 hPa top: the
+    setcoef clamps and the key-species limits."""
+    from rrtmg_sw_amd.workloads import extreme
+    cols = extreme()
+    mr, _ = ob.run_mcica(cols, "ref")
+    r = ob.run_rrtmg(cols, mr, which="ref")
+    o = ob.run_rrtmg(cols, mr, which="orc")
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+
+
 @pytest.mark.parametrize("isolvar,scon,opts", [
     (-1, 0.0, {}), (-1, 1361.0, {"bndsolvar": np.linspace(0.9, 1.1, 14)}),
     (0, 0.0, {}), (1, 1361.0, {"indsolvar": [1.2, 0.8], "solcycfrac": 0.3}),
