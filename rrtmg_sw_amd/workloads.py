@@ -251,6 +251,24 @@ def extreme(seed=95):
     return cols
 
 
+def with_cloud_optics(cols, inflag, iceflag, seed=0):
+    """The columns' clouds under another cloud-optics option: ice sizes clipped into the bounds of
+    iceflag 1 (13-130 um), 2 (5-131 um) or 3 (5-140 um); for inflag 0 per-band optical depth,
+    single-scattering albedo, asymmetry and forward fraction (asymmetry squared) drawn for every
+    cloudy layer, as a GCM passing its own cloud optics would."""
+    cols.inflgsw, cols.iceflgsw = inflag, iceflag
+    lo, hi = {1: (13.0, 130.0), 2: (5.0, 131.0), 3: (5.0, 140.0)}[iceflag]
+    cols.rei = _f(np.clip(cols.rei, lo, hi))
+    if inflag == 0:
+        r = np.random.default_rng(seed)
+        cl = (cols.cldfrac > 0)[None]
+        cols.tauc = _f(np.where(cl, r.uniform(0.1, 20.0, cols.tauc.shape), 0.0))
+        cols.ssac = _f(np.where(cl, r.uniform(0.8, 1.0, cols.ssac.shape), 1.0))
+        cols.asmc = _f(np.where(cl, r.uniform(0.7, 0.9, cols.asmc.shape), 0.0))
+        cols.fsfc = _f(cols.asmc ** 2)
+    return cols
+
+
 def workload(name, ncol=None):
     """Columns of a named workload (optionally only its first `ncol` columns)."""
     w = dict(WORKLOADS[name])

@@ -134,6 +134,24 @@ def test_ice_parameterizations_bitexact(iceflg):
         assert np.array_equal(r[k], o[k]), k
 
 
+@pytest.mark.parametrize("irng", [0, 1])
+def test_mcica_given_optics_bitexact(irng):
+    """inflag = 0 under McICA: per-band optical properties copied to the cloudy sub-columns and
+    delta-scaled by cldprmc (src/rrtmg_sw_cldprmc.f90:149-164)."""
+    from rrtmg_sw_amd.workloads import with_cloud_optics
+    cols = with_cloud_optics(make_columns(8, 40, icld=2, seed=41), 0, 3, seed=41)
+    cols.irng = irng
+    mo, _ = ob.run_mcica(cols, "orc")
+    mr, _ = ob.run_mcica(cols, "ref")
+    for k in mo:
+        assert np.array_equal(mo[k], mr[k]), k
+    r = ob.run_rrtmg(cols, mr, which="ref")
+    o = ob.run_rrtmg(cols, mr, which="orc")
+    for k in OUTS:
+        assert np.array_equal(r[k], o[k]), k
+    assert np.abs(o["swuflx"] - o["swuflxc"]).max() > 1.0
+
+
 def test_get_alpha_bitexact():
     cols = make_columns(10, 30, icld=5, seed=5)
     lat = np.linspace(-80, 80, 10)
