@@ -11,7 +11,9 @@
   returns at once); rrtmgsw_finalize then rrtmgsw_init (same results after).
 * Night and grazing sun: coszen from -0.5 to 1e-12, and extreme atmospheres (temperatures off
   the reference grid, no water vapour or ozone, a 0.001 hPa top), against the oracle on every
-  forward driver, the adjoint finite.
+  forward driver, the adjoint finite; the mixed form against FP64 and the exact-exponential form
+  finite on both. The adjoint under random, maximum and exponential overlap and the Mersenne
+  Twister generator (store = checkpoint-7, TOA-flux directional derivative).
 * Layer counts outside 4..256 and the reference's STOP conditions, each with its error code
   and message: invalid icld (src/mcica_subcol_gen_sw.f90:262-263), inflag = 1 with McICA
   (src/rrtmg_sw_cldprmc.f90:166), pressures not ordered bottom-up for the KISS seed
@@ -352,3 +354,62 @@ def test_extreme_atmospheres(gpu_lib):
         g = _adjoint(api, cols, d, st, sh, ck)
         for k, v in g.items():
             assert torch.isfinite(v).all(), (ck, k)
+
+
+@pytest.mark.parametrize("case", ["night", "extreme"])
+def test_precision_forms_on_edges(gpu_lib, case):
+    """The mixed FP32/FP64 form within 0.1 W/m2 and 0.01 K/day of the FP64 form, and the
+    exact-exponential form finite, on the night / grazing-sun and the extreme columns."""
+    from rrtmg_sw_amd.workloads import extreme
+    api = gpu_lib
+    if case == "night":
+        cols = make_columns(8, 60, icld=2, iaer=10, seed=91)
+        cols.coszen = np.asfortranarray(np.array([-0.5, 0.0, 1e-12, 1e-9, 1e-6, 1e-3, 0.02, 1.0]))
+    else:
+        cols = extreme()
+    f64 = _fused(api, cols)
+    mixed = _fused(api, cols, "mixed")
+    e = _err(mixed, f64)
+    print(case, "mixed vs fp64", e)
+    for k in FLUX:
+        assert e[k] <= 0.1, k
+    for k in HR:
+        assert e[k] <= 0.01, k
+    exact = _fused(api, cols, "exact")
+    for k, v in exact.items():
+        assert np.isfinite(v).all(), k
+
+
+@pytest.mark.parametrize("icld,irng", [(1, 1), (3, 1), (4, 0)])
+def test_adjoint_other_overlaps(gpu_lib, icld, irng):
+    """The adjoint with random (1), maximum (3) and exponential (4) overlap, and with the Mersenne
+    Twister generator: store and checkpoint-7 bit-identical, the TOA-flux directional derivative
+    within the bar of the other adjoint tests."""
+    import torch
+    api = gpu_lib
+    cols = make_columns(16, 40, icld=icld, iaer=10, seed=60 + icld)
+    cols.irng = irng
+    N, L = cols.ncol, cols.nlay
+    print(f"adjoint icld={icld} irng={irng}", flush=True)
+    d = api.device_inputs(cols)
+    one = torch.ones(N, dtype=torch.float64, device="cuda")
+    zero_hr = torch.zeros(L, N, dtype=torch.float64, device="cuda")
+    a = _adjoint(api, cols, d, one, zero_hr, 0)
+    b = _adjoint(api, cols, d, one, zero_hr, 7)
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    rng = np.random.default_rng(icld)
+    dirs = {k: 1e-4 * getattr(cols, k) * rng.uniform(-1, 1, getattr(cols, k).shape)
+            for k in KINDS + ALB}
+    plus, minus = cols.subset(np.arange(N)), cols.subset(np.arange(N))
+    for k, dx in dirs.items():
+        getattr(plus, k)[...] += dx
+        getattr(minus, k)[...] -= dx
+    up_p, _ = _toa_and_hr(api, plus)
+    up_m, _ = _toa_and_hr(api, minus)
+    g = {k: api.to_host(v) for k, v in a.items()}
+    lin = sum((g[k] * dirs[k]).sum(axis=1) for k in KINDS)
+    lin = lin + sum(g["alb"][:, m] * dirs[k] for m, k in enumerate(ALB))
+    rel = np.abs(lin - (up_p - up_m) / 2) / np.abs((up_p - up_m) / 2)
+    print("J1: sorted relative errors " + " ".join(f"{x:.1e}" for x in np.sort(rel)))
+    assert np.quantile(rel, 0.9) <= 1e-4 and rel.max() <= 1e-3, rel
