@@ -195,10 +195,20 @@ def get_alpha(iplon, ncol, nlayers, icld, idcor, decorr_con, dz, lat, juldat, cl
 PRECISION = {"fp64": 0, "mixed": 1, "exact": 2}
 
 
-def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None):
+def _solvar(bndsolvar, indsolvar, solcycfrac):
+    """The optional solar-variability arguments as host arrays (read on the host by every entry
+    point), or None."""
+    return [None if bndsolvar is None else np.ascontiguousarray(bndsolvar, dtype=np.float64),
+            None if indsolvar is None else np.ascontiguousarray(indsolvar, dtype=np.float64),
+            None if solcycfrac is None else np.array([solcycfrac], dtype=np.float64)]
+
+
+def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None, bndsolvar=None, indsolvar=None,
+                   solcycfrac=None):
     """mcica_subcol_sw + rrtmg_sw fused on the GPU for the workload `cols` (a
     workloads.Columns) whose arrays are already resident: `dev_arrays` maps input names to
-    device tensors (see device_inputs). Returns the output tensors."""
+    device tensors (see device_inputs). bndsolvar (14), indsolvar (2) and solcycfrac are the
+    reference's optional solar-variability arguments, on the host. Returns the output tensors."""
     N, L = cols.ncol, cols.nlay
     d = dev_arrays
     if out is None:
@@ -215,7 +225,8 @@ def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None):
     args += [_ptr(d.get(k)) for k in _lib.FUSED_CLD]
     args += [_ptr(d.get(k)) for k in _lib.SW_AER]
     args += [_ptr(out[k]) for k in _lib.SW_OUT]
-    args += [None, None, None, C.c_int(PRECISION[precision]), _stream()]
+    sv = _solvar(bndsolvar, indsolvar, solcycfrac)
+    args += [_ptr(x) for x in sv] + [C.c_int(PRECISION[precision]), _stream()]
     check(_lib.lib().rrtmgsw_mcica_sw_dev(*args))
     return out
 
@@ -235,7 +246,8 @@ def _fused_inputs(cols, d, irng):
 ADJ_INPUTS = ("tlay", "h2ovmr", "o3vmr", "clwp", "ciwp")
 
 
-def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None, ckpt=0):
+def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None, ckpt=0, bndsolvar=None,
+                           indsolvar=None, solcycfrac=None):
     """Adjoint of the fused call (exact-exponential FP64 forward, masks fixed):
     J = sum seed_toa * swuflx(:, top) + sum seed_hr * swhr. seed_toa is a device tensor of
     shape (ncol,), seed_hr one of Fortran shape (ncol, nlay) (i.e. torch shape (nlay, ncol)).
@@ -248,7 +260,8 @@ def mcica_rrtmg_sw_adjoint(cols, dev_arrays, seed_toa, seed_hr, out=None, ckpt=0
         out["alb"] = _new(True, (N, 4))
     ir = C.c_int(cols.irng)
     args = _fused_inputs(cols, dev_arrays, ir)
-    args += [None, None, None, _ptr(seed_toa, N), _ptr(seed_hr, N * L)]
+    sv = _solvar(bndsolvar, indsolvar, solcycfrac)
+    args += [_ptr(x) for x in sv] + [_ptr(seed_toa, N), _ptr(seed_hr, N * L)]
     args += [_ptr(out[k]) for k in ADJ_INPUTS] + [_ptr(out["alb"]), C.c_int(ckpt), _stream()]
     check(_lib.lib().rrtmgsw_mcica_sw_adjoint_dev(*args))
     return out

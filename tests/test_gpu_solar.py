@@ -2,7 +2,8 @@
 Earth-Sun distance (dyofyr > 0, earth_sun, src/rrtmg_sw_rad.f90:844-868), a given solar
 constant, and every solar-variability branch of inatm_sw (isolvar -1 to 3 with the optional
 bndsolvar, indsolvar and solcycfrac, src/rrtmg_sw_rad.f90:1248-1512), on clear and cloudy McICA
-columns (host arrays) and on the non-McICA driver. Bars as in the parity tests; the oracle matches
+columns (host arrays, and the fused device entry), on the non-McICA driver and through the
+adjoint (linear in the incident flux). Bars as in the parity tests; the oracle matches
 the compiled reference bit for bit on the same options (test_oracle_reference.py)."""
 import numpy as np
 import pytest
@@ -48,3 +49,30 @@ def test_solar_options_clear_and_nomcica(gpu_lib):
     nref, _, _ = ob.run_nomcica(oc, solvar=sv)
     got = api.rrtmg_sw_nomcica(**nm_args(oc), bndsolvar=sv[0], indsolvar=sv[1], solcycfrac=sv[2])
     assert_close(got, nref, "nomcica isolvar=3")
+
+
+def test_solar_options_fused_and_adjoint(gpu_lib):
+    """The fused device entry takes the same optional arguments (host arrays) as rrtmg_sw, and the
+    adjoint sees them: scaling every band's source by 2 (isolvar = -1, bndsolvar = 2) doubles
+    every gradient, as the path is linear in the incident flux."""
+    import torch
+    api = gpu_lib
+    cols = make_columns(16, 40, icld=5, iaer=10, seed=53)
+    cols.dyofyr = 100
+    d = api.device_inputs(cols)
+    mc, _ = ob.run_mcica(cols)
+    for isolvar, scon, opts in SOLVAR:
+        cols.isolvar, cols.scon = isolvar, scon
+        ref = ob.run_rrtmg(cols, mc, **opts)
+        out = api.mcica_rrtmg_sw(cols, d, **opts)
+        torch.cuda.synchronize()
+        assert_close({k: api.to_host(v) for k, v in out.items()}, ref, f"fused isolvar={isolvar}")
+    cols.isolvar, cols.scon = -1, 0.0
+    st = torch.ones(cols.ncol, dtype=torch.float64, device="cuda")
+    sh = torch.ones(cols.nlay, cols.ncol, dtype=torch.float64, device="cuda")
+    g1 = api.mcica_rrtmg_sw_adjoint(cols, d, st, sh, bndsolvar=np.ones(14))
+    g2 = api.mcica_rrtmg_sw_adjoint(cols, d, st, sh, bndsolvar=np.full(14, 2.0))
+    torch.cuda.synchronize()
+    for k in g1:
+        assert g1[k].abs().max() > 0, k
+        assert torch.allclose(g2[k], 2.0 * g1[k], rtol=1e-13, atol=0.0), k
