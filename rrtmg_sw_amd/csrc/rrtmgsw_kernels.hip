@@ -682,51 +682,57 @@ hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s
 }
 
 // ------------------------------------------------------------------------------------------
-// Mersenne Twister MT19937 stream (src/mcica_random_numbers.f90:77-306): one workgroup keeps
-// the 624-word state in LDS and advances it in the three dependency-free phases of a block
-// (227 + 227 + 170 words), writing tempered outputs in order.
+// Mersenne Twister MT19937 stream (src/mcica_random_numbers.f90:77-306): one workgroup generates
+// the single stream in order, 454 words per barrier, tempered outputs written as they appear.
 // ------------------------------------------------------------------------------------------
+// Barrier that orders this workgroup's LDS accesses only: the wave's LDS operations complete,
+// then all waves meet. Unlike __syncthreads it does not wait for the wave's global stores, which
+// no other wave reads. The memory clobber keeps the compiler from moving accesses across it.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __global__ void __launch_bounds__(256) mt_kernel(uint32_t seed, long long ndraw, uint32_t *out) {
-  __shared__ uint32_t mt[624];
+  // MT19937 (Matsumoto and Nishimura 1998) written as the recurrence over raw words
+  // x[r] = x[r - 227] ^ twist(x[r - 624], x[r - 623]), r >= 624, output r - 624 = temper(x[r]).
+  // A word depends on words at least 227 back, so thread t computes words r = 624 + base + t and
+  // r + 227 (the second uses the first as its x[r - 227]) from words written before the batch:
+  // 454 words per barrier, held in a 2048-word ring whose written and read windows of a batch,
+  // [624 + base, 1078 + base) and [base, 624 + base), never meet.
+  __shared__ uint32_t ring[2048];
+  constexpr uint32_t MSK = 2047;
   const int t = threadIdx.x;
   if (t == 0) {
-    mt[0] = seed;
-    for (int i = 1; i < 624; i++) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+    uint32_t x = seed;
+    ring[0] = x;
+    for (int i = 1; i < 624; i++) {
+      x = 1812433253u * (x ^ (x >> 30)) + (uint32_t)i;
+      ring[i] = x;
+    }
   }
   __syncthreads();
-  for (long long base = 0; base < ndraw; base += 624) {
-    // phase k in [0,227): uses old mt[k], mt[k+1], mt[k+397]
-    // phase k in [227,454): mt[k+397-624] = new values from phase 1
-    // phase k in [454,623) and k = 623 (uses new mt[0]); mt[k+1] old for k < 623
-    for (int ph = 0; ph < 3; ph++) {
-      int k0 = ph == 0 ? 0 : (ph == 1 ? 227 : 454), k1 = ph == 0 ? 227 : (ph == 1 ? 454 : 623);
-      uint32_t nv = 0;
-      int k = k0 + t;
-      if (k < k1) {
-        uint32_t y = (mt[k] & 0x80000000u) | (mt[k + 1] & 0x7fffffffu);
-        nv = mt[(k + 397) % 624] ^ (y >> 1) ^ ((mt[k + 1] & 1u) ? 0x9908b0dfu : 0u);
-      }
-      __syncthreads();
-      if (k < k1) mt[k] = nv;
-      __syncthreads();
+  auto twist = [](uint32_t a, uint32_t b) {
+    const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    return (y >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
+  };
+  auto temper = [](uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    return y ^ (y >> 18);
+  };
+  for (long long base = 0; base < ndraw; base += 454) {
+    if (t < 227) {
+      const uint32_t r = (uint32_t)(624 + base + t);   // ring positions are r mod 2048
+      const uint32_t a1 = ring[(r - 624) & MSK], b1 = ring[(r - 623) & MSK], c1 = ring[(r - 227) & MSK];
+      const uint32_t a2 = ring[(r - 397) & MSK], b2 = ring[(r - 396) & MSK];
+      const uint32_t x1 = c1 ^ twist(a1, b1);
+      const uint32_t x2 = x1 ^ twist(a2, b2);
+      ring[r & MSK] = x1;
+      ring[(r + 227) & MSK] = x2;
+      const long long o1 = base + t, o2 = o1 + 227;
+      if (o1 < ndraw) out[o1] = temper(x1);
+      if (o2 < ndraw) out[o2] = temper(x2);
     }
-    if (t == 0) {
-      uint32_t y = (mt[623] & 0x80000000u) | (mt[0] & 0x7fffffffu);
-      mt[623] = mt[396] ^ (y >> 1) ^ ((mt[0] & 1u) ? 0x9908b0dfu : 0u);
-    }
-    __syncthreads();
-    for (int i = t; i < 624; i += blockDim.x) {
-      long long o = base + i;
-      if (o < ndraw) {
-        uint32_t y = mt[i];
-        y ^= y >> 11;
-        y ^= (y << 7) & 0x9d2c5680u;
-        y ^= (y << 15) & 0xefc60000u;
-        y ^= y >> 18;
-        out[o] = y;
-      }
-    }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
