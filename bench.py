@@ -13,7 +13,8 @@ Both precision forms are timed in the same run; `value` is the one --precision n
 default), and `forms` carries both, their speed ratio and the mixed form's largest error
 against the FP64 form (fluxes W/m2, heating rates K/day). `adjoint` carries the adjoint's cost
 against the forward on the 4096 x 60 exponential-random-overlap + aerosol workload; `nomcica`
-the non-McICA driver on the same columns made overcast; `host_abi` the reference's own two
+the non-McICA driver on the same columns made overcast; `mt` the fused call with the Mersenne Twister
+generator; `host_abi` the reference's own two
 calls on host arrays (mcica_subcol_sw, then rrtmg_sw on the materialised sub-column arrays),
 PCIe copies included; `cpu_baseline_all_cores` the CPU leg on 16 worker processes over the
 whole workload.
@@ -189,6 +190,21 @@ def nomcica_form(args, name="overcast8192x72"):
                         f"{cols.ncol} columns x {cols.nlay} layers, every cloudy layer overcast",
             "ms_per_step": ms, "columns_per_s": cols.ncol / (ms * 1e-3),
             "kernel": "rt_kernel<0, 2>"}
+
+
+def mt_form(cols, d, steps=3, warmup=1):
+    """The same fused call with the Mersenne Twister generator (irng = 1): one global MT19937
+    stream per call, generated in order by one workgroup, then the same kernel."""
+    from rrtmg_sw_amd import api
+    irng = cols.irng
+    cols.irng = 1
+    try:
+        out = api.mcica_rrtmg_sw(cols, d)
+        ms = _time(lambda: api.mcica_rrtmg_sw(cols, d, out=out), steps, warmup)
+    finally:
+        cols.irng = irng
+    return {"workload": f"mcica8192x72 with irng = 1 (one MT19937 stream per call)",
+            "ms_per_step": ms, "columns_per_s": cols.ncol / (ms * 1e-3)}
 
 
 def host_abi_form(cols, steps=3, warmup=1):
@@ -378,6 +394,7 @@ def main():
     line["adjoint"] = adjoint_ratio(args)
     line["nomcica"] = nomcica_form(args)
     if args.workload == "mcica8192x72":
+        line["mt"] = mt_form(cols, d)
         del d
         torch.cuda.empty_cache()
         line["host_abi"] = host_abi_form(cols)
