@@ -25,7 +25,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", os.path.join(ROOT, "include")]
-SOURCES = ["rrtmgsw_kernels.hip", "rrtmgsw_adjoint.hip", "rrtmgsw_api.cpp", "rrtmgsw_tables.cpp"]
+SOURCES = ["rrtmgsw_kernels.hip", "rrtmgsw_adjoint.hip", "rrtmgsw_api.cpp", "rrtmgsw_tables.cpp",
+           "rrtmgsw_mtjump.cpp"]
 
 
 def _run(cmd, verbose):

@@ -54,6 +54,7 @@ struct Context {
   std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork;
   std::map<hipStream_t, std::unique_ptr<std::mutex>> calls;   // one call at a time per stream
   std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
+  uint32_t *d_mtjump = nullptr;   // MT19937 jump-ahead: set-bit positions of x^J mod P, on first use
 } ctx;
 
 // A call's scratch, status word and generator buffers are its stream's from their acquisition
@@ -482,6 +483,8 @@ int rrtmgsw_finalize(void) {
   }
   for (auto &kv : ctx.jumps) hipFree(kv.second);
   ctx.jumps.clear();
+  if (ctx.d_mtjump) hipFree(ctx.d_mtjump);
+  ctx.d_mtjump = nullptr;
   ctx.ready = false;
   return 0;
 }
@@ -943,7 +946,7 @@ int rrtmgsw_column_mcica_sw(int nlay, int nsamples, int icld, int irng, int idel
     e = hipMemcpy(d_cj, cj.data(), cj.size() * sizeof(KissJump), hipMemcpyHostToDevice);
   if (e == hipSuccess && ir)
     for (size_t c = 0; c < C && e == hipSuccess; c++)
-      e = launch_mt_stream((uint32_t)((c + 2) * NGPT), (long long)ndraw, d_mt + c * ndraw, nullptr);
+      e = launch_mt_stream((uint32_t)((c + 2) * NGPT), (long long)ndraw, d_mt + c * ndraw, nullptr, nullptr);
   if (e != hipSuccess) {
     hipFree(d);
     return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
@@ -1007,11 +1010,29 @@ static int mcica_prepare(int ncol, int nlay, int icld, int permuteseed, int irng
   if (irng == 0) return kiss_jumps(permuteseed, nlay, per_sub, s, jumps);
   long long ndraw = (long long)NGPT * ncol * (icld == 3 ? 1 : (icld >= 4 ? 2LL * nlay : nlay));
   void *buf = nullptr;
+  const uint32_t *coef = nullptr;
+  const char *seq = std::getenv("RRTMGSW_MT_SEQUENTIAL");   // 1: one workgroup, no jump-ahead
+  const bool jump = mt_extra_words(ndraw) > 0 && !(seq && std::atoi(seq) != 0);
   {
     std::lock_guard<std::mutex> lk(ctx.mu);
-    if (int rc = stream_buffer(ctx.mtbuf, s, (size_t)ndraw * 4, &buf)) return rc;
+    if (jump) {
+      if (!ctx.d_mtjump) {
+        // the set bits of x^J mod P as a list {n, i_1, ..., i_n}
+        std::vector<uint32_t> c = mt_jump_coefficients(MT_LOG2_SEG);
+        if (c.size() != 624) return fail(RRTMGSW_ERR_ARGUMENT, "MT19937 jump-ahead polynomial not found");
+        std::vector<uint32_t> list(1, 0);
+        for (uint32_t i = 0; i < 19937; i++)
+          if ((c[i >> 5] >> (i & 31)) & 1u) list.push_back(i);
+        list[0] = (uint32_t)(list.size() - 1);
+        HIPCHK(hipMalloc(&ctx.d_mtjump, list.size() * sizeof(uint32_t)));
+        HIPCHK(hipMemcpy(ctx.d_mtjump, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      }
+      coef = ctx.d_mtjump;
+    }
+    const size_t words = (size_t)ndraw + (jump ? mt_extra_words(ndraw) : 0);
+    if (int rc = stream_buffer(ctx.mtbuf, s, words * 4, &buf)) return rc;
   }
-  HIPCHK(launch_mt_stream((uint32_t)permuteseed, ndraw, (uint32_t *)buf, s));
+  HIPCHK(launch_mt_stream((uint32_t)permuteseed, ndraw, (uint32_t *)buf, coef, s));
   *mt = (const uint32_t *)buf;
   return 0;
 }

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace rrtmgsw {
 
 constexpr int NGPT = 112;      // reduced g-points (modules/parrrsw.f90:36)
@@ -162,6 +164,12 @@ hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s
 hipError_t launch_rt_spectral(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
-hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, hipStream_t s);
+// MT19937 stream of ndraw tempered words to out. With coef (device: {n, i_1, ..., i_n}, the set
+// bits of mt_jump_coefficients(MT_LOG2_SEG)) and ndraw > 2^(MT_LOG2_SEG + 1), out must hold
+// ndraw + mt_extra_words(ndraw) words: segment start states go after the stream.
+constexpr int MT_LOG2_SEG = 21;
+size_t mt_extra_words(long long ndraw);
+hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const uint32_t *coef, hipStream_t s);
+std::vector<uint32_t> mt_jump_coefficients(int log2_jump);   // host, rrtmgsw_mtjump.cpp
 
 }  // namespace rrtmgsw

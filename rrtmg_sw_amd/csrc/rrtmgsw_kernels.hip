@@ -682,62 +682,138 @@ hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s
 }
 
 // ------------------------------------------------------------------------------------------
-// Mersenne Twister MT19937 stream (src/mcica_random_numbers.f90:77-306): one workgroup generates
-// the single stream in order, 454 words per barrier, tempered outputs written as they appear.
+// Mersenne Twister MT19937 stream (src/mcica_random_numbers.f90:77-306): the single stream of a
+// call, generated in order by one workgroup, or for long streams in segments of 2^MT_LOG2_SEG
+// words, one workgroup each, from start states found by jump-ahead.
 // ------------------------------------------------------------------------------------------
 // Barrier that orders this workgroup's LDS accesses only: the wave's LDS operations complete,
 // then all waves meet. Unlike __syncthreads it does not wait for the wave's global stores, which
 // no other wave reads. The memory clobber keeps the compiler from moving accesses across it.
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-__global__ void __launch_bounds__(256) mt_kernel(uint32_t seed, long long ndraw, uint32_t *out) {
-  // MT19937 (Matsumoto and Nishimura 1998) written as the recurrence over raw words
-  // x[r] = x[r - 227] ^ twist(x[r - 624], x[r - 623]), r >= 624, output r - 624 = temper(x[r]).
-  // A word depends on words at least 227 back, so thread t computes words r = 624 + base + t and
-  // r + 227 (the second uses the first as its x[r - 227]) from words written before the batch:
-  // 454 words per barrier, held in a 2048-word ring whose written and read windows of a batch,
-  // [624 + base, 1078 + base) and [base, 624 + base), never meet.
-  __shared__ uint32_t ring[2048];
+// The stream continued from the 624 raw words in ring[0 .. 624) (ring of 2048 words in LDS):
+// `count` tempered outputs to out. MT19937 is run as the recurrence over raw words
+// x[r] = x[r - 227] ^ twist(x[r - 624], x[r - 623]), output r - 624 = temper(x[r]). A word depends
+// on words at least 227 back, so thread t computes words r = 624 + base + t and r + 227 (the
+// second takes the first as its x[r - 227]) from words written before the batch: 454 words per
+// barrier, the batch's written and read windows [624 + base, 1078 + base) and [base, 624 + base)
+// never meeting in the ring.
+__device__ inline uint32_t mt_twist(uint32_t a, uint32_t b) {
+  const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+  return (y >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
+}
+__device__ inline uint32_t mt_temper(uint32_t y) {
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  return y ^ (y >> 18);
+}
+__device__ inline void mt_ring_run(uint32_t *ring, long long count, uint32_t *out) {
   constexpr uint32_t MSK = 2047;
   const int t = threadIdx.x;
-  if (t == 0) {
-    uint32_t x = seed;
-    ring[0] = x;
-    for (int i = 1; i < 624; i++) {
-      x = 1812433253u * (x ^ (x >> 30)) + (uint32_t)i;
-      ring[i] = x;
-    }
-  }
-  __syncthreads();
-  auto twist = [](uint32_t a, uint32_t b) {
-    const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
-    return (y >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
-  };
-  auto temper = [](uint32_t y) {
-    y ^= y >> 11;
-    y ^= (y << 7) & 0x9d2c5680u;
-    y ^= (y << 15) & 0xefc60000u;
-    return y ^ (y >> 18);
-  };
-  for (long long base = 0; base < ndraw; base += 454) {
+  for (long long base = 0; base < count; base += 454) {
     if (t < 227) {
       const uint32_t r = (uint32_t)(624 + base + t);   // ring positions are r mod 2048
       const uint32_t a1 = ring[(r - 624) & MSK], b1 = ring[(r - 623) & MSK], c1 = ring[(r - 227) & MSK];
       const uint32_t a2 = ring[(r - 397) & MSK], b2 = ring[(r - 396) & MSK];
-      const uint32_t x1 = c1 ^ twist(a1, b1);
-      const uint32_t x2 = x1 ^ twist(a2, b2);
+      const uint32_t x1 = c1 ^ mt_twist(a1, b1);
+      const uint32_t x2 = x1 ^ mt_twist(a2, b2);
       ring[r & MSK] = x1;
       ring[(r + 227) & MSK] = x2;
       const long long o1 = base + t, o2 = o1 + 227;
-      if (o1 < ndraw) out[o1] = temper(x1);
-      if (o2 < ndraw) out[o2] = temper(x2);
+      if (o1 < count) out[o1] = mt_temper(x1);
+      if (o2 < count) out[o2] = mt_temper(x2);
     }
     lds_barrier();
   }
 }
+__device__ inline void mt_init(uint32_t seed, uint32_t *w) {   // init_genrand
+  uint32_t x = seed;
+  w[0] = x;
+  for (int i = 1; i < 624; i++) {
+    x = 1812433253u * (x ^ (x >> 30)) + (uint32_t)i;
+    w[i] = x;
+  }
+}
 
-hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, hipStream_t s) {
-  hipLaunchKernelGGL(mt_kernel, dim3(1), dim3(256), 0, s, seed, ndraw, out);
+// The whole stream in one workgroup.
+__global__ void __launch_bounds__(256) mt_kernel(uint32_t seed, long long ndraw, uint32_t *out) {
+  __shared__ uint32_t ring[2048];
+  if (threadIdx.x == 0) mt_init(seed, ring);
+  __syncthreads();
+  mt_ring_run(ring, ndraw, out);
+}
+
+// Jump-ahead (x^J mod P, coefficients c_i from the host): the 624-word window J = 2^MT_LOG2_SEG
+// raw words ahead is the XOR of the windows i ahead over the set bits c_i, i < 19937. One
+// workgroup walks the segment starts: from a start window it extends the raw words to
+// 624 + 19936, forms the next start (thread w owns word w), and writes every start to `starts`.
+constexpr int MT_EXT = 624 + 19936;   // raw words a jump reads
+// coef = {n, i_1, ..., i_n}: the positions of the set bits c_i, each below 19937.
+__global__ void __launch_bounds__(640) mt_chain_kernel(uint32_t seed, int nseg,
+                                                       const uint32_t *__restrict__ coef, uint32_t *starts) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t msm[];
+  uint32_t *buf = msm;
+  const int t = threadIdx.x;
+  const int nb = (int)coef[0];
+  const uint32_t *bits = coef + 1;
+  if (t == 0) mt_init(seed, buf);
+  __syncthreads();
+  for (int i = t; i < 624; i += blockDim.x) starts[i] = buf[i];
+  for (int k = 1; k < nseg; k++) {
+    for (int r0 = 624; r0 < MT_EXT; r0 += 227) {
+      const int r = r0 + t;
+      if (t < 227 && r < MT_EXT) buf[r] = buf[r - 227] ^ mt_twist(buf[r - 624], buf[r - 623]);
+      __syncthreads();
+    }
+    uint32_t acc = 0;
+    if (t < 624) {
+      // eight independent LDS reads in flight per step; i <= 19936, so i + t stays below MT_EXT
+      int j = 0;
+      for (; j + 8 <= nb; j += 8) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v ^= buf[bits[j + q] + t];
+        acc ^= v;
+      }
+      for (; j < nb; j++) acc ^= buf[bits[j] + t];
+    }
+    __syncthreads();
+    if (t < 624) {
+      buf[t] = acc;
+      starts[(size_t)k * 624 + t] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Segment k of a jumped stream: outputs [k J, min((k + 1) J, ndraw)) from start window k.
+__global__ void __launch_bounds__(256) mt_segment_kernel(const uint32_t *starts, long long ndraw, uint32_t *out) {
+  __shared__ uint32_t ring[2048];
+  const long long k = blockIdx.x, o0 = k << MT_LOG2_SEG;
+  const long long cnt = ndraw - o0 < (1LL << MT_LOG2_SEG) ? ndraw - o0 : (1LL << MT_LOG2_SEG);
+  for (int i = threadIdx.x; i < 624; i += blockDim.x) ring[i] = starts[k * 624 + i];
+  __syncthreads();
+  mt_ring_run(ring, cnt, out + o0);
+}
+
+size_t mt_extra_words(long long ndraw) {
+  const long long J = 1LL << MT_LOG2_SEG;
+  return ndraw > 2 * J ? (size_t)((ndraw + J - 1) / J) * 624 : 0;
+}
+
+hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const uint32_t *coef,
+                            hipStream_t s) {
+  const long long J = 1LL << MT_LOG2_SEG;
+  if (coef && ndraw > 2 * J) {
+    const int nseg = (int)((ndraw + J - 1) / J);
+    uint32_t *starts = out + ndraw;   // the caller sized out for ndraw + mt_extra_words(ndraw)
+    hipLaunchKernelGGL(mt_chain_kernel, dim3(1), dim3(640), MT_EXT * sizeof(uint32_t), s, seed, nseg, coef,
+                       starts);
+    hipLaunchKernelGGL(mt_segment_kernel, dim3(nseg), dim3(256), 0, s, starts, ndraw, out);
+  } else {
+    hipLaunchKernelGGL(mt_kernel, dim3(1), dim3(256), 0, s, seed, ndraw, out);
+  }
   return hipGetLastError();
 }
 
