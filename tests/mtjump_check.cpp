@@ -1,0 +1,43 @@
+// Checker for the MT19937 jump-ahead of rrtmg_sw_amd/csrc/rrtmgsw_mtjump.cpp (compiled with it by
+// tests/test_mtjump.py): for jumps of 2^5, 2^12 and 2^21 words and three seeds, the window of 624
+// raw words formed from the coefficients equals the sequentially generated window (all bits that
+// are part of the state), and 5000 words generated from it equal the sequential stream. Prints the
+// mismatch count per case and exits non-zero on any mismatch.
+#include <cstdio>
+#include <cstdint>
+#include <vector>
+#include <chrono>
+namespace rrtmgsw { std::vector<uint32_t> mt_jump_coefficients(int log2_jump); }
+static std::vector<uint32_t> raw(uint32_t seed, size_t n) {
+  std::vector<uint32_t> x(n); x[0] = seed;
+  for (size_t i = 1; i < 624; i++) x[i] = 1812433253u * (x[i-1] ^ (x[i-1] >> 30)) + (uint32_t)i;
+  for (size_t r = 624; r < n; r++) { uint32_t y = (x[r-624] & 0x80000000u) | (x[r-623] & 0x7fffffffu);
+    x[r] = x[r-227] ^ (y >> 1) ^ ((x[r-623] & 1u) ? 0x9908b0dfu : 0u); }
+  return x;
+}
+int main() {
+  for (int lj : {5, 12, 21}) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto c = rrtmgsw::mt_jump_coefficients(lj);
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (c.empty()) { printf("no polynomial\n"); return 1; }
+    size_t J = (size_t)1 << lj;
+    for (uint32_t seed : {112u, 5489u, 4357u}) {
+      auto x = raw(seed, J + 624 + 5000 + 20000);
+      // window at J from windows i < 19937
+      std::vector<uint32_t> w(624, 0);
+      for (int i = 0; i < 19937; i++) if ((c[i >> 5] >> (i & 31)) & 1u) for (int k = 0; k < 624; k++) w[k] ^= x[i + k];
+      int bad = 0;
+      for (int k = 1; k < 624; k++) bad += w[k] != x[J + k];
+      bad += (w[0] & 0x80000000u) != (x[J] & 0x80000000u);
+      // generate forward from the jumped window
+      std::vector<uint32_t> y(w); y.resize(624 + 5000);
+      for (size_t r = 624; r < y.size(); r++) { uint32_t v = (y[r-624] & 0x80000000u) | (y[r-623] & 0x7fffffffu);
+        y[r] = y[r-227] ^ (v >> 1) ^ ((y[r-623] & 1u) ? 0x9908b0dfu : 0u); }
+      for (size_t r = 624; r < y.size(); r++) bad += y[r] != x[J + r];
+      printf("log2J=%d seed=%u mismatches=%d (%.2f s for the coefficients)\n", lj, seed, bad, dt);
+      if (bad) return 2;
+    }
+  }
+  return 0;
+}
