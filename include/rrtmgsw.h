@@ -205,6 +205,26 @@ int rrtmgsw_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, dou
                       const double *dz, const double *lat, int juldat, const double *cldfrac,
                       double *alpha);
 
+/* Replaces setcoef_sw(...), src/rrtmg_sw_setcoef.f90:49-305, for ncol columns at once on device
+ * pointers, fed as rrtmg_sw feeds it: inatm_sw's dry-air and molecular amounts
+ * (src/rrtmg_sw_rad.f90:1531-1586) are formed from play, plev (ncol, nlay+1), tlay and the volume
+ * mixing ratios (ncol, nlay), as the forward kernels form them. It runs the forward kernels' own
+ * setcoef code and writes its results, in the reference's dummy order: laytrop and laylow (ncol),
+ * then jp, jt, jt1 (1-based), the column amounts (molecules / cm2 times 1e-20), fac00..fac11, the
+ * self- and foreign-continuum factors, fractions and indices (ncol, nlay) each; and last laysolfr
+ * (ncol, 14, 1-based), the layer at which taumol_sw takes each band's solar source
+ * (src/rrtmg_sw_taumol.f90, e.g. :351-356, :559-564). Any output may be NULL (not written).
+ * Not produced: layswtch (always 0 in the reference), coln2o and co2mult, which no later step of
+ * the path reads. Synchronizes the stream before returning. */
+int rrtmgsw_setcoef_dev(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                        const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                        const double *ch4vmr, const double *o2vmr, int *laytrop, int *laylow,
+                        int *jp, int *jt, int *jt1, double *colch4, double *colco2, double *colh2o,
+                        double *colmol, double *colo2, double *colo3, double *fac00, double *fac01,
+                        double *fac10, double *fac11, double *selffac, double *selffrac,
+                        int *indself, double *forfac, double *forfrac, int *indfor, int *laysolfr,
+                        void *stream);
+
 /* mcica_subcol_sw followed by rrtmg_sw on device pointers, without materializing the seven
  * (112, ncol, nlay) sub-column arrays: the sub-column masks are generated inside the radiative
  * transfer kernel from the same KISS / Mersenne-Twister streams (no counterpart in the

@@ -544,6 +544,15 @@ static spec_t spec(double col1, double col2, double strrat, double mult, const c
 #define CROSS_LO(ref) (lay + 1 < nlay && c->jp[lay] < (ref) && c->jp[lay + 1] >= (ref))
 #define CROSS_HI(ref) (lay >= 1 && c->jp[lay - 1] < (ref) && c->jp[lay] >= (ref))
 
+/* Where orc_setcoef_gcm asks for them, the final laysolfr of each band (0-based band index) is
+ * recorded here: the layer whose solar source taumol_sw keeps (the last crossing wins, as in the
+ * reference's loops). */
+static int *g_laysolfr_out;
+#define RECORD_LAYSOLFR(ib, v)                    \
+  do {                                            \
+    if (g_laysolfr_out) g_laysolfr_out[ib] = (v); \
+  } while (0)
+
 static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, double *sfluxzen,
                    double *taug, double *taur) {
   const int laytrop = c->laytrop;
@@ -577,6 +586,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(0, laysolfr);
   }
 
   /* ---- band 17, :386-525 ---- */
@@ -605,6 +615,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(1, laysolfr);
   }
 
   /* ---- bands 18, 19, 21, 22, 24: lower binary species, laysolfr in the lower loop ---- */
@@ -643,6 +654,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(ib, laysolfr);
     for (int lay = laytrop; lay < nlay; lay++) {
       int nspb = NSPB[ib];
       int ind0 = IND0B(nspb) + 1, ind1 = IND1B(nspb) + 1;
@@ -686,6 +698,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(4, laysolfr);
     for (int lay = laytrop; lay < nlay; lay++) {
       int ind0 = IND0B(NSPB[4]) + 1, ind1 = IND1B(NSPB[4]) + 1;
       double tauray = c->colmol[lay] * b->rayl;
@@ -715,6 +728,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(7, laysolfr);
     for (int lay = laytrop; lay < nlay; lay++)
       for (int ig = 1; ig <= b->ng; ig++) {
         TG(lay, iw0 + ig - 1) = 0.;
@@ -738,6 +752,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(9, laysolfr);
     for (int lay = laytrop; lay < nlay; lay++)
       for (int ig = 1; ig <= b->ng; ig++) {
         TR(lay, iw0 + ig - 1) = c->colmol[lay] * b->raylv[ig - 1];
@@ -747,6 +762,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
 
   /* ---- band 26, :1414-1474 ---- */
   b = &B[10], iw0 = 80;
+  RECORD_LAYSOLFR(10, laytrop);
   for (int lay = 0; lay < nlay; lay++)
     for (int ig = 1; ig <= b->ng; ig++) {
       if (lay < laytrop && lay + 1 == laytrop) solar_plain(b, sol, iw0 + ig - 1, ig, 1.0, sfluxzen, ssi);
@@ -777,6 +793,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         if (lay + 1 == laysolfr) solar_plain(b, sol, iw0 + ig - 1, ig, scalekur, sfluxzen, ssi);
       }
     }
+    RECORD_LAYSOLFR(11, laysolfr);
   }
 
   /* ---- band 28, :1568-1692 ---- */
@@ -803,6 +820,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(12, laysolfr);
   }
 
   /* ---- band 29, :1695-1785 ---- */
@@ -832,6 +850,7 @@ static void taumol(int nlay, const coef_t *c, const solar_t *sol, double *ssi, d
         TR(lay, iw0 + ig - 1) = tauray;
       }
     }
+    RECORD_LAYSOLFR(13, laysolfr);
   }
 }
 
@@ -974,7 +993,14 @@ static int cldprmc(int nlay, int inflag, int iceflag, int liqflag, const double 
 /* ------------------------------------------------------------------------------------
  * exp-table transmittance used by reftra and spcvmc (e.g. src/rrtmg_sw_spcvmc.f90:558-566)
  * ---------------------------------------------------------------------------------- */
+/* Exact-exponential mode (orc_set_exact_exp): exp(-ze) in place of the table and its Taylor
+ * branch, in reftra_sw's two transmittances and spcvmc_sw's direct beam alike -- the smooth
+ * forward that the adjoint differentiates (the table is a step function of ze). */
+static int g_exact_exp;
+void orc_set_exact_exp(int on) { g_exact_exp = on != 0; }
+
 static inline double tbl_exp(double ze) {
+  if (g_exact_exp) return exp(-ze);
   if (ze <= OD_LO) return 1. - ze + 0.5 * ze * ze;
   double tblind = ze / (BPADE + ze);
   int itind = (int)(TBLINT * tblind + 0.5);
@@ -2075,6 +2101,52 @@ void orc_setcoef(int nlay, const double *pavel, const double *tavel, const doubl
   double *da[] = {c.colh2o, c.colco2, c.colo3, c.coln2o, c.colch4, c.colo2, c.colmol, c.co2mult,
                   c.fac00, c.fac01, c.fac10, c.fac11, c.selffac, c.selffrac, c.forfac, c.forfrac};
   for (int k = 0; k < 16; k++) memcpy(dout + k * nlay, da[k], sizeof(double) * nlay);
+  coef_free(&c);
+}
+
+/* setcoef_sw for ncol columns fed as rrtmg_sw feeds it (src/rrtmg_sw_rad.f90:1531-1586 for coldry
+ * and wkl, then src/rrtmg_sw_setcoef.f90:49-305), plus the laysolfr of each band that taumol_sw
+ * finds (e.g. src/rrtmg_sw_taumol.f90:351-356, :559-564). Integer outputs iout:
+ * laytrop (ncol), laylow (ncol), laysolfr (ncol, 14), then jp, jt, jt1, indself, indfor (ncol, nlay)
+ * each. dout: colh2o colco2 colo3 colch4 colo2 colmol fac00 fac01 fac10 fac11 selffac selffrac
+ * forfac forfrac, (ncol, nlay) each. All (column fastest). */
+void orc_setcoef_gcm(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                     const double *ch4vmr, const double *o2vmr, int *iout, double *dout) {
+  const int N = ncol, L = nlay;
+  const double amd = 28.9660, amw = 18.0160;
+  coef_t c = coef_alloc(L);
+  double *coldry = malloc(8 * L), *wkl = calloc(MXMOL * L, 8), *pavel = malloc(8 * L), *tavel = malloc(8 * L);
+  double *taug = malloc(8 * L * NGPT), *taur = malloc(8 * L * NGPT), ssi[NGPT], sfz[NGPT];
+  solar_t sol = {-1, 1., 1., 1., NULL, NULL, NULL};
+  int ls[NBND];
+  for (int i = 0; i < N; i++) {
+    for (int l = 0; l < L; l++) {
+      pavel[l] = play[i + N * l], tavel[l] = tlay[i + N * l];
+      double *w = wkl + MXMOL * l;
+      for (int m = 0; m < MXMOL; m++) w[m] = 0.;
+      w[0] = h2ovmr[i + N * l], w[1] = co2vmr[i + N * l], w[2] = o3vmr[i + N * l];
+      w[5] = ch4vmr[i + N * l], w[6] = o2vmr[i + N * l];
+      double amm = (1. - w[0]) * amd + w[0] * amw;
+      coldry[l] = (plev[i + N * l] - plev[i + N * (l + 1)]) * 1.e3 * AVOGAD / (1.e2 * GRAV * amm * (1. + w[0]));
+      for (int m = 0; m < 7; m++) w[m] = coldry[l] * w[m];
+    }
+    setcoef(L, pavel, tavel, coldry, wkl, &c);
+    for (int b = 0; b < NBND; b++) ls[b] = -1;
+    g_laysolfr_out = ls;
+    taumol(L, &c, &sol, ssi, sfz, taug, taur);
+    g_laysolfr_out = NULL;
+    iout[i] = c.laytrop, iout[N + i] = c.laylow;
+    for (int b = 0; b < NBND; b++) iout[2 * N + i + N * b] = ls[b];
+    int *ia[] = {c.jp, c.jt, c.jt1, c.indself, c.indfor};
+    for (int k = 0; k < 5; k++)
+      for (int l = 0; l < L; l++) iout[(2 + NBND) * N + (size_t)N * L * k + i + N * l] = ia[k][l];
+    double *da[] = {c.colh2o, c.colco2, c.colo3, c.colch4, c.colo2, c.colmol, c.fac00, c.fac01,
+                    c.fac10, c.fac11, c.selffac, c.selffrac, c.forfac, c.forfrac};
+    for (int k = 0; k < 14; k++)
+      for (int l = 0; l < L; l++) dout[(size_t)N * L * k + i + N * l] = da[k][l];
+  }
+  free(coldry), free(wkl), free(pavel), free(tavel), free(taug), free(taur);
   coef_free(&c);
 }
 

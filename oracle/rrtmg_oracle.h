@@ -112,6 +112,12 @@ int orc_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, double 
                   const double *dz, const double *lat, int juldat, const double *cldfrac,
                   double *alpha);
 
+/* Exact-exponential mode on (1) or off (0, the reference): every exponential-table transmittance
+ * of reftra_sw (src/rrtmg_sw_reftra.f90:203-209, :277-295) and of spcvmc_sw's direct beam
+ * (src/rrtmg_sw_spcvmc.f90:558-582) becomes exp(-x), its Taylor branch included. Not part of
+ * the reference: the smooth forward that the adjoint differentiates. */
+void orc_set_exact_exp(int on);
+
 /* ---- per-routine entry points (one column), used by the unit-level tests ---- */
 
 /* setcoef_sw -- src/rrtmg_sw_setcoef.f90:49-305. wkl is (39, nlay).
@@ -120,6 +126,15 @@ int orc_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, double 
  *       fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac (16) + spare. */
 void orc_setcoef(int nlay, const double *pavel, const double *tavel, const double *coldry,
                  const double *wkl, int *iout, double *dout);
+
+/* setcoef_sw for ncol columns fed as rrtmg_sw feeds it (inatm_sw's coldry / wkl from the profiles,
+ * src/rrtmg_sw_rad.f90:1531-1586), with the laysolfr of each band that taumol_sw finds.
+ * iout: laytrop (ncol), laylow (ncol), laysolfr (ncol, 14), jp, jt, jt1, indself, indfor
+ * (ncol, nlay each). dout: colh2o colco2 colo3 colch4 colo2 colmol fac00 fac01 fac10 fac11 selffac
+ * selffrac forfac forfrac (ncol, nlay each). Column index fastest. */
+void orc_setcoef_gcm(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                     const double *ch4vmr, const double *o2vmr, int *iout, double *dout);
 
 /* taumol_sw -- src/rrtmg_sw_taumol.f90:50-1787 on the outputs of orc_setcoef.
  * taug, taur: (nlay, 112); ssi, sfluxzen: (112). svar_bnd: 3 x 29 (f, s, i). */

@@ -56,6 +56,8 @@ def lib():
         [C.c_int] * 5 + [_ip] + [_dp] * 16 +
         [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int] + [_dp] * 10 +
         [_dp] * 4 + [_dp] * 3 + [_dp] * 2 + [_dp] * 6 + [C.c_int, _vp])
+    L.rrtmgsw_setcoef_dev.argtypes = ([C.c_int, C.c_int] + [_dp] * 8 + [_ip] * 5 + [_dp] * 12 +
+                                      [_ip, _dp, _dp, _ip, _ip, _vp])
     L.rrtmgsw_adjoint_work_bytes.argtypes = [C.c_int, C.c_int]
     L.rrtmgsw_adjoint_work_bytes.restype = C.c_size_t
     _lib = L
@@ -66,7 +68,8 @@ EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_s
             "rrtmgsw_sw_dev", "rrtmgsw_sw_nomcica", "rrtmgsw_sw_nomcica_dev", "rrtmgsw_column_sw",
             "rrtmgsw_column_mcica_sw",
             "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
-            "rrtmgsw_get_alpha", "rrtmgsw_mcica_sw_dev", "rrtmgsw_mcica_sw_adjoint_dev", "rrtmgsw_adjoint_work_bytes"]
+            "rrtmgsw_get_alpha", "rrtmgsw_setcoef_dev", "rrtmgsw_mcica_sw_dev",
+            "rrtmgsw_mcica_sw_adjoint_dev", "rrtmgsw_adjoint_work_bytes"]
 
 
 def check(rc):

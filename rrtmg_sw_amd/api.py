@@ -192,6 +192,37 @@ def get_alpha(iplon, ncol, nlayers, icld, idcor, decorr_con, dz, lat, juldat, cl
     return alpha
 
 
+# setcoef_sw's outputs in the reference's dummy order (src/rrtmg_sw_setcoef.f90:49-53), as
+# rrtmgsw_setcoef_dev writes them, then laysolfr; True marks the int32 ones
+SETCOEF_OUT = [("laytrop", True), ("laylow", True), ("jp", True), ("jt", True), ("jt1", True),
+               ("colch4", False), ("colco2", False), ("colh2o", False), ("colmol", False),
+               ("colo2", False), ("colo3", False), ("fac00", False), ("fac01", False),
+               ("fac10", False), ("fac11", False), ("selffac", False), ("selffrac", False),
+               ("indself", True), ("forfac", False), ("forfrac", False), ("indfor", True),
+               ("laysolfr", True)]
+
+
+def setcoef_sw(ncol, nlay, play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr):
+    """setcoef_sw (src/rrtmg_sw_setcoef.f90:49-305) for every column on the GPU, fed from
+    inatm_sw's layer amounts of the given device profiles (torch tensors in reversed Fortran
+    dims). Returns device tensors: laytrop, laylow (ncol,); laysolfr (14, ncol) (Fortran
+    (ncol, 14)); the rest (nlay, ncol) (Fortran (ncol, nlay)); indices 1-based int32."""
+    ip = C.POINTER(C.c_int)
+    out = {}
+    for name, is_int in SETCOEF_OUT:
+        shape = (ncol,) if name in ("laytrop", "laylow") else ((NBND, ncol) if name == "laysolfr"
+                                                               else (nlay, ncol))
+        out[name] = torch.zeros(shape, dtype=torch.int32 if is_int else torch.float64, device="cuda")
+    args = [C.c_int(ncol), C.c_int(nlay)]
+    args += [_ptr(x) for x in (play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr)]
+    for name, is_int in SETCOEF_OUT:
+        t = out[name]
+        args.append(C.cast(C.c_void_p(t.data_ptr()), ip) if is_int else _ptr(t))
+    args.append(_stream())
+    check(_lib.lib().rrtmgsw_setcoef_dev(*args))
+    return out
+
+
 PRECISION = {"fp64": 0, "mixed": 1, "exact": 2}
 
 

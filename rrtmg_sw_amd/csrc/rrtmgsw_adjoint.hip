@@ -410,11 +410,11 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   const bool active = lane < WAVE_G;
 
   // ---- prologue (as the forward) + adjoint accumulators and level weights ----
-  if (tid == 0) v.misc[M_LAYTROP] = 0;
+  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_LAYLOW] = 0;
   for (int i = tid; i < L * AC_N; i += blockDim.x) acc[i] = 0.0;
   if (tid < 4) albacc[tid] = 0.0;
   __syncthreads();
-  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, c, lay, v, &v.misc[M_LAYTROP]);
+  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
   for (int j = tid; j <= L; j += blockDim.x) {
     // top-based level j is bottom-based level i = L - j; swhr(k) = (net(k+1) - net(k)) hf / pdp(k)
     // for k < L - 1 (the top layer's heating rate is zero)

@@ -47,7 +47,7 @@ struct Context {
   double heatfac = 0.0;
   HostTables host;
   DeviceTables dev{};
-  double *d_tab = nullptr, *d_exp = nullptr;
+  double *d_tab = nullptr, *d_exp = nullptr, *d_pthr = nullptr;
   float *d_tab32 = nullptr, *d_exp32 = nullptr, *d_rexp32 = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
@@ -452,10 +452,12 @@ int rrtmgsw_init(double cpdair) {
     HIPCHK(up32(H.exp_tbl, &ctx.d_exp32, false));
     HIPCHK(up32(H.exp_tbl, &ctx.d_rexp32, true));
   }
+  HIPCHK(hipMalloc(&ctx.d_pthr, H.pthr.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(ctx.d_pthr, H.pthr.data(), H.pthr.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx.d_gp, H.gp.size() * sizeof(GPoint)));
   HIPCHK(hipMemcpy(ctx.d_gp, H.gp.data(), H.gp.size() * sizeof(GPoint), hipMemcpyHostToDevice));
   DeviceTables &D = ctx.dev;
-  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.gp = ctx.d_gp;
+  D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.gp = ctx.d_gp, D.pthr = ctx.d_pthr;
   D.tab32 = ctx.d_tab32, D.exp32 = ctx.d_exp32, D.rexp32 = ctx.d_rexp32;
   D.extliq1 = H.extliq1, D.ssaliq1 = H.ssaliq1, D.asyliq1 = H.asyliq1;
   D.extice2 = H.extice2, D.ssaice2 = H.ssaice2, D.asyice2 = H.asyice2;
@@ -477,6 +479,7 @@ int rrtmgsw_finalize(void) {
   hipFree(ctx.d_exp32);
   hipFree(ctx.d_rexp32);
   hipFree(ctx.d_gp);
+  hipFree(ctx.d_pthr);
   for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.adjwork}) {
     for (auto &kv : *m) hipFree(kv.second.ptr);
     m->clear();
@@ -1255,6 +1258,38 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
   a.swuflx = swuflx, a.swdflx = swdflx, a.swhr = swhr, a.swuflxc = swuflxc, a.swdflxc = swdflxc;
   a.swhrc = swhrc;
   return run_rt(a, t, s, precision);
+}
+
+int rrtmgsw_setcoef_dev(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                        const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                        const double *ch4vmr, const double *o2vmr, int *laytrop, int *laylow,
+                        int *jp, int *jt, int *jt1, double *colch4, double *colco2, double *colh2o,
+                        double *colmol, double *colo2, double *colo3, double *fac00, double *fac01,
+                        double *fac10, double *fac11, double *selffac, double *selffrac,
+                        int *indself, double *forfac, double *forfrac, int *indfor, int *laysolfr,
+                        void *stream) {
+  if (int rc = check_init()) return rc;
+  if (ncol < 0 || nlay < 1 || nlay > MAX_LAY)
+    return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 1..256 and ncol >= 0");
+  if (ncol == 0) return 0;
+  const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr};
+  for (const void *p : need)
+    if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  RtArgs a{};
+  a.ncol = ncol, a.nlay = nlay, a.col0 = 0, a.ncol_chunk = ncol;
+  a.play = play, a.plev = plev, a.tlay = tlay, a.h2ovmr = h2ovmr, a.o3vmr = o3vmr;
+  a.co2vmr = co2vmr, a.ch4vmr = ch4vmr, a.o2vmr = o2vmr;
+  SetcoefOut o{};
+  o.laytrop = laytrop, o.laylow = laylow, o.laysolfr = laysolfr;
+  o.jp = jp, o.jt = jt, o.jt1 = jt1, o.indself = indself, o.indfor = indfor;
+  o.fac00 = fac00, o.fac01 = fac01, o.fac10 = fac10, o.fac11 = fac11;
+  o.selffac = selffac, o.selffrac = selffrac, o.forfac = forfac, o.forfrac = forfrac;
+  o.colh2o = colh2o, o.colco2 = colco2, o.colo3 = colo3, o.colch4 = colch4, o.colo2 = colo2;
+  o.colmol = colmol;
+  HIPCHK(launch_setcoef(a, o, ctx.dev, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
 }
 
 size_t rrtmgsw_adjoint_work_bytes(int nlay, int ckpt) {

@@ -69,6 +69,7 @@ struct ColView {
 // misc slots
 constexpr int M_LAYTROP = 0, M_LAYSOL = 1;   // laysolfr of band b at misc[M_LAYSOL + b]
 constexpr int M_PARTIAL = 15;                 // non-McICA: a layer has a partial cloud fraction
+constexpr int M_LAYLOW = 16;                  // setcoef_sw's laylow (layers with log p >= 6.62)
 
 // ------------------------------------------------------------------------------------------
 // inatm_sw + setcoef_sw for one layer (src/rrtmg_sw_rad.f90:1531-1586,
@@ -96,7 +97,21 @@ __device__ static const double TREF[59] = {
     2.1435e+02, 2.0887e+02, 2.0340e+02, 1.9792e+02, 1.9290e+02, 1.8809e+02, 1.8329e+02,
     1.7849e+02, 1.7394e+02, 1.7212e+02};
 
-__device__ static void setcoef_layer(const RtArgs &a, int c, int lay, ColView &v, int *ntrop) {
+// jp and the troposphere / lowest-layer tests are taken from the pressure thresholds of the host
+// libm's log (DeviceTables::pthr): the device log estimates jp, the thresholds correct it, so the
+// indices are the reference's bit for bit even where p lies within an ulp of a switch. The
+// fraction fp uses the device log (within an ulp or two of the host's).
+__device__ inline int jp_index(double p, double plog, const double *__restrict__ pthr) {
+#pragma clang fp contract(off)
+  int jp = (int)(36. - 5 * (plog + 0.04));
+  jp = jp < 1 ? 1 : (jp > 58 ? 58 : jp);
+  // the reference's jp is 1 + #{k = 2..58 : p < pthr[k - 2]} (the thresholds decrease with k)
+  while (jp < 58 && p < pthr[PTHR_JP + jp - 1]) jp++;
+  while (jp > 1 && !(p < pthr[PTHR_JP + jp - 2])) jp--;
+  return jp;
+}
+__device__ static void setcoef_layer(const RtArgs &a, const DeviceTables &T, int c, int lay, ColView &v,
+                                     int *ntrop) {
 #pragma clang fp contract(off)
   const int C = a.ncol;
   const double amd = 28.9660, amw = 18.0160, avogad = 6.02214199e+23, grav = 9.8066;
@@ -120,8 +135,7 @@ __device__ static void setcoef_layer(const RtArgs &a, int c, int lay, ColView &v
   }
 
   double plog = log(p);
-  int jp = (int)(36. - 5 * (plog + 0.04));
-  jp = jp < 1 ? 1 : (jp > 58 ? 58 : jp);
+  int jp = jp_index(p, plog, T.pthr);
   int jp1 = jp + 1;
   double fp = 5. * (PREFLOG[jp - 1] - plog);
   int jt = (int)(3. + (t - TREF[jp - 1]) / 15.);
@@ -134,8 +148,9 @@ __device__ static void setcoef_layer(const RtArgs &a, int c, int lay, ColView &v
   double scalefac = p * (296. / 1013.) / t;
   double forfac = scalefac / (1. + water), forfrac, selffac, selffrac;
   int indfor, indself;
-  if (plog > 4.56) {
+  if (p >= T.pthr[PTHR_TROP]) {   // log p > 4.56
     atomicAdd(ntrop, 1);
+    if (p >= T.pthr[PTHR_LOW]) atomicAdd(&v.misc[M_LAYLOW], 1);   // log p >= 6.62
     double factor = (332.0 - t) / 36.0;
     indfor = (int)factor;
     indfor = indfor < 1 ? 1 : (indfor > 2 ? 2 : indfor);

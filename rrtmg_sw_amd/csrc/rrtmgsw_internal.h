@@ -18,6 +18,10 @@ constexpr int SCR_COEFS = 6;   // elimination coefficients kept per (layer, g-po
 // (nlay, SCR_COEFS, NGPT), then the cloud optics (tau, ssa, asymmetry) of each (layer, band) of
 // the fused entry (nlay, NBND, 3).
 constexpr size_t scr_col_doubles(int nlay) { return (size_t)nlay * (SCR_COEFS * NGPT + 4 * NBND); }
+// Pressure thresholds of setcoef_sw's decisions (src/rrtmg_sw_setcoef.f90:169-176, :208-210):
+// pthr[PTHR_JP + k - 2] (k = 2..58) is the smallest pressure at which jp < k, pthr[PTHR_TROP] the
+// smallest at which log p > 4.56, pthr[PTHR_LOW] the smallest at which log p >= 6.62.
+constexpr int PTHR_JP = 0, PTHR_TROP = 57, PTHR_LOW = 58, NPTHR = 64;
 constexpr double PADE = 0.278;
 constexpr double OD_LO = 0.06;
 
@@ -69,6 +73,7 @@ struct DeviceTables {
   const double *exp_tbl;    // NTBL+1 entries
   const float *tab32, *exp32, *rexp32;   // FP32 copies of tab, exp_tbl and 1 / exp_tbl (mixed form)
   const GPoint *gp;         // NGPT records
+  const double *pthr;       // NPTHR pressure thresholds of setcoef_sw's index decisions
   // cloud optics offsets (tables (n, 16:29) column-major: value(i, band) at off + (i-1) + n*(band-16))
   int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;
   int32_t abari, bbari, cbari, dbari, ebari, fbari;
@@ -143,6 +148,15 @@ struct McicaArgs {
   int *status;
 };
 
+// Outputs of rrtmgsw_setcoef_dev (NULL: not written): laytrop, laylow (ncol); laysolfr (ncol, 14);
+// the rest (ncol, nlay).
+struct SetcoefOut {
+  int *laytrop, *laylow, *laysolfr;
+  int *jp, *jt, *jt1, *indself, *indfor;
+  double *fac00, *fac01, *fac10, *fac11, *selffac, *selffrac, *forfac, *forfrac;
+  double *colh2o, *colco2, *colo3, *colch4, *colo2, *colmol;
+};
+
 // Adjoint of the fused path (rrtmgsw_adjoint.hip): seeds of the total-sky TOA upward flux and
 // heating rates in, gradients with respect to T, H2O, O3, in-cloud water paths and albedos out.
 struct AdjArgs {
@@ -164,6 +178,7 @@ hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s
 hipError_t launch_rt_spectral(const RtArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
+hipError_t launch_setcoef(const RtArgs &a, const SetcoefOut &o, const DeviceTables &t, hipStream_t s);
 // MT19937 stream of ndraw tempered words to out. With coef (device: {n, i_1, ..., i_n}, the set
 // bits of mt_jump_coefficients(MT_LOG2_SEG)) and ndraw > 2^(MT_LOG2_SEG + 1), out must hold
 // ndraw + mt_extra_words(ndraw) words: segment start states go after the stream.

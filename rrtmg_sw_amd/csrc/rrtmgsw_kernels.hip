@@ -106,7 +106,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   const int gi = active ? g : wave * WAVE_G;
 
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
-  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0;
+  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0, v.misc[M_LAYLOW] = 0;
   if (MIXED)
     for (int w = tid; w < (L + 31) / 32; w += blockDim.x) v.thin[w] = 0u;
   __syncthreads();
@@ -116,7 +116,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       const double pdp = a.plev[c + (size_t)C * lay] - a.plev[c + (size_t)C * (lay + 1)];
       if (pdp < a.thin_pdp) atomicOr(&v.thin[j >> 5], 1u << (j & 31));
     }
-  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, c, lay, v, &v.misc[M_LAYTROP]);
+  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
   __syncthreads();
   const int laytrop = v.misc[M_LAYTROP];
   if (tid < NBND) {
@@ -637,6 +637,54 @@ hipError_t launch_rt_mixed(const RtArgs &a, const DeviceTables &t, hipStream_t s
 hipError_t launch_rt_exact(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
   // the exact-exponential form is offered by the fused entry point only
   return a.fused && !a.layer_clouds ? launch_rt<2, CLD_FUSED>(a, t, s) : hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------------------------
+// setcoef_sw (src/rrtmg_sw_setcoef.f90:49-305) for every column, fed from inatm_sw's layer
+// amounts: the forward kernels' own prologue (setcoef_layer for every layer, laytrop,
+// find_laysolfr of each band), one workgroup per column, its LDS results written out.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) setcoef_kernel(RtArgs a, DeviceTables T, SetcoefOut o) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int L = a.nlay, C = a.ncol;
+  ColLayout lo(L, 0, false);
+  ColView v(smem, lo);
+  const int c = a.col0 + blockIdx.x, tid = threadIdx.x;
+  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_LAYLOW] = 0;
+  __syncthreads();
+  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+  __syncthreads();
+  const int laytrop = v.misc[M_LAYTROP];
+  if (tid < NBND) {
+    const int first[NBND] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100};
+    v.misc[M_LAYSOL + tid] = find_laysolfr(T.gp[first[tid]], v, laytrop, L);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (o.laytrop) o.laytrop[c] = laytrop;
+    if (o.laylow) o.laylow[c] = v.misc[M_LAYLOW];
+  }
+  if (tid < NBND && o.laysolfr) o.laysolfr[c + (size_t)C * tid] = v.misc[M_LAYSOL + tid];
+  int *const iv[5] = {o.jp, o.jt, o.jt1, o.indself, o.indfor};
+  const int *const is[5] = {v.jp, v.jt, v.jt1, v.inds, v.indf};
+  // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac, then h2o co2 o3 ch4 o2 colmol
+  double *const dv[14] = {o.fac00,   o.fac01,  o.fac10,  o.fac11,  o.selffac, o.selffrac, o.forfac,
+                          o.forfrac, o.colh2o, o.colco2, o.colo3,  o.colch4,  o.colo2,    o.colmol};
+  for (int lay = tid; lay < L; lay += blockDim.x) {
+    const size_t q = c + (size_t)C * lay;
+    for (int k = 0; k < 5; k++)
+      if (iv[k]) iv[k][q] = is[k][lay];
+    for (int k = 0; k < 8; k++)
+      if (dv[k]) dv[k][q] = v.f(k, lay);
+    for (int k = 0; k < 6; k++)
+      if (dv[8 + k]) dv[8 + k][q] = v.c(k, lay);
+  }
+}
+
+hipError_t launch_setcoef(const RtArgs &a, const SetcoefOut &o, const DeviceTables &t, hipStream_t s) {
+  ColLayout lo(a.nlay, 0, false);
+  hipLaunchKernelGGL(setcoef_kernel, dim3(a.ncol_chunk), dim3(128), lo.total, s, a, t, o);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------

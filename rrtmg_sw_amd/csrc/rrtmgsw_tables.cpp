@@ -301,6 +301,66 @@ bool build_tables(HostTables &H, std::string &err) {
     H.exp_tbl[itr] = exp(-tau_tbl);
     if (H.exp_tbl[itr] <= 1.e-20) H.exp_tbl[itr] = 1.e-20;
   }
+  return pressure_thresholds(H.pthr, err);
+}
+
+// ---- pressure thresholds of setcoef_sw's decisions on log(p) ----
+// setcoef_sw decides jp = int(36 - 5 (log p + 0.04)) (src/rrtmg_sw_setcoef.f90:169-176) and the
+// troposphere / lowest-layer tests log p > 4.56, log p >= 6.62 (:208-210) on the host libm's
+// log. Each decision is a step function of p, so it is fixed by the pressure at which it
+// switches: the smallest double p on the far side, found here by bisection over the ordered bit
+// patterns of positive doubles with the same libm log the reference and the oracle call. A
+// device log that differs from it by an ulp can then no longer move an index.
+namespace {
+int jp_raw(double p) {
+#pragma clang fp contract(off)
+  double plog = log(p);
+  return (int)(36. - 5 * (plog + 0.04));
+}
+double next_up(double x) { return nextafter(x, INFINITY); }
+// Smallest p in (lo, hi] with pred(p), given !pred(lo) and pred(hi) and a single switch.
+template <typename F>
+double switch_point(F pred, double lo, double hi) {
+  uint64_t a, b;
+  memcpy(&a, &lo, 8), memcpy(&b, &hi, 8);
+  while (b - a > 1) {
+    uint64_t m = a + (b - a) / 2;
+    double x;
+    memcpy(&x, &m, 8);
+    if (pred(x)) b = m;
+    else a = m;
+  }
+  double r;
+  memcpy(&r, &b, 8);
+  return r;
+}
+// The predicate switches exactly once within +-64 ulps of t (the log's rounding could only
+// break monotonicity within an ulp or two of the switch).
+template <typename F>
+bool clean_switch(F pred, double t) {
+  double x = t;
+  for (int i = 0; i < 64; i++) x = nextafter(x, 0.0);
+  for (int i = -64; i < 64; i++, x = next_up(x))
+    if (pred(x) != (i >= 0)) return false;
+  return true;
+}
+}  // namespace
+
+bool pressure_thresholds(std::vector<double> &pthr, std::string &err) {
+  pthr.assign(NPTHR, 0.0);
+  const double lo = 1.e-300, hi = 1.e300;
+  for (int k = 2; k <= 58; k++) {   // T_k: the smallest p with jp_raw(p) < k
+    auto pred = [k](double p) { return jp_raw(p) < k; };
+    double t = switch_point(pred, lo, hi);
+    if (!clean_switch(pred, t)) return err = "jp threshold is not a single switch", false;
+    pthr[PTHR_JP + k - 2] = t;
+  }
+  auto trop = [](double p) { return log(p) > 4.56; };
+  auto low = [](double p) { return log(p) >= 6.62; };
+  pthr[PTHR_TROP] = switch_point(trop, lo, hi);
+  pthr[PTHR_LOW] = switch_point(low, lo, hi);
+  if (!clean_switch(trop, pthr[PTHR_TROP]) || !clean_switch(low, pthr[PTHR_LOW]))
+    return err = "troposphere threshold is not a single switch", false;
   return true;
 }
 

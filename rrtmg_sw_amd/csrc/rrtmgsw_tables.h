@@ -12,10 +12,14 @@ struct HostTables {
   std::vector<double> exp_tbl;   // NTBL + 1
   std::vector<GPoint> gp;        // NGPT
   std::vector<double> rwgt;      // 224 reduction weights
+  std::vector<double> pthr;      // NPTHR pressure thresholds of setcoef_sw's index decisions
   int32_t extliq1, ssaliq1, asyliq1, extice2, ssaice2, asyice2, extice3, ssaice3, asyice3, fdlice3;
   int32_t abari, bbari, cbari, dbari, ebari, fbari, rsrtaua, rsrpiza, rsrasya;
 };
 
 bool build_tables(HostTables &h, std::string &err);
+// The pressures (hPa) at which setcoef_sw's jp, troposphere and lowest-layer decisions switch,
+// on the host libm's log (layout PTHR_*).
+bool pressure_thresholds(std::vector<double> &pthr, std::string &err);
 
 }  // namespace rrtmgsw

@@ -30,6 +30,11 @@ def _inputs(dev, xs, alb):
 class McicaShortwave(torch.autograd.Function):
     @staticmethod
     def forward(ctx, tlay, h2ovmr, o3vmr, clwp, ciwp, alb, cols, dev, precision="exact"):
+        # the adjoint is the derivative of the exact-exponential forward only: the table forward
+        # is piecewise constant in the optical depths, the mixed form another function again
+        if precision != "exact":
+            raise ValueError("McicaShortwave differentiates the exact-exponential forward: "
+                             "precision must be 'exact'")
         d = _inputs(dev, (tlay, h2ovmr, o3vmr, clwp, ciwp), alb)
         out = api.mcica_rrtmg_sw(cols, d, precision=precision)
         ctx.cols, ctx.d = cols, d

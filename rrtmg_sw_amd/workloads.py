@@ -153,7 +153,7 @@ def make_columns(ncol, nlay, icld=0, iaer=0, seed=0, psfc=1013.0, ptop=0.01, jit
     dz = zlev[:, 1:] - zlev[:, :-1]
     h2o = q0[:, None] * np.exp(-zlay / 2000.0)
     o3 = 8e-6 * np.exp(-0.5 * np.log(play / 10.0) ** 2)
-    ones = np.ones((C, L))
+    ones = np.ones((C, L), order="F")
     cols = Columns(
         ncol=C, nlay=L, icld=icld, iaer=iaer,
         play=_f(play), plev=_f(plev), tlay=_f(tlay), tlev=_f(tlev), tsfc=_f(T0),
@@ -184,17 +184,19 @@ def make_columns(ncol, nlay, icld=0, iaer=0, seed=0, psfc=1013.0, ptop=0.01, jit
         rei = np.where(mask, rei_d, 60.0)
     cols.cldfrac, cols.clwp, cols.ciwp = _f(cldfrac), _f(clwp), _f(ciwp)
     cols.rel, cols.rei = _f(rel), _f(rei)
-    cols.tauc = _f(np.zeros((NBND, C, L)))
-    cols.ssac = _f(np.ones((NBND, C, L)))
-    cols.asmc = _f(np.zeros((NBND, C, L)))
-    cols.fsfc = _f(np.zeros((NBND, C, L)))
-    cols.alpha = _f(np.zeros((C, L)))
+    cols.tauc = np.zeros((NBND, C, L), order="F")
+    cols.ssac = np.ones((NBND, C, L), order="F")
+    cols.asmc = np.zeros((NBND, C, L), order="F")
+    cols.fsfc = np.zeros((NBND, C, L), order="F")
+    cols.alpha = np.zeros((C, L), order="F")
     if icld in (4, 5):
         cols.decorr_len = _f(rng.uniform(decorr[0], decorr[1], C))
         cols.alpha = _f(alpha_for(cols))
-    tauaer = np.zeros((C, L, NBND))
-    ssaaer = np.ones((C, L, NBND))
-    asmaer = np.zeros((C, L, NBND))
+    # (ncol, nlay, nbnd) arrays are built as C-ordered (nbnd, nlay, ncol) and transposed: the
+    # Fortran layout without a copy
+    tauaer = np.zeros((C, L, NBND), order="F")
+    ssaaer = np.ones((C, L, NBND), order="F")
+    asmaer = np.zeros((C, L, NBND), order="F")
     if iaer == 10:
         aod = rng.lognormal(np.log(0.1), 0.5, (C, NBND))
         ssa = rng.uniform(0.85, 0.99, (C, NBND))
@@ -202,11 +204,11 @@ def make_columns(ncol, nlay, icld=0, iaer=0, seed=0, psfc=1013.0, ptop=0.01, jit
         low = zlay < 3000.0
         w = np.where(low, dz, 0.0)
         w = w / np.maximum(w.sum(axis=1, keepdims=True), 1e-30)
-        tauaer = w[:, :, None] * aod[:, None, :]
-        ssaaer = np.broadcast_to(ssa[:, None, :], (C, L, NBND)).copy()
-        asmaer = np.broadcast_to(g[:, None, :], (C, L, NBND)).copy()
-    cols.tauaer, cols.ssaaer, cols.asmaer = _f(tauaer), _f(ssaaer), _f(asmaer)
-    cols.ecaer = _f(np.zeros((C, L, NAER)))
+        tauaer = np.multiply(w.T[None, :, :], aod.T[:, None, :], order="C").T
+        ssaaer = np.ascontiguousarray(np.broadcast_to(ssa.T[:, None, :], (NBND, L, C))).T
+        asmaer = np.ascontiguousarray(np.broadcast_to(g.T[:, None, :], (NBND, L, C))).T
+    cols.tauaer, cols.ssaaer, cols.asmaer = tauaer, ssaaer, asmaer
+    cols.ecaer = np.zeros((C, L, NAER), order="F")
     return cols
 
 

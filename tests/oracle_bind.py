@@ -156,6 +156,32 @@ def run_mcica(cols, which="orc"):
     return o, irng.value
 
 
+SETCOEF_INT = ["jp", "jt", "jt1", "indself", "indfor"]
+SETCOEF_DBL = ["colh2o", "colco2", "colo3", "colch4", "colo2", "colmol", "fac00", "fac01",
+               "fac10", "fac11", "selffac", "selffrac", "forfac", "forfrac"]
+
+
+def run_setcoef(ncol, nlay, play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr):
+    """The oracle's setcoef_sw for every column, fed from inatm_sw's layer amounts (Fortran
+    (ncol, nlay) / (ncol, nlay+1) float64 arrays). Returns laytrop, laylow (ncol), laysolfr
+    (ncol, 14) and the per-layer arrays (ncol, nlay), in the reference's 1-based indices."""
+    N, L = ncol, nlay
+    iout = np.zeros((2 + NBND + 5 * L) * N, dtype=np.int32)
+    dout = np.zeros(14 * N * L, dtype=np.float64)
+    arrs = [np.asfortranarray(a, dtype=np.float64) for a in (play, plev, tlay, h2ovmr, o3vmr,
+                                                             co2vmr, ch4vmr, o2vmr)]
+    orc().orc_setcoef_gcm(C.c_int(N), C.c_int(L), *[_p(a) for a in arrs], _ip_of(iout),
+                          _p(dout))
+    o = {"laytrop": iout[:N].copy(), "laylow": iout[N:2 * N].copy(),
+         "laysolfr": iout[2 * N:(2 + NBND) * N].reshape((N, NBND), order="F").copy()}
+    base = (2 + NBND) * N
+    for k, name in enumerate(SETCOEF_INT):
+        o[name] = iout[base + k * N * L: base + (k + 1) * N * L].reshape((N, L), order="F").copy()
+    for k, name in enumerate(SETCOEF_DBL):
+        o[name] = dout[k * N * L:(k + 1) * N * L].reshape((N, L), order="F").copy()
+    return o
+
+
 def empty_mcica(cols):
     G, N, L = NGPT, cols.ncol, cols.nlay
     o = {k: _out((G, N, L)) for k in ("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl", "asmcmcl",

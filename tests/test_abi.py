@@ -25,3 +25,14 @@ def test_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     from rrtmg_sw_amd import _lib
     assert sorted(_lib.EXPORTED) == names
+
+
+def test_autograd_wrapper_refuses_other_forms():
+    """The adjoint is the derivative of the exact-exponential forward: the torch.autograd.Function
+    refuses the table and mixed forms before anything runs."""
+    import torch
+    from rrtmg_sw_amd.torch_adjoint import McicaShortwave
+    x = torch.zeros(2, 3, dtype=torch.float64)
+    for p in ("fp64", "mixed"):
+        with pytest.raises(ValueError):
+            McicaShortwave.apply(x, x, x, x, x, x, None, {}, p)

@@ -2,8 +2,10 @@
 IN_AER_RRTM in, OUTPUT_RRTM out, radiative transfer on the GPU through rrtmgsw_column_sw) on the
 reference's own example suite: every printed block of the expected outputs (broadband and, where
 the case prints them, bands 16-29; upward, diffuse and direct downward, downward and net flux,
-heating rate) within 0.1 W/m2 and 0.01 K/day. The expected outputs are printed to 1e-4 W/m2 and
-1e-5 K/day; the largest differences are printed."""
+heating rate) equal to the printed precision: the expected outputs are printed to 1e-4 W/m2
+and 1e-5 K/day, and the program's own printed values may differ from them by at most half a
+unit in the last digit (0.5e-4 + 2e-6 W/m2, 0.5e-5 + 2e-7 K/day, the oracle's bar for the same
+cases). The largest differences are printed."""
 import os
 import shutil
 import subprocess
@@ -44,7 +46,7 @@ def test_column_program_example_suite(tmp_path, name):
         flux = np.abs(rows[:, :5] - exp[b][:, :5]).max()
         hr = np.abs(rows[:, 5] - exp[b][:, 5]).max()
         worst = (max(worst[0], flux), max(worst[1], hr))
-        assert flux <= 0.1, (name, b, flux)
-        assert hr <= 0.01, (name, b, hr)
+        assert flux <= 0.5e-4 + 2e-6, (name, b, flux)
+        assert hr <= 0.5e-5 + 2e-7, (name, b, hr)
     print(f"{name}: {len(got)} blocks, max flux diff {worst[0]:.2e} W/m2, max HR diff {worst[1]:.2e} K/day",
           flush=True)

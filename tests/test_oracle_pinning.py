@@ -25,6 +25,7 @@ import pytest
 import scipy.linalg
 
 import oracle_bind as ob
+from rrtmg_sw_amd.workloads import workload
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _dp = ob._p
@@ -191,6 +192,47 @@ def test_reftra_matches_two_stream_solution():
         worst = max(worst, d)
         assert d < 2e-3, (tau, w, g, mu, (R, eR), (T, eT), (Rd, eRd), (Td, eTd))
     assert worst > 0.0
+
+
+def test_reftra_exact_exponential_matches_two_stream_solution():
+    """In its exact-exponential mode (the forward the adjoint differentiates) the oracle's reftra
+    is the PIFM two-stream solution itself: agreement to rounding (1e-11) instead of the table's
+    1e-3, on the general branch (away from the removable singularity k mu = 1) and on the
+    conservative one (w = 1, checked through R + T = 1 and the exact solution)."""
+    ob.orc().orc_set_exact_exp(1)
+    try:
+        worst = 0.0
+        for tau, w, g, mu in GRID + [(t, 1.0, g, mu) for t in (0.1, 2.0) for g in (0.0, 0.7)
+                                     for mu in (0.4, 1.0)]:
+            k = math.sqrt(max(((8 - w * (5 + 3 * g)) / 4) ** 2 - (3 * w * (1 - g) / 4) ** 2, 0.0))
+            if w < 1.0 and abs(1 - k * mu) < 0.05:
+                continue
+            R, Rd, T, Td = [v[0] for v in _reftra([tau], [w], [g], mu)]
+            eR, eT, eRd, eTd = _two_stream(tau, w, g, mu)
+            d = max(abs(R - eR), abs(T - eT), abs(Rd - eRd), abs(Td - eTd))
+            worst = max(worst, d)
+            assert d < 1e-11, (tau, w, g, mu, (R, eR), (T, eT), (Rd, eRd), (Td, eTd))
+    finally:
+        ob.orc().orc_set_exact_exp(0)
+    assert worst > 0.0
+
+
+def test_exact_exponential_forward_near_table_forward():
+    """End to end, the exact-exponential forward differs from the reference's table forward only
+    by the table's quantisation: under 5 W/m2, and not at all where the mode is off again."""
+    cols = workload("expran4096x60", ncol=8)
+    mc, _ = ob.run_mcica(cols)
+    a = ob.run_rrtmg(cols, mc)
+    ob.orc().orc_set_exact_exp(1)
+    try:
+        b = ob.run_rrtmg(cols, mc)
+    finally:
+        ob.orc().orc_set_exact_exp(0)
+    c = ob.run_rrtmg(cols, mc)
+    d = max(float(np.abs(a[k] - b[k]).max()) for k in ("swuflx", "swdflx"))
+    assert 1e-6 < d < 5.0, d
+    for k in ("swuflx", "swdflx", "swhr"):
+        assert np.array_equal(a[k], c[k]), k
 
 
 def test_reftra_special_cases():
