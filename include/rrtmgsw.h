@@ -248,6 +248,37 @@ int rrtmgsw_mcica_sw_dev(int ncol, int nlay, int icld, int iaer, int permuteseed
                          double *swhrc, const double *bndsolvar, const double *indsolvar,
                          const double *solcycfrac, int precision, void *stream);
 
+/* Bytes of the device array rrtmgsw_mcica_sw_dev_noalloc works in (one per GPU, 256 MiB). */
+#define RRTMGSW_NOALLOC_SCRATCH_BYTES ((size_t)256 * 1024 * 1024)
+
+/* rrtmgsw_mcica_sw_dev that allocates nothing: no counterpart in the reference. It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors as rrtmgsw_mcica_sw_dev;
+ * the caller still passes every input and output. Its working memory -- the status word, the
+ * KISS jump records, the Mersenne Twister stream of irng = 1 and the per-column elimination
+ * coefficients -- lives in one file-scope device array of RRTMGSW_NOALLOC_SCRATCH_BYTES per GPU,
+ * set up by rrtmgsw_init on the device current then (the entry point fails on any other); the
+ * columns run in launches of as many as the array holds (640 at 72 layers). A Mersenne Twister
+ * stream (112 x ncol x nlay words, twice that for icld 4/5) that leaves no room for one column
+ * is refused with RRTMGSW_ERR_ARGUMENT. The array keeps the last call's contents; calls share
+ * it, so they run one at a time (a lock), and the call synchronizes its stream before returning. */
+int rrtmgsw_mcica_sw_dev_noalloc(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                                 const double *play, const double *plev, const double *tlay,
+                                 const double *tlev, const double *tsfc, const double *h2ovmr,
+                                 const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                                 const double *n2ovmr, const double *o2vmr, const double *asdir,
+                                 const double *asdif, const double *aldir, const double *aldif,
+                                 const double *coszen, double adjes, int dyofyr, double scon,
+                                 int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                                 const double *cldfrac, const double *ciwp, const double *clwp,
+                                 const double *rei, const double *rel, const double *tauc,
+                                 const double *ssac, const double *asmc, const double *fsfc,
+                                 const double *alpha, const double *tauaer, const double *ssaaer,
+                                 const double *asmaer, const double *ecaer, double *swuflx,
+                                 double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
+                                 double *swhrc, const double *bndsolvar, const double *indsolvar,
+                                 const double *solcycfrac, int precision, void *stream);
+
 /* Adjoint of rrtmgsw_mcica_sw_dev (precision RRTMGSW_FP64_EXP) for the objective
  *   J = sum_c seed_toa(c) * swuflx(c, nlay+1) + sum_(c,k) seed_hr(c, k) * swhr(c, k)
  * (total sky; seeds are device arrays (ncol) and (ncol, nlay)). Writes dJ/d tlay, h2ovmr,

@@ -52,6 +52,7 @@ def lib():
                                        [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                         C.c_int] + [_dp] * 10 + [_dp] * 4 + [_dp] * 6 + [_dp] * 3 +
                                        [C.c_int, _vp])
+    L.rrtmgsw_mcica_sw_dev_noalloc.argtypes = L.rrtmgsw_mcica_sw_dev.argtypes
     L.rrtmgsw_mcica_sw_adjoint_dev.argtypes = (
         [C.c_int] * 5 + [_ip] + [_dp] * 16 +
         [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int] + [_dp] * 10 +
@@ -69,6 +70,7 @@ EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_s
             "rrtmgsw_column_mcica_sw",
             "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",
             "rrtmgsw_get_alpha", "rrtmgsw_setcoef_dev", "rrtmgsw_mcica_sw_dev",
+            "rrtmgsw_mcica_sw_dev_noalloc",
             "rrtmgsw_mcica_sw_adjoint_dev", "rrtmgsw_adjoint_work_bytes"]
 
 

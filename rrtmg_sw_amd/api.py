@@ -235,11 +235,13 @@ def _solvar(bndsolvar, indsolvar, solcycfrac):
 
 
 def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None, bndsolvar=None, indsolvar=None,
-                   solcycfrac=None):
+                   solcycfrac=None, noalloc=False):
     """mcica_subcol_sw + rrtmg_sw fused on the GPU for the workload `cols` (a
     workloads.Columns) whose arrays are already resident: `dev_arrays` maps input names to
     device tensors (see device_inputs). bndsolvar (14), indsolvar (2) and solcycfrac are the
-    reference's optional solar-variability arguments, on the host. Returns the output tensors."""
+    reference's optional solar-variability arguments, on the host. noalloc selects the entry
+    point that allocates no device memory (rrtmgsw_mcica_sw_dev_noalloc). Returns the output
+    tensors."""
     N, L = cols.ncol, cols.nlay
     d = dev_arrays
     if out is None:
@@ -258,7 +260,8 @@ def mcica_rrtmg_sw(cols, dev_arrays, precision="fp64", out=None, bndsolvar=None,
     args += [_ptr(out[k]) for k in _lib.SW_OUT]
     sv = _solvar(bndsolvar, indsolvar, solcycfrac)
     args += [_ptr(x) for x in sv] + [C.c_int(PRECISION[precision]), _stream()]
-    check(_lib.lib().rrtmgsw_mcica_sw_dev(*args))
+    L = _lib.lib()
+    check((L.rrtmgsw_mcica_sw_dev_noalloc if noalloc else L.rrtmgsw_mcica_sw_dev)(*args))
     return out
 
 

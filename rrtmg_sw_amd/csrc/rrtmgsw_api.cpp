@@ -21,6 +21,12 @@
 
 using namespace rrtmgsw;
 
+// Scratch of rrtmgsw_mcica_sw_dev_noalloc: a file-scope device array, one copy per GPU that loads
+// the library, held until the process ends. Host code never passes this name to a kernel: its
+// device address is looked up once per device (hipGetSymbolAddress, at rrtmgsw_init) and kept in
+// the context.
+static __device__ __align__(256) char scratch[RRTMGSW_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -55,6 +61,8 @@ struct Context {
   std::map<hipStream_t, std::unique_ptr<std::mutex>> calls;   // one call at a time per stream
   std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
   uint32_t *d_mtjump = nullptr;   // MT19937 jump-ahead: set-bit positions of x^J mod P, on first use
+  std::map<int, char *> noalloc;  // device -> address of `scratch` on that device
+  std::mutex noalloc_mu;          // calls of the no-alloc entry share their GPU's array: one at a time
 } ctx;
 
 // A call's scratch, status word and generator buffers are its stream's from their acquisition
@@ -377,6 +385,24 @@ struct SwCall {
 // 0.003 K/day at 3 hPa).
 constexpr double THIN_PDP_HPA = 3.0;
 
+// The status word cleared (by a kernel), then the forward kernel of the given form over column
+// chunks of at most `chunk` columns, all on stream s, into a.scratch / a.status.
+hipError_t launch_rt_chunks(RtArgs a, const DeviceTables &tables, hipStream_t s, int precision, int chunk,
+                            bool spectral = false) {
+  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, a.status);
+  a.thin_pdp = THIN_PDP_HPA;
+  hipError_t e = hipGetLastError();
+  for (int c0 = 0; c0 < a.ncol && e == hipSuccess; c0 += chunk) {
+    a.col0 = c0;
+    a.ncol_chunk = std::min(chunk, a.ncol - c0);
+    e = spectral                        ? launch_rt_spectral(a, tables, s)
+        : precision == RRTMGSW_MIXED     ? launch_rt_mixed(a, tables, s)
+        : precision == RRTMGSW_FP64_EXP ? launch_rt_exact(a, tables, s)
+                                        : launch_rt_fp64(a, tables, s);
+  }
+  return e;
+}
+
 // spectral: the column program's per-band outputs (a.bandu/bandd/banddir); the per-g level terms
 // of a chunk go to a buffer allocated for the call.
 int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int precision = RRTMGSW_FP64,
@@ -394,19 +420,9 @@ int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int pr
     chunk = (int)std::min<size_t>((size_t)chunk, std::max<size_t>(1, ((size_t)1 << 30) / (per * sizeof(double))));
     HIPCHK(hipMalloc(&a.gspec, (size_t)chunk * per * sizeof(double)));
   }
-  hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, (int *)st);
   a.scratch = (double *)scr;
   a.status = (int *)st;
-  a.thin_pdp = THIN_PDP_HPA;
-  hipError_t e = hipSuccess;
-  for (int c0 = 0; c0 < base.ncol && e == hipSuccess; c0 += chunk) {
-    a.col0 = c0;
-    a.ncol_chunk = std::min(chunk, base.ncol - c0);
-    e = spectral                        ? launch_rt_spectral(a, tables, s)
-        : precision == RRTMGSW_MIXED     ? launch_rt_mixed(a, tables, s)
-        : precision == RRTMGSW_FP64_EXP ? launch_rt_exact(a, tables, s)
-                                        : launch_rt_fp64(a, tables, s);
-  }
+  hipError_t e = launch_rt_chunks(a, tables, s, precision, chunk, spectral);
   int rc = e != hipSuccess ? fail(RRTMGSW_ERR_HIP, hipGetErrorString(e)) : finish(s, (int *)st);
   if (spectral) {
     hipStreamSynchronize(s);
@@ -466,6 +482,11 @@ int rrtmgsw_init(double cpdair) {
   D.ebari = H.ebari, D.fbari = H.fbari;
   D.rsrtaua = H.rsrtaua, D.rsrpiza = H.rsrpiza, D.rsrasya = H.rsrasya;
   D.kiss_jump = nullptr;
+  {
+    char *p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    ctx.noalloc[ctx.device] = p;
+  }
   ctx.ready = true;
   return 0;
 }
@@ -1179,7 +1200,8 @@ static int fused_setup(int ncol, int nlay, int icld, int iaer, int permuteseed, 
                        const double *asmc, const double *fsfc, const double *alpha,
                        const double *tauaer, const double *ssaaer, const double *asmaer,
                        const double *ecaer, const double *bndsolvar, const double *indsolvar,
-                       const double *solcycfrac, hipStream_t s, RtArgs &a, DeviceTables &t) {
+                       const double *solcycfrac, hipStream_t s, RtArgs &a, DeviceTables &t,
+                       bool prepare = true) {
   if (icld < 0 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "MCICA_SUBCOL: INVALID ICLD");
   if (ncol < 0 || nlay < 4 || nlay > MAX_LAY)
     return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and ncol >= 0");
@@ -1212,7 +1234,7 @@ static int fused_setup(int ncol, int nlay, int icld, int iaer, int permuteseed, 
     return fail(RRTMGSW_ERR_ARGUMENT, "tauaer, ssaaer and asmaer are required when iaer = 10");
   if (iaer == 6 && !ecaer) return fail(RRTMGSW_ERR_ARGUMENT, "ecaer is required when iaer = 6");
   const uint32_t *mt = nullptr, *jumps = nullptr;
-  if (icld >= 1)
+  if (icld >= 1 && prepare)
     if (int rc = mcica_prepare(ncol, nlay, icld, permuteseed, ir, s, &mt, &jumps)) return rc;
   a.mt_stream = mt;
   t.kiss_jump = jumps;
@@ -1290,6 +1312,91 @@ int rrtmgsw_setcoef_dev(int ncol, int nlay, const double *play, const double *pl
   HIPCHK(launch_setcoef(a, o, ctx.dev, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
+}
+
+int rrtmgsw_mcica_sw_dev_noalloc(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
+                                 const double *play, const double *plev, const double *tlay,
+                                 const double *tlev, const double *tsfc, const double *h2ovmr,
+                                 const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                                 const double *n2ovmr, const double *o2vmr, const double *asdir,
+                                 const double *asdif, const double *aldir, const double *aldif,
+                                 const double *coszen, double adjes, int dyofyr, double scon,
+                                 int isolvar, int inflgsw, int iceflgsw, int liqflgsw,
+                                 const double *cldfrac, const double *ciwp, const double *clwp,
+                                 const double *rei, const double *rel, const double *tauc,
+                                 const double *ssac, const double *asmc, const double *fsfc,
+                                 const double *alpha, const double *tauaer, const double *ssaaer,
+                                 const double *asmaer, const double *ecaer, double *swuflx,
+                                 double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
+                                 double *swhrc, const double *bndsolvar, const double *indsolvar,
+                                 const double *solcycfrac, int precision, void *stream) {
+  if (int rc = check_init()) return rc;
+  if (precision != RRTMGSW_FP64 && precision != RRTMGSW_MIXED && precision != RRTMGSW_FP64_EXP)
+    return fail(RRTMGSW_ERR_ARGUMENT, "precision must be RRTMGSW_FP64, RRTMGSW_MIXED or RRTMGSW_FP64_EXP");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    auto it = ctx.noalloc.find(dev);
+    if (it == ctx.noalloc.end())
+      return fail(RRTMGSW_ERR_ARGUMENT, "the current device is not the one rrtmgsw_init set up");
+    base = it->second;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> one(ctx.noalloc_mu);
+  RtArgs a{};
+  DeviceTables t = ctx.dev;
+  if (ncol > 0) {
+    const void *outs[] = {swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+    for (const void *p : outs)
+      if (!p) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  }
+  if (int rc = fused_setup(ncol, nlay, icld, iaer, permuteseed, irng, play, plev, tlay, h2ovmr, o3vmr,
+                           co2vmr, ch4vmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr,
+                           scon, isolvar, inflgsw, iceflgsw, liqflgsw, cldfrac, ciwp, clwp, rei, rel,
+                           tauc, ssac, asmc, fsfc, alpha, tauaer, ssaaer, asmaer, ecaer, bndsolvar,
+                           indsolvar, solcycfrac, s, a, t, false))
+    return rc;
+  (void)tlev, (void)tsfc, (void)n2ovmr;
+  if (ncol == 0) return 0;
+  // the array: status word | KISS jumps (112 records) | MT19937 stream (irng = 1) | per-column
+  // elimination coefficients and cloud optics of one launch's columns
+  const size_t jump_off = 256, mt_off = jump_off + ((NGPT * sizeof(KissJump) + 255) / 256) * 256;
+  const int per_sub = icld == 3 ? 1 : (icld >= 4 ? 2 * nlay : nlay);
+  const bool mt = icld >= 1 && a.mc_irng == 1;
+  const size_t ndraw = mt ? (size_t)NGPT * ncol * per_sub : 0;
+  const size_t rt_off = mt_off + ((ndraw * 4 + 255) / 256) * 256;
+  const size_t per_col = scr_col_doubles(nlay) * sizeof(double);
+  if (rt_off + per_col > RRTMGSW_NOALLOC_SCRATCH_BYTES)
+    return fail(RRTMGSW_ERR_ARGUMENT,
+                "rrtmgsw_mcica_sw_dev_noalloc: the Mersenne Twister stream of this call (" +
+                    std::to_string(ndraw * 4) + " bytes) does not fit its " +
+                    std::to_string((size_t)RRTMGSW_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
+  const int chunk = (int)std::min<size_t>(
+      (size_t)ncol, launch_column_limit((RRTMGSW_NOALLOC_SCRATCH_BYTES - rt_off) / per_col));
+  if (icld >= 1) {
+    if (a.mc_irng == 0) {
+      KissJump J[NGPT];   // per-sub-column jumps, on the host stack
+      const uint64_t W = permuteseed > 0 ? (uint64_t)permuteseed : 0;
+      for (int g = 0; g < NGPT; g++) J[g] = kiss_jump_of(W + (uint64_t)g * (uint64_t)per_sub);
+      HIPCHK(hipMemcpyAsync(base + jump_off, J, sizeof J, hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));   // J leaves scope
+      t.kiss_jump = reinterpret_cast<const uint32_t *>(base + jump_off);
+    } else {
+      // one workgroup generates the stream in order (no jump-ahead table)
+      uint32_t *w = reinterpret_cast<uint32_t *>(base + mt_off);
+      HIPCHK(launch_mt_stream((uint32_t)permuteseed, (long long)ndraw, w, nullptr, s));
+      a.mt_stream = w;
+    }
+  }
+  a.swuflx = swuflx, a.swdflx = swdflx, a.swhr = swhr, a.swuflxc = swuflxc, a.swdflxc = swdflxc;
+  a.swhrc = swhrc;
+  a.status = reinterpret_cast<int *>(base);
+  a.scratch = reinterpret_cast<double *>(base + rt_off);
+  hipError_t e = launch_rt_chunks(a, t, s, precision, chunk);
+  if (e != hipSuccess) return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
+  return finish(s, a.status);
 }
 
 size_t rrtmgsw_adjoint_work_bytes(int nlay, int ckpt) {
