@@ -54,7 +54,6 @@ struct Context {
   HostTables host;
   DeviceTables dev{};
   double *d_tab = nullptr, *d_exp = nullptr, *d_pthr = nullptr;
-  float *d_tab32 = nullptr, *d_exp32 = nullptr, *d_rexp32 = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
   std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork;
@@ -377,20 +376,12 @@ struct SwCall {
   double *swuflx, *swdflx, *swhr, *swuflxc, *swdflxc, *swhrc;
 };
 
-// Mixed form: layers whose pressure thickness is below this (hPa) run entirely in FP64 (gas
-// optics included). A layer's heating rate is heatfac / pdp times its flux divergence, and an
-// FP32 optical depth moves about one exponential-table lookup in a thousand by one table step
-// (3.6e-4 of a transmittance), a flux divergence change of up to ~0.02 W/m2: below ~3 hPa that
-// alone exceeds 0.01 K/day (measured on 8192 x 72 McICA columns: 0.012 K/day at 1 hPa,
-// 0.003 K/day at 3 hPa).
-constexpr double THIN_PDP_HPA = 3.0;
 
 // The status word cleared (by a kernel), then the forward kernel of the given form over column
 // chunks of at most `chunk` columns, all on stream s, into a.scratch / a.status.
 hipError_t launch_rt_chunks(RtArgs a, const DeviceTables &tables, hipStream_t s, int precision, int chunk,
                             bool spectral = false) {
   hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, s, a.status);
-  a.thin_pdp = THIN_PDP_HPA;
   hipError_t e = hipGetLastError();
   for (int c0 = 0; c0 < a.ncol && e == hipSuccess; c0 += chunk) {
     a.col0 = c0;
@@ -455,26 +446,12 @@ int rrtmgsw_init(double cpdair) {
   HIPCHK(hipMemcpy(ctx.d_tab, H.tab.data(), H.tab.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx.d_exp, H.exp_tbl.size() * sizeof(double)));
   HIPCHK(hipMemcpy(ctx.d_exp, H.exp_tbl.data(), H.exp_tbl.size() * sizeof(double), hipMemcpyHostToDevice));
-  {
-    // FP32 copies for the mixed form
-    auto up32 = [](const std::vector<double> &src, float **dst, bool recip) -> hipError_t {
-      std::vector<float> f(src.size());
-      for (size_t i = 0; i < f.size(); i++) f[i] = (float)(recip ? 1.0 / src[i] : src[i]);
-      hipError_t e = hipMalloc(dst, f.size() * sizeof(float));
-      if (e != hipSuccess) return e;
-      return hipMemcpy(*dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice);
-    };
-    HIPCHK(up32(H.tab, &ctx.d_tab32, false));
-    HIPCHK(up32(H.exp_tbl, &ctx.d_exp32, false));
-    HIPCHK(up32(H.exp_tbl, &ctx.d_rexp32, true));
-  }
   HIPCHK(hipMalloc(&ctx.d_pthr, H.pthr.size() * sizeof(double)));
   HIPCHK(hipMemcpy(ctx.d_pthr, H.pthr.data(), H.pthr.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx.d_gp, H.gp.size() * sizeof(GPoint)));
   HIPCHK(hipMemcpy(ctx.d_gp, H.gp.data(), H.gp.size() * sizeof(GPoint), hipMemcpyHostToDevice));
   DeviceTables &D = ctx.dev;
   D.tab = ctx.d_tab, D.exp_tbl = ctx.d_exp, D.gp = ctx.d_gp, D.pthr = ctx.d_pthr;
-  D.tab32 = ctx.d_tab32, D.exp32 = ctx.d_exp32, D.rexp32 = ctx.d_rexp32;
   D.extliq1 = H.extliq1, D.ssaliq1 = H.ssaliq1, D.asyliq1 = H.asyliq1;
   D.extice2 = H.extice2, D.ssaice2 = H.ssaice2, D.asyice2 = H.asyice2;
   D.extice3 = H.extice3, D.ssaice3 = H.ssaice3, D.asyice3 = H.asyice3, D.fdlice3 = H.fdlice3;
@@ -496,9 +473,6 @@ int rrtmgsw_finalize(void) {
   if (!ctx.ready) return 0;
   hipFree(ctx.d_tab);
   hipFree(ctx.d_exp);
-  hipFree(ctx.d_tab32);
-  hipFree(ctx.d_exp32);
-  hipFree(ctx.d_rexp32);
   hipFree(ctx.d_gp);
   hipFree(ctx.d_pthr);
   for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.adjwork}) {

@@ -24,9 +24,10 @@ constexpr double CLDMIN = 1.e-20;           // src/rrtmg_sw_cldprmc.f90:118
 struct ColLayout {
   int L;
   int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
-  int o_fac, o_col, o_red, o_reda, o_thin, o_mask, total;   // double blocks / mask words
-  // fluxes = false leaves out the per-level flux sums (red, reda) of the forward kernels
-  __host__ __device__ ColLayout(int nlay, int nmaskw, bool fluxes = true) {
+  int o_fac, o_col, o_red, o_reda, o_mask, total;     // double blocks / mask words
+  // fluxes = false leaves out the per-level flux sums (red, reda) of the forward kernels; mixed
+  // holds them in FP32, with the layers' absorbed fluxes (red: 6 a level, reda: 4 a layer)
+  __host__ __device__ ColLayout(int nlay, int nmaskw, bool fluxes = true, bool mixed = false) {
     L = nlay;
     int ib = 0;
     o_jp = ib, ib += L;
@@ -38,9 +39,9 @@ struct ColLayout {
     int bytes = ((ib * 4 + 15) / 16) * 16;
     o_fac = bytes, bytes += 8 * L * 8;     // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac
     o_col = bytes, bytes += 6 * L * 8;     // h2o co2 o3 ch4 o2 colmol
-    o_red = bytes, bytes += fluxes ? 2 * (L + 1) * 4 * 8 : 0;
-    o_reda = bytes, bytes += fluxes ? 2 * L * 2 * 8 : 0;
-    o_thin = bytes, bytes += ((L + 31) / 32) * 4;
+    o_red = bytes, bytes += !fluxes ? 0 : mixed ? 2 * (L + 1) * 6 * 4 : 2 * (L + 1) * 4 * 8;
+    o_reda = bytes, bytes += !fluxes ? 0 : mixed ? 2 * L * 4 * 4 : 2 * L * 2 * 8;
+    bytes = ((bytes + 15) / 16) * 16;
     o_mask = bytes, bytes += nmaskw * NGPT * 4;
     total = bytes;
   }
@@ -50,7 +51,8 @@ struct ColView {
   int L;
   int *jp, *jt, *jt1, *inds, *indf, *misc;
   double *fac, *col, *red, *reda;
-  uint32_t *mask, *thin;
+  float *redf, *redaf;   // the mixed form's FP32 sums, in place of red / reda
+  uint32_t *mask;
   __device__ ColView(char *smem, const ColLayout &lo) {
     L = lo.L;
     int *ip = reinterpret_cast<int *>(smem);
@@ -60,8 +62,9 @@ struct ColView {
     col = reinterpret_cast<double *>(smem + lo.o_col);
     red = reinterpret_cast<double *>(smem + lo.o_red);
     reda = reinterpret_cast<double *>(smem + lo.o_reda);
+    redf = reinterpret_cast<float *>(smem + lo.o_red);
+    redaf = reinterpret_cast<float *>(smem + lo.o_reda);
     mask = reinterpret_cast<uint32_t *>(smem + lo.o_mask);
-    thin = reinterpret_cast<uint32_t *>(smem + lo.o_thin);
   }
   __device__ double &f(int k, int lay) { return fac[k * L + lay]; }   // 0..7 as listed above
   __device__ double &c(int s, int lay) { return col[s * L + lay]; }   // 0..4 species, 5 colmol
@@ -758,19 +761,39 @@ __device__ inline double tbl_exp1(double ze, const double *__restrict__ exp_tbl)
 // with the reference's |zdenr| <= 1e-8 test as |(1 - k^2 mu0^2) D| <= 1e-8 e1 (R = 1e-8, T = e2,
 // and then rD = 1 / D). e2 = table(min(tau / mu0, 500)) is also the layer's direct-beam
 // transmittance: every table entry past index 9275 is the floor 1e-20, reached at tau/mu0 > 46.
-__device__ inline RT4 reftra_e(double zto1, double zw, double zg, double prmuz, double e2,
-                               const double *__restrict__ exp_tbl) {
-  const double zwcrit = 0.9999995, eps = 1.e-08;
-  RT4 o;
+// The FP64 part of reftra_e that decides its table lookups: the PIFM coefficients, the
+// conservative-scattering decision and, for the general branch, k and e1 = exp(-k tau). The
+// mixed form runs this same code, so its lookups pick the FP64 form's table entries.
+struct RtFront {
+  double g1, g2, g3, g4, k, kmu, e1;
+  bool cons;
+};
+__device__ inline RtFront reftra_front(double zto1, double zw, double zg, double prmuz,
+                                       const double *__restrict__ exp_tbl) {
+  const double zwcrit = 0.9999995;
+  RtFront f;
   double zg3 = 3. * zg;
-  double g1 = (8. - zw * (5. + zg3)) * 0.25;
-  double g2 = 3. * (zw * (1. - zg)) * 0.25;
-  double g3 = (2. - zg3 * prmuz) * 0.25;
-  double g4 = 1. - g3;
+  f.g1 = (8. - zw * (5. + zg3)) * 0.25;
+  f.g2 = 3. * (zw * (1. - zg)) * 0.25;
+  f.g3 = (2. - zg3 * prmuz) * 0.25;
+  f.g4 = 1. - f.g3;
   double omg = 1. - zg;
   double den2 = omg * omg - (1. - zw) * zg * zg;
-  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
-  if (cons) {  // conservative scattering, :188-228
+  f.cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+  f.k = 0., f.kmu = 0., f.e1 = 1.;
+  if (!f.cons) {
+    f.k = sqrt(f.g1 * f.g1 - f.g2 * f.g2);
+    f.kmu = f.k * prmuz;
+    f.e1 = tbl_exp1(fmin(f.k * zto1, 500.), exp_tbl);
+  }
+  return f;
+}
+// The rest of reftra_e in FP64, from the front.
+__device__ inline RT4 reftra_back(const RtFront &f, double zto1, double zw, double prmuz, double e2) {
+  const double eps = 1.e-08;
+  RT4 o;
+  const double g1 = f.g1, g2 = f.g2, g3 = f.g3, g4 = f.g4;
+  if (f.cons) {  // conservative scattering, :188-228
     double za1 = g1 * prmuz - g3;
     double zgt = g1 * zto1;
     double inv = frcp(1. + zgt);
@@ -780,11 +803,9 @@ __device__ inline RT4 reftra_e(double zto1, double zw, double zg, double prmuz, 
     o.td = 1. - o.rd;
     if (e2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
   } else {
+    const double k = f.k, kmu = f.kmu, e1 = f.e1;
     double a1 = g1 * g4 + g2 * g3;
     double a2 = g1 * g3 + g2 * g4;
-    double k = sqrt(g1 * g1 - g2 * g2);
-    double kmu = k * prmuz;
-    double e1 = tbl_exp1(fmin(k * zto1, 500.), exp_tbl);
     double ee = e1 * e1;
     double D = (k + g1) - (g1 - k) * ee;
     double P = 1. - kmu * kmu;
@@ -800,6 +821,10 @@ __device__ inline RT4 reftra_e(double zto1, double zw, double zg, double prmuz, 
     o.td = 2. * k * e1 * PrD;
   }
   return o;
+}
+__device__ inline RT4 reftra_e(double zto1, double zw, double zg, double prmuz, double e2,
+                               const double *__restrict__ exp_tbl) {
+  return reftra_back(reftra_front(zto1, zw, zg, prmuz, exp_tbl), zto1, zw, prmuz, e2);
 }
 
 // sweep_layer with a reciprocal from frcp; A (the level's ztdn) and B (prdnd) are the state
@@ -979,113 +1004,71 @@ __device__ inline double wave_sum2d(double v0, double v1, int lane) {
 }
 
 // ------------------------------------------------------------------------------------------
-// FP32 pieces of the mixed form. The layer optics carry the co-albedo c = 1 - w explicitly
-// (from the absorbing optical depths, so it keeps full relative precision when w is close to
-// one), which removes the cancellations of the two-stream expressions in FP32:
-//   gamma1 - gamma2 = 2c (PIFM), so k = sqrt(2c (gamma1 + gamma2));
-//   (k + gamma1) / e - (gamma1 - k) e = gamma1 (1 - e^2)/e + k (1/e + e) with 1 - e from the
-//   expansion or the table, and likewise for the diffuse denominator;
-//   zwo >= zwcrit  <=>  c (1 - zwcrit h^2) <= 1 - zwcrit,  h = g / (1 - g)
-// (algebraically the reference's quantities, src/rrtmg_sw_reftra.f90:164-322).
+// The mixed form's layer. Its FP64 front -- the delta-scaled optics, the two-stream
+// coefficients and k, the conservative-scattering decision and every exponential-table lookup --
+// is the FP64 form's own arithmetic (reftra_front, tbl_exp1), so its lookups pick the FP64
+// form's table entries. Its FP32 back evaluates the PIFM solution (src/rrtmg_sw_reftra.f90:233-322)
+// in the deficits d = 1 - e of the two transmittances. With e1 = 1 - d1 (e1 = exp(-k tau)),
+// e2 = 1 - d2 (e2 = exp(-tau / mu0)), s1 = 1 - e1^2 = d1 (2 - d1), the reference's numerators,
+// multiplied through by e1, lose the terms that cancel at tau = 0:
+//   R    =  w [B2 s1 + C2 (d1 + d2 - d1 d2)] rD,
+//   Tdif = -w [C1 (d1 - d2) + (1 - d2) B1 s1] rD,       T = e2 + Tdif,
+//   B2 = (1 + k mu0)(a2 - k g3), C2 = 2k (g3 - a2 mu0), B1 = (1 - k mu0)(a1 - k g4),
+//   C1 = 2k (g4 + a1 mu0), rD = 1 / ((1 - k^2 mu0^2) D'),  D' = k (1 + e1^2) + g1 s1,
+//   Rd = g2 s1 / D',  Td = 2k e1 / D'
+// (every term O(tau) with no cancellation), and the absorptances of the direct and the diffuse
+// beam, with c = 1 - w carried explicitly (gamma1 - gamma2 = 2c),
+//   a  = 1 - R - T   = d2 - (R + Tdif),
+//   ad = 1 - Rd - Td = d1 (k d1 + 2c (2 - d1)) / D',
+// keep their accuracy relative to tau / mu0 however thin the layer. FP64 remains in the
+// conservative branch and on lanes with |1 - k mu0| < 2e-3, where 1 - (k mu0)^2 vanishes with
+// the numerators and the table's steps make the quotient large (they run reftra_back).
 // ------------------------------------------------------------------------------------------
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-
-struct Exf {
-  float e, r, om;   // exp(-x) from the table (or its expansion), 1/e, 1 - e
+struct RTm {
+  float r, rd, t, td, a, ad;
 };
-__device__ inline Exf tbl_expf(float ze, const float *__restrict__ et, const float *__restrict__ ret) {
-  if (ze <= float(OD_LO)) {
-    float om = ze - 0.5f * ze * ze;
-    float e = 1.f - om;
-    return {e, rcpf(e), om};
-  }
-  float tblind = ze * rcpf(float(BPADE) + ze);
-  int itind = (int)(10000.f * tblind + 0.5f);
-  itind = itind < 0 ? 0 : (itind > NTBL ? NTBL : itind);
-  float e = et[itind];
-  return {e, ret[itind], 1.f - e};
+constexpr double SING_BAND = 2.e-3;   // |1 - k mu0| below which a lane runs the FP64 back
+// 1 - e of a table transmittance e = tbl_exp1(ze): the expansion's ze - ze^2 / 2 below od_lo
+__device__ inline float tbl_deficit(double ze, double e) {
+  return float(ze <= OD_LO ? ze - 0.5 * ze * ze : 1. - e);
 }
-
-struct RT4f {
-  float r, rd, t, td;
-};
-// |1 - k mu0| < 2e-3 for the layer's two-stream eigenvalue k (general branch only; the
-// conservative branch has no such denominator).
-__device__ inline bool near_singular(float zw, float zc, float zg, float mu0) {
-  float zgamma1 = (8.f - zw * (5.f + 3.f * zg)) * 0.25f;
-  float zgamma2 = 3.f * (zw * (1.f - zg)) * 0.25f;
-  float k = __builtin_sqrtf(2.f * zc * (zgamma1 + zgamma2));
-  return fabsf(1.f - k * mu0) < 2.e-3f;
-}
-__device__ inline RT4f reftra_f(float zto1, float zw, float zc, float zg, float prmuz, float rmu,
-                                const float *__restrict__ et, const float *__restrict__ ret) {
-  const float zwcrit = 0.9999995f, eps = 1.e-08f;
-  RT4f o;
-  float zg3 = 3.f * zg;
-  float zgamma1 = (8.f - zw * (5.f + zg3)) * 0.25f;
-  float zgamma2 = 3.f * (zw * (1.f - zg)) * 0.25f;
-  float zgamma3 = (2.f - zg3 * prmuz) * 0.25f;
-  float zgamma4 = 1.f - zgamma3;
-  bool cons;
-  if (zg != 1.f) {
-    float h = zg * rcpf(1.f - zg);
-    float h2 = h * h;
-    float den = 1.f - zc * h2;
-    cons = zw > 0.f && den > 0.f && zc * ((1.f - h2) + (1.f - zwcrit) * h2) <= (1.f - zwcrit);
-  } else {
-    cons = zw >= zwcrit;
+__device__ inline RTm reftra_mixed(const RtFront &f, double zto1, double zw, float zc, double prmuz,
+                                   double e2, float d2) {
+  RTm o;
+  if (f.cons || fabs(1. - f.kmu) < SING_BAND) {
+    const RT4 q = reftra_back(f, zto1, zw, prmuz, e2);
+    o.r = float(q.r), o.rd = float(q.rd), o.t = float(q.t), o.td = float(q.td);
+    o.a = float(1. - q.r - q.t), o.ad = float(1. - q.rd - q.td);
+    return o;
   }
-  if (cons) {
-    float za1 = zgamma1 * prmuz - zgamma3;
-    float zgt = zgamma1 * zto1;
-    Exf e2 = tbl_expf(fminf(zto1 * rmu, 500.f), et, ret);
-    float inv = rcpf(1.f + zgt);
-    o.r = (zgt - za1 * e2.om) * inv;
-    o.t = 1.f - o.r;
-    o.rd = zgt * inv;
-    o.td = 1.f - o.rd;
-    if (e2.e == 1.f) o.r = 0.f, o.t = 1.f, o.rd = 0.f, o.td = 1.f;
-  } else {
-    float za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
-    float za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
-    float zrk = __builtin_sqrtf(2.f * zc * (zgamma1 + zgamma2));
-    float zrp = zrk * prmuz;
-    float zrp1 = 1.f + zrp, zrm1 = 1.f - zrp, zrk2 = 2.f * zrk;
-    float zrpp = 1.f - zrp * zrp;
-    float zr1 = zrm1 * (za2 + zrk * zgamma3);
-    float zr2 = zrp1 * (za2 - zrk * zgamma3);
-    float zr3 = zrk2 * (zgamma3 - za2 * prmuz);
-    float zt1 = zrp1 * (za1 + zrk * zgamma4);
-    float zt2 = zrm1 * (za1 - zrk * zgamma4);
-    float zt3 = zrk2 * (zgamma4 + za1 * prmuz);
-    Exf e1 = tbl_expf(fminf(zrk * zto1, 500.f), et, ret);
-    Exf e2 = tbl_expf(fminf(zto1 * rmu, 500.f), et, ret);
-    // zr4 / e1 + zr5 e1 = zrpp (g1 (1/e1 - e1) + k (1/e1 + e1)), 1/e1 - e1 = (1 - e1)(1 + e1)/e1
-    float zdenr = zrpp * (zgamma1 * (e1.om * (1.f + e1.e) * e1.r) + zrk * (e1.r + e1.e));
-    if (zdenr >= -eps && zdenr <= eps) {
-      o.r = eps;
-      o.t = e2.e;
-    } else {
-      float rden = rcpf(zdenr);
-      o.r = zw * (zr1 * e1.r - zr2 * e1.e - zr3 * e2.e) * rden;
-      o.t = e2.e - e2.e * zw * (zt1 * e1.r - zt2 * e1.e - zt3 * e2.r) * rden;
-    }
-    // (k + g1) - (g1 - k) e1^2 = k (1 + e1^2) + g1 (1 - e1^2)
-    float om2 = e1.om * (1.f + e1.e);
-    float zdend = rcpf(zrk * (1.f + e1.e * e1.e) + zgamma1 * om2);
-    o.rd = zgamma2 * om2 * zdend;
-    o.td = zrk2 * e1.e * zdend;
-  }
+  const float w = float(zw), mu = float(prmuz), k = float(f.k), kmu = float(f.kmu);
+  const float g1 = float(f.g1), g2 = float(f.g2), g3 = float(f.g3), g4 = float(f.g4);
+  const float a1 = g1 * g4 + g2 * g3, a2 = g1 * g3 + g2 * g4;
+  const float d1 = tbl_deficit(fmin(f.k * zto1, 500.), f.e1), e1 = float(f.e1), e2f = float(e2);
+  const float s1 = d1 * (2.f - d1);
+  const float Dp = k * (1.f + e1 * e1) + g1 * s1;
+  const float rDp = rcpf(Dp);
+  const float rD = rcpf(1.f - kmu * kmu) * rDp;
+  const float B2 = (1.f + kmu) * (a2 - k * g3), C2 = 2.f * k * (g3 - a2 * mu);
+  const float B1 = (1.f - kmu) * (a1 - k * g4), C1 = 2.f * k * (g4 + a1 * mu);
+  const float NR = B2 * s1 + C2 * (d1 + d2 - d1 * d2);
+  const float NT = C1 * (d1 - d2) + (1.f - d2) * B1 * s1;
+  o.r = w * NR * rD;
+  const float tdif = -w * NT * rD;
+  o.t = e2f + tdif;
+  o.rd = g2 * s1 * rDp;
+  o.td = 2.f * k * e1 * rDp;
+  o.a = d2 - (o.r + tdif);
+  o.ad = d1 * (k * d1 + 2.f * zc * (2.f - d1)) * rDp;
   return o;
 }
 
+// The elimination step of sweep_r in FP32 (state and coefficients), B = prdnd before the layer.
 struct SweepF {
   float tdbt, ztdn, prdnd;
 };
-__device__ inline void sweep_layer_f(SweepF &s, const RT4f &l, float dbt, float &A, float &B,
-                                     float &Cc, float &E) {
-  A = s.ztdn;
-  B = s.prdnd;
+__device__ inline void sweep_f(SweepF &s, const RTm &l, float dbt, float &Cc, float &E) {
   float a = s.ztdn - s.tdbt;
   float zreflect = rcpf(1.f - l.rd * s.prdnd);
   Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
@@ -1095,6 +1078,37 @@ __device__ inline void sweep_layer_f(SweepF &s, const RT4f &l, float dbt, float 
   s.tdbt = dbt * s.tdbt;
   s.ztdn = ztdn;
   s.prdnd = prdnd;
+}
+
+// Wavefront sums in FP32 (the DPP pattern of wave_sum4d / wave_sum2d, one 32-bit move a value).
+template <int CTRL>
+__device__ inline float dpp_f(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+__device__ inline float cross_rows_f(float x) {
+  x += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x401F));
+  auto p = __builtin_amdgcn_permlane32_swap((unsigned)__float_as_int(x), (unsigned)__float_as_int(x),
+                                            false, false);
+  return __int_as_float((int)p[0]) + __int_as_float((int)p[1]);
+}
+__device__ inline float wave_sum4f(float v0, float v1, float v2, float v3, int lane) {
+  const bool b3 = lane & 8, b2 = lane & 4;
+  float ka = b3 ? v2 : v0, kb = b3 ? v3 : v1;
+  float sa = b3 ? v0 : v2, sb = b3 ? v1 : v3;
+  float w0 = ka + dpp_f<DPP_MIRROR>(sa);
+  float w1 = kb + dpp_f<DPP_MIRROR>(sb);
+  float x = (b2 ? w1 : w0) + dpp_f<DPP_HALF_MIRROR>(b2 ? w0 : w1);
+  x += dpp_f<DPP_XOR1>(x);
+  x += dpp_f<DPP_XOR2>(x);
+  return cross_rows_f(x);
+}
+__device__ inline float wave_sum2f(float v0, float v1, int lane) {
+  const bool b3 = lane & 8;
+  float x = (b3 ? v1 : v0) + dpp_f<DPP_MIRROR>(b3 ? v0 : v1);
+  x += dpp_f<DPP_HALF_MIRROR>(x);
+  x += dpp_f<DPP_XOR1>(x);
+  x += dpp_f<DPP_XOR2>(x);
+  return cross_rows_f(x);
 }
 
 // ------------------------------------------------------------------------------------------

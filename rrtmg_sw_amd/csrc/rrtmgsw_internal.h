@@ -71,7 +71,6 @@ struct GPoint {
 struct DeviceTables {
   const double *tab;        // k-tables, solar tables, cloud optics
   const double *exp_tbl;    // NTBL+1 entries
-  const float *tab32, *exp32, *rexp32;   // FP32 copies of tab, exp_tbl and 1 / exp_tbl (mixed form)
   const GPoint *gp;         // NGPT records
   const double *pthr;       // NPTHR pressure thresholds of setcoef_sw's index decisions
   // cloud optics offsets (tables (n, 16:29) column-major: value(i, band) at off + (i-1) + n*(band-16))
@@ -136,7 +135,6 @@ struct RtArgs {
   double *swuflx, *swdflx, *swhr, *swuflxc, *swdflxc, *swhrc;
   double *scratch;               // per (column, layer, coefficient, g) elimination coefficients
   int *status;                   // first error code raised inside a kernel
-  double thin_pdp;               // mixed form: layers thinner than this (hPa) run in FP64
   SwScalars sc;
 };
 

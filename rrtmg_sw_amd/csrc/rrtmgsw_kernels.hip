@@ -59,6 +59,98 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Pass 2 and epilogue of the mixed form (FP32 state and coefficients, FP32 sums over g).
+// Level j (top-based) gets the sums of zinc U, zinc B U (clear and total) and layer j the rest
+// of its absorbed flux, zinc ad (B_j U_j + U_(j+1)); the heating rate of a layer is heatfac / pdp
+// times its absorbed flux (pass 1's part + this one) summed over g: the layer's net-flux
+// divergence formed without the difference of two nearly equal fluxes.
+// ------------------------------------------------------------------------------------------
+__device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const double *scr, int gi,
+                                      int lane, int wave, int ibnd, float zinc, int jfirst,
+                                      const SweepF &scf, const SweepF &stf) {
+  const int L = a.nlay, C = a.ncol;
+  const bool vis = ibnd >= 9 && ibnd <= 12;
+  const float albdir = float(vis ? a.asdir[c] : a.aldir[c]);
+  const float albdif = float(vis ? a.asdif[c] : a.aldif[c]);
+  float uc = (albdir * scf.tdbt + albdif * (scf.ztdn - scf.tdbt)) * rcpf(1.f - albdif * scf.prdnd);
+  float ut = (albdir * stf.tdbt + albdif * (stf.ztdn - stf.tdbt)) * rcpf(1.f - albdif * stf.prdnd);
+  {
+    const float dc = scf.ztdn + scf.prdnd * uc, dt = stf.ztdn + stf.prdnd * ut;
+    const float s = wave_sum4f(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
+    float *r = v.redf + (wave * (L + 1) + L) * 6;
+    if (lane < 16 && (lane & 3) == 0) r[lane >> 2] = s;
+    if (lane == 0) r[4] = 0.f, r[5] = 0.f;
+  }
+  struct Co {
+    float cB, cC, cE, cA, tB, tC, tE, tA;
+  };
+  auto fetch = [&](int j) {
+    Co o;
+    const int jj = j < 0 ? 0 : j;
+    const float *p = reinterpret_cast<const float *>(scr - gi + (size_t)jj * SCR_COEFS * NGPT) + gi;
+    o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT], o.cA = p[3 * NGPT];
+    o.tB = 0.f, o.tC = 0.f, o.tE = 0.f, o.tA = 0.f;
+    if (jj >= jfirst) o.tB = p[4 * NGPT], o.tC = p[5 * NGPT], o.tE = p[6 * NGPT], o.tA = p[7 * NGPT];
+    return o;
+  };
+  auto step = [&](int j, const Co &o) {
+    const bool tot = j >= jfirst;
+    const float ub_t = ut, ub_c = uc;   // upward flux at the layer's bottom
+    ut = tot ? o.tC + o.tE * ut : o.cC + o.cE * ut;
+    const float bu_t = (tot ? o.tB : o.cB) * ut, ad_t = tot ? o.tA : o.cA;
+    uc = o.cC + o.cE * uc;
+    const float bu_c = o.cB * uc;
+    const float s4 = wave_sum4f(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
+    const float s2 = wave_sum2f(zinc * (ad_t * (bu_t + ub_t)), zinc * (o.cA * (bu_c + ub_c)), lane);
+    float *r = v.redf + (wave * (L + 1) + j) * 6;
+    if (lane < 16 && (lane & 3) == 0) r[lane >> 2] = s4;
+    if (lane == 0 || lane == 8) r[4 + (lane >> 3)] = s2;
+  };
+  Co r0 = fetch(L - 1), r1 = fetch(L - 2), r2 = fetch(L - 3), r3 = fetch(L - 4);
+  int j = L - 1;
+  for (; j >= 3; j -= 4) {
+    step(j, r0);
+    r0 = fetch(j - 4);
+    step(j - 1, r1);
+    r1 = fetch(j - 5);
+    step(j - 2, r2);
+    r2 = fetch(j - 6);
+    step(j - 3, r3);
+    r3 = fetch(j - 7);
+  }
+  if (j >= 0) step(j, r0);
+  if (j >= 1) step(j - 1, r1);
+  if (j >= 2) step(j - 2, r2);
+  __syncthreads();
+  // epilogue: fluxes at the nlay+1 levels; heating rates from the layers' absorbed fluxes
+  const int tid = threadIdx.x;
+  for (int jl = tid; jl <= L; jl += blockDim.x) {
+    const float *q0 = v.redf + (0 * (L + 1) + jl) * 6, *q1 = v.redf + (1 * (L + 1) + jl) * 6;
+    double fu = double(q0[0]) + q1[0], fd = double(q0[1]) + q1[1];
+    double cu = double(q0[2]) + q1[2], cd = double(q0[3]) + q1[3];
+    if (jl < L) {
+      const float *p0 = v.redaf + (0 * L + jl) * 4, *p1 = v.redaf + (1 * L + jl) * 4;
+      fd += double(p0[0]) + p1[0];
+      cd += double(p0[1]) + p1[1];
+    }
+    const size_t q = c + (size_t)C * (L - jl);
+    a.swuflx[q] = fu, a.swdflx[q] = fd, a.swuflxc[q] = cu, a.swdflxc[q] = cd;
+  }
+  for (int i = tid; i < L; i += blockDim.x) {
+    const int jl = L - 1 - i;   // layer i (bottom-based) is layer jl from the top
+    const float *q0 = v.redf + (0 * (L + 1) + jl) * 6, *q1 = v.redf + (1 * (L + 1) + jl) * 6;
+    const float *p0 = v.redaf + (0 * L + jl) * 4, *p1 = v.redaf + (1 * L + jl) * 4;
+    const double abs_t = (double(p0[2]) + p1[2]) + (double(q0[4]) + q1[4]);
+    const double abs_c = (double(p0[3]) + p1[3]) + (double(q0[5]) + q1[5]);
+    const double pdp = a.plev[c + (size_t)C * i] - a.plev[c + (size_t)C * (i + 1)];
+    const double zdpgcp = a.sc.heatfac / pdp;
+    const size_t q = c + (size_t)C * i;
+    a.swhr[q] = i == L - 1 ? 0.0 : abs_t * zdpgcp;
+    a.swhrc[q] = i == L - 1 ? 0.0 : abs_c * zdpgcp;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // FP64 forms: MODE 0 = the reference's exponential table, MODE 2 = exact exponentials (the
 // smooth forward the adjoint differentiates). CLD, where the clouds come from:
 //   CLD_SUBCOL  the materialised (112, ncol, nlay) McICA sub-column arrays (rrtmg_sw), cloud
@@ -90,7 +182,8 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   constexpr bool LAYERCLD = CLD == CLD_LAYER;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
-  ColLayout lo(L, FUSED ? (L + 31) / 32 : 0);
+  static_assert(!MIXED || CLD == CLD_FUSED, "the mixed form is the fused entry's");
+  ColLayout lo(L, FUSED ? (L + 31) / 32 : 0, true, MIXED);
   ColView v(smem, lo);
   // XCD-aware column order: consecutive columns run on the same XCD (its own L2), so the
   // strided (ncol, nlay) reads and writes of neighbouring columns share cache lines.
@@ -107,15 +200,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
 
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
   if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0, v.misc[M_LAYLOW] = 0;
-  if (MIXED)
-    for (int w = tid; w < (L + 31) / 32; w += blockDim.x) v.thin[w] = 0u;
   __syncthreads();
-  if (MIXED)   // layers (top-based j) thin enough in pressure to need FP64 flux divergences
-    for (int j = tid; j < L; j += blockDim.x) {
-      const int lay = L - 1 - j;
-      const double pdp = a.plev[c + (size_t)C * lay] - a.plev[c + (size_t)C * (lay + 1)];
-      if (pdp < a.thin_pdp) atomicOr(&v.thin[j >> 5], 1u << (j & 31));
-    }
   for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
   __syncthreads();
   const int laytrop = v.misc[M_LAYTROP];
@@ -194,20 +279,25 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   // stays identical to the clear sky until its first cloudy layer (jfirst): above it the
   // total-sky sweep is not computed and nothing is stored for it.
   Sweep sc_ = {1.0, 1.0, 0.0}, st_ = {1.0, 1.0, 0.0};
+  // the mixed form's FP32 sweep state (clear, total)
+  SweepF scf = {1.f, 1.f, 0.f}, stf = {1.f, 1.f, 0.f};
   int jfirst = L;
   double *scr = a.scratch + (size_t)cl * scr_col_doubles(L) + gi;
   // A layer's coefficients are stored once the next layer's table reads have been issued, so that
   // the waits for those reads do not include the stores (the vector memory counter retires in
-  // issue order). The mixed form stores its thick layers' coefficients as FP32.
+  // issue order). The mixed form stores FP32 B, C, E and the diffuse absorptance ad per sky in
+  // the same slots (8 floats in the 6 doubles of a layer).
   double pend[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  float pendf[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int pend_j = -1;
-  bool pend_t = false, pend_f = false;
+  bool pend_t = false;
   auto flush = [&]() {
     if (active && pend_j >= 0) {
-      if (MIXED && pend_f) {
+      if constexpr (MIXED) {
         float *p = reinterpret_cast<float *>(scr - gi + (size_t)pend_j * SCR_COEFS * NGPT) + gi;
-        p[0] = float(pend[0]), p[NGPT] = float(pend[1]), p[2 * NGPT] = float(pend[2]);
-        if (pend_t) p[3 * NGPT] = float(pend[3]), p[4 * NGPT] = float(pend[4]), p[5 * NGPT] = float(pend[5]);
+        p[0] = pendf[0], p[NGPT] = pendf[1], p[2 * NGPT] = pendf[2], p[3 * NGPT] = pendf[3];
+        if (pend_t)
+          p[4 * NGPT] = pendf[4], p[5 * NGPT] = pendf[5], p[6 * NGPT] = pendf[6], p[7 * NGPT] = pendf[7];
       } else {
         double *p = scr + (size_t)pend_j * SCR_COEFS * NGPT;
         p[0] = pend[0], p[NGPT] = pend[1], p[2 * NGPT] = pend[2];
@@ -344,84 +434,65 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     }
     pend[0] = Bc, pend[1] = Cc, pend[2] = Ec, pend[3] = Bt, pend[4] = Ct, pend[5] = Et;
   };
-  // The mixed form's FP32 layer (thick layers): optics with the co-albedo carried explicitly,
-  // reftra_f and the FP32 elimination step on the FP64 sweep state. Where k mu0 is close to one
-  // the two-stream solution divides by 1 - (k mu0)^2, a removable singularity that turns the
-  // exponential table's steps into a large reflectance: such lanes redo the layer in FP64 from
-  // FP64 gas optics.
-  auto body32 = [&](auto lower_tag, const GasLane &q, int j, float taugf, float taurf, const LayIn &in,
-                    double &At, double &Ac) {
-    constexpr bool LOWER = decltype(lower_tag)::value;
-    const int lay = L - 1 - j;
-    const float taf = float(in.ta), waf = float(in.wa), gaf = float(in.ga);
-    float tauc = taurf + taugf + taf;
-    const float scat = taurf + taf * waf;
-    const float coal = taugf + taf * (1.f - waf);
-    float gcc = gaf * waf * taf * rcpf(scat);
-    const float rtau = rcpf(tauc);
-    float omc = scat * rtau, ccl = coal * rtau;
-    const float zf = gcc * gcc, zwf = omc * zf;
-    const float r1 = rcpf(1.f - zwf);
-    tauc = (1.f - zwf) * tauc;
-    omc = (omc - zwf) * r1;
-    ccl = ccl * r1;
-    gcc = gcc * rcpf(1.f + gcc);   // (g - g^2) / (1 - g^2)
-    const float mu0f = float(mu0);
-    bool sing = near_singular(omc, ccl, gcc, mu0f);
-    const float ctf = float(in.ct), cwf = float(in.cw), cgf = float(in.cg), ccw = float(1.0 - in.cw);
-    float tauo = 0.f, omo = 0.f, cco = 0.f, go = 0.f;
-    if (in.cf > REPCLC) {
-      tauo = tauc + ctf;
-      const float scato = tauc * omc + ctf * cwf;
-      const float coalo = tauc * ccl + ctf * ccw;
-      go = (ctf * cwf * cgf + tauc * omc * gcc) * rcpf(scato);
-      const float rto = rcpf(tauo);
-      omo = scato * rto, cco = coalo * rto;
-      sing = sing || near_singular(omo, cco, go, mu0f);
-    }
-    if (sing) {
-      double taug, taur;
-      gas_finish<LOWER, double>(q, gas_fetch<LOWER, double>(q, T.tab, v, lay), v, lay, taug, taur);
-      body64(j, taug, taur, in, At, Ac);
-      return;
-    }
-    const float rmuf = float(rmu);
-    const RT4f rc = reftra_f(tauc, omc, ccl, gcc, mu0f, rmuf, T.exp32, T.rexp32);
-    const float dbc = tbl_expf(tauc * rmuf, T.exp32, T.rexp32).e;
-    Ac = sc_.ztdn;
-    const double Bc = sc_.prdnd;
-    if (in.cf != 0.0 && jfirst == L) {
+  // The mixed form's layer: the FP64 front of body64 (delta-scaled optics, reftra_front, every
+  // table lookup), the FP32 back (reftra_mixed) and the FP32 elimination step. Besides the
+  // coefficients it leaves in pendf the layer's absorbed flux known in pass 1,
+  // P = a S + ad (ztdn - S) (S = tdbt, the direct beam at the layer top), per sky; pass 2 adds
+  // ad (B U_top + U_bottom). Returns ztdn (total, clear) at the layer top.
+  auto bodymx = [&](int j, double taug, double taur, const LayIn &in, float &At, float &Ac, float &Pt,
+                    float &Pc) {
+    const double ta = in.ta, wa = in.wa, ga = in.ga, cf = in.cf, ct = in.ct, cw = in.cw, cg = in.cg;
+    const double tau0 = taur + taug + ta;
+    const double sct = taur * 1.0 + ta * wa;
+    const double r1 = frcp(sct * tau0);
+    const double g0 = ga * wa * ta * tau0 * r1;
+    const double w0 = sct * sct * r1;
+    const double zwf = w0 * (g0 * g0);
+    const double r2 = frcp((1.0 - zwf) * (1.0 + g0));
+    const double tauc = (1.0 - zwf) * tau0;
+    const double omc = (w0 - zwf) * (1.0 + g0) * r2;
+    const double gcc = g0 * (1.0 - zwf) * r2;
+    // co-albedo 1 - omc = (1 - w0)(1 + g0) r2 with 1 - w0 = (taug + ta (1 - wa)) / tau0
+    const double ccl = (taug + ta * (1.0 - wa)) * sct * r1 * (1.0 + g0) * r2;
+    const double ze2 = fmin(tauc * rmu, 500.);
+    const double e2 = tbl_exp1(ze2, T.exp_tbl);
+    const RTm rc = reftra_mixed(reftra_front(tauc, omc, gcc, mu0, T.exp_tbl), tauc, omc, float(ccl), mu0,
+                                e2, tbl_deficit(ze2, e2));
+    const float dbc = float(e2);
+    Ac = scf.ztdn;
+    Pc = rc.a * scf.tdbt + rc.ad * (scf.ztdn - scf.tdbt);
+    const float Bc = scf.prdnd;
+    if (cf != 0.0 && jfirst == L) {
       jfirst = j;
-      st_ = sc_;
+      stf = scf;
     }
-    SweepF sf = {float(sc_.tdbt), float(sc_.ztdn), float(sc_.prdnd)};
-    float Af, Bf, Cc, Ec;
-    sweep_layer_f(sf, rc, dbc, Af, Bf, Cc, Ec);
-    sc_ = {double(sf.tdbt), double(sf.ztdn), double(sf.prdnd)};
-    At = Ac;
-    double Bt = Bc;
-    float Ct = Cc, Et = Ec;
+    float Cc, Ec;
+    sweep_f(scf, rc, dbc, Cc, Ec);
+    At = Ac, Pt = Pc;
+    float Bt = Bc, Ct = Cc, Et = Ec, adt = rc.ad;
     if (jfirst <= j) {
-      At = st_.ztdn, Bt = st_.prdnd;
-      RT4f rt = rc;
+      At = stf.ztdn, Bt = stf.prdnd;
+      RTm rt = rc;
       float dbt = dbc;
-      if (in.cf != 0.0) {
-        const float dbo = tbl_expf(tauo * rmuf, T.exp32, T.rexp32).e;
-        RT4f ro = {0.f, 0.f, 1.f, 1.f};
-        if (in.cf > REPCLC) ro = reftra_f(tauo, omo, cco, go, mu0f, rmuf, T.exp32, T.rexp32);
-        const float zcl = float(1.0 - in.cf), cff = float(in.cf);
-        rt.r = zcl * rc.r + cff * ro.r;
-        rt.rd = zcl * rc.rd + cff * ro.rd;
-        rt.t = zcl * rc.t + cff * ro.t;
-        rt.td = zcl * rc.td + cff * ro.td;
-        dbt = zcl * dbc + cff * dbo;
+      if (cf != 0.0) {   // a cloudy sub-column cell (cf = 1): the total sky is the cloudy one
+        const double tauo = tauc + ct;
+        const double so = tauc * omc + ct * cw;
+        const double r3 = frcp(so * tauo);
+        const double go = (ct * cw * cg + tauc * omc * gcc) * tauo * r3;
+        const double omo = so * so * r3;
+        const double cco = (tauc * ccl + ct * (1.0 - cw)) * so * r3;   // 1 - omo
+        const double zo = fmin(tauo * rmu, 500.);
+        const double e2o = tbl_exp1(zo, T.exp_tbl);
+        rt = reftra_mixed(reftra_front(tauo, omo, go, mu0, T.exp_tbl), tauo, omo, float(cco), mu0, e2o,
+                          tbl_deficit(zo, e2o));
+        dbt = float(e2o);
       }
-      SweepF tf = {float(st_.tdbt), float(st_.ztdn), float(st_.prdnd)};
-      float Atf, Btf;
-      sweep_layer_f(tf, rt, dbt, Atf, Btf, Ct, Et);
-      st_ = {double(tf.tdbt), double(tf.ztdn), double(tf.prdnd)};
+      Pt = rt.a * stf.tdbt + rt.ad * (stf.ztdn - stf.tdbt);
+      sweep_f(stf, rt, dbt, Ct, Et);
+      adt = rt.ad;
     }
-    pend[0] = Bc, pend[1] = Cc, pend[2] = Ec, pend[3] = Bt, pend[4] = Ct, pend[5] = Et;
+    pendf[0] = Bc, pendf[1] = Cc, pendf[2] = Ec, pendf[3] = rc.ad;
+    pendf[4] = Bt, pendf[5] = Ct, pendf[6] = Et, pendf[7] = adt;
   };
   // column program: the per-g level terms of the per-band fluxes, and the direct beam without
   // delta scaling (src/rrtmg_sw_spcvmc.f90:449-470: the table transmittance of the unscaled
@@ -437,7 +508,20 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   auto layer = [&](auto lower_tag, const GasLane &q, int j) {
     constexpr bool LOWER = decltype(lower_tag)::value;
     const int lay = L - 1 - j;
-    const bool thin = MIXED && ((v.thin[j >> 5] >> (j & 31)) & 1u);   // uniform over the workgroup
+    if constexpr (MIXED) {
+      const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
+      flush();
+      double taug, taur;
+      gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
+      float At, Ac, Pt, Pc;
+      bodymx(j, taug, taur, inputs(lay), At, Ac, Pt, Pc);
+      pend_j = j, pend_t = jfirst <= j;
+      // sums over g of the incident-flux-weighted ztdn and pass-1 absorbed flux, both skies
+      const float zf = float(zinc);
+      const float sa = wave_sum4f(zf * At, zf * Ac, zf * Pt, zf * Pc, lane);
+      if (lane < 16 && (lane & 3) == 0) v.redaf[(wave * L + j) * 4 + (lane >> 2)] = sa;
+      return;
+    }
     double At, Ac;
     if (SPEC) {
       const double Tdir = jfirst <= j ? st_.tdbt : sc_.tdbt;   // total-sky beam at level j
@@ -452,12 +536,6 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       const double dmc = tbl_exp1(tau0 / mu0, T.exp_tbl);
       const double dmo = tbl_exp1((tau0 + in.co) / mu0, T.exp_tbl);
       tnd = (in.cf != 0.0 ? (1.0 - in.cf) * dmc + in.cf * dmo : dmc) * tnd;
-    } else if (MIXED && !thin) {
-      const GasFetch<float> gf = gas_fetch<LOWER, float>(q, T.tab32, v, lay);
-      flush();
-      float taugf, taurf;
-      gas_finish<LOWER, float>(q, gf, v, lay, taugf, taurf);
-      body32(lower_tag, q, j, taugf, taurf, inputs(lay), At, Ac);
     } else {
       const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
       flush();
@@ -465,7 +543,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
       body64(j, taug, taur, inputs(lay), At, Ac);
     }
-    pend_j = j, pend_t = jfirst <= j, pend_f = MIXED && !thin;
+    pend_j = j, pend_t = jfirst <= j;
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
     const double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
     if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
@@ -481,9 +559,13 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   regime(std::integral_constant<bool, false>(), 0, jsplit);
   regime(std::integral_constant<bool, true>(), jsplit, L);
   flush();
-  if (jfirst == L) st_ = sc_;
+  if (jfirst == L) st_ = sc_, stf = scf;
   spec_level(L, st_.ztdn, st_.tdbt);
   if (err) atomicCAS(a.status, 0, err);
+  if constexpr (MIXED) {
+    rt_pass2_mixed(a, v, c, scr, gi, lane, wave, ibnd, float(zinc), jfirst, scf, stf);
+    return;
+  }
 
   // ---- pass 2: bottom to top ----
   // surface (src/rrtmg_sw_vrtqdr.f90:163-169 with prup = albdir, prupd = albdif)
@@ -517,15 +599,9 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     Co o;
     const int jj = j < 0 ? 0 : j;
     o.tB = 0.0, o.tC = 0.0, o.tE = 0.0;
-    if (MIXED && !((v.thin[jj >> 5] >> (jj & 31)) & 1u)) {   // FP32 coefficients (uniform branch)
-      const float *p = reinterpret_cast<const float *>(scr - gi + (size_t)jj * SCR_COEFS * NGPT) + gi;
-      o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
-      if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
-    } else {
-      const double *p = scr + (size_t)jj * SCR_COEFS * NGPT;
-      o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
-      if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
-    }
+    const double *p = scr + (size_t)jj * SCR_COEFS * NGPT;
+    o.cB = p[0], o.cC = p[NGPT], o.cE = p[2 * NGPT];
+    if (jj >= jfirst) o.tB = p[3 * NGPT], o.tC = p[4 * NGPT], o.tE = p[5 * NGPT];
     return o;
   };
   auto step = [&](int j, const Co &o) {
@@ -564,7 +640,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
 }
 
 template <int MODE, int CLD>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MODE == 0 && CLD != CLD_SUBCOL ? 4 : 3)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu((MODE == 0 && CLD != CLD_SUBCOL) || MODE == 1 ? 4 : 3)))
 rt_kernel(RtArgs a, DeviceTables T) {
   rt_body<MODE, CLD, false>(a, T);
 }

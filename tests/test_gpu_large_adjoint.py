@@ -61,8 +61,6 @@ def test_large_forward_against_oracle(gpu_lib, big):
         assert max(e[k] for k in FLUX) <= ftol and max(e[k] for k in HR) <= htol, (prec, e)
 
 
-@pytest.mark.xfail(strict=True, reason="the mixed form misses the bar at 137 layers (0.31 W/m2, "
-                   "0.19 K/day measured); its rebuild is the fix")
 def test_large_mixed_against_fp64(gpu_lib, big):
     import torch
     api = gpu_lib
