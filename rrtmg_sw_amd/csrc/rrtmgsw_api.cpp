@@ -59,7 +59,14 @@ struct Context {
   std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork;
   std::map<hipStream_t, std::unique_ptr<std::mutex>> calls;   // one call at a time per stream
   std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
-  uint32_t *d_mtjump = nullptr;   // MT19937 jump-ahead: set-bit positions of x^J mod P, on first use
+  // MT19937 jump-ahead tables by segment length (log2): the set bits of x^(k J) mod P for
+  // k = 1 .. count, built on first use and extended when a longer stream needs more segments
+  struct MtJump {
+    int count = 0;
+    uint32_t *bits = nullptr, *offs = nullptr;
+  };
+  std::map<int, MtJump> mtjump;
+  std::vector<void *> retired;    // tables replaced by longer ones: a call on another stream may still read them
   std::map<int, char *> noalloc;  // device -> address of `scratch` on that device
   std::mutex noalloc_mu;          // calls of the no-alloc entry share their GPU's array: one at a time
 } ctx;
@@ -481,8 +488,10 @@ int rrtmgsw_finalize(void) {
   }
   for (auto &kv : ctx.jumps) hipFree(kv.second);
   ctx.jumps.clear();
-  if (ctx.d_mtjump) hipFree(ctx.d_mtjump);
-  ctx.d_mtjump = nullptr;
+  for (auto &kv : ctx.mtjump) hipFree(kv.second.bits), hipFree(kv.second.offs);
+  ctx.mtjump.clear();
+  for (void *p : ctx.retired) hipFree(p);
+  ctx.retired.clear();
   ctx.ready = false;
   return 0;
 }
@@ -1008,29 +1017,43 @@ static int mcica_prepare(int ncol, int nlay, int icld, int permuteseed, int irng
   if (irng == 0) return kiss_jumps(permuteseed, nlay, per_sub, s, jumps);
   long long ndraw = (long long)NGPT * ncol * (icld == 3 ? 1 : (icld >= 4 ? 2LL * nlay : nlay));
   void *buf = nullptr;
-  const uint32_t *coef = nullptr;
+  MtJumpTable jt{};
   const char *seq = std::getenv("RRTMGSW_MT_SEQUENTIAL");   // 1: one workgroup, no jump-ahead
   const bool jump = mt_extra_words(ndraw) > 0 && !(seq && std::atoi(seq) != 0);
   {
     std::lock_guard<std::mutex> lk(ctx.mu);
     if (jump) {
-      if (!ctx.d_mtjump) {
-        // the set bits of x^J mod P as a list {n, i_1, ..., i_n}
-        std::vector<uint32_t> c = mt_jump_coefficients(MT_LOG2_SEG);
-        if (c.size() != 624) return fail(RRTMGSW_ERR_ARGUMENT, "MT19937 jump-ahead polynomial not found");
-        std::vector<uint32_t> list(1, 0);
-        for (uint32_t i = 0; i < 19937; i++)
-          if ((c[i >> 5] >> (i & 31)) & 1u) list.push_back(i);
-        list[0] = (uint32_t)(list.size() - 1);
-        HIPCHK(hipMalloc(&ctx.d_mtjump, list.size() * sizeof(uint32_t)));
-        HIPCHK(hipMemcpy(ctx.d_mtjump, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      const int lj = mt_log2_seg(ndraw);
+      const long long J = 1LL << lj;
+      const int need = (int)((ndraw + J - 1) / J) - 1;
+      Context::MtJump &t = ctx.mtjump[lj];
+      if (t.count < need) {
+        // x^(k J) mod P for every segment start k, as lists of set bits with offsets
+        const int count = std::max(need, 2 * t.count);
+        std::vector<uint32_t> c = mt_jump_multiples(lj, count);
+        if (c.size() != (size_t)count * 624)
+          return fail(RRTMGSW_ERR_ARGUMENT, "MT19937 jump-ahead polynomial not found");
+        std::vector<uint32_t> bits, offs(1, 0);
+        for (int k = 0; k < count; k++) {
+          const uint32_t *w = c.data() + (size_t)k * 624;
+          for (uint32_t i = 0; i < 19937; i++)
+            if ((w[i >> 5] >> (i & 31)) & 1u) bits.push_back(i);
+          offs.push_back((uint32_t)bits.size());
+        }
+        if (t.bits) ctx.retired.push_back(t.bits), ctx.retired.push_back(t.offs);
+        t.bits = t.offs = nullptr, t.count = 0;
+        HIPCHK(hipMalloc(&t.bits, bits.size() * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&t.offs, offs.size() * sizeof(uint32_t)));
+        HIPCHK(hipMemcpy(t.bits, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.offs, offs.data(), offs.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        t.count = count;
       }
-      coef = ctx.d_mtjump;
+      jt.log2_seg = lj, jt.count = t.count, jt.bits = t.bits, jt.offs = t.offs;
     }
     const size_t words = (size_t)ndraw + (jump ? mt_extra_words(ndraw) : 0);
     if (int rc = stream_buffer(ctx.mtbuf, s, words * 4, &buf)) return rc;
   }
-  HIPCHK(launch_mt_stream((uint32_t)permuteseed, ndraw, (uint32_t *)buf, coef, s));
+  HIPCHK(launch_mt_stream((uint32_t)permuteseed, ndraw, (uint32_t *)buf, jump ? &jt : nullptr, s));
   *mt = (const uint32_t *)buf;
   return 0;
 }

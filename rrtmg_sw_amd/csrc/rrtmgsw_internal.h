@@ -177,12 +177,19 @@ hipError_t launch_rt_spectral(const RtArgs &a, const DeviceTables &t, hipStream_
 hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s);
 hipError_t launch_setcoef(const RtArgs &a, const SetcoefOut &o, const DeviceTables &t, hipStream_t s);
-// MT19937 stream of ndraw tempered words to out. With coef (device: {n, i_1, ..., i_n}, the set
-// bits of mt_jump_coefficients(MT_LOG2_SEG)) and ndraw > 2^(MT_LOG2_SEG + 1), out must hold
-// ndraw + mt_extra_words(ndraw) words: segment start states go after the stream.
-constexpr int MT_LOG2_SEG = 21;
+// The set bits of x^(k J) mod P (J = 2^log2_seg, P the MT19937 characteristic polynomial) for
+// k = 1 .. count, on the device: bits[offs[k - 1] .. offs[k]).
+struct MtJumpTable {
+  int log2_seg, count;
+  const uint32_t *bits, *offs;
+};
+// MT19937 stream of ndraw tempered words to out, in segments of 2^mt_log2_seg(ndraw) words when
+// jt (for that segment length, count >= segments - 1) is given and the stream holds more than two;
+// then out must hold ndraw + mt_extra_words(ndraw) words: segment start states go after the stream.
+int mt_log2_seg(long long ndraw);
 size_t mt_extra_words(long long ndraw);
-hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const uint32_t *coef, hipStream_t s);
-std::vector<uint32_t> mt_jump_coefficients(int log2_jump);   // host, rrtmgsw_mtjump.cpp
+hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const MtJumpTable *jt, hipStream_t s);
+std::vector<uint32_t> mt_jump_coefficients(int log2_jump);          // host, rrtmgsw_mtjump.cpp
+std::vector<uint32_t> mt_jump_multiples(int log2_jump, int count);  // host, rrtmgsw_mtjump.cpp
 
 }  // namespace rrtmgsw

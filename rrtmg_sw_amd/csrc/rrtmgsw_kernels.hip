@@ -807,8 +807,8 @@ hipError_t launch_mcica(const McicaArgs &a, const DeviceTables &t, hipStream_t s
 
 // ------------------------------------------------------------------------------------------
 // Mersenne Twister MT19937 stream (src/mcica_random_numbers.f90:77-306): the single stream of a
-// call, generated in order by one workgroup, or for long streams in segments of 2^MT_LOG2_SEG
-// words, one workgroup each, from start states found by jump-ahead.
+// call, generated in order by one workgroup, or for long streams in segments of 2^19 words or
+// more, one workgroup each, from start states found by jump-ahead.
 // ------------------------------------------------------------------------------------------
 // Barrier that orders this workgroup's LDS accesses only: the wave's LDS operations complete,
 // then all waves meet. Unlike __syncthreads it does not wait for the wave's global stores, which
@@ -868,73 +868,75 @@ __global__ void __launch_bounds__(256) mt_kernel(uint32_t seed, long long ndraw,
   mt_ring_run(ring, ndraw, out);
 }
 
-// Jump-ahead (x^J mod P, coefficients c_i from the host): the 624-word window J = 2^MT_LOG2_SEG
-// raw words ahead is the XOR of the windows i ahead over the set bits c_i, i < 19937. One
-// workgroup walks the segment starts: from a start window it extends the raw words to
-// 624 + 19936, forms the next start (thread w owns word w), and writes every start to `starts`.
+// Jump-ahead (Haramoto et al. 2008): with x^(k J) = sum_i c_i x^i mod P (the set bits i listed by
+// the host), the 624-word window k J raw words ahead is the XOR of the windows i ahead. Every
+// segment start is formed at once from the stream's own start: workgroup k - 1 extends the raw
+// words from the seed to 624 + 19936, then forms start k (thread w owns word w).
 constexpr int MT_EXT = 624 + 19936;   // raw words a jump reads
-// coef = {n, i_1, ..., i_n}: the positions of the set bits c_i, each below 19937.
-__global__ void __launch_bounds__(640) mt_chain_kernel(uint32_t seed, int nseg,
-                                                       const uint32_t *__restrict__ coef, uint32_t *starts) {
+// bits[offs[k - 1] .. offs[k]) are the set bits of x^(k J) mod P, each below 19937.
+__global__ void __launch_bounds__(640) mt_starts_kernel(uint32_t seed, const uint32_t *__restrict__ bits,
+                                                        const uint32_t *__restrict__ offs, uint32_t *starts) {
   extern __shared__ __attribute__((aligned(16))) uint32_t msm[];
   uint32_t *buf = msm;
-  const int t = threadIdx.x;
-  const int nb = (int)coef[0];
-  const uint32_t *bits = coef + 1;
+  const int t = threadIdx.x, k = blockIdx.x + 1;
   if (t == 0) mt_init(seed, buf);
   __syncthreads();
-  for (int i = t; i < 624; i += blockDim.x) starts[i] = buf[i];
-  for (int k = 1; k < nseg; k++) {
-    for (int r0 = 624; r0 < MT_EXT; r0 += 227) {
-      const int r = r0 + t;
-      if (t < 227 && r < MT_EXT) buf[r] = buf[r - 227] ^ mt_twist(buf[r - 624], buf[r - 623]);
-      __syncthreads();
-    }
-    uint32_t acc = 0;
-    if (t < 624) {
-      // eight independent LDS reads in flight per step; i <= 19936, so i + t stays below MT_EXT
-      int j = 0;
-      for (; j + 8 <= nb; j += 8) {
-        uint32_t v = 0;
+  if (k == 1)
+    for (int i = t; i < 624; i += blockDim.x) starts[i] = buf[i];
+  for (int r0 = 624; r0 < MT_EXT; r0 += 227) {
+    const int r = r0 + t;
+    if (t < 227 && r < MT_EXT) buf[r] = buf[r - 227] ^ mt_twist(buf[r - 624], buf[r - 623]);
+    __syncthreads();
+  }
+  if (t < 624) {
+    // eight independent LDS reads in flight per step; i <= 19936, so i + t stays below MT_EXT
+    const uint32_t b0 = offs[k - 1], b1 = offs[k];
+    uint32_t acc = 0, j = b0;
+    for (; j + 8 <= b1; j += 8) {
+      uint32_t v = 0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) v ^= buf[bits[j + q] + t];
-        acc ^= v;
-      }
-      for (; j < nb; j++) acc ^= buf[bits[j] + t];
+      for (int q = 0; q < 8; q++) v ^= buf[bits[j + q] + t];
+      acc ^= v;
     }
-    __syncthreads();
-    if (t < 624) {
-      buf[t] = acc;
-      starts[(size_t)k * 624 + t] = acc;
-    }
-    __syncthreads();
+    for (; j < b1; j++) acc ^= buf[bits[j] + t];
+    starts[(size_t)k * 624 + t] = acc;
   }
 }
 
 // Segment k of a jumped stream: outputs [k J, min((k + 1) J, ndraw)) from start window k.
-__global__ void __launch_bounds__(256) mt_segment_kernel(const uint32_t *starts, long long ndraw, uint32_t *out) {
+__global__ void __launch_bounds__(256) mt_segment_kernel(const uint32_t *starts, long long ndraw, int log2_seg,
+                                                         uint32_t *out) {
   __shared__ uint32_t ring[2048];
-  const long long k = blockIdx.x, o0 = k << MT_LOG2_SEG;
-  const long long cnt = ndraw - o0 < (1LL << MT_LOG2_SEG) ? ndraw - o0 : (1LL << MT_LOG2_SEG);
+  const long long k = blockIdx.x, o0 = k << log2_seg, J = 1LL << log2_seg;
+  const long long cnt = ndraw - o0 < J ? ndraw - o0 : J;
   for (int i = threadIdx.x; i < 624; i += blockDim.x) ring[i] = starts[k * 624 + i];
   __syncthreads();
   mt_ring_run(ring, cnt, out + o0);
 }
 
+// Segments of 2^19 words or more: at most 256 of them.
+int mt_log2_seg(long long ndraw) {
+  int l = 19;
+  while ((ndraw >> l) > 256) l++;
+  return l;
+}
+
 size_t mt_extra_words(long long ndraw) {
-  const long long J = 1LL << MT_LOG2_SEG;
+  const long long J = 1LL << mt_log2_seg(ndraw);
   return ndraw > 2 * J ? (size_t)((ndraw + J - 1) / J) * 624 : 0;
 }
 
-hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const uint32_t *coef,
+hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const MtJumpTable *jt,
                             hipStream_t s) {
-  const long long J = 1LL << MT_LOG2_SEG;
-  if (coef && ndraw > 2 * J) {
+  const int lj = mt_log2_seg(ndraw);
+  const long long J = 1LL << lj;
+  if (jt && ndraw > 2 * J) {
     const int nseg = (int)((ndraw + J - 1) / J);
+    if (jt->log2_seg != lj || jt->count < nseg - 1) return hipErrorInvalidValue;
     uint32_t *starts = out + ndraw;   // the caller sized out for ndraw + mt_extra_words(ndraw)
-    hipLaunchKernelGGL(mt_chain_kernel, dim3(1), dim3(640), MT_EXT * sizeof(uint32_t), s, seed, nseg, coef,
-                       starts);
-    hipLaunchKernelGGL(mt_segment_kernel, dim3(nseg), dim3(256), 0, s, starts, ndraw, out);
+    hipLaunchKernelGGL(mt_starts_kernel, dim3(nseg - 1), dim3(640), MT_EXT * sizeof(uint32_t), s, seed,
+                       jt->bits, jt->offs, starts);
+    hipLaunchKernelGGL(mt_segment_kernel, dim3(nseg), dim3(256), 0, s, starts, ndraw, lj, out);
   } else {
     hipLaunchKernelGGL(mt_kernel, dim3(1), dim3(256), 0, s, seed, ndraw, out);
   }

@@ -7,7 +7,8 @@
 #include <cstdint>
 #include <vector>
 #include <chrono>
-namespace rrtmgsw { std::vector<uint32_t> mt_jump_coefficients(int log2_jump); }
+namespace rrtmgsw { std::vector<uint32_t> mt_jump_coefficients(int log2_jump);
+                    std::vector<uint32_t> mt_jump_multiples(int log2_jump, int count); }
 static std::vector<uint32_t> raw(uint32_t seed, size_t n) {
   std::vector<uint32_t> x(n); x[0] = seed;
   for (size_t i = 1; i < 624; i++) x[i] = 1812433253u * (x[i-1] ^ (x[i-1] >> 30)) + (uint32_t)i;
@@ -37,6 +38,25 @@ int main() {
       for (size_t r = 624; r < y.size(); r++) bad += y[r] != x[J + r];
       printf("log2J=%d seed=%u mismatches=%d (%.2f s for the coefficients)\n", lj, seed, bad, dt);
       if (bad) return 2;
+    }
+  }
+  // x^(k J) mod P for k = 1..4 (J = 2^12) and k = 1..3 (J = 2^19): the window k J ahead
+  for (auto [lj, cnt] : {std::pair<int, int>{12, 4}, std::pair<int, int>{19, 3}}) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto m = rrtmgsw::mt_jump_multiples(lj, cnt);
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (m.size() != (size_t)cnt * 624) { printf("no multiples\n"); return 1; }
+    size_t J = (size_t)1 << lj;
+    auto x = raw(5489u, cnt * J + 624 + 20000);
+    for (int k = 1; k <= cnt; k++) {
+      const uint32_t *c = m.data() + (size_t)(k - 1) * 624;
+      std::vector<uint32_t> w(624, 0);
+      for (int i = 0; i < 19937; i++) if ((c[i >> 5] >> (i & 31)) & 1u) for (int q = 0; q < 624; q++) w[q] ^= x[i + q];
+      int bad = 0;
+      for (int q = 1; q < 624; q++) bad += w[q] != x[k * J + q];
+      bad += (w[0] & 0x80000000u) != (x[k * J] & 0x80000000u);
+      printf("multiple k=%d log2J=%d mismatches=%d (%.2f s for all %d)\n", k, lj, bad, dt, cnt);
+      if (bad) return 3;
     }
   }
   return 0;
