@@ -8,11 +8,13 @@
 #include <tuple>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rrtmgsw.h"
@@ -56,7 +58,7 @@ struct Context {
   double *d_tab = nullptr, *d_exp = nullptr, *d_pthr = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
-  std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork;
+  std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork, hostbuf;
   std::map<hipStream_t, std::unique_ptr<std::mutex>> calls;   // one call at a time per stream
   std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
   // MT19937 jump-ahead tables by segment length (log2): the set bits of x^(k J) mod P for
@@ -96,6 +98,172 @@ int stream_buffer(std::map<hipStream_t, Scratch> &m, hipStream_t s, size_t bytes
   }
   *out = b.ptr;
   return 0;
+}
+
+// ---------------- host-array transfers ----------------
+// The host-array entry points take the caller's ordinary (pageable) arrays. Transfers pass
+// through two pinned staging buffers allocated once (hipHostMalloc): while the DMA engine moves
+// one chunk, host threads copy the next chunk (host to device) or the previous one (device to
+// host) between the caller's array and the other buffer.
+constexpr size_t STAGE_BYTES = (size_t)32 << 20;
+
+// A fixed set of host threads for large copies, started on first use, never stopped (they wait
+// on a condition variable when idle).
+struct CopyPool {
+  std::mutex m;
+  std::condition_variable go, done;
+  int nth = 0, pending = 0;
+  uint64_t gen = 0;
+  char *dst = nullptr;
+  const char *src = nullptr;
+  size_t n = 0, part = 0;
+  void start() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    nth = (int)std::min<unsigned>(15, hw > 1 ? hw - 1 : 0);
+    for (int t = 1; t <= nth; t++) std::thread([this, t] { worker(t); }).detach();
+  }
+  void slice(int t, char *d, const char *s2, size_t total, size_t p) {
+    const size_t o = (size_t)t * p;
+    if (o < total) memcpy(d + o, s2 + o, std::min(p, total - o));
+  }
+  void worker(int t) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+      go.wait(lk, [&] { return gen != seen; });
+      seen = gen;
+      char *d = dst;
+      const char *s2 = src;
+      const size_t total = n, p = part;
+      lk.unlock();
+      slice(t, d, s2, total, p);
+      lk.lock();
+      if (--pending == 0) done.notify_one();
+    }
+  }
+  void copy(void *d, const void *s2, size_t bytes) {
+    if (nth == 0 || bytes < ((size_t)4 << 20)) {
+      memcpy(d, s2, bytes);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(m);
+    dst = (char *)d, src = (const char *)s2, n = bytes;
+    part = ((bytes / (nth + 1)) + 4095) & ~(size_t)4095;
+    pending = nth;
+    gen++;
+    go.notify_all();
+    lk.unlock();
+    slice(0, dst, src, bytes, part);
+    lk.lock();
+    done.wait(lk, [&] { return pending == 0; });
+  }
+};
+
+struct Stager {
+  std::mutex mu;   // one host-array transfer at a time
+  char *pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2];
+  bool live[2] = {false, false};
+  bool ready = false;
+  CopyPool *pool = nullptr;   // leaked on purpose: its threads outlive any static destructor
+} stager;
+
+int stager_init() {
+  if (stager.ready) return 0;
+  for (int b = 0; b < 2; b++) {
+    HIPCHK(hipHostMalloc((void **)&stager.pin[b], STAGE_BYTES, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&stager.ev[b], hipEventDisableTiming));
+  }
+  stager.pool = new CopyPool();
+  stager.pool->start();
+  stager.ready = true;
+  return 0;
+}
+int stage_wait(int b) {
+  if (stager.live[b]) {
+    HIPCHK(hipEventSynchronize(stager.ev[b]));
+    stager.live[b] = false;
+  }
+  return 0;
+}
+// host -> device, queued on s ahead of the work queued after it
+int staged_h2d(void *dev, const void *host, size_t bytes, hipStream_t s) {
+  size_t i = 0;
+  for (size_t off = 0; off < bytes; off += STAGE_BYTES, i++) {
+    const int b = (int)(i & 1);
+    const size_t n = std::min(STAGE_BYTES, bytes - off);
+    if (int rc = stage_wait(b)) return rc;
+    stager.pool->copy(stager.pin[b], (const char *)host + off, n);
+    HIPCHK(hipMemcpyAsync((char *)dev + off, stager.pin[b], n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(stager.ev[b], s));
+    stager.live[b] = true;
+  }
+  return 0;
+}
+// device -> host after the work queued on s before it; returns with the data in host memory
+int staged_d2h(void *host, const void *dev, size_t bytes, hipStream_t s) {
+  size_t i = 0, prev_off = 0, prev_n = 0;
+  int prev_b = -1;
+  for (size_t off = 0; off < bytes; off += STAGE_BYTES, i++) {
+    const int b = (int)(i & 1);
+    const size_t n = std::min(STAGE_BYTES, bytes - off);
+    if (int rc = stage_wait(b)) return rc;
+    HIPCHK(hipMemcpyAsync(stager.pin[b], (const char *)dev + off, n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(stager.ev[b], s));
+    stager.live[b] = true;
+    if (prev_b >= 0) {
+      if (int rc = stage_wait(prev_b)) return rc;
+      stager.pool->copy((char *)host + prev_off, stager.pin[prev_b], prev_n);
+    }
+    prev_b = b, prev_off = off, prev_n = n;
+  }
+  if (prev_b >= 0) {
+    if (int rc = stage_wait(prev_b)) return rc;
+    stager.pool->copy((char *)host + prev_off, stager.pin[prev_b], prev_n);
+  }
+  return 0;
+}
+
+// The host-array form of an entry point: the inputs that are present staged into one persistent
+// device buffer, the device entry run (run(inputs, outputs), on the null stream, which it
+// synchronizes), the outputs staged back. An input with n > 0 and no array is an error.
+struct HostIn {
+  const double *h;
+  size_t n;
+};
+struct HostOut {
+  double *h;
+  size_t n;
+};
+template <typename F>
+int host_call(const HostIn *in, int nin, const HostOut *out, int nout, F run) {
+  std::lock_guard<std::mutex> hl(stager.mu);
+  if (int rc = stager_init()) return rc;
+  size_t total = 0;
+  for (int i = 0; i < nin; i++) total += in[i].n;
+  for (int i = 0; i < nout; i++) total += out[i].n;
+  void *dv = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (int rc = stream_buffer(ctx.hostbuf, nullptr, std::max<size_t>(total, 1) * sizeof(double), &dv)) return rc;
+  }
+  double *d = (double *)dv;
+  std::vector<const double *> din(nin);
+  std::vector<double *> dout(nout);
+  size_t off = 0;
+  for (int i = 0; i < nin; i++) {
+    din[i] = in[i].n ? d + off : nullptr;
+    if (in[i].n) {
+      if (!in[i].h) return fail(RRTMGSW_ERR_ARGUMENT, "a required input array is NULL");
+      if (int rc = staged_h2d(d + off, in[i].h, in[i].n * sizeof(double), nullptr)) return rc;
+    }
+    off += in[i].n;
+  }
+  for (int i = 0; i < nout; i++) dout[i] = d + off, off += out[i].n;
+  int rc = run(din.data(), dout.data());
+  for (int i = 0; rc == 0 && i < nout; i++)
+    if (out[i].n) rc = staged_d2h(out[i].h, dout[i], out[i].n * sizeof(double), nullptr);
+  return rc;
 }
 
 // ---------------- KISS jump-ahead (host) ----------------
@@ -482,7 +650,7 @@ int rrtmgsw_finalize(void) {
   hipFree(ctx.d_exp);
   hipFree(ctx.d_gp);
   hipFree(ctx.d_pthr);
-  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.adjwork}) {
+  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.adjwork, &ctx.hostbuf}) {
     for (auto &kv : *m) hipFree(kv.second.ptr);
     m->clear();
   }
@@ -572,10 +740,7 @@ int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer, const double *play, con
     return 0;
   }
   const size_t C = ncol, L = nlay, G = NGPT;
-  struct In {
-    const double *h;
-    size_t n;
-  };
+  using In = HostIn;
   // the driver's clamps (src/rrtmg_sw_rad.f90:589, :597) decide which arrays are read
   int ic = (*icld < 0 || *icld > 3) ? 2 : *icld;
   int ia = (*iaer != 0 && *iaer != 6 && *iaer != 10) ? 0 : *iaer;
@@ -591,47 +756,17 @@ int rrtmgsw_sw(int ncol, int nlay, int *icld, int *iaer, const double *play, con
              {tauaer, ia == 10 ? C * L * NBND : 0}, {ssaaer, ia == 10 ? C * L * NBND : 0},
              {asmaer, ia == 10 ? C * L * NBND : 0}, {ecaer, ia == 6 ? C * L * NAER : 0}};
   const int NIN = sizeof in / sizeof *in;
-  size_t total = 0;
-  for (auto &x : in) total += x.n;
-  size_t nout = 4 * C * (L + 1) + 2 * C * L;
-  double *d = nullptr;
-  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
-  const double *dp[NIN];
-  size_t off = 0;
-  for (int i = 0; i < NIN; i++) {
-    dp[i] = in[i].n ? d + off : nullptr;
-    if (in[i].n) {
-      if (!in[i].h) {
-        hipFree(d);
-        return fail(RRTMGSW_ERR_ARGUMENT, "a required input array is NULL");
-      }
-      hipError_t e = hipMemcpy(d + off, in[i].h, in[i].n * sizeof(double), hipMemcpyHostToDevice);
-      if (e != hipSuccess) {
-        hipFree(d);
-        return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
-      }
-    }
-    off += in[i].n;
-  }
-  double *o = d + total;
-  double *ou = o, *od = o + C * (L + 1), *ouc = o + 2 * C * (L + 1), *odc = o + 3 * C * (L + 1);
-  double *oh = o + 4 * C * (L + 1), *ohc = oh + C * L;
-  int rc = rrtmgsw_sw_dev(ncol, nlay, icld, iaer, dp[0], dp[1], dp[2], nullptr, nullptr, dp[3], dp[4],
+  const HostOut out[] = {{swuflx, C * (L + 1)}, {swdflx, C * (L + 1)}, {swuflxc, C * (L + 1)},
+                         {swdflxc, C * (L + 1)}, {swhr, C * L}, {swhrc, C * L}};
+  for (const HostOut &o : out)
+    if (!o.h) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  return host_call(in, NIN, out, 6, [&](const double *const *dp, double *const *o) {
+    return rrtmgsw_sw_dev(ncol, nlay, icld, iaer, dp[0], dp[1], dp[2], nullptr, nullptr, dp[3], dp[4],
                           dp[5], dp[6], nullptr, dp[7], dp[8], dp[9], dp[10], dp[11], dp[12], adjes,
                           dyofyr, scon, isolvar, inflgsw, iceflgsw, liqflgsw, dp[13], dp[14], dp[15],
                           dp[16], dp[17], dp[18], dp[19], dp[20], dp[21], dp[22], dp[23], dp[24], dp[25],
-                          ou, od, oh, ouc, odc, ohc, bndsolvar, indsolvar, solcycfrac, nullptr);
-  if (rc == 0) {
-    hipError_t e = hipMemcpy(swuflx, ou, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swdflx, od, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swuflxc, ouc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swdflxc, odc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swhr, oh, C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swhrc, ohc, C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
-  }
-  hipFree(d);
-  return rc;
+                          o[0], o[1], o[4], o[2], o[3], o[5], bndsolvar, indsolvar, solcycfrac, nullptr);
+  });
 }
 
 // Non-McICA driver (src/rrtmg_sw_rad.nomcica.f90:97-816): layer cloud fractions and cldprop_sw
@@ -715,10 +850,7 @@ int rrtmgsw_sw_nomcica(int ncol, int nlay, int *icld, int *iaer, const double *p
     return 0;
   }
   const size_t C = ncol, L = nlay;
-  struct In {
-    const double *h;
-    size_t n;
-  };
+  using In = HostIn;
   int ic = (*icld < 0 || *icld > 3) ? 2 : *icld;
   int ia = (*iaer != 0 && *iaer != 6 && *iaer != 10) ? 0 : *iaer;
   const size_t cl = ic >= 1 ? C * L : 0, cb = ic >= 1 ? NBND * C * L : 0;
@@ -729,48 +861,18 @@ int rrtmgsw_sw_nomcica(int ncol, int nlay, int *icld, int *iaer, const double *p
              {tauaer, ia == 10 ? C * L * NBND : 0}, {ssaaer, ia == 10 ? C * L * NBND : 0},
              {asmaer, ia == 10 ? C * L * NBND : 0}, {ecaer, ia == 6 ? C * L * NAER : 0}};
   const int NIN = sizeof in / sizeof *in;
-  size_t total = 0;
-  for (auto &x : in) total += x.n;
-  size_t nout = 4 * C * (L + 1) + 2 * C * L;
-  double *d = nullptr;
-  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
-  const double *dp[NIN];
-  size_t off = 0;
-  for (int i = 0; i < NIN; i++) {
-    dp[i] = in[i].n ? d + off : nullptr;
-    if (in[i].n) {
-      if (!in[i].h) {
-        hipFree(d);
-        return fail(RRTMGSW_ERR_ARGUMENT, "a required input array is NULL");
-      }
-      hipError_t e = hipMemcpy(d + off, in[i].h, in[i].n * sizeof(double), hipMemcpyHostToDevice);
-      if (e != hipSuccess) {
-        hipFree(d);
-        return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
-      }
-    }
-    off += in[i].n;
-  }
-  double *o = d + total;
-  double *ou = o, *od = o + C * (L + 1), *ouc = o + 2 * C * (L + 1), *odc = o + 3 * C * (L + 1);
-  double *oh = o + 4 * C * (L + 1), *ohc = oh + C * L;
-  int rc = rrtmgsw_sw_nomcica_dev(ncol, nlay, icld, iaer, dp[0], dp[1], dp[2], nullptr, nullptr, dp[3],
+  const HostOut out[] = {{swuflx, C * (L + 1)}, {swdflx, C * (L + 1)}, {swuflxc, C * (L + 1)},
+                         {swdflxc, C * (L + 1)}, {swhr, C * L}, {swhrc, C * L}};
+  for (const HostOut &o : out)
+    if (!o.h) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  return host_call(in, NIN, out, 6, [&](const double *const *dp, double *const *o) {
+    return rrtmgsw_sw_nomcica_dev(ncol, nlay, icld, iaer, dp[0], dp[1], dp[2], nullptr, nullptr, dp[3],
                                   dp[4], dp[5], dp[6], nullptr, dp[7], dp[8], dp[9], dp[10], dp[11],
                                   dp[12], adjes, dyofyr, scon, isolvar, inflgsw, iceflgsw, liqflgsw,
                                   dp[13], dp[14], dp[15], dp[16], dp[17], dp[18], dp[19], dp[20], dp[21],
-                                  dp[22], dp[23], dp[24], dp[25], ou, od, oh, ouc, odc, ohc, bndsolvar,
-                                  indsolvar, solcycfrac, nullptr);
-  if (rc == 0) {
-    hipError_t e = hipMemcpy(swuflx, ou, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swdflx, od, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swuflxc, ouc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swdflxc, odc, C * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swhr, oh, C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(swhrc, ohc, C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
-  }
-  hipFree(d);
-  return rc;
+                                  dp[22], dp[23], dp[24], dp[25], o[0], o[1], o[4], o[2], o[3], o[5],
+                                  bndsolvar, indsolvar, solcycfrac, nullptr);
+  });
 }
 
 // Column program without McICA (src/rrtmg_sw.1col.f90:540-1000 for imca = 0): layer amounts in,
@@ -1118,44 +1220,19 @@ int rrtmgsw_mcica_subcol(int iplon, int ncol, int nlay, int icld, int permutesee
     return 0;
   }
   const size_t C = ncol, L = nlay, G = NGPT;
-  const double *hin[] = {play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, alpha};
-  size_t nin[] = {C * L, C * L, C * L, C * L, C * L, C * L, NBND * C * L, NBND * C * L,
-                  NBND * C * L, NBND * C * L, C * L};
-  size_t total = 0;
-  for (size_t n : nin) total += n;
-  size_t nout = 7 * G * C * L + 2 * C * L;
-  double *d = nullptr;
-  HIPCHK(hipMalloc(&d, (total + nout) * sizeof(double)));
-  const double *dp[11];
-  size_t off = 0;
-  for (int i = 0; i < 11; i++) {
-    dp[i] = d + off;
-    hipError_t e = hipMemcpy(d + off, hin[i], nin[i] * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      hipFree(d);
-      return fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
-    }
-    off += nin[i];
-  }
-  double *o = d + total;
-  double *oc = o, *oi = o + G * C * L, *ol = o + 2 * G * C * L, *ot = o + 3 * G * C * L;
-  double *os = o + 4 * G * C * L, *oa = o + 5 * G * C * L, *of = o + 6 * G * C * L;
-  double *ore = o + 7 * G * C * L, *orl = ore + C * L;
-  int rc = rrtmgsw_mcica_subcol_dev(iplon, ncol, nlay, icld, permuteseed, irng, dp[0], dp[1], dp[2],
-                                    dp[3], dp[4], dp[5], dp[6], dp[7], dp[8], dp[9], dp[10], oc, oi, ol,
-                                    ore, orl, ot, os, oa, of, nullptr);
-  if (rc == 0) {
-    double *hout[] = {cldfmcl, ciwpmcl, clwpmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl};
-    double *dout[] = {oc, oi, ol, ot, os, oa, of};
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 7 && e == hipSuccess; i++)
-      e = hipMemcpy(hout[i], dout[i], G * C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(reicmcl, ore, C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(relqmcl, orl, C * L * sizeof(double), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(RRTMGSW_ERR_HIP, hipGetErrorString(e));
-  }
-  hipFree(d);
-  return rc;
+  const HostIn in[] = {{play, C * L}, {cldfrac, C * L}, {ciwp, C * L}, {clwp, C * L}, {rei, C * L},
+                       {rel, C * L}, {tauc, NBND * C * L}, {ssac, NBND * C * L}, {asmc, NBND * C * L},
+                       {fsfc, NBND * C * L}, {alpha, icld >= 4 ? C * L : 0}};
+  const HostOut out[] = {{cldfmcl, G * C * L}, {ciwpmcl, G * C * L}, {clwpmcl, G * C * L},
+                         {reicmcl, C * L}, {relqmcl, C * L}, {taucmcl, G * C * L},
+                         {ssacmcl, G * C * L}, {asmcmcl, G * C * L}, {fsfcmcl, G * C * L}};
+  for (const HostOut &o : out)
+    if (!o.h) return fail(RRTMGSW_ERR_ARGUMENT, "a required array argument is NULL");
+  return host_call(in, 11, out, 9, [&](const double *const *dp, double *const *o) {
+    return rrtmgsw_mcica_subcol_dev(iplon, ncol, nlay, icld, permuteseed, irng, dp[0], dp[1], dp[2],
+                                    dp[3], dp[4], dp[5], dp[6], dp[7], dp[8], dp[9], dp[10], o[0], o[1],
+                                    o[2], o[3], o[4], o[5], o[6], o[7], o[8], nullptr);
+  });
 }
 
 int rrtmgsw_get_alpha(int iplon, int ncol, int nlayers, int icld, int idcor, double decorr_con,
