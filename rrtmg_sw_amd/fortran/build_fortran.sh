@@ -9,13 +9,15 @@ HERE=$(cd "$(dirname "$0")" && pwd)
 PKG=$(dirname "$HERE")
 FC=${FC:-amdflang}
 mkdir -p "$HERE/mod"
-$FC -O2 -fPIC -shared -J "$HERE/mod" -o "$HERE/librrtmgsw_fortran.so" "$HERE/rrtmg_sw_hip.f90" \
+$FC -O2 -fPIC -shared -J "$HERE/mod" -o "$HERE/librrtmgsw_fortran.so" "$HERE/rrtmgsw_lead.f90" \
+    "$HERE/rrtmg_sw_hip.f90" \
     -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN/..'
-$FC -O2 -I "$HERE/mod" -o "$HERE/example_gcm" "$HERE/example_gcm.f90" \
+$FC -O2 -J "$HERE/mod" -o "$HERE/example_gcm" "$HERE/example_pad.f90" "$HERE/example_gcm.f90" \
     -L"$HERE" -lrrtmgsw_fortran -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN' -Wl,-rpath,'$ORIGIN/..'
 mkdir -p "$HERE/mod_nomcica"
 $FC -O2 -fPIC -shared -J "$HERE/mod_nomcica" -o "$HERE/librrtmgsw_fortran_nomcica.so" \
-    "$HERE/rrtmg_sw_hip_nomcica.f90" -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN/..'
-$FC -O2 -I "$HERE/mod_nomcica" -o "$HERE/example_gcm_nomcica" "$HERE/example_gcm_nomcica.f90" \
+    "$HERE/rrtmgsw_lead.f90" "$HERE/rrtmg_sw_hip_nomcica.f90" -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN/..'
+$FC -O2 -J "$HERE/mod_nomcica" -o "$HERE/example_gcm_nomcica" "$HERE/example_pad.f90" \
+    "$HERE/example_gcm_nomcica.f90" \
     -L"$HERE" -lrrtmgsw_fortran_nomcica -L"$PKG" -lrrtmgsw_hip -Wl,-rpath,'$ORIGIN' \
     -Wl,-rpath,'$ORIGIN/..'

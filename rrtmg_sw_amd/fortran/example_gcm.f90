@@ -2,7 +2,9 @@
 ! reference: rrtmg_sw_ini once, then mcica_subcol_sw and rrtmg_sw per chunk of columns
 ! (README.md of the reference, "GCM" usage). Inputs are read from a raw stream file and the six
 ! outputs written to another, so that tests can drive it:
-!   example_gcm <input.bin> <output.bin>
+!   example_gcm <input.bin> <output.bin> [pad]
+! With pad > 0 every array is re-homed in one with pcols = ncol + pad columns (the extra columns
+! hold junk), as a GCM's pcols-sized arrays are, and the outputs' first ncol columns are written.
 ! input: 11 int32 (ncol nlay icld iaer permuteseed irng dyofyr isolvar inflgsw iceflgsw liqflgsw),
 !        2 real64 (adjes scon), then play plev tlay tlev tsfc h2o o3 co2 ch4 n2o o2 asdir asdif
 !        aldir aldif coszen cldfrac ciwp clwp rei rel tauc ssac asmc fsfc alpha tauaer ssaaer
@@ -11,12 +13,13 @@ program example_gcm
   use rrtmg_sw_init, only : rrtmg_sw_ini
   use mcica_subcol_gen_sw, only : mcica_subcol_sw
   use rrtmg_sw_rad, only : rrtmg_sw
+  use example_pad
   implicit none
   integer, parameter :: rb = selected_real_kind(12), ngptsw = 112, nbndsw = 14, naerec = 6
   integer :: hdr(11), ncol, nlay, icld, iaer, permuteseed, irng, dyofyr, isolvar
-  integer :: inflgsw, iceflgsw, liqflgsw, u
+  integer :: inflgsw, iceflgsw, liqflgsw, u, pad, pcols
   real(rb) :: sc(2)
-  character(len=512) :: fin, fout
+  character(len=512) :: fin, fout, arg
   real(rb), allocatable :: play(:,:), plev(:,:), tlay(:,:), tlev(:,:), tsfc(:), h2o(:,:), o3(:,:)
   real(rb), allocatable :: co2(:,:), ch4(:,:), n2o(:,:), o2(:,:), asdir(:), asdif(:), aldir(:)
   real(rb), allocatable :: aldif(:), coszen(:), cldfrac(:,:), ciwp(:,:), clwp(:,:), rei(:,:)
@@ -28,6 +31,11 @@ program example_gcm
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  pad = 0
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    read(arg, *) pad
+  end if
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) hdr, sc
   ncol = hdr(1); nlay = hdr(2); icld = hdr(3); iaer = hdr(4); permuteseed = hdr(5)
@@ -45,11 +53,23 @@ program example_gcm
           coszen, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, alpha, tauaer, ssaaer, &
           asmaer, ecaer
   close(u)
-  allocate(cldfmcl(ngptsw,ncol,nlay), ciwpmcl(ngptsw,ncol,nlay), clwpmcl(ngptsw,ncol,nlay))
-  allocate(taucmcl(ngptsw,ncol,nlay), ssacmcl(ngptsw,ncol,nlay), asmcmcl(ngptsw,ncol,nlay))
-  allocate(fsfcmcl(ngptsw,ncol,nlay), reicmcl(ncol,nlay), relqmcl(ncol,nlay))
-  allocate(swuflx(ncol,nlay+1), swdflx(ncol,nlay+1), swhr(ncol,nlay))
-  allocate(swuflxc(ncol,nlay+1), swdflxc(ncol,nlay+1), swhrc(ncol,nlay))
+  pcols = ncol + pad
+  if (pad > 0) then
+    call pad2(play, pcols); call pad2(plev, pcols); call pad2(tlay, pcols)
+    call pad2(tlev, pcols); call pad1(tsfc, pcols); call pad2(h2o, pcols); call pad2(o3, pcols)
+    call pad2(co2, pcols); call pad2(ch4, pcols); call pad2(n2o, pcols); call pad2(o2, pcols)
+    call pad1(asdir, pcols); call pad1(asdif, pcols); call pad1(aldir, pcols)
+    call pad1(aldif, pcols); call pad1(coszen, pcols); call pad2(cldfrac, pcols)
+    call pad2(ciwp, pcols); call pad2(clwp, pcols); call pad2(rei, pcols); call pad2(rel, pcols)
+    call pad3b(tauc, pcols); call pad3b(ssac, pcols); call pad3b(asmc, pcols)
+    call pad3b(fsfc, pcols); call pad2(alpha, pcols); call pad3a(tauaer, pcols)
+    call pad3a(ssaaer, pcols); call pad3a(asmaer, pcols); call pad3a(ecaer, pcols)
+  end if
+  allocate(cldfmcl(ngptsw,pcols,nlay), ciwpmcl(ngptsw,pcols,nlay), clwpmcl(ngptsw,pcols,nlay))
+  allocate(taucmcl(ngptsw,pcols,nlay), ssacmcl(ngptsw,pcols,nlay), asmcmcl(ngptsw,pcols,nlay))
+  allocate(fsfcmcl(ngptsw,pcols,nlay), reicmcl(pcols,nlay), relqmcl(pcols,nlay))
+  allocate(swuflx(pcols,nlay+1), swdflx(pcols,nlay+1), swhr(pcols,nlay))
+  allocate(swuflxc(pcols,nlay+1), swdflxc(pcols,nlay+1), swhrc(pcols,nlay))
   cldfmcl = 0; ciwpmcl = 0; clwpmcl = 0; taucmcl = 0; ssacmcl = 1; asmcmcl = 0; fsfcmcl = 0
   reicmcl = 0; relqmcl = 0
 
@@ -65,6 +85,7 @@ program example_gcm
                 ciwpmcl, clwpmcl, reicmcl, relqmcl, tauaer, ssaaer, asmaer, ecaer, &
                 swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(u) icld, iaer, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc
+  write(u) icld, iaer, swuflx(1:ncol,:), swdflx(1:ncol,:), swhr(1:ncol,:), swuflxc(1:ncol,:), &
+           swdflxc(1:ncol,:), swhrc(1:ncol,:)
   close(u)
 end program example_gcm

@@ -2,15 +2,18 @@
 ! calls the reference's non-McICA driver: rrtmg_sw_ini once, then rrtmg_sw with layer clouds.
 ! Same stream files as example_gcm (the McICA-only header fields permuteseed and irng and the
 ! alpha array are read and ignored):
-!   example_gcm_nomcica <input.bin> <output.bin>
+!   example_gcm_nomcica <input.bin> <output.bin> [pad]
+! pad re-homes the arrays in pcols = ncol + pad columns, as in example_gcm.
 program example_gcm_nomcica
   use rrtmg_sw_init, only : rrtmg_sw_ini
   use rrtmg_sw_rad, only : rrtmg_sw
+  use example_pad
   implicit none
   integer, parameter :: rb = selected_real_kind(12), nbndsw = 14, naerec = 6
   integer :: hdr(11), ncol, nlay, icld, iaer, dyofyr, isolvar, inflgsw, iceflgsw, liqflgsw, u
+  integer :: pad, pcols
   real(rb) :: sc(2)
-  character(len=512) :: fin, fout
+  character(len=512) :: fin, fout, arg
   real(rb), allocatable :: play(:,:), plev(:,:), tlay(:,:), tlev(:,:), tsfc(:), h2o(:,:), o3(:,:)
   real(rb), allocatable :: co2(:,:), ch4(:,:), n2o(:,:), o2(:,:), asdir(:), asdif(:), aldir(:)
   real(rb), allocatable :: aldif(:), coszen(:), cldfr(:,:), ciwp(:,:), clwp(:,:), rei(:,:)
@@ -20,6 +23,11 @@ program example_gcm_nomcica
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  pad = 0
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    read(arg, *) pad
+  end if
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) hdr, sc
   ncol = hdr(1); nlay = hdr(2); icld = hdr(3); iaer = hdr(4); dyofyr = hdr(7); isolvar = hdr(8)
@@ -36,8 +44,20 @@ program example_gcm_nomcica
           coszen, cldfr, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, alpha, tauaer, ssaaer, &
           asmaer, ecaer
   close(u)
-  allocate(swuflx(ncol,nlay+1), swdflx(ncol,nlay+1), swhr(ncol,nlay))
-  allocate(swuflxc(ncol,nlay+1), swdflxc(ncol,nlay+1), swhrc(ncol,nlay))
+  pcols = ncol + pad
+  if (pad > 0) then
+    call pad2(play, pcols); call pad2(plev, pcols); call pad2(tlay, pcols)
+    call pad2(tlev, pcols); call pad1(tsfc, pcols); call pad2(h2o, pcols); call pad2(o3, pcols)
+    call pad2(co2, pcols); call pad2(ch4, pcols); call pad2(n2o, pcols); call pad2(o2, pcols)
+    call pad1(asdir, pcols); call pad1(asdif, pcols); call pad1(aldir, pcols)
+    call pad1(aldif, pcols); call pad1(coszen, pcols); call pad2(cldfr, pcols)
+    call pad2(ciwp, pcols); call pad2(clwp, pcols); call pad2(rei, pcols); call pad2(rel, pcols)
+    call pad3b(tauc, pcols); call pad3b(ssac, pcols); call pad3b(asmc, pcols)
+    call pad3b(fsfc, pcols); call pad3a(tauaer, pcols); call pad3a(ssaaer, pcols)
+    call pad3a(asmaer, pcols); call pad3a(ecaer, pcols)
+  end if
+  allocate(swuflx(pcols,nlay+1), swdflx(pcols,nlay+1), swhr(pcols,nlay))
+  allocate(swuflxc(pcols,nlay+1), swdflxc(pcols,nlay+1), swhrc(pcols,nlay))
 
   call rrtmg_sw_ini(1004.64_rb)
   call rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, &
@@ -45,6 +65,7 @@ program example_gcm_nomcica
                 inflgsw, iceflgsw, liqflgsw, cldfr, tauc, ssac, asmc, fsfc, ciwp, clwp, rei, rel, &
                 tauaer, ssaaer, asmaer, ecaer, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(u) icld, iaer, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc
+  write(u) icld, iaer, swuflx(1:ncol,:), swdflx(1:ncol,:), swhr(1:ncol,:), swuflxc(1:ncol,:), &
+           swdflxc(1:ncol,:), swhrc(1:ncol,:)
   close(u)
 end program example_gcm_nomcica

@@ -6,7 +6,9 @@
 !   module rrtmg_sw_init   rrtmg_sw_ini(cpdair)   src/rrtmg_sw_init.f90:47
 !   module rrtmg_sw_rad    rrtmg_sw(...)          src/rrtmg_sw_rad.nomcica.f90:97-111
 ! Each routine forwards to the C-ABI of librrtmgsw_hip.so (include/rrtmgsw.h) through
-! iso_c_binding; a non-zero return stops with the reference's STOP message.
+! iso_c_binding; arrays larger than the counts (pcols-sized arrays with ncol columns in use) pass
+! their leading blocks, which the reference reads (rrtmgsw_lead). A non-zero return stops with
+! the reference's STOP message.
 
 module rrtmgsw_capi_nomcica
   use iso_c_binding
@@ -69,6 +71,7 @@ end module rrtmg_sw_init
 module rrtmg_sw_rad
   use iso_c_binding
   use rrtmgsw_capi_nomcica
+  use rrtmgsw_lead
   implicit none
   integer, parameter, private :: rb = selected_real_kind(12), im = selected_int_kind(8)
   public :: rrtmg_sw
@@ -105,6 +108,12 @@ contains
     real(kind=rb), target :: bsv(14), isv(2), scf
     type(c_ptr) :: pb, pi, pf
     integer(c_int) :: ic, ia
+    integer, parameter :: nbndsw = 14, naerec = 6
+    type(buf1), target :: i1(6)
+    type(buf2), target :: i2(15), o2(6)
+    type(buf3), target :: i3(8)
+    type(c_ptr) :: p(35), q(6)
+    logical :: uc, ua, ue, u
     pb = c_null_ptr
     pi = c_null_ptr
     pf = c_null_ptr
@@ -122,15 +131,59 @@ contains
     end if
     ic = icld
     ia = iaer
-    call rrtmgsw_check(c_rrtmgsw_sw_nomcica(ncol, nlay, ic, ia, c_loc(play), c_loc(plev), &
-         c_loc(tlay), c_loc(tlev), c_loc(tsfc), c_loc(h2ovmr), c_loc(o3vmr), c_loc(co2vmr), &
-         c_loc(ch4vmr), c_loc(n2ovmr), c_loc(o2vmr), c_loc(asdir), c_loc(asdif), c_loc(aldir), &
-         c_loc(aldif), c_loc(coszen), adjes, dyofyr, scon, isolvar, inflgsw, iceflgsw, liqflgsw, &
-         c_loc(cldfr), c_loc(taucld), c_loc(ssacld), c_loc(asmcld), c_loc(fsfcld), c_loc(cicewp), &
-         c_loc(cliqwp), c_loc(reice), c_loc(reliq), c_loc(tauaer), c_loc(ssaaer), c_loc(asmaer), &
-         c_loc(ecaer), c_loc(swuflx), c_loc(swdflx), c_loc(swhr), c_loc(swuflxc), c_loc(swdflxc), &
-         c_loc(swhrc), pb, pi, pf))
+    ! which arrays the call reads: icld outside 0..3 becomes 2, iaer outside 0, 6, 10 becomes 0
+    ! (src/rrtmg_sw_rad.nomcica.f90:560, :568); every cloud array when icld >= 1 (:1579)
+    u = ncol > 0
+    uc = icld /= 0 .and. u
+    ua = iaer == 10 .and. u
+    ue = iaer == 6 .and. u
+    p(1) = lead2(play, ncol, nlay, i2(1), u)
+    p(2) = lead2(plev, ncol, nlay + 1, i2(2), u)
+    p(3) = lead2(tlay, ncol, nlay, i2(3), u)
+    p(4) = lead2(tlev, ncol, nlay + 1, i2(4), .false.)
+    p(5) = lead1(tsfc, ncol, i1(1), .false.)
+    p(6) = lead2(h2ovmr, ncol, nlay, i2(5), u)
+    p(7) = lead2(o3vmr, ncol, nlay, i2(6), u)
+    p(8) = lead2(co2vmr, ncol, nlay, i2(7), u)
+    p(9) = lead2(ch4vmr, ncol, nlay, i2(8), u)
+    p(10) = lead2(n2ovmr, ncol, nlay, i2(9), .false.)
+    p(11) = lead2(o2vmr, ncol, nlay, i2(10), u)
+    p(12) = lead1(asdir, ncol, i1(2), u)
+    p(13) = lead1(asdif, ncol, i1(3), u)
+    p(14) = lead1(aldir, ncol, i1(4), u)
+    p(15) = lead1(aldif, ncol, i1(5), u)
+    p(16) = lead1(coszen, ncol, i1(6), u)
+    p(17) = lead2(cldfr, ncol, nlay, i2(11), uc)
+    p(18) = lead3(taucld, nbndsw, ncol, nlay, i3(1), uc)
+    p(19) = lead3(ssacld, nbndsw, ncol, nlay, i3(2), uc)
+    p(20) = lead3(asmcld, nbndsw, ncol, nlay, i3(3), uc)
+    p(21) = lead3(fsfcld, nbndsw, ncol, nlay, i3(4), uc)
+    p(22) = lead2(cicewp, ncol, nlay, i2(12), uc)
+    p(23) = lead2(cliqwp, ncol, nlay, i2(13), uc)
+    p(24) = lead2(reice, ncol, nlay, i2(14), uc)
+    p(25) = lead2(reliq, ncol, nlay, i2(15), uc)
+    p(26) = lead3(tauaer, ncol, nlay, nbndsw, i3(5), ua)
+    p(27) = lead3(ssaaer, ncol, nlay, nbndsw, i3(6), ua)
+    p(28) = lead3(asmaer, ncol, nlay, nbndsw, i3(7), ua)
+    p(29) = lead3(ecaer, ncol, nlay, naerec, i3(8), ue)
+    q(1) = outb2(swuflx, ncol, nlay + 1, o2(1))
+    q(2) = outb2(swdflx, ncol, nlay + 1, o2(2))
+    q(3) = outb2(swhr, ncol, nlay, o2(3))
+    q(4) = outb2(swuflxc, ncol, nlay + 1, o2(4))
+    q(5) = outb2(swdflxc, ncol, nlay + 1, o2(5))
+    q(6) = outb2(swhrc, ncol, nlay, o2(6))
+    call rrtmgsw_check(c_rrtmgsw_sw_nomcica(ncol, nlay, ic, ia, p(1), p(2), p(3), p(4), p(5), &
+         p(6), p(7), p(8), p(9), p(10), p(11), p(12), p(13), p(14), p(15), p(16), adjes, dyofyr, &
+         scon, isolvar, inflgsw, iceflgsw, liqflgsw, p(17), p(18), p(19), p(20), p(21), p(22), &
+         p(23), p(24), p(25), p(26), p(27), p(28), p(29), q(1), q(2), q(3), q(4), q(5), q(6), &
+         pb, pi, pf))
     icld = ic
     iaer = ia
+    call back2(swuflx, o2(1))
+    call back2(swdflx, o2(2))
+    call back2(swhr, o2(3))
+    call back2(swuflxc, o2(4))
+    call back2(swdflxc, o2(5))
+    call back2(swhrc, o2(6))
   end subroutine rrtmg_sw
 end module rrtmg_sw_rad

@@ -1,7 +1,9 @@
 """The Fortran drop-in: a host program compiled against the modules rrtmg_sw_init,
 mcica_subcol_gen_sw and rrtmg_sw_rad (same routine names and dummies as the reference) calls
 mcica_subcol_sw + rrtmg_sw on the GPU; a second one, compiled against the non-McICA driver's
-modules, calls its rrtmg_sw. Outputs are checked against the oracle."""
+modules, calls its rrtmg_sw. Outputs are checked against the oracle, also with every array held in
+one of pcols = ncol + pad columns whose extra columns hold junk (a GCM's pcols-sized arrays): the
+drop-in reads the leading ncol columns, as the reference does."""
 import os
 import subprocess
 
@@ -49,8 +51,8 @@ def _read_outputs(path, cols):
 NAMES = ["swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"]
 
 
-@pytest.mark.parametrize("icld,iaer", [(2, 6), (0, 0)])
-def test_fortran_host_program_nomcica(tmp_path, icld, iaer):
+@pytest.mark.parametrize("icld,iaer,pad", [(2, 6, 0), (0, 0, 0), (2, 6, 7)])
+def test_fortran_host_program_nomcica(tmp_path, icld, iaer, pad):
     """The non-McICA driver's modules (rrtmg_sw_hip_nomcica.f90): same names as the reference's
     src/rrtmg_sw_rad.nomcica.f90; layer clouds, overcast only."""
     if not os.path.exists(EXE_NM):
@@ -59,7 +61,8 @@ def test_fortran_host_program_nomcica(tmp_path, icld, iaer):
     cols.ecaer = np.asfortranarray(np.random.default_rng(8).uniform(0.0, 0.2, cols.ecaer.shape))
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     _write_inputs(fin, cols, icld, iaer)
-    r = subprocess.run([EXE_NM, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE_NM, str(fin), str(fout), str(pad)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     ic, ia, got = _read_outputs(fout, cols)
     ref, ic_r, ia_r = ob.run_nomcica(cols, icld=icld, iaer=iaer)
@@ -84,9 +87,12 @@ def test_fortran_nomcica_partial_cloud_stops(tmp_path):
     assert "PARTIAL CLOUD NOT ALLOWED" in r.stdout + r.stderr
 
 
-@pytest.mark.parametrize("icld,iaer", [(2, 0), (5, 10), (0, 0)])
-def test_fortran_host_program(tmp_path, icld, iaer):
+@pytest.mark.parametrize("icld,iaer,pad", [(2, 0, 0), (5, 10, 0), (0, 0, 0), (2, 6, 5),
+                                           (5, 10, 3)])
+def test_fortran_host_program(tmp_path, icld, iaer, pad):
     cols = make_columns(20, 45, icld=icld, iaer=iaer, seed=70 + icld)
+    if iaer == 6:
+        cols.ecaer = np.asfortranarray(np.random.default_rng(9).uniform(0.0, 0.2, cols.ecaer.shape))
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     hdr = np.array([cols.ncol, cols.nlay, icld, iaer, cols.permuteseed, cols.irng, cols.dyofyr,
                     cols.isolvar, cols.inflgsw, cols.iceflgsw, cols.liqflgsw], dtype=np.int32)
@@ -95,7 +101,8 @@ def test_fortran_host_program(tmp_path, icld, iaer):
         f.write(np.array([cols.adjes, cols.scon], dtype=np.float64).tobytes())
         for k in ORDER:
             f.write(np.asarray(getattr(cols, k), dtype=np.float64).ravel(order="F").tobytes())
-    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, str(fin), str(fout), str(pad)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = open(fout, "rb").read()
     ic, ia = np.frombuffer(raw[:8], dtype=np.int32)
