@@ -58,9 +58,8 @@ struct Context {
   double *d_tab = nullptr, *d_exp = nullptr, *d_pthr = nullptr;
   GPoint *d_gp = nullptr;
   std::mutex mu;
-  std::map<hipStream_t, Scratch> scratch, status, mtbuf, adjwork, hostbuf;
+  std::map<hipStream_t, Scratch> scratch, status, mtbuf, kissbuf, adjwork, hostbuf;
   std::map<hipStream_t, std::unique_ptr<std::mutex>> calls;   // one call at a time per stream
-  std::map<std::tuple<int, int, int>, uint32_t *> jumps;  // (permuteseed, nlay, draws/layer)
   // MT19937 jump-ahead tables by segment length (log2): the set bits of x^(k J) mod P for
   // k = 1 .. count, built on first use and extended when a longer stream needs more segments
   struct MtJump {
@@ -274,10 +273,10 @@ int host_call(const HostIn *in, int nin, const HostOut *out, int nout, F run) {
 struct Affine {
   uint32_t a, c;
 };
-Affine affine_compose(Affine f, Affine g) {  // f(g(x))
+__host__ __device__ Affine affine_compose(Affine f, Affine g) {  // f(g(x))
   return {f.a * g.a, f.a * g.c + f.c};
 }
-Affine affine_pow(Affine f, uint64_t n) {
+__host__ __device__ Affine affine_pow(Affine f, uint64_t n) {
   Affine r = {1u, 0u};
   while (n) {
     if (n & 1) r = affine_compose(f, r);
@@ -289,18 +288,18 @@ Affine affine_pow(Affine f, uint64_t n) {
 struct Gf2 {
   uint32_t col[32];
 };
-uint32_t gf2_apply(const Gf2 &m, uint32_t x) {
+__host__ __device__ uint32_t gf2_apply(const Gf2 &m, uint32_t x) {
   uint32_t r = 0;
   for (int i = 0; i < 32; i++)
     if ((x >> i) & 1u) r ^= m.col[i];
   return r;
 }
-Gf2 gf2_compose(const Gf2 &a, const Gf2 &b) {  // a(b(x))
+__host__ __device__ Gf2 gf2_compose(const Gf2 &a, const Gf2 &b) {  // a(b(x))
   Gf2 r;
   for (int i = 0; i < 32; i++) r.col[i] = gf2_apply(a, b.col[i]);
   return r;
 }
-Gf2 gf2_pow(Gf2 m, uint64_t n) {
+__host__ __device__ Gf2 gf2_pow(Gf2 m, uint64_t n) {
   Gf2 r;
   for (int i = 0; i < 32; i++) r.col[i] = 1u << i;
   while (n) {
@@ -310,7 +309,7 @@ Gf2 gf2_pow(Gf2 m, uint64_t n) {
   }
   return r;
 }
-uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+__host__ __device__ uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
   uint64_t r = 1 % m;
   b %= m;
   while (e) {
@@ -324,7 +323,7 @@ uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
 // The state map of n kissvec draws (src/mcica_subcol_gen_sw.f90:721-753): an affine map for the
 // congruential part, a GF(2) matrix power for the xorshift, modular powers for the two
 // multiply-with-carry parts.
-KissJump kiss_jump_of(uint64_t n) {
+__host__ __device__ KissJump kiss_jump_of(uint64_t n) {
   Gf2 xs;
   for (int i = 0; i < 32; i++) {
     uint32_t k = 1u << i;
@@ -338,32 +337,35 @@ KissJump kiss_jump_of(uint64_t n) {
   Affine a = affine_pow(lcg, n);
   Gf2 m = gf2_pow(xs, n);
   J.a1 = a.a, J.c1 = a.c;
-  memcpy(J.m2, m.col, sizeof m.col);
+  for (int i = 0; i < 32; i++) J.m2[i] = m.col[i];
   J.p3 = powmod(18000, n, 1179647999ull);
   J.p4 = powmod(30903, n, 2025259007ull);
   return J;
 }
 
-// Device table of the per-sub-column KISS jumps for (permuteseed, nlay, per_sub), built once and
-// kept for the process (calls from several threads share the cache under ctx.mu).
-int kiss_jumps(int permuteseed, int nlay, int per_sub, hipStream_t s, const uint32_t **out) {
-  std::lock_guard<std::mutex> lk(ctx.mu);
-  auto key = std::make_tuple(permuteseed, nlay, per_sub);
-  auto it = ctx.jumps.find(key);
-  if (it != ctx.jumps.end()) {
-    *out = it->second;
-    return 0;
-  }
-  std::vector<KissJump> J(NGPT);
-  uint64_t W = permuteseed > 0 ? (uint64_t)permuteseed : 0;
-  for (int g = 0; g < NGPT; g++) J[g] = kiss_jump_of(W + (uint64_t)g * (uint64_t)per_sub);
-  uint32_t *d = nullptr;
-  HIPCHK(hipMalloc(&d, sizeof(KissJump) * NGPT));
-  HIPCHK(hipMemcpyAsync(d, J.data(), sizeof(KissJump) * NGPT, hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
-  ctx.jumps[key] = d;
-  *out = d;
+// The jump of each g-point's sub-column, g*per_sub + w draws into its column's stream, one
+// lane per g-point.
+__global__ void __launch_bounds__(128) kiss_jumps_kernel(uint64_t w, uint64_t per_sub, KissJump *out) {
+  const int g = threadIdx.x;
+  if (g < NGPT) out[g] = kiss_jump_of(w + (uint64_t)g * per_sub);
+}
+
+// The per-sub-column KISS jumps for (permuteseed, draws per sub-column), formed on the GPU into
+// the stream's own buffer ahead of the work queued after it: no host copy, no wait, no cache.
+int kiss_jumps(int permuteseed, int per_sub, hipStream_t s, KissJump *buf) {
+  const uint64_t w = permuteseed > 0 ? (uint64_t)permuteseed : 0;
+  hipLaunchKernelGGL(kiss_jumps_kernel, dim3(1), dim3(128), 0, s, w, (uint64_t)per_sub, buf);
+  HIPCHK(hipGetLastError());
   return 0;
+}
+int kiss_jumps(int permuteseed, int per_sub, hipStream_t s, const uint32_t **out) {
+  void *buf = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (int rc = stream_buffer(ctx.kissbuf, s, sizeof(KissJump) * NGPT, &buf)) return rc;
+  }
+  *out = (const uint32_t *)buf;
+  return kiss_jumps(permuteseed, per_sub, s, (KissJump *)buf);
 }
 
 // ---------------- solar variability and flux adjustment (host) ----------------
@@ -650,12 +652,10 @@ int rrtmgsw_finalize(void) {
   hipFree(ctx.d_exp);
   hipFree(ctx.d_gp);
   hipFree(ctx.d_pthr);
-  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.adjwork, &ctx.hostbuf}) {
+  for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.kissbuf, &ctx.adjwork, &ctx.hostbuf}) {
     for (auto &kv : *m) hipFree(kv.second.ptr);
     m->clear();
   }
-  for (auto &kv : ctx.jumps) hipFree(kv.second);
-  ctx.jumps.clear();
   for (auto &kv : ctx.mtjump) hipFree(kv.second.bits), hipFree(kv.second.offs);
   ctx.mtjump.clear();
   for (void *p : ctx.retired) hipFree(p);
@@ -703,7 +703,6 @@ int rrtmgsw_sw_dev(int ncol, int nlay, int *icld, int *iaer, const double *play,
   a.sc.heatfac = ctx.heatfac;
   a.sc.icld = *icld, a.sc.iaer = *iaer;
   a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
-  if (*icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
   const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
                         aldir, aldif, coszen, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
   for (const void *p : need)
@@ -999,7 +998,6 @@ int rrtmgsw_column_mcica_sw(int nlay, int nsamples, int icld, int irng, int idel
     return fail(RRTMGSW_ERR_ARGUMENT, "nlay must lie in 4..256 and nsamples >= 0");
   if (icld < 1 || icld > 5) return fail(RRTMGSW_ERR_ICLD, "RRTMG_SW: INVALID ICLD");
   if (idelm != 0 && idelm != 1) return fail(RRTMGSW_ERR_ARGUMENT, "idelm must be 0 or 1");
-  if (inflag == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
   if (nsamples == 0) return 0;
   const void *need[] = {pavel, tavel, pz, coldry, wkl, albedo, cldfrac, ciwp, clwp, rei, rel,
                         tauc, ssac, asmc, fsfc, fup, fdn, fdir};
@@ -1062,7 +1060,7 @@ int rrtmgsw_column_mcica_sw(int nlay, int nsamples, int icld, int irng, int idel
   }
   const uint32_t *jumps = nullptr;
   if (ir == 0)
-    if (int rc = kiss_jumps(0, nlay, per_sub, nullptr, &jumps)) {
+    if (int rc = kiss_jumps(0, per_sub, nullptr, &jumps)) {
       hipFree(d);
       return rc;
     }
@@ -1116,7 +1114,7 @@ static int mcica_prepare(int ncol, int nlay, int icld, int permuteseed, int irng
   *mt = nullptr;
   *jumps = nullptr;
   int per_sub = icld == 3 ? 1 : (icld >= 4 ? 2 * nlay : nlay);
-  if (irng == 0) return kiss_jumps(permuteseed, nlay, per_sub, s, jumps);
+  if (irng == 0) return kiss_jumps(permuteseed, per_sub, s, jumps);
   long long ndraw = (long long)NGPT * ncol * (icld == 3 ? 1 : (icld >= 4 ? 2LL * nlay : nlay));
   void *buf = nullptr;
   MtJumpTable jt{};
@@ -1295,7 +1293,6 @@ static int fused_setup(int ncol, int nlay, int icld, int iaer, int permuteseed, 
   a.sc.heatfac = ctx.heatfac;
   a.sc.icld = icld >= 1 ? 1 : 0, a.sc.iaer = iaer;
   a.sc.inflag = inflgsw, a.sc.iceflag = iceflgsw, a.sc.liqflag = liqflgsw;
-  if (icld >= 1 && inflgsw == 1) return fail(RRTMGSW_ERR_INFLAG, "INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA");
   if (ncol == 0) return 0;
   const void *need[] = {play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, o2vmr, asdir, asdif,
                         aldir, aldif, coszen};
@@ -1451,11 +1448,9 @@ int rrtmgsw_mcica_sw_dev_noalloc(int ncol, int nlay, int icld, int iaer, int per
       (size_t)ncol, launch_column_limit((RRTMGSW_NOALLOC_SCRATCH_BYTES - rt_off) / per_col));
   if (icld >= 1) {
     if (a.mc_irng == 0) {
-      KissJump J[NGPT];   // per-sub-column jumps, on the host stack
-      const uint64_t W = permuteseed > 0 ? (uint64_t)permuteseed : 0;
-      for (int g = 0; g < NGPT; g++) J[g] = kiss_jump_of(W + (uint64_t)g * (uint64_t)per_sub);
-      HIPCHK(hipMemcpyAsync(base + jump_off, J, sizeof J, hipMemcpyHostToDevice, s));
-      HIPCHK(hipStreamSynchronize(s));   // J leaves scope
+      // the per-sub-column jumps, formed on the GPU ahead of the columns' launches
+      if (int rc = kiss_jumps(permuteseed, per_sub, s, reinterpret_cast<KissJump *>(base + jump_off)))
+        return rc;
       t.kiss_jump = reinterpret_cast<const uint32_t *>(base + jump_off);
     } else {
       // one workgroup generates the stream in order (no jump-ahead table)

@@ -412,7 +412,8 @@ __device__ static CloudOpt cloud_optics_in(const DeviceTables &T, const SwScalar
 }
 
 // cldprmc_sw for one (g, layer) (src/rrtmg_sw_cldprmc.f90:141-347): a cell failing the cloud test
-// keeps its McICA inputs; inflag = 1 is the reference's STOP.
+// keeps its McICA inputs; a cell passing it with inflag = 1 is the reference's STOP (a call whose
+// cells all fail the test runs); another inflag than 0, 1 and 2 keeps the inputs.
 __device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &sc, int band,
                                         double cldf, double ciwp, double clwp, double rei,
                                         double rel, double tauc, double ssac, double asmc,
@@ -420,10 +421,8 @@ __device__ static CloudOpt cloud_optics(const DeviceTables &T, const SwScalars &
   const CloudOpt keep = {tauc, ssac, asmc, tauc};
   double cwp = ciwp + clwp;
   if (!(cldf >= CLDMIN && (cwp >= CLDMIN || tauc >= CLDMIN))) return keep;
-  if (sc.inflag != 0 && sc.inflag != 2) {
-    err = RRTMGSW_ERR_INFLAG;
-    return keep;
-  }
+  if (sc.inflag == 1) err = RRTMGSW_ERR_INFLAG;
+  if (sc.inflag != 0 && sc.inflag != 2) return keep;
   return cloud_optics_in(T, sc, band, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc, err);
 }
 

@@ -285,6 +285,39 @@ def test_reference_stops(gpu_lib):
     assert_close(got, ob.run_rrtmg(cols, mc), "after errors")
 
 
+def test_inflag_one_without_cloudy_cells(gpu_lib):
+    """inflag = 1 stops only in a cell that passes the cloud test (src/rrtmg_sw_cldprmc.f90:145-166):
+    cloud-free columns (cloud fraction 0, water paths left as they are) run through the fused entry,
+    mcica_subcol_sw + rrtmg_sw and the adjoint and agree with the oracle; one cloudy layer stops."""
+    import torch
+    api = gpu_lib
+    cols = make_columns(6, 40, icld=2, iaer=0, seed=32)
+    cols.cldfrac = np.zeros_like(cols.cldfrac, order="F")
+    cols.inflgsw = 1
+    mc, _ = ob.run_mcica(cols)
+    assert not mc["cldfmcl"].any()
+    ref = ob.run_rrtmg(cols, mc)
+    out = api.mcica_rrtmg_sw(cols, api.device_inputs(cols))
+    torch.cuda.synchronize()
+    assert_close({k: api.to_host(v) for k, v in out.items()}, ref, "fused, inflag 1")
+    gmc = api.mcica_subcol_sw(**mc_args(cols))
+    assert_close(api.rrtmg_sw(**sw_args(cols, gmc)), ref, "host ABI, inflag 1")
+    st = torch.ones(cols.ncol, dtype=torch.float64, device="cuda")
+    sh = torch.ones(cols.nlay, cols.ncol, dtype=torch.float64, device="cuda")
+    g = api.mcica_rrtmg_sw_adjoint(cols, api.device_inputs(cols), st, sh)
+    torch.cuda.synchronize()
+    assert all(torch.isfinite(v).all() for v in g.values())
+    bad = cols.subset(np.arange(6))
+    f = bad.cldfrac.copy()
+    f[3, np.argmax(bad.ciwp[3] + bad.clwp[3])] = 1.0   # a layer holding condensate
+    bad.cldfrac = np.asfortranarray(f)
+    with pytest.raises(RuntimeError, match="INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA"):
+        ob.run_rrtmg(bad, ob.run_mcica(bad)[0])
+    with pytest.raises(api.RRTMGSWError, match="INFLAG = 1 OPTION NOT AVAILABLE WITH MCICA") as e:
+        api.mcica_rrtmg_sw(bad, api.device_inputs(bad))
+    assert e.value.code == 3
+
+
 def test_finalize_and_reinit(gpu_lib):
     """rrtmgsw_finalize frees the tables and buffers; calls then return RRTMGSW_ERR_NOT_INITIALIZED
     until rrtmgsw_init, after which results are bit-for-bit those before."""
