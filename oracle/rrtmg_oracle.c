@@ -2251,7 +2251,8 @@ static int column_run(int nlay, const double *pavel, const double *tavel, const 
   for (int l = 0; l < L; l++)
     for (int m = 0; m < 7; m++) wkl[m + MXMOL * l] = wkl7[m + 7 * l];
   setcoef(L, pavel, tavel, coldry, wkl, &c);
-  double cossza = zenith == 0. ? 1.e-10 : zenith;
+  double cossza = zenith;
+  if (cossza == 0.) cossza = 1.e-10;
   double alb[NBND];
   for (int ib = 0; ib < NBND; ib++) alb[ib] = 1. - semiss[ib];
   double *zc0 = calloc((size_t)L * NGPT, 8), *zw1 = malloc(8 * (size_t)L * NGPT);

@@ -56,6 +56,7 @@ struct Context {
   HostTables host;
   DeviceTables dev{};
   double *d_tab = nullptr, *d_exp = nullptr, *d_pthr = nullptr;
+  void *d_kisspow = nullptr;      // KissPow: the KISS maps of 2^k draws
   GPoint *d_gp = nullptr;
   std::mutex mu;
   std::map<hipStream_t, Scratch> scratch, status, mtbuf, kissbuf, adjwork, hostbuf;
@@ -273,10 +274,10 @@ int host_call(const HostIn *in, int nin, const HostOut *out, int nout, F run) {
 struct Affine {
   uint32_t a, c;
 };
-__host__ __device__ Affine affine_compose(Affine f, Affine g) {  // f(g(x))
+Affine affine_compose(Affine f, Affine g) {  // f(g(x))
   return {f.a * g.a, f.a * g.c + f.c};
 }
-__host__ __device__ Affine affine_pow(Affine f, uint64_t n) {
+Affine affine_pow(Affine f, uint64_t n) {
   Affine r = {1u, 0u};
   while (n) {
     if (n & 1) r = affine_compose(f, r);
@@ -288,18 +289,18 @@ __host__ __device__ Affine affine_pow(Affine f, uint64_t n) {
 struct Gf2 {
   uint32_t col[32];
 };
-__host__ __device__ uint32_t gf2_apply(const Gf2 &m, uint32_t x) {
+uint32_t gf2_apply(const Gf2 &m, uint32_t x) {
   uint32_t r = 0;
   for (int i = 0; i < 32; i++)
     if ((x >> i) & 1u) r ^= m.col[i];
   return r;
 }
-__host__ __device__ Gf2 gf2_compose(const Gf2 &a, const Gf2 &b) {  // a(b(x))
+Gf2 gf2_compose(const Gf2 &a, const Gf2 &b) {  // a(b(x))
   Gf2 r;
   for (int i = 0; i < 32; i++) r.col[i] = gf2_apply(a, b.col[i]);
   return r;
 }
-__host__ __device__ Gf2 gf2_pow(Gf2 m, uint64_t n) {
+Gf2 gf2_pow(Gf2 m, uint64_t n) {
   Gf2 r;
   for (int i = 0; i < 32; i++) r.col[i] = 1u << i;
   while (n) {
@@ -309,7 +310,7 @@ __host__ __device__ Gf2 gf2_pow(Gf2 m, uint64_t n) {
   }
   return r;
 }
-__host__ __device__ uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
   uint64_t r = 1 % m;
   b %= m;
   while (e) {
@@ -320,10 +321,8 @@ __host__ __device__ uint32_t powmod(uint64_t b, uint64_t e, uint64_t m) {
   return (uint32_t)r;
 }
 
-// The state map of n kissvec draws (src/mcica_subcol_gen_sw.f90:721-753): an affine map for the
-// congruential part, a GF(2) matrix power for the xorshift, modular powers for the two
-// multiply-with-carry parts.
-__host__ __device__ KissJump kiss_jump_of(uint64_t n) {
+// The one-draw maps of the four kissvec components (src/mcica_subcol_gen_sw.f90:721-753).
+Gf2 kiss_xorshift() {
   Gf2 xs;
   for (int i = 0; i < 32; i++) {
     uint32_t k = 1u << i;
@@ -332,29 +331,76 @@ __host__ __device__ KissJump kiss_jump_of(uint64_t n) {
     k ^= k << 5;
     xs.col[i] = k;
   }
-  const Affine lcg = {69069u, 1327217885u};
+  return xs;
+}
+const Affine KISS_LCG = {69069u, 1327217885u};
+constexpr uint64_t KISS_M3 = 1179647999ull, KISS_M4 = 2025259007ull;   // 18000 2^16 - 1, 30903 2^16 - 1
+
+// The state map of n kissvec draws: an affine map for the congruential part, a GF(2) matrix
+// power for the xorshift, modular powers for the two multiply-with-carry parts.
+KissJump kiss_jump_of(uint64_t n) {
   KissJump J;
-  Affine a = affine_pow(lcg, n);
-  Gf2 m = gf2_pow(xs, n);
+  Affine a = affine_pow(KISS_LCG, n);
+  Gf2 m = gf2_pow(kiss_xorshift(), n);
   J.a1 = a.a, J.c1 = a.c;
-  for (int i = 0; i < 32; i++) J.m2[i] = m.col[i];
-  J.p3 = powmod(18000, n, 1179647999ull);
-  J.p4 = powmod(30903, n, 2025259007ull);
+  memcpy(J.m2, m.col, sizeof m.col);
+  J.p3 = powmod(18000, n, KISS_M3);
+  J.p4 = powmod(30903, n, KISS_M4);
   return J;
 }
 
-// The jump of each g-point's sub-column, g*per_sub + w draws into its column's stream, one
-// lane per g-point.
-__global__ void __launch_bounds__(128) kiss_jumps_kernel(uint64_t w, uint64_t per_sub, KissJump *out) {
-  const int g = threadIdx.x;
-  if (g < NGPT) out[g] = kiss_jump_of(w + (uint64_t)g * per_sub);
+// The maps of 2^k draws, k = 0..63, built once at initialisation: the jump of n draws is the
+// composition of those of n's set bits (all powers of one map commute).
+struct KissPow {
+  uint32_t m2[64][32];   // xorshift matrix columns
+  uint32_t a1[64], c1[64], p3[64], p4[64];
+};
+KissPow kiss_pow_table() {
+  KissPow P;
+  Gf2 m = kiss_xorshift();
+  Affine a = KISS_LCG;
+  uint64_t p3 = 18000, p4 = 30903;
+  for (int k = 0; k < 64; k++) {
+    memcpy(P.m2[k], m.col, sizeof m.col);
+    P.a1[k] = a.a, P.c1[k] = a.c, P.p3[k] = (uint32_t)p3, P.p4[k] = (uint32_t)p4;
+    m = gf2_compose(m, m), a = affine_compose(a, a);
+    p3 = p3 * p3 % KISS_M3, p4 = p4 * p4 % KISS_M4;
+  }
+  return P;
+}
+
+// The jump records of the 112 sub-columns, n = w + g per_sub draws into the column's stream:
+// 32 lanes per g-point, lane i forming column i of the xorshift matrix by applying the maps of
+// n's set bits (k < kmax, the bit length of the largest n) to the identity's column i; lane 0
+// also forms the affine and multiply-with-carry parts.
+__global__ void __launch_bounds__(64) kiss_jumps_kernel(const KissPow *P, uint64_t w, uint64_t per_sub,
+                                                        int kmax, KissJump *out) {
+  const int lane = threadIdx.x & 31, g = blockIdx.x * 2 + (threadIdx.x >> 5);
+  if (g >= NGPT) return;
+  const uint64_t n = w + (uint64_t)g * per_sub;
+  uint32_t col = 1u << lane, a = 1u, c = 0u;
+  uint64_t q3 = 1, q4 = 1;
+  for (int k = 0; k < kmax; k++) {
+    if (!((n >> k) & 1u)) continue;
+    const uint32_t *m = P->m2[k];
+    uint32_t r = 0;
+    for (int j = 0; j < 32; j++) r ^= m[j] & (0u - ((col >> j) & 1u));
+    col = r;
+    c = P->a1[k] * c + P->c1[k], a = P->a1[k] * a;
+    q3 = q3 * P->p3[k] % KISS_M3, q4 = q4 * P->p4[k] % KISS_M4;
+  }
+  out[g].m2[lane] = col;
+  if (lane == 0) out[g].a1 = a, out[g].c1 = c, out[g].p3 = (uint32_t)q3, out[g].p4 = (uint32_t)q4;
 }
 
 // The per-sub-column KISS jumps for (permuteseed, draws per sub-column), formed on the GPU into
-// the stream's own buffer ahead of the work queued after it: no host copy, no wait, no cache.
+// `buf` ahead of the work queued after it on s: no host copy, no wait, no cache.
 int kiss_jumps(int permuteseed, int per_sub, hipStream_t s, KissJump *buf) {
   const uint64_t w = permuteseed > 0 ? (uint64_t)permuteseed : 0;
-  hipLaunchKernelGGL(kiss_jumps_kernel, dim3(1), dim3(128), 0, s, w, (uint64_t)per_sub, buf);
+  const uint64_t nmax = w + (uint64_t)(NGPT - 1) * (uint64_t)per_sub;
+  const int kmax = nmax ? 64 - __builtin_clzll(nmax) : 0;
+  hipLaunchKernelGGL(kiss_jumps_kernel, dim3((NGPT + 1) / 2), dim3(64), 0, s,
+                     (const KissPow *)ctx.d_kisspow, w, (uint64_t)per_sub, kmax, buf);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -625,6 +671,11 @@ int rrtmgsw_init(double cpdair) {
   HIPCHK(hipMemcpy(ctx.d_exp, H.exp_tbl.data(), H.exp_tbl.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx.d_pthr, H.pthr.size() * sizeof(double)));
   HIPCHK(hipMemcpy(ctx.d_pthr, H.pthr.data(), H.pthr.size() * sizeof(double), hipMemcpyHostToDevice));
+  {
+    const KissPow P = kiss_pow_table();
+    HIPCHK(hipMalloc(&ctx.d_kisspow, sizeof P));
+    HIPCHK(hipMemcpy(ctx.d_kisspow, &P, sizeof P, hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMalloc(&ctx.d_gp, H.gp.size() * sizeof(GPoint)));
   HIPCHK(hipMemcpy(ctx.d_gp, H.gp.data(), H.gp.size() * sizeof(GPoint), hipMemcpyHostToDevice));
   DeviceTables &D = ctx.dev;
@@ -652,6 +703,7 @@ int rrtmgsw_finalize(void) {
   hipFree(ctx.d_exp);
   hipFree(ctx.d_gp);
   hipFree(ctx.d_pthr);
+  hipFree(ctx.d_kisspow);
   for (auto *m : {&ctx.scratch, &ctx.status, &ctx.mtbuf, &ctx.kissbuf, &ctx.adjwork, &ctx.hostbuf}) {
     for (auto &kv : *m) hipFree(kv.second.ptr);
     m->clear();

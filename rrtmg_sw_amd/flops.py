@@ -1,6 +1,12 @@
 """Algorithmic floating-point operation count of the GCM-mode shortwave path.
 
-The count is the reference algorithm's own work (not what a particular implementation issues):
+workload_flops(name) is the counted figure: the +, -, *, / and sqrt that the oracle executes on the
+whole workload (mcica_subcol_sw + rrtmg_sw), counted by its operation-counting build and stored
+in data/flops_counted.json by oracle/count_flops.py. The tally below is an independent reading of
+the reference's sources, routine by routine, kept as a cross-check (they agree within 10 %) and
+for workloads the table does not hold.
+
+The tally is the reference algorithm's own work (not what a particular implementation issues):
 each +, -, *, / and sqrt counts 1; comparisons, min/max, table gathers, integer and random-number
 arithmetic count 0. It is tallied routine by routine from the reference sources:
 
@@ -46,3 +52,19 @@ def fused_bytes_per_column(nlay, iaer=0):
     aer = 42 * L if iaer == 10 else 0
     out = 4 * (L + 1) + 2 * L
     return 8 * (prof + cloud + aer + out)
+
+
+_COUNTED = None
+
+
+def workload_flops(name):
+    """Counted FLOPs of one call on the named workload, or None when it was not counted."""
+    global _COUNTED
+    if _COUNTED is None:
+        import json
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "flops_counted.json")
+        with open(path) as f:
+            _COUNTED = json.load(f)
+    rec = _COUNTED.get(name)
+    return None if rec is None else rec["total"]

@@ -1,4 +1,5 @@
-"""Finite-difference checks of the adjoint along a direction (test infrastructure).
+"""Finite-difference check of the adjoint along a direction, on the GPU's own exact-exponential
+forward (the function the adjoint differentiates).
 
 The exact-exponential forward is piecewise smooth: its tables are interpolated linearly within
 cells, and reftra's conservative / general branch switch is a small jump. A central difference

@@ -1,5 +1,6 @@
-"""ctypes bindings for the checkers: the C oracle (oracle/liborc.so) and the compiled reference
-(oracle/_ref/librrtmg_ref.so). Test infrastructure only."""
+"""ctypes bindings for the checkers: the C oracle (oracle/liborc.so), its operation-counting build
+(oracle/liborc_count.so) and the compiled reference (oracle/_ref/librrtmg_ref.so). Test
+infrastructure only."""
 import ctypes as C
 import os
 
@@ -29,7 +30,10 @@ def _ip_of(a):
     return a.ctypes.data_as(_ip)
 
 
+COUNT_PATH = os.path.join(ROOT, "oracle", "liborc_count.so")
+
 _orc = None
+_cnt = None
 _ref = None
 _ref_nm = None
 
@@ -37,21 +41,46 @@ _ref_nm = None
 def orc():
     global _orc
     if _orc is None:
-        lib = C.CDLL(ORC_PATH)
-        lib.orc_init.argtypes = [C.c_char_p, C.c_double]
-        lib.orc_last_error.restype = C.c_char_p
-        lib.orc_exp_tbl.restype = C.c_double
-        lib.orc_exp_tbl.argtypes = [C.c_int]
-        lib.orc_table.argtypes = [C.c_char_p, _dp]
-        lib.orc_kiss_step.restype = C.c_double
-        lib.orc_mt_first_int.restype = C.c_uint
-        lib.orc_mt_first_int.argtypes = [C.c_int, C.c_long]
-        lib.orc_mt_reals.argtypes = [C.c_int, C.c_long, _dp]
-        rc = lib.orc_init(BLOB.encode(), CPDAIR)
-        if rc:
-            raise RuntimeError(lib.orc_last_error().decode())
-        _orc = lib
+        _orc = _load_orc(ORC_PATH)
     return _orc
+
+
+def orc_count():
+    """The oracle's operation-counting build (oracle/orc_count.cpp): the same interface, plus
+    orc_fp_reset / orc_fp_counts."""
+    global _cnt
+    if _cnt is None:
+        _cnt = _load_orc(COUNT_PATH)
+        _cnt.orc_fp_counts.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    return _cnt
+
+
+def fp_counts():
+    """(+-*/ and sqrt, exp/log/pow) counted by the counting build since its last reset."""
+    a, b = C.c_uint64(0), C.c_uint64(0)
+    orc_count().orc_fp_counts(C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def _lib(which):
+    return orc_count() if which == "count" else orc()
+
+
+def _load_orc(path):
+    lib = C.CDLL(path)
+    lib.orc_init.argtypes = [C.c_char_p, C.c_double]
+    lib.orc_last_error.restype = C.c_char_p
+    lib.orc_exp_tbl.restype = C.c_double
+    lib.orc_exp_tbl.argtypes = [C.c_int]
+    lib.orc_table.argtypes = [C.c_char_p, _dp]
+    lib.orc_kiss_step.restype = C.c_double
+    lib.orc_mt_first_int.restype = C.c_uint
+    lib.orc_mt_first_int.argtypes = [C.c_int, C.c_long]
+    lib.orc_mt_reals.argtypes = [C.c_int, C.c_long, _dp]
+    rc = lib.orc_init(BLOB.encode(), CPDAIR)
+    if rc:
+        raise RuntimeError(lib.orc_last_error().decode())
+    return lib
 
 
 def ref_available():
@@ -111,10 +140,11 @@ def run_nomcica(cols, which="orc", icld=None, iaer=None, solvar=None):
         sv = [np.ascontiguousarray(solvar[0], dtype=np.float64),
               np.ascontiguousarray(solvar[1], dtype=np.float64),
               np.array([solvar[2]], dtype=np.float64)]
-    if which == "orc":
-        rc = orc().orc_rrtmg_sw_nomcica(*common, *([None] * 3 if sv is None else [_p(x) for x in sv]))
+    if which in ("orc", "count"):
+        lib = _lib(which)
+        rc = lib.orc_rrtmg_sw_nomcica(*common, *([None] * 3 if sv is None else [_p(x) for x in sv]))
         if rc:
-            raise RuntimeError(orc().orc_last_error().decode())
+            raise RuntimeError(lib.orc_last_error().decode())
     else:
         if sv is None:
             sv = [np.ones(NBND), np.ones(2), np.zeros(1)]
@@ -147,10 +177,10 @@ def run_mcica(cols, which="orc"):
             _p(o["cldfmcl"]), _p(o["ciwpmcl"]), _p(o["clwpmcl"]), _p(o["reicmcl"]),
             _p(o["relqmcl"]), _p(o["taucmcl"]), _p(o["ssacmcl"]), _p(o["asmcmcl"]),
             _p(o["fsfcmcl"])]
-    if which == "orc":
-        rc = orc().orc_mcica_subcol_sw(*args)
+    if which in ("orc", "count"):
+        rc = _lib(which).orc_mcica_subcol_sw(*args)
         if rc:
-            raise RuntimeError(orc().orc_last_error().decode())
+            raise RuntimeError(_lib(which).orc_last_error().decode())
     else:
         ref().ref_mcica_subcol_sw(*args)
     return o, irng.value
@@ -217,15 +247,16 @@ def run_rrtmg(cols, mc=None, which="orc", bndsolvar=None, indsolvar=None, solcyc
               _p(mc["reicmcl"]), _p(mc["relqmcl"]), _p(cols.tauaer), _p(cols.ssaaer),
               _p(cols.asmaer), _p(cols.ecaer), _p(out["swuflx"]), _p(out["swdflx"]),
               _p(out["swhr"]), _p(out["swuflxc"]), _p(out["swdflxc"]), _p(out["swhrc"])]
-    if which == "orc":
+    if which in ("orc", "count"):
         bu = bd = dird = None
         if bands:
             bu, bd = _out((N, L + 1, NBND)), _out((N, L + 1, NBND))
             dird = _out((N, L + 1))
-        rc = orc().orc_rrtmg_sw_ext(*common, _p(bnd), _p(ind), _p(frac), _p(bu), _p(bd),
-                                    _p(dird), C.c_int(idelm))
+        lib = _lib(which)
+        rc = lib.orc_rrtmg_sw_ext(*common, _p(bnd), _p(ind), _p(frac), _p(bu), _p(bd),
+                                  _p(dird), C.c_int(idelm))
         if rc:
-            raise RuntimeError(orc().orc_last_error().decode())
+            raise RuntimeError(lib.orc_last_error().decode())
         if bands:
             out["bandu"], out["bandd"], out["dirdflx"] = bu, bd, dird
     else:

@@ -264,9 +264,9 @@ def test_adjoint_directional_derivative(gpu_lib):
     """<dJ/dx, dx> against finite differences along a random relative direction touching every
     input at once (dx = 1e-4 |x| r, r uniform in [-1, 1]): the sum over all elements stands far
     above the rounding floor, also for the heating-rate sum whose elements mostly do not. The
-    quotients are those of fd_util.directional; each column within 1e-4 relative of one."""
+    quotients are those of gradcheck.directional; each column within 1e-4 relative of one."""
     import torch
-    import fd_util
+    from rrtmg_sw_amd import gradcheck
     api = gpu_lib
     cols = _fd_sample()
     N, L = cols.ncol, cols.nlay
@@ -281,7 +281,7 @@ def test_adjoint_directional_derivative(gpu_lib):
         return {k: api.to_host(v) for k, v in g.items()}
 
     objs = {1: (np.ones(N), np.zeros((N, L))), 2: (np.zeros(N), np.ones((N, L)))}
-    for name, (rel, n1, n8) in fd_util.directional(api, cols, dirs, objs, adjoint).items():
+    for name, (rel, n1, n8) in gradcheck.directional(api, cols, dirs, objs, adjoint).items():
         print(f"J{name} directional over {N} columns: relative error median {np.median(rel):.2e}, "
               f"max {rel.max():.2e}; {n1} columns on a one-sided quotient, {n8} at step / 8")
         assert rel.max() <= 1e-4, (name, np.sort(rel)[-5:])

@@ -91,7 +91,7 @@ def test_large_store_equals_checkpoint(gpu_lib, big):
 
 def test_large_directional_derivative(gpu_lib, big):
     import torch
-    import fd_util
+    from rrtmg_sw_amd import gradcheck
     api = gpu_lib
     cols, _, sample = big
     sub = cols.subset(sample)
@@ -107,7 +107,7 @@ def test_large_directional_derivative(gpu_lib, big):
             torch.cuda.synchronize()
             return {k: api.to_host(v) for k, v in g.items()}
 
-        for name, (rel, n1, n8) in fd_util.directional(api, sub, dirs, objs, adjoint).items():
+        for name, (rel, n1, n8) in gradcheck.directional(api, sub, dirs, objs, adjoint).items():
             print(f"expran65536x137 J{name} ckpt {ck} directional over {N} columns: relative "
                   f"error median {np.median(rel):.2e}, max {rel.max():.2e}; {n1} one-sided, "
                   f"{n8} at step / 8")

@@ -69,6 +69,18 @@ def test_mcica_subcol_bitexact(gpu_lib, icld, irng):
         assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("icld,permuteseed", [(2, 0), (2, 1), (4, 113), (1, 5000), (5, 100003)])
+def test_mcica_subcol_permuteseeds(gpu_lib, icld, permuteseed):
+    """KISS sub-column jumps over the seed range: permuteseed warm-up draws, then per_sub draws
+    per g-point (the jumps compose the maps of 2^k draws for n's set bits)."""
+    cols = make_columns(12, 48, icld=icld, iaer=0, seed=250 + icld)
+    cols.permuteseed = permuteseed
+    got = gpu_lib.mcica_subcol_sw(**mc_args(cols))
+    ref, _ = ob.run_mcica(cols)
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), (k, permuteseed)
+
+
 def test_mcica_subcol_full_width_kiss(gpu_lib):
     """KISS masks of 256 columns of the 8192 x 72 workload (per-column streams, so a column
     subset is exactly the full run's result)."""
