@@ -386,8 +386,14 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
 // Doubles of work area per column: (UB (L) + per-layer states and gas optics (5 L, store mode) or
 // checkpoints (3 ceil(L/K)) and a segment buffer (3 K)) times NGPT, then the cloud table CJ
 // (L, NBND, CJ_N).
+// A layer's record for the reverse sweep, per g-point: the sweep state at its top (tdbt, ztdn,
+// prdnd) and its gas optics (taug, taur); in checkpoint mode also its two-stream solution
+// (r, rd, t, td, dbt), which the segment's re-sweep has just formed. (In store mode the reverse
+// sweep recomputes the solution: keeping it for every layer measured slower, the extra HBM
+// traffic costing more than the recomputation.)
+constexpr int REC_S = 5, REC_N = 10;
 __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
-  size_t states = K == 0 ? (size_t)L * 5 : (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
+  size_t states = K == 0 ? (size_t)L * REC_S : (size_t)((L + K - 1) / K) * 3 + (size_t)K * REC_N;
   return ((size_t)L + states) * NGPT + (size_t)L * NBND * CJ_N;
 }
 
@@ -480,14 +486,20 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   const double albdif = vis ? a.asdif[c] : a.aldif[c];
   const double zinc = active ? incident_flux(gp, T.tab, v, a.sc, mu0) : 0.0;
 
-  // Per-column work area: UB (L, NGPT) = dJ/dU_j, then either every sweep state
-  // S (L, 3, NGPT) (store mode) or one state per K layers CK (ceil(L/K), 3, NGPT) plus a
-  // segment buffer SG (K, 3, NGPT) (checkpoint mode, K = A.ckpt).
+  // Per-column work area: UB (L, NGPT) = dJ/dU_j, then either every layer's record
+  // S (L, REC_S, NGPT) (store mode) or one sweep state per K layers CK (ceil(L/K), 3, NGPT)
+  // plus the records of one K-layer segment SG (K, REC_N, NGPT) (checkpoint mode, K = A.ckpt).
   double *S = UB + (size_t)L * NGPT;
   double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
-  const int SW = K == 0 ? 5 : 3;   // store mode: tdbt ztdn prdnd taug taur per layer
-  auto sp = [&](int j, int k) -> double & { return S[((size_t)j * SW + k) * NGPT + g]; };
-  auto sg = [&](int j, int k) -> double & { return SG[((size_t)j * 3 + k) * NGPT + g]; };
+  auto ck = [&](int i, int k) -> double & { return S[((size_t)i * 3 + k) * NGPT + g]; };
+  // record k of layer j: S in store mode, SG (segment-relative) in checkpoint mode
+  auto rec = [&](int j, int k) -> double & {
+    return K == 0 ? S[((size_t)j * REC_S + k) * NGPT + g] : SG[((size_t)(j % K) * REC_N + k) * NGPT + g];
+  };
+  auto put_rec = [&](int j, const Sweep &s, double taug, double taur, const LayerV &ly) {
+    rec(j, 0) = s.tdbt, rec(j, 1) = s.ztdn, rec(j, 2) = s.prdnd, rec(j, 3) = taug, rec(j, 4) = taur;
+    if (K > 0) rec(j, 5) = ly.r, rec(j, 6) = ly.rd, rec(j, 7) = ly.t, rec(j, 8) = ly.td, rec(j, 9) = ly.dbt;
+  };
 
   // Gas optics of layer j with the lane's constants of the loop's regime (q), or of the other
   // regime's, read from the g-point record, for a layer on the other side of laytrop (checkpoint
@@ -515,17 +527,11 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   Sweep st = {1.0, 1.0, 0.0};
   double carry = 0.0;
   auto f1_layer = [&](auto lower_tag, const GasLane &q, int j) {
-    if (active) {
-      if (K == 0) {
-        sp(j, 0) = st.tdbt, sp(j, 1) = st.ztdn, sp(j, 2) = st.prdnd;
-      } else if (j % K == 0) {
-        sp(j / K, 0) = st.tdbt, sp(j / K, 1) = st.ztdn, sp(j / K, 2) = st.prdnd;
-      }
-    }
+    if (active && K > 0 && j % K == 0) ck(j / K, 0) = st.tdbt, ck(j / K, 1) = st.ztdn, ck(j / K, 2) = st.prdnd;
     LayerIn in;
     double taug, taur;
     LayerV ly = layer_at(lower_tag, q, j, in, taug, taur);
-    if (active && K == 0) sp(j, 3) = taug, sp(j, 4) = taur;
+    if (active && K == 0) put_rec(j, st, taug, taur, ly);
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     double Aa, Bb, Cc, Ee;
     sweep_layer(st, l, ly.dbt, Aa, Bb, Cc, Ee);
@@ -579,37 +585,37 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   };
   for (int j = L - 1; j >= 0; j--) {
     if (K > 0 && (j == L - 1 || (j + 1) % K == 0)) {
-      // checkpoint mode: recompute the sweep states of this segment from its checkpoint
+      // checkpoint mode: the records of this segment, re-swept from its checkpoint with the
+      // forward sweep's own functions (the same values as F1's, so the same as store mode's)
       const int j0 = (j / K) * K;
-      Sweep ss = {active ? sp(j0 / K, 0) : 1.0, active ? sp(j0 / K, 1) : 1.0, active ? sp(j0 / K, 2) : 0.0};
-      for (int jj = j0; jj < j; jj++) {
-        if (active) sg(jj - j0, 0) = ss.tdbt, sg(jj - j0, 1) = ss.ztdn, sg(jj - j0, 2) = ss.prdnd;
+      Sweep ss = {active ? ck(j0 / K, 0) : 1.0, active ? ck(j0 / K, 1) : 1.0, active ? ck(j0 / K, 2) : 0.0};
+      for (int jj = j0; jj <= j; jj++) {
         double taug, taur;
         gas_rec(jj, taug, taur);
         const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - jj, active);
         LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
-        RT4 l = {ly.r, ly.rd, ly.t, ly.td};
-        double Aa, Bb, Cc, Ee;
-        sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
+        if (active) put_rec(jj, ss, taug, taur, ly);
+        if (jj < j) {
+          RT4 l = {ly.r, ly.rd, ly.t, ly.td};
+          double Aa, Bb, Cc, Ee;
+          sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
+        }
       }
-      if (active) sg(j - j0, 0) = ss.tdbt, sg(j - j0, 1) = ss.ztdn, sg(j - j0, 2) = ss.prdnd;
     }
-    Sweep sj;
-    if (K == 0)
-      sj = {active ? sp(j, 0) : 1.0, active ? sp(j, 1) : 1.0, active ? sp(j, 2) : 0.0};
-    else
-      sj = {active ? sg(j % K, 0) : 1.0, active ? sg(j % K, 1) : 1.0, active ? sg(j % K, 2) : 0.0};
+    // the layer's record: sweep state at its top, gas optics, and in checkpoint mode the
+    // two-stream solution (store mode recomputes it with the same function, so both modes see
+    // the same values)
+    const Sweep sj = {active ? rec(j, 0) : 1.0, active ? rec(j, 1) : 1.0, active ? rec(j, 2) : 0.0};
+    const double taug = active ? rec(j, 3) : 1., taur = active ? rec(j, 4) : 0.1;
     const double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
     const int lay = L - 1 - j;
-    double taug, taur;
-    // gas optics stored (store mode) or recomputed; aerosol, cloud and layer optics recomputed
-    // at one call site for both modes (identical rounding)
-    if (K == 0)
-      taug = active ? sp(j, 3) : 1., taur = active ? sp(j, 4) : 0.1;
-    else
-      gas_rec(j, taug, taur);
     const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
-    const LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
+    LayerV ly;
+    if (K == 0)
+      ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
+    else
+      ly = {active ? rec(j, 5) : 0., active ? rec(j, 6) : 0., active ? rec(j, 7) : 1.,
+            active ? rec(j, 8) : 1., active ? rec(j, 9) : 1.};
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     // C_j, E_j and U_j from the layer
     Sweep tmp = sj;
