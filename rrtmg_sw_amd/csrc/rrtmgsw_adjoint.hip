@@ -110,7 +110,7 @@ __device__ inline void gas_adj_r(const GasLane &q, const double *__restrict__ ta
     mol_b = tr_b * q.rayl;
   }
   const double cm_b = tg_b * q.mcoef;
-  const double c1 = v.col[q.sp1 + lay], c2 = v.col[q.sp2 + lay];
+  const double c1 = v.c(q.s1, lay), c2 = v.c(q.s2, lay);
   double c1_b = q.bin ? 0.0 : kmul_b * q.kmult, c2_b = 0.0;
   key_spec_adj(c1, c2, q.strrat, LOWER ? 8.0 : 4.0, q.bin, fs_b, q.bin ? kmul_b : 0.0, c1_b, c2_b);
 #pragma unroll

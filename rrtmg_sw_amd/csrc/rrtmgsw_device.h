@@ -66,8 +66,10 @@ struct ColView {
     redaf = reinterpret_cast<float *>(smem + lo.o_reda);
     mask = reinterpret_cast<uint32_t *>(smem + lo.o_mask);
   }
-  __device__ double &f(int k, int lay) { return fac[k * L + lay]; }   // 0..7 as listed above
-  __device__ double &c(int s, int lay) { return col[s * L + lay]; }   // 0..4 species, 5 colmol
+  // layer-major: a layer's eight fractions (its six columns) are adjacent, so the gas optics
+  // reads them from one address with immediate offsets
+  __device__ double &f(int k, int lay) { return fac[lay * 8 + k]; }   // 0..7 as listed above
+  __device__ double &c(int s, int lay) { return col[lay * 6 + s]; }   // 0..4 species, 5 colmol
 };
 // misc slots
 constexpr int M_LAYTROP = 0, M_LAYSOL = 1;   // laysolfr of band b at misc[M_LAYSOL + b]
@@ -846,8 +848,7 @@ __device__ inline void sweep_r(Sweep &s, const RT4 &l, double dbt, double &Cc, d
 // gas_optics, e.g. src/rrtmg_sw_taumol.f90:287-375 lower and :357-383 upper).
 struct GasLane {
   uint32_t kb, kjs, kt, kp, ks;   // key table: base, species step, temperature, pressure, row
-  int sp1, sp2, spm;              // LDS column offsets of the key pair and the minor absorber
-  int s1, s2, sm;                 // their species indices
+  int s1, s2, sm;                 // species of the key pair and the minor absorber (0..4)
   double strrat, kmult, mcoef, rayl;
   bool bin, rayl_js, cont;        // binary key; Rayleigh interpolated in js; continuum present
   uint32_t selfb, selfs, forb, fors, rab, ras;
@@ -881,7 +882,6 @@ __device__ inline GasLane gas_lane(const GPoint &g, int L) {
   q.s1 = LOWER ? g.lo_s1 : g.hi_s1;
   q.s2 = LOWER ? g.lo_s2 : g.hi_s2;
   q.sm = LOWER ? g.lo_minor : g.hi_minor;
-  q.sp1 = q.s1 * L, q.sp2 = q.s2 * L, q.spm = q.sm * L;
   q.strrat = LOWER ? g.lo_strrat : g.hi_strrat;
   q.kmult = LOWER ? g.lo_mult : g.hi_mult;
   q.mcoef = LOWER ? g.lo_minor_coef : g.hi_minor_coef;
@@ -917,7 +917,7 @@ __device__ inline GasFetch<R> gas_fetch(const GasLane &q, const R *__restrict__ 
   GasFetch<R> f;
   const int jp = uni(LOWER ? min(v.jp[lay], 12) : max(v.jp[lay], 13));
   const int jt = uni(v.jt[lay]), jt1 = uni(v.jt1[lay]);
-  const R c1 = R(v.col[q.sp1 + lay]), c2 = R(v.col[q.sp2 + lay]);
+  const R c1 = R(v.c(q.s1, lay)), c2 = R(v.c(q.s2, lay));
   SpecT<R> s = key_spec_r<R>(c1, c2, R(q.strrat), R(LOWER ? 8 : 4), q.bin);
   f.kmul = q.bin ? s.speccomb : c1 * R(q.kmult);
   f.fs = s.fs;
@@ -955,7 +955,7 @@ __device__ inline void gas_finish(const GasLane &q, const GasFetch<R> &f, ColVie
   R cont = uni(R(v.f(6, lay))) * (f.fo0 + uni(R(v.f(7, lay))) * (f.fo1 - f.fo0));
   if (LOWER) cont = uni(R(v.f(4, lay))) * (f.s0 + uni(R(v.f(5, lay))) * (f.s1 - f.s0)) + cont;
   cont = q.cont ? cont : R(0);
-  taug = f.kmul * k + colh2o * cont + R(v.col[q.spm + lay]) * R(q.mcoef);
+  taug = f.kmul * k + colh2o * cont + R(v.c(q.sm, lay)) * R(q.mcoef);
   R ray = R(q.rayl);
   if (LOWER) ray = q.rayl_js ? f.r0 + fs * (f.r1 - f.r0) : ray;
   taur = colmol * ray;
