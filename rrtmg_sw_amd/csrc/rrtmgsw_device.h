@@ -484,6 +484,14 @@ __device__ inline double frcp(double x) {
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
 }
+// One Newton-Raphson step (relative error the square of the estimate's, below 1e-13): for the
+// forward forms' optics and adding-method denominators, whose reciprocals only scale fluxes;
+// the reciprocals that pick a table entry or a branch keep frcp.
+__device__ inline double frcp1(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
 
 // ------------------------------------------------------------------------------------------
 // Layer reflectance, diffuse reflectance, transmittance and diffuse transmittance.
@@ -797,7 +805,7 @@ __device__ inline RT4 reftra_back(const RtFront &f, double zto1, double zw, doub
   if (f.cons) {  // conservative scattering, :188-228
     double za1 = g1 * prmuz - g3;
     double zgt = g1 * zto1;
-    double inv = frcp(1. + zgt);
+    double inv = frcp1(1. + zgt);
     o.r = (zgt - za1 * (1. - e2)) * inv;
     o.t = 1. - o.r;
     o.rd = zgt * inv;
@@ -812,7 +820,7 @@ __device__ inline RT4 reftra_back(const RtFront &f, double zto1, double zw, doub
     double P = 1. - kmu * kmu;
     bool tiny = fabs(P * D) <= eps * e1;
     P = tiny ? 1. : P;
-    double rD = frcp(P * D);
+    double rD = frcp1(P * D);
     double NR = (1. - kmu) * (a2 + k * g3) - (1. + kmu) * (a2 - k * g3) * ee - 2. * k * (g3 - a2 * prmuz) * (e1 * e2);
     double NT = e2 * ((1. + kmu) * (a1 + k * g4) - (1. - kmu) * (a1 - k * g4) * ee) - 2. * k * (g4 + a1 * prmuz) * e1;
     o.r = tiny ? eps : zw * NR * rD;
@@ -832,7 +840,7 @@ __device__ inline RT4 reftra_e(double zto1, double zw, double zg, double prmuz, 
 // before the layer.
 __device__ inline void sweep_r(Sweep &s, const RT4 &l, double dbt, double &Cc, double &E) {
   double a = s.ztdn - s.tdbt;
-  double zreflect = frcp(1. - l.rd * s.prdnd);
+  double zreflect = frcp1(1. - l.rd * s.prdnd);
   Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
   E = l.td * zreflect;
   double ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;

@@ -370,11 +370,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     const double ta = in.ta, wa = in.wa, ga = in.ga, cf = in.cf, ct = in.ct, cw = in.cw, cg = in.cg;
     const double tau0 = taur + taug + ta;
     const double sct = taur * 1.0 + ta * wa;
-    const double r1 = frcp(sct * tau0);
+    const double r1 = frcp1(sct * tau0);
     const double g0 = ga * wa * ta * tau0 * r1;
     const double w0 = sct * sct * r1;
     const double zwf = w0 * (g0 * g0);
-    const double r2 = frcp((1.0 - zwf) * (1.0 + g0));
+    const double r2 = frcp1((1.0 - zwf) * (1.0 + g0));
     const double tauc = (1.0 - zwf) * tau0;
     const double omc = (w0 - zwf) * (1.0 + g0) * r2;
     const double gcc = g0 * (1.0 - zwf) * r2;
@@ -408,7 +408,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
         // total-sky optics (:500-505): go = (ct cw cg + tauc omc gcc) / so, omo = so / tauo
         const double tauo = tauc + ct;
         const double so = tauc * omc + ct * cw;
-        const double r3 = frcp(so * tauo);
+        const double r3 = frcp1(so * tauo);
         const double go = (ct * cw * cg + tauc * omc * gcc) * tauo * r3;
         const double omo = so * so * r3;
         RT4 ro;
@@ -444,11 +444,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     const double ta = in.ta, wa = in.wa, ga = in.ga, cf = in.cf, ct = in.ct, cw = in.cw, cg = in.cg;
     const double tau0 = taur + taug + ta;
     const double sct = taur * 1.0 + ta * wa;
-    const double r1 = frcp(sct * tau0);
+    const double r1 = frcp1(sct * tau0);
     const double g0 = ga * wa * ta * tau0 * r1;
     const double w0 = sct * sct * r1;
     const double zwf = w0 * (g0 * g0);
-    const double r2 = frcp((1.0 - zwf) * (1.0 + g0));
+    const double r2 = frcp1((1.0 - zwf) * (1.0 + g0));
     const double tauc = (1.0 - zwf) * tau0;
     const double omc = (w0 - zwf) * (1.0 + g0) * r2;
     const double gcc = g0 * (1.0 - zwf) * r2;
@@ -477,7 +477,7 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       if (cf != 0.0) {   // a cloudy sub-column cell (cf = 1): the total sky is the cloudy one
         const double tauo = tauc + ct;
         const double so = tauc * omc + ct * cw;
-        const double r3 = frcp(so * tauo);
+        const double r3 = frcp1(so * tauo);
         const double go = (ct * cw * cg + tauc * omc * gcc) * tauo * r3;
         const double omo = so * so * r3;
         const double cco = (tauc * ccl + ct * (1.0 - cw)) * so * r3;   // 1 - omo
