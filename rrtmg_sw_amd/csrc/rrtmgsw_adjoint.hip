@@ -420,7 +420,13 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   for (int i = tid; i < L * AC_N; i += blockDim.x) acc[i] = 0.0;
   if (tid < 4) albacc[tid] = 0.0;
   __syncthreads();
-  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+  for (int lay = tid; lay < L; lay += blockDim.x) {
+    setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+    if (a.sc.icld >= 1) {   // staged for the mask drawing
+      v.cf[lay] = a.cldfrac[c + (size_t)C * lay];
+      v.al[lay] = a.mc_icld >= 4 ? a.alpha[c + (size_t)C * lay] : 0.0;
+    }
+  }
   for (int j = tid; j <= L; j += blockDim.x) {
     // top-based level j is bottom-based level i = L - j; swhr(k) = (net(k+1) - net(k)) hf / pdp(k)
     // for k < L - 1 (the top layer's heating rate is zero)
@@ -446,7 +452,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   }
   int err = 0;
   if (a.sc.icld >= 1 && active)
-    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
+    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, v.cf, v.al, 1,
                        a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
   __syncthreads();
   // cloud optics and their water-path derivatives for each (layer, band) with a cloudy sub-column

@@ -24,7 +24,7 @@ constexpr double CLDMIN = 1.e-20;           // src/rrtmg_sw_cldprmc.f90:118
 struct ColLayout {
   int L;
   int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
-  int o_fac, o_col, o_red, o_reda, o_mask, total;     // double blocks / mask words
+  int o_fac, o_col, o_red, o_reda, o_cf, o_mask, total;   // double blocks / mask words
   // fluxes = false leaves out the per-level flux sums (red, reda) of the forward kernels; mixed
   // holds them in FP32, with the layers' absorbed fluxes (red: 6 a level, reda: 4 a layer)
   __host__ __device__ ColLayout(int nlay, int nmaskw, bool fluxes = true, bool mixed = false) {
@@ -42,6 +42,7 @@ struct ColLayout {
     o_red = bytes, bytes += !fluxes ? 0 : mixed ? 2 * (L + 1) * 6 * 4 : 2 * (L + 1) * 4 * 8;
     o_reda = bytes, bytes += !fluxes ? 0 : mixed ? 2 * L * 4 * 4 : 2 * L * 2 * 8;
     bytes = ((bytes + 15) / 16) * 16;
+    o_cf = bytes, bytes += nmaskw > 0 ? 2 * L * 8 : 0;   // masks drawn here: cldfrac, alpha
     o_mask = bytes, bytes += nmaskw * NGPT * 4;
     total = bytes;
   }
@@ -51,6 +52,7 @@ struct ColView {
   int L;
   int *jp, *jt, *jt1, *inds, *indf, *misc;
   double *fac, *col, *red, *reda;
+  double *cf, *al;       // the column's cloud fractions and overlap parameters (mask drawing)
   float *redf, *redaf;   // the mixed form's FP32 sums, in place of red / reda
   uint32_t *mask;
   __device__ ColView(char *smem, const ColLayout &lo) {
@@ -64,6 +66,7 @@ struct ColView {
     reda = reinterpret_cast<double *>(smem + lo.o_reda);
     redf = reinterpret_cast<float *>(smem + lo.o_red);
     redaf = reinterpret_cast<float *>(smem + lo.o_reda);
+    cf = reinterpret_cast<double *>(smem + lo.o_cf), al = cf + L;
     mask = reinterpret_cast<uint32_t *>(smem + lo.o_mask);
   }
   // layer-major: a layer's eight fractions (its six columns) are adjacent, so the gas optics
@@ -1160,8 +1163,10 @@ __device__ inline double mt_real(uint32_t u) { return (double)u / (4294967296.0 
 // Sample mode (the column program's McICA loop, src/rrtmg_sw.1col.f90:577-600, with every sample
 // a workgroup): col_jump[c] advances column c's KISS streams by its own permuteseed, and each
 // column reads an MT19937 stream of its own, mt + c * mt_stride, indexed as a single column.
+// cf[l * cs] and al[l * cs] are the column's cloud fraction and overlap parameter of layer l
+// (the caller's arrays at column c with stride ncol, or the copies staged in LDS with stride 1).
 __device__ static int column_masks(int icld, int irng, int permuteseed, int C, int L, int c, int g,
-                                   const double *play, const double *cldfrac, const double *alpha,
+                                   const double *play, const double *cf, const double *al, size_t cs,
                                    const uint32_t *mt, const KissJump *jumps, uint32_t *mask,
                                    const KissJump *col_jump = nullptr, size_t mt_stride = 0) {
 #pragma clang fp contract(off)
@@ -1181,8 +1186,9 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
   double cdf_prev = 0.0, r3 = 0.0;
   if (icld == 3) r3 = irng == 0 ? kiss_draw(k) : mt_real(mt[(size_t)g * mC + mc]);
   uint32_t word = 0;
+  double cldf_prev = 0.0;
   for (int l = 0; l < L; l++) {
-    double cldf = cldfrac[c + (size_t)C * l];
+    double cldf = cf[l * cs];
     if (cldf < CLDMIN) cldf = 0.0;
     double cdf, cdf2 = 0.0;
     if (icld == 3) {
@@ -1200,15 +1206,15 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
     }
     if (l > 0) {
       if (icld == 2) {  // :541-551
-        double cprev = cldfrac[c + (size_t)C * (l - 1)];
-        if (cprev < CLDMIN) cprev = 0.0;
+        const double cprev = cldf_prev;   // layer l - 1's fraction, clamped as above
         if (cdf_prev > 1. - cprev) cdf = cdf_prev;
         else cdf = cdf * (1. - cprev);
       } else if (icld >= 4) {  // :607-611, :647-651
-        if (cdf2 < alpha[c + (size_t)C * l]) cdf = cdf_prev;
+        if (cdf2 < al[l * cs]) cdf = cdf_prev;
       }
     }
     cdf_prev = cdf;
+    cldf_prev = cldf;
     if (cdf >= 1. - cldf) word |= 1u << (l & 31);
     if ((l & 31) == 31 || l == L - 1) {
       mask[(l >> 5) * NGPT + g] = word;

@@ -201,7 +201,15 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
   if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0, v.misc[M_LAYLOW] = 0;
   __syncthreads();
-  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+  for (int lay = tid; lay < L; lay += blockDim.x) {
+    setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+    // the column's cloud fractions and overlap parameters, for the mask drawing below (its
+    // layer-by-layer reads then wait on LDS, not on a load from memory)
+    if (FUSED && a.sc.icld >= 1) {
+      v.cf[lay] = a.cldfrac[c + (size_t)C * lay];
+      v.al[lay] = a.mc_icld >= 4 ? a.alpha[c + (size_t)C * lay] : 0.0;
+    }
+  }
   __syncthreads();
   const int laytrop = v.misc[M_LAYTROP];
   if (tid < NBND) {
@@ -211,11 +219,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   int err = 0;
   if (FUSED && a.sc.icld >= 1 && active) {
     if (SPEC)   // the column program's McICA samples
-      err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
+      err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, v.cf, v.al, 1,
                          a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask,
                          reinterpret_cast<const KissJump *>(a.col_jump), a.mt_stride);
     else
-      err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
+      err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, v.cf, v.al, 1,
                          a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
   }
   if (LAYERCLD && a.sc.icld >= 1)   // src/rrtmg_sw_rad.nomcica.f90:617-622
@@ -773,8 +781,9 @@ __global__ void __launch_bounds__(128) mcica_kernel(McicaArgs a, DeviceTables T)
   const int C = a.ncol, L = a.nlay;
   const bool active = g < NGPT;
   if (active) {
-    int e = column_masks(a.icld, a.irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac, a.alpha,
-                         a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), mask);
+    int e = column_masks(a.icld, a.irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac + c,
+                         a.alpha ? a.alpha + c : nullptr, (size_t)C, a.mt_stream,
+                         reinterpret_cast<const KissJump *>(T.kiss_jump), mask);
     if (e) atomicCAS(a.status, 0, e);
     // band of g (ngb, src/rrtmg_sw_init.f90:340-353)
     int ibnd = T.gp[g].ibnd;
