@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   const RtArgs &a = A.f;
   const int L = a.nlay, C = a.ncol;
   const int nmw = (L + 31) / 32;
-  ColLayout lo(L, nmw, false);
+  ColLayout lo(L, nmw, false, false, false);
   ColView v(smem, lo);
   double *acc = reinterpret_cast<double *>(smem + ((lo.total + 15) & ~15));   // (L, AC_N)
   double *wU = acc + (size_t)L * AC_N, *wD = wU + (L + 1);     // level weights, top-based
@@ -420,13 +420,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   for (int i = tid; i < L * AC_N; i += blockDim.x) acc[i] = 0.0;
   if (tid < 4) albacc[tid] = 0.0;
   __syncthreads();
-  for (int lay = tid; lay < L; lay += blockDim.x) {
-    setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
-    if (a.sc.icld >= 1) {   // staged for the mask drawing
-      v.cf[lay] = a.cldfrac[c + (size_t)C * lay];
-      v.al[lay] = a.mc_icld >= 4 ? a.alpha[c + (size_t)C * lay] : 0.0;
-    }
-  }
+  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
   for (int j = tid; j <= L; j += blockDim.x) {
     // top-based level j is bottom-based level i = L - j; swhr(k) = (net(k+1) - net(k)) hf / pdp(k)
     // for k < L - 1 (the top layer's heating rate is zero)
@@ -452,7 +446,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   }
   int err = 0;
   if (a.sc.icld >= 1 && active)
-    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, v.cf, v.al, 1,
+    // the masks read the cloud fractions from memory: staged in LDS they would take the
+    // workgroup past the 40 KB at which four fit a CU at 137 layers
+    err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac + c,
+                       a.alpha ? a.alpha + c : nullptr, (size_t)C,
                        a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
   __syncthreads();
   // cloud optics and their water-path derivatives for each (layer, band) with a cloudy sub-column
@@ -712,7 +709,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 }
 
 size_t adj_lds_bytes(int nlay) {
-  ColLayout lo(nlay, (nlay + 31) / 32, false);
+  ColLayout lo(nlay, (nlay + 31) / 32, false, false, false);
   return ((lo.total + 15) & ~15) + sizeof(double) * ((size_t)nlay * AC_N + 2 * (nlay + 1) + 4);
 }
 

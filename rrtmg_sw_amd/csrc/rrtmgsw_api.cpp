@@ -583,7 +583,7 @@ size_t launch_column_limit(size_t n) {
 }
 
 int chunk_columns(int ncol, int nlay) {
-  size_t per = scr_col_doubles(nlay) * sizeof(double);
+  size_t per = scr_alloc_doubles(nlay) * sizeof(double);
   size_t cap = (size_t)4 << 30;
   size_t n = cap / per;
   if (n < 64) n = 64;
@@ -625,7 +625,7 @@ int run_rt(const RtArgs &base, const DeviceTables &tables, hipStream_t s, int pr
   void *scr = nullptr, *st = nullptr;
   {
     std::lock_guard<std::mutex> lk(ctx.mu);
-    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * scr_col_doubles(base.nlay) * sizeof(double), &scr)) return rc;
+    if (int rc = stream_buffer(ctx.scratch, s, (size_t)chunk * scr_alloc_doubles(base.nlay) * sizeof(double), &scr)) return rc;
     if (int rc = stream_buffer(ctx.status, s, 256, &st)) return rc;
   }
   RtArgs a = base;
@@ -1490,7 +1490,7 @@ int rrtmgsw_mcica_sw_dev_noalloc(int ncol, int nlay, int icld, int iaer, int per
   const bool mt = icld >= 1 && a.mc_irng == 1;
   const size_t ndraw = mt ? (size_t)NGPT * ncol * per_sub : 0;
   const size_t rt_off = mt_off + ((ndraw * 4 + 255) / 256) * 256;
-  const size_t per_col = scr_col_doubles(nlay) * sizeof(double);
+  const size_t per_col = scr_alloc_doubles(nlay) * sizeof(double);
   if (rt_off + per_col > RRTMGSW_NOALLOC_SCRATCH_BYTES)
     return fail(RRTMGSW_ERR_ARGUMENT,
                 "rrtmgsw_mcica_sw_dev_noalloc: the Mersenne Twister stream of this call (" +

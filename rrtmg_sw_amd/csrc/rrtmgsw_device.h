@@ -27,7 +27,9 @@ struct ColLayout {
   int o_fac, o_col, o_red, o_reda, o_cf, o_mask, total;   // double blocks / mask words
   // fluxes = false leaves out the per-level flux sums (red, reda) of the forward kernels; mixed
   // holds them in FP32, with the layers' absorbed fluxes (red: 6 a level, reda: 4 a layer)
-  __host__ __device__ ColLayout(int nlay, int nmaskw, bool fluxes = true, bool mixed = false) {
+  // stage = false leaves out the staged cloud fractions (cf, al) of the mask drawing
+  __host__ __device__ ColLayout(int nlay, int nmaskw, bool fluxes = true, bool mixed = false,
+                                bool stage = true) {
     L = nlay;
     int ib = 0;
     o_jp = ib, ib += L;
@@ -42,7 +44,7 @@ struct ColLayout {
     o_red = bytes, bytes += !fluxes ? 0 : mixed ? 2 * (L + 1) * 6 * 4 : 2 * (L + 1) * 4 * 8;
     o_reda = bytes, bytes += !fluxes ? 0 : mixed ? 2 * L * 4 * 4 : 2 * L * 2 * 8;
     bytes = ((bytes + 15) / 16) * 16;
-    o_cf = bytes, bytes += nmaskw > 0 ? 2 * L * 8 : 0;   // masks drawn here: cldfrac, alpha
+    o_cf = bytes, bytes += nmaskw > 0 && stage ? 2 * L * 8 : 0;   // masks drawn here: cldfrac, alpha
     o_mask = bytes, bytes += nmaskw * NGPT * 4;
     total = bytes;
   }
@@ -1187,23 +1189,10 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
   if (icld == 3) r3 = irng == 0 ? kiss_draw(k) : mt_real(mt[(size_t)g * mC + mc]);
   uint32_t word = 0;
   double cldf_prev = 0.0;
-  for (int l = 0; l < L; l++) {
+  // one layer: its draws (cdf, cdf2), the overlap recurrence, the mask bit
+  auto layer = [&](int l, double cdf, double cdf2) {
     double cldf = cf[l * cs];
     if (cldf < CLDMIN) cldf = 0.0;
-    double cdf, cdf2 = 0.0;
-    if (icld == 3) {
-      cdf = r3;
-    } else if (irng == 0) {
-      cdf = kiss_draw(k);
-      if (icld >= 4) cdf2 = kiss_draw(k);
-    } else {
-      size_t base = ((size_t)g * mC + mc) * L + l;
-      if (icld >= 4) {
-        cdf = mt_real(mt[2 * base]);
-        cdf2 = mt_real(mt[2 * base + 1]);
-      } else
-        cdf = mt_real(mt[base]);
-    }
     if (l > 0) {
       if (icld == 2) {  // :541-551
         const double cprev = cldf_prev;   // layer l - 1's fraction, clamped as above
@@ -1219,6 +1208,39 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
     if ((l & 31) == 31 || l == L - 1) {
       mask[(l >> 5) * NGPT + g] = word;
       word = 0;
+    }
+  };
+  if (irng == 0 || icld == 3) {
+    for (int l = 0; l < L; l++) {
+      double cdf = r3, cdf2 = 0.0;
+      if (icld != 3) {
+        cdf = kiss_draw(k);
+        if (icld >= 4) cdf2 = kiss_draw(k);
+      }
+      layer(l, cdf, cdf2);
+    }
+  } else {
+    // the Mersenne Twister stream's words of four layers requested together (they are
+    // consecutive: draw index ((g C + c) L + l), pairs for icld 4/5)
+    for (int l0 = 0; l0 < L; l0 += 4) {
+      const int nl = min(4, L - l0);
+      const size_t base = ((size_t)g * mC + mc) * L + l0;
+      uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+      if (icld >= 4) {
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (u < 2 * nl) w[u] = mt[2 * base + u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (u < nl) w[u] = mt[base + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (u < nl) {
+          if (icld >= 4) layer(l0 + u, mt_real(w[2 * u]), mt_real(w[2 * u + 1]));
+          else layer(l0 + u, mt_real(w[u]), 0.0);
+        }
     }
   }
   return 0;

@@ -15,9 +15,15 @@ constexpr int MAX_LAY = 256;   // layer limit of the per-column LDS staging
 constexpr int NTBL = 10000;    // exp lookup table, modules/rrsw_tbl.f90:28
 constexpr int SCR_COEFS = 6;   // elimination coefficients kept per (layer, g-point): B C E for two skies
 // Per-column scratch of the forward kernels, in doubles: the elimination coefficients
-// (nlay, SCR_COEFS, NGPT), then the cloud optics (tau, ssa, asymmetry) of each (layer, band) of
-// the fused entry (nlay, NBND, 3).
+// (nlay, SCR_COEFS, NGPT), then the cloud optics (tau, ssa, asymmetry, unscaled tau) of each
+// (layer, band) (nlay, NBND, 4). After the launch's columns come each column's per-level flux
+// sums over g (scr_red_doubles: 2 waves x (nlay + 1) x 4, then 2 x nlay x 2; the mixed form keeps
+// FP32 sums in the same space), used instead of LDS where a column's LDS would otherwise limit
+// the workgroups per CU below what the registers allow (137 layers, for instance).
 constexpr size_t scr_col_doubles(int nlay) { return (size_t)nlay * (SCR_COEFS * NGPT + 4 * NBND); }
+constexpr size_t scr_red_doubles(int nlay) { return 8 * ((size_t)nlay + 1) + 4 * (size_t)nlay; }
+// what one column of a launch takes of the scratch buffer
+constexpr size_t scr_alloc_doubles(int nlay) { return scr_col_doubles(nlay) + scr_red_doubles(nlay); }
 // Pressure thresholds of setcoef_sw's decisions (src/rrtmg_sw_setcoef.f90:169-176, :208-210):
 // pthr[PTHR_JP + k - 2] (k = 2..58) is the smallest pressure at which jp < k, pthr[PTHR_TROP] the
 // smallest at which log p > 4.56, pthr[PTHR_LOW] the smallest at which log p >= 6.62.

@@ -26,16 +26,17 @@
 namespace rrtmgsw {
 
 // Broadband fluxes at the nlay+1 levels and heating rates (src/rrtmg_sw_rad.f90:805-836,
-// top layer 0) from the per-level sums over g of both waves in LDS.
-__device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
+// top layer 0) from the per-level sums over g of both waves (red, reda: in LDS or the column's
+// scratch).
+__device__ inline void rt_epilogue(const RtArgs &a, const double *red, const double *reda, int c) {
   const int L = a.nlay, C = a.ncol, tid = threadIdx.x;
   // level j (top-based): up = red[.][0], down = red[.][1] (+ reda total), clear in 2, 3 (+ reda clear)
   auto level = [&](int j, double &fu, double &fd, double &cu, double &cd) {
-    const double *r0 = v.red + (0 * (L + 1) + j) * 4, *r1 = v.red + (1 * (L + 1) + j) * 4;
+    const double *r0 = red + (0 * (L + 1) + j) * 4, *r1 = red + (1 * (L + 1) + j) * 4;
     fu = r0[0] + r1[0], fd = r0[1] + r1[1], cu = r0[2] + r1[2], cd = r0[3] + r1[3];
     if (j < L) {
-      fd += v.reda[(0 * L + j) * 2 + 0] + v.reda[(1 * L + j) * 2 + 0];
-      cd += v.reda[(0 * L + j) * 2 + 1] + v.reda[(1 * L + j) * 2 + 1];
+      fd += reda[(0 * L + j) * 2 + 0] + reda[(1 * L + j) * 2 + 0];
+      cd += reda[(0 * L + j) * 2 + 1] + reda[(1 * L + j) * 2 + 1];
     }
   };
   for (int j = tid; j <= L; j += blockDim.x) {
@@ -65,9 +66,9 @@ __device__ inline void rt_epilogue(const RtArgs &a, ColView &v, int c) {
 // times its absorbed flux (pass 1's part + this one) summed over g: the layer's net-flux
 // divergence formed without the difference of two nearly equal fluxes.
 // ------------------------------------------------------------------------------------------
-__device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const double *scr, int gi,
-                                      int lane, int wave, int ibnd, float zinc, int jfirst,
-                                      const SweepF &scf, const SweepF &stf) {
+__device__ inline void rt_pass2_mixed(const RtArgs &a, float *redf, const float *redaf, int c,
+                                      const double *scr, int gi, int lane, int wave, int ibnd,
+                                      float zinc, int jfirst, const SweepF &scf, const SweepF &stf) {
   const int L = a.nlay, C = a.ncol;
   const bool vis = ibnd >= 9 && ibnd <= 12;
   const float albdir = float(vis ? a.asdir[c] : a.aldir[c]);
@@ -77,7 +78,7 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const 
   {
     const float dc = scf.ztdn + scf.prdnd * uc, dt = stf.ztdn + stf.prdnd * ut;
     const float s = wave_sum4f(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
-    float *r = v.redf + (wave * (L + 1) + L) * 6;
+    float *r = redf + (wave * (L + 1) + L) * 6;
     if (lane < 16 && (lane & 3) == 0) r[lane >> 2] = s;
     if (lane == 0) r[4] = 0.f, r[5] = 0.f;
   }
@@ -102,7 +103,7 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const 
     const float bu_c = o.cB * uc;
     const float s4 = wave_sum4f(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
     const float s2 = wave_sum2f(zinc * (ad_t * (bu_t + ub_t)), zinc * (o.cA * (bu_c + ub_c)), lane);
-    float *r = v.redf + (wave * (L + 1) + j) * 6;
+    float *r = redf + (wave * (L + 1) + j) * 6;
     if (lane < 16 && (lane & 3) == 0) r[lane >> 2] = s4;
     if (lane == 0 || lane == 8) r[4 + (lane >> 3)] = s2;
   };
@@ -125,11 +126,11 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const 
   // epilogue: fluxes at the nlay+1 levels; heating rates from the layers' absorbed fluxes
   const int tid = threadIdx.x;
   for (int jl = tid; jl <= L; jl += blockDim.x) {
-    const float *q0 = v.redf + (0 * (L + 1) + jl) * 6, *q1 = v.redf + (1 * (L + 1) + jl) * 6;
+    const float *q0 = redf + (0 * (L + 1) + jl) * 6, *q1 = redf + (1 * (L + 1) + jl) * 6;
     double fu = double(q0[0]) + q1[0], fd = double(q0[1]) + q1[1];
     double cu = double(q0[2]) + q1[2], cd = double(q0[3]) + q1[3];
     if (jl < L) {
-      const float *p0 = v.redaf + (0 * L + jl) * 4, *p1 = v.redaf + (1 * L + jl) * 4;
+      const float *p0 = redaf + (0 * L + jl) * 4, *p1 = redaf + (1 * L + jl) * 4;
       fd += double(p0[0]) + p1[0];
       cd += double(p0[1]) + p1[1];
     }
@@ -138,8 +139,8 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const 
   }
   for (int i = tid; i < L; i += blockDim.x) {
     const int jl = L - 1 - i;   // layer i (bottom-based) is layer jl from the top
-    const float *q0 = v.redf + (0 * (L + 1) + jl) * 6, *q1 = v.redf + (1 * (L + 1) + jl) * 6;
-    const float *p0 = v.redaf + (0 * L + jl) * 4, *p1 = v.redaf + (1 * L + jl) * 4;
+    const float *q0 = redf + (0 * (L + 1) + jl) * 6, *q1 = redf + (1 * (L + 1) + jl) * 6;
+    const float *p0 = redaf + (0 * L + jl) * 4, *p1 = redaf + (1 * L + jl) * 4;
     const double abs_t = (double(p0[2]) + p1[2]) + (double(q0[4]) + q1[4]);
     const double abs_c = (double(p0[3]) + p1[3]) + (double(q0[5]) + q1[5]);
     const double pdp = a.plev[c + (size_t)C * i] - a.plev[c + (size_t)C * (i + 1)];
@@ -173,7 +174,9 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, ColView &v, int c, const 
 // a.gspec for per-band fluxes, the direct beam with or without delta scaling, per-band surface
 // albedo, layer amounts given directly.
 enum : int { CLD_SUBCOL = 0, CLD_FUSED = 1, CLD_LAYER = 2 };
-template <int MODE, int CLD, bool SPEC>
+// GSUM: the waves' per-level flux sums go to the column's scratch instead of LDS (a compile-time
+// choice, so the layer loops carry no branch or extra registers for it).
+template <int MODE, int CLD, bool SPEC, bool GSUM = false>
 __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, const DeviceTables &T) {
   static_assert(!SPEC || (MODE == 0 && CLD != CLD_SUBCOL), "spectral output: FP64 layer-cloud or fused form");
   constexpr bool EXACT = MODE == 2;
@@ -183,7 +186,8 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.nlay, C = a.ncol;
   static_assert(!MIXED || CLD == CLD_FUSED, "the mixed form is the fused entry's");
-  ColLayout lo(L, FUSED ? (L + 31) / 32 : 0, true, MIXED);
+  constexpr bool gsum = GSUM;
+  ColLayout lo(L, FUSED ? (L + 31) / 32 : 0, !gsum, MIXED);
   ColView v(smem, lo);
   // XCD-aware column order: consecutive columns run on the same XCD (its own L2), so the
   // strided (ncol, nlay) reads and writes of neighbouring columns share cache lines.
@@ -194,6 +198,9 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = wave * WAVE_G + lane;
   const bool active = lane < WAVE_G;
+  // the waves' per-level flux sums: in LDS (v.red ...), or in the column's scratch (gsum)
+  double *const gred = a.scratch + (size_t)a.ncol_chunk * scr_col_doubles(L) + (size_t)cl * scr_red_doubles(L);
+  double *const greda = gred + 8 * (L + 1);
   // Idle lanes (56-63) mirror their own wave's first g-point: every scratch location a lane
   // reads in pass 2 is written by its own wave in pass 1 (there is no barrier between the passes).
   const int gi = active ? g : wave * WAVE_G;
@@ -527,7 +534,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       // sums over g of the incident-flux-weighted ztdn and pass-1 absorbed flux, both skies
       const float zf = float(zinc);
       const float sa = wave_sum4f(zf * At, zf * Ac, zf * Pt, zf * Pc, lane);
-      if (lane < 16 && (lane & 3) == 0) v.redaf[(wave * L + j) * 4 + (lane >> 2)] = sa;
+      if (lane < 16 && (lane & 3) == 0) {
+        const int q = (wave * L + j) * 4 + (lane >> 2);
+        if (gsum) reinterpret_cast<float *>(greda)[q] = sa;
+        else v.redaf[q] = sa;
+      }
       return;
     }
     double At, Ac;
@@ -554,7 +565,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     pend_j = j, pend_t = jfirst <= j;
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
     const double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
-    if (lane == 0 || lane == 8) v.reda[(wave * L + j) * 2 + (lane >> 3)] = sa;
+    if (lane == 0 || lane == 8) {
+      const int q = (wave * L + j) * 2 + (lane >> 3);
+      if (gsum) greda[q] = sa;
+      else v.reda[q] = sa;
+    }
   };
   // Each layer issues its gas-optics table reads, then stores the coefficients of the layer
   // above.
@@ -571,7 +586,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   spec_level(L, st_.ztdn, st_.tdbt);
   if (err) atomicCAS(a.status, 0, err);
   if constexpr (MIXED) {
-    rt_pass2_mixed(a, v, c, scr, gi, lane, wave, ibnd, float(zinc), jfirst, scf, stf);
+    if (gsum)
+      rt_pass2_mixed(a, reinterpret_cast<float *>(gred), reinterpret_cast<const float *>(greda), c, scr,
+                     gi, lane, wave, ibnd, float(zinc), jfirst, scf, stf);
+    else
+      rt_pass2_mixed(a, v.redf, v.redaf, c, scr, gi, lane, wave, ibnd, float(zinc), jfirst, scf, stf);
     return;
   }
 
@@ -591,7 +610,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   double dt = st_.ztdn + st_.prdnd * ut;
   {
     const double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
-    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + L) * 4 + (lane >> 2)] = s;
+    if (lane < 16 && (lane & 3) == 0) {
+      const int q = (wave * (L + 1) + L) * 4 + (lane >> 2);
+      if (gsum) gred[q] = s;
+      else v.red[q] = s;
+    }
   }
   if (SPEC && active) {
     gs[(size_t)(L * 4 + 2) * NGPT] = zinc * ut;
@@ -625,7 +648,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       gs[(size_t)(j * 4 + 3) * NGPT] = zinc * bu_t;
     }
     const double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
-    if (lane < 16 && (lane & 3) == 0) v.red[(wave * (L + 1) + j) * 4 + (lane >> 2)] = s;
+    if (lane < 16 && (lane & 3) == 0) {
+      const int q = (wave * (L + 1) + j) * 4 + (lane >> 2);
+      if (gsum) gred[q] = s;
+      else v.red[q] = s;
+    }
   };
   Co r0 = fetch(L - 1), r1 = fetch(L - 2), r2 = fetch(L - 3), r3 = fetch(L - 4);
   int j = L - 1;
@@ -644,13 +671,16 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   if (j >= 1) step(j - 1, r1);
   if (j >= 2) step(j - 2, r2);
   __syncthreads();
-  rt_epilogue(a, v, c);
+  if (gsum)
+    rt_epilogue(a, gred, greda, c);
+  else
+    rt_epilogue(a, v.red, v.reda, c);
 }
 
-template <int MODE, int CLD>
+template <int MODE, int CLD, bool GSUM>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu((MODE == 0 && CLD != CLD_SUBCOL) || MODE == 1 ? 4 : 3)))
 rt_kernel(RtArgs a, DeviceTables T) {
-  rt_body<MODE, CLD, false>(a, T);
+  rt_body<MODE, CLD, false, GSUM>(a, T);
 }
 
 // The column program's forms: per-band and direct-beam outputs, with the non-McICA clouds
@@ -664,11 +694,24 @@ rt_column_mcica_kernel(RtArgs a, DeviceTables T) {
   rt_body<0, CLD_FUSED, true>(a, T);
 }
 
+// The fused forms' flux sums go to the column's scratch when keeping them in LDS would leave room
+// for fewer than the eight workgroups per CU that their registers allow (4 waves per SIMD); the
+// other forms keep them in LDS.
+static bool sums_global(int nlay, int nmw, bool mixed) {
+  const int per_cu = 160 * 1024 / 8;
+  return ColLayout(nlay, nmw, true, mixed).total > per_cu;
+}
+
 template <int MODE, int CLD>
 static hipError_t launch_rt(const RtArgs &a, const DeviceTables &t, hipStream_t s) {
-  ColLayout lo(a.nlay, CLD == CLD_FUSED ? (a.nlay + 31) / 32 : 0);
+  const int nmw = CLD == CLD_FUSED ? (a.nlay + 31) / 32 : 0;
+  const bool gsum = CLD == CLD_FUSED && sums_global(a.nlay, nmw, MODE == 1);
+  ColLayout lo(a.nlay, nmw, !gsum, MODE == 1);
   int per = (a.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL((rt_kernel<MODE, CLD>), dim3(per * 8), dim3(128), lo.total, s, a, t);
+  if (gsum)
+    hipLaunchKernelGGL((rt_kernel<MODE, CLD, CLD == CLD_FUSED>), dim3(per * 8), dim3(128), lo.total, s, a, t);
+  else
+    hipLaunchKernelGGL((rt_kernel<MODE, CLD, false>), dim3(per * 8), dim3(128), lo.total, s, a, t);
   return hipGetLastError();
 }
 
@@ -883,33 +926,54 @@ __global__ void __launch_bounds__(256) mt_kernel(uint32_t seed, long long ndraw,
 // words from the seed to 624 + 19936, then forms start k (thread w owns word w).
 constexpr int MT_EXT = 624 + 19936;   // raw words a jump reads
 // bits[offs[k - 1] .. offs[k]) are the set bits of x^(k J) mod P, each below 19937.
+// Segment k's start window: the XOR of the windows i words into the stream over the set bits i
+// of x^(k J) mod P. MT_PARTS workgroups share a segment, each taking a slice of its bits and
+// XOR-ing its partial window into the start (zeroed before the launch; XOR makes the order of
+// the parts immaterial). Each workgroup generates the stream's first MT_EXT words in LDS and
+// stages its bit list there in chunks (the threads of a wave read the same bit).
+constexpr int MT_PARTS = 4, MT_BCHUNK = 4096;
 __global__ void __launch_bounds__(640) mt_starts_kernel(uint32_t seed, const uint32_t *__restrict__ bits,
                                                         const uint32_t *__restrict__ offs, uint32_t *starts) {
   extern __shared__ __attribute__((aligned(16))) uint32_t msm[];
-  uint32_t *buf = msm;
-  const int t = threadIdx.x, k = blockIdx.x + 1;
+  uint32_t *buf = msm, *bl = msm + MT_EXT;
+  const int t = threadIdx.x, k = blockIdx.x / MT_PARTS + 1, part = blockIdx.x % MT_PARTS;
   if (t == 0) mt_init(seed, buf);
   __syncthreads();
-  if (k == 1)
+  if (k == 1 && part == 0)
     for (int i = t; i < 624; i += blockDim.x) starts[i] = buf[i];
   for (int r0 = 624; r0 < MT_EXT; r0 += 227) {
     const int r = r0 + t;
     if (t < 227 && r < MT_EXT) buf[r] = buf[r - 227] ^ mt_twist(buf[r - 624], buf[r - 623]);
     __syncthreads();
   }
-  if (t < 624) {
-    // eight independent LDS reads in flight per step; i <= 19936, so i + t stays below MT_EXT
-    const uint32_t b0 = offs[k - 1], b1 = offs[k];
-    uint32_t acc = 0, j = b0;
-    for (; j + 8 <= b1; j += 8) {
-      uint32_t v = 0;
+  const uint32_t b0 = offs[k - 1], n = offs[k] - b0;
+  const uint32_t p0 = b0 + (uint32_t)((uint64_t)n * part / MT_PARTS);
+  const uint32_t p1 = b0 + (uint32_t)((uint64_t)n * (part + 1) / MT_PARTS);
+  uint32_t acc = 0;
+  for (uint32_t c0 = p0; c0 < p1; c0 += MT_BCHUNK) {
+    const uint32_t cn = min((uint32_t)MT_BCHUNK, p1 - c0);
+    __syncthreads();   // the previous chunk is consumed
+    for (uint32_t i = t; i < cn; i += blockDim.x) bl[i] = bits[c0 + i];
+    __syncthreads();
+    if (t < 624) {
+      // eight independent LDS reads in flight per step; a bit is at most 19936, so bit + t stays
+      // below MT_EXT
+      uint32_t j = 0;
+      for (; j + 8 <= cn; j += 8) {
+        uint32_t v = 0;
 #pragma unroll
-      for (int q = 0; q < 8; q++) v ^= buf[bits[j + q] + t];
-      acc ^= v;
+        for (int q = 0; q < 8; q++) v ^= buf[bl[j + q] + t];
+        acc ^= v;
+      }
+      for (; j < cn; j++) acc ^= buf[bl[j] + t];
     }
-    for (; j < b1; j++) acc ^= buf[bits[j] + t];
-    starts[(size_t)k * 624 + t] = acc;
   }
+  if (t < 624) atomicXor(&starts[(size_t)k * 624 + t], acc);
+}
+
+__global__ void __launch_bounds__(256) zero_words(uint32_t *p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 // Segment k of a jumped stream: outputs [k J, min((k + 1) J, ndraw)) from start window k.
@@ -943,8 +1007,10 @@ hipError_t launch_mt_stream(uint32_t seed, long long ndraw, uint32_t *out, const
     const int nseg = (int)((ndraw + J - 1) / J);
     if (jt->log2_seg != lj || jt->count < nseg - 1) return hipErrorInvalidValue;
     uint32_t *starts = out + ndraw;   // the caller sized out for ndraw + mt_extra_words(ndraw)
-    hipLaunchKernelGGL(mt_starts_kernel, dim3(nseg - 1), dim3(640), MT_EXT * sizeof(uint32_t), s, seed,
-                       jt->bits, jt->offs, starts);
+    const size_t nz = (size_t)(nseg - 1) * 624;
+    hipLaunchKernelGGL(zero_words, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, starts + 624, nz);
+    hipLaunchKernelGGL(mt_starts_kernel, dim3((nseg - 1) * MT_PARTS), dim3(640),
+                       (MT_EXT + MT_BCHUNK) * sizeof(uint32_t), s, seed, jt->bits, jt->offs, starts);
     hipLaunchKernelGGL(mt_segment_kernel, dim3(nseg), dim3(256), 0, s, starts, ndraw, lj, out);
   } else {
     hipLaunchKernelGGL(mt_kernel, dim3(1), dim3(256), 0, s, seed, ndraw, out);

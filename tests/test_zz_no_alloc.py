@@ -30,7 +30,7 @@ def _same(ref, got, what):
 
 def test_noalloc_matches_allocating_entry(gpu_lib):
     api = gpu_lib
-    big = workload("mcica8192x72")                      # 13 launches of 640 columns
+    big = workload("mcica8192x72")                      # 14 launches of 629 columns
     small = workload("expran4096x60", ncol=300)         # one launch, icld 5 + aerosols
     for prec in ("fp64", "mixed", "exact"):
         ref, got = _pair(api, big, prec)
