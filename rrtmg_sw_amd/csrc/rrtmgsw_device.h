@@ -23,7 +23,7 @@ constexpr double CLDMIN = 1.e-20;           // src/rrtmg_sw_cldprmc.f90:118
 // ------------------------------------------------------------------------------------------
 struct ColLayout {
   int L;
-  int o_jp, o_jt, o_jt1, o_inds, o_indf, o_misc;      // int arrays (L each), misc (32 ints)
+  int o_ix, o_misc;   // packed setcoef indices (L words), misc (32 ints)
   int o_fac, o_col, o_red, o_reda, o_cf, o_mask, total;   // double blocks / mask words
   // fluxes = false leaves out the per-level flux sums (red, reda) of the forward kernels; mixed
   // holds them in FP32, with the layers' absorbed fluxes (red: 6 a level, reda: 4 a layer)
@@ -32,11 +32,7 @@ struct ColLayout {
                                 bool stage = true) {
     L = nlay;
     int ib = 0;
-    o_jp = ib, ib += L;
-    o_jt = ib, ib += L;
-    o_jt1 = ib, ib += L;
-    o_inds = ib, ib += L;
-    o_indf = ib, ib += L;
+    o_ix = ib, ib += L;
     o_misc = ib, ib += 32;
     int bytes = ((ib * 4 + 15) / 16) * 16;
     o_fac = bytes, bytes += 8 * L * 8;     // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac
@@ -52,7 +48,8 @@ struct ColLayout {
 
 struct ColView {
   int L;
-  int *jp, *jt, *jt1, *inds, *indf, *misc;
+  uint32_t *ix;   // a layer's jp | jt << 8 | jt1 << 12 | indself << 16 | indfor << 24
+  int *misc;
   double *fac, *col, *red, *reda;
   double *cf, *al;       // the column's cloud fractions and overlap parameters (mask drawing)
   float *redf, *redaf;   // the mixed form's FP32 sums, in place of red / reda
@@ -60,8 +57,7 @@ struct ColView {
   __device__ ColView(char *smem, const ColLayout &lo) {
     L = lo.L;
     int *ip = reinterpret_cast<int *>(smem);
-    jp = ip + lo.o_jp, jt = ip + lo.o_jt, jt1 = ip + lo.o_jt1, inds = ip + lo.o_inds;
-    indf = ip + lo.o_indf, misc = ip + lo.o_misc;
+    ix = reinterpret_cast<uint32_t *>(ip + lo.o_ix), misc = ip + lo.o_misc;
     fac = reinterpret_cast<double *>(smem + lo.o_fac);
     col = reinterpret_cast<double *>(smem + lo.o_col);
     red = reinterpret_cast<double *>(smem + lo.o_red);
@@ -75,6 +71,11 @@ struct ColView {
   // reads them from one address with immediate offsets
   __device__ double &f(int k, int lay) { return fac[lay * 8 + k]; }   // 0..7 as listed above
   __device__ double &c(int s, int lay) { return col[lay * 6 + s]; }   // 0..4 species, 5 colmol
+  // the packed indices: 0 jp, 1 jt, 2 jt1, 3 indself, 4 indfor (all small and non-negative)
+  __device__ static int idx(uint32_t w, int k) {
+    return k == 0 ? w & 0xff : k == 1 ? (w >> 8) & 0xf : k == 2 ? (w >> 12) & 0xf : k == 3 ? (w >> 16) & 0xff : w >> 24;
+  }
+  __device__ int jp(int lay) const { return ix[lay] & 0xff; }
 };
 // misc slots
 constexpr int M_LAYTROP = 0, M_LAYSOL = 1;   // laysolfr of band b at misc[M_LAYSOL + b]
@@ -185,7 +186,8 @@ __device__ static void setcoef_layer(const RtArgs &a, const DeviceTables &T, int
   if (colch4 == 0.) colch4 = 1.e-32 * coldry;
   if (colo2 == 0.) colo2 = 1.e-32 * coldry;
   double compfp = 1. - fp;
-  v.jp[lay] = jp, v.jt[lay] = jt, v.jt1[lay] = jt1, v.inds[lay] = indself, v.indf[lay] = indfor;
+  v.ix[lay] = (uint32_t)jp | (uint32_t)jt << 8 | (uint32_t)jt1 << 12 | (uint32_t)indself << 16 |
+              (uint32_t)indfor << 24;
   v.f(0, lay) = compfp * (1. - ft);   // fac00
   v.f(1, lay) = fp * (1. - ft1);      // fac01
   v.f(2, lay) = compfp * ft;          // fac10
@@ -202,12 +204,12 @@ __device__ static int find_laysolfr(const GPoint &g, const ColView &v, int laytr
   if (!g.sol_upper) {
     int ls = laytrop;
     for (int lay = 1; lay <= laytrop; lay++)
-      if (lay + 1 <= L && v.jp[lay - 1] < ref && v.jp[lay] >= ref) ls = min(lay + 1, laytrop);
+      if (lay + 1 <= L && v.jp(lay - 1) < ref && v.jp(lay) >= ref) ls = min(lay + 1, laytrop);
     return ls < 1 ? 1 : ls;
   }
   int ls = L;
   for (int lay = laytrop + 1; lay <= L; lay++)
-    if (lay >= 2 && v.jp[lay - 2] < ref && v.jp[lay - 1] >= ref) ls = lay;
+    if (lay >= 2 && v.jp(lay - 2) < ref && v.jp(lay - 1) >= ref) ls = lay;
   return ls;
 }
 
@@ -928,8 +930,9 @@ template <bool LOWER, typename R>
 __device__ inline GasFetch<R> gas_fetch(const GasLane &q, const R *__restrict__ tab, ColView &v, int lay) {
 #pragma clang fp contract(off)
   GasFetch<R> f;
-  const int jp = uni(LOWER ? min(v.jp[lay], 12) : max(v.jp[lay], 13));
-  const int jt = uni(v.jt[lay]), jt1 = uni(v.jt1[lay]);
+  const uint32_t w = uni((int)v.ix[lay]);
+  const int jp = LOWER ? min(ColView::idx(w, 0), 12) : max(ColView::idx(w, 0), 13);
+  const int jt = ColView::idx(w, 1), jt1 = ColView::idx(w, 2);
   const R c1 = R(v.c(q.s1, lay)), c2 = R(v.c(q.s2, lay));
   SpecT<R> s = key_spec_r<R>(c1, c2, R(q.strrat), R(LOWER ? 8 : 4), q.bin);
   f.kmul = q.bin ? s.speccomb : c1 * R(q.kmult);
@@ -939,11 +942,11 @@ __device__ inline GasFetch<R> gas_fetch(const GasLane &q, const R *__restrict__ 
   const uint32_t o1 = q.kb + __umul24(jpr + 1, q.kp) + __umul24(jt1 - 1, q.kt) + __umul24(s.js - 1, q.ks);
   f.a0 = ldt(tab, o0), f.a1 = ldt(tab, o0 + q.kjs), f.a2 = ldt(tab, o0 + q.kt), f.a3 = ldt(tab, o0 + q.kt + q.kjs);
   f.b0 = ldt(tab, o1), f.b1 = ldt(tab, o1 + q.kjs), f.b2 = ldt(tab, o1 + q.kt), f.b3 = ldt(tab, o1 + q.kt + q.kjs);
-  const int jf = uni(LOWER ? min(v.indf[lay], 2) : min(max(v.indf[lay], 1), 3));
+  const int jf = LOWER ? min(ColView::idx(w, 4), 2) : min(max(ColView::idx(w, 4), 1), 3);
   f.fo0 = ldt(tab, q.forb + __umul24(jf - 1, q.fors)), f.fo1 = ldt(tab, q.forb + __umul24(jf, q.fors));
   f.s0 = R(0), f.s1 = R(0), f.r0 = R(0), f.r1 = R(0);
   if (LOWER) {
-    const int is = uni(max(v.inds[lay], 1));
+    const int is = max(ColView::idx(w, 3), 1);
     f.s0 = ldt(tab, q.selfb + __umul24(is - 1, q.selfs)), f.s1 = ldt(tab, q.selfb + __umul24(is, q.selfs));
     f.r0 = ldt(tab, q.rab + __umul24(s.js - 1, q.ras)), f.r1 = ldt(tab, q.rab + __umul24(s.js, q.ras));
   }

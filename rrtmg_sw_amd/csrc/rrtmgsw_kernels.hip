@@ -793,14 +793,13 @@ __global__ void __launch_bounds__(128) setcoef_kernel(RtArgs a, DeviceTables T, 
   }
   if (tid < NBND && o.laysolfr) o.laysolfr[c + (size_t)C * tid] = v.misc[M_LAYSOL + tid];
   int *const iv[5] = {o.jp, o.jt, o.jt1, o.indself, o.indfor};
-  const int *const is[5] = {v.jp, v.jt, v.jt1, v.inds, v.indf};
   // fac00 fac01 fac10 fac11 selffac selffrac forfac forfrac, then h2o co2 o3 ch4 o2 colmol
   double *const dv[14] = {o.fac00,   o.fac01,  o.fac10,  o.fac11,  o.selffac, o.selffrac, o.forfac,
                           o.forfrac, o.colh2o, o.colco2, o.colo3,  o.colch4,  o.colo2,    o.colmol};
   for (int lay = tid; lay < L; lay += blockDim.x) {
     const size_t q = c + (size_t)C * lay;
     for (int k = 0; k < 5; k++)
-      if (iv[k]) iv[k][q] = is[k][lay];
+      if (iv[k]) iv[k][q] = ColView::idx(v.ix[lay], k);
     for (int k = 0; k < 8; k++)
       if (dv[k]) dv[k][q] = v.f(k, lay);
     for (int k = 0; k < 6; k++)
