@@ -570,8 +570,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     adif_b -= den_b * st.prdnd;
     sb.prdnd -= den_b * albdif;
     // reduce over g: vis bands -> asdir / asdif, the others -> aldir / aldif
-    double s = wave_sum4d(vis ? adir_b : 0.0, vis ? adif_b : 0.0, vis ? 0.0 : adir_b, vis ? 0.0 : adif_b, lane);
-    if (lane < 16 && (lane & 3) == 0) atomicAdd(&albacc[lane >> 2], s);
+    double s = wave_sum4d(vis ? adir_b : 0.0, vis ? adif_b : 0.0, vis ? 0.0 : adir_b, vis ? 0.0 : adif_b);
+    if ((lane & 15) == 0) atomicAdd(&albacc[lane >> 4], s);
   }
 
   // ---- R1: bottom -> top. The lane's gas-optics constants are read from its g-point record
@@ -644,12 +644,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
         la[AC_CIWP] = xb[2] * o[6] + xb[3] * o[7] + xb[4] * o[8];
       }
     }
-#pragma unroll
-    for (int q4 = 0; q4 < AC_N / 4; q4++) {
-      double s = wave_sum4d(la[4 * q4], la[4 * q4 + 1], la[4 * q4 + 2], la[4 * q4 + 3], lane);
-      // one term per wave and entry, onto zero: a + b rounds as b + a, whichever wave is first
-      if (lane < 16 && (lane & 3) == 0) atomicAdd(&acc[(size_t)lay * AC_N + 4 * q4 + (lane >> 2)], s);
-    }
+    static_assert(AC_N == 16, "one wave_sum16d per layer");
+    const double s = wave_sum16d(la, lane);
+    // one term per wave and entry, onto zero: a + b rounds as b + a, whichever wave is first
+    if ((lane & 3) == 0) atomicAdd(&acc[(size_t)lay * AC_N + (lane >> 2)], s);
   }
 
   // ---- incident flux: zinc = adjflux src mu0, src interpolated in the key-species parameter

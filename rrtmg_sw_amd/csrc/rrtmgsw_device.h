@@ -989,33 +989,58 @@ __device__ inline double dpp_d(double x) {
   return __hiloint2double(hi, lo);
 }
 constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141;
-__device__ inline double cross_rows(double x) {
-  int lo = __double2loint(x), hi = __double2hiint(x);
-  x += __hiloint2double(__builtin_amdgcn_ds_swizzle(hi, 0x401F), __builtin_amdgcn_ds_swizzle(lo, 0x401F));
-  lo = __double2loint(x), hi = __double2hiint(x);
-  auto l2 = __builtin_amdgcn_permlane32_swap((unsigned)lo, (unsigned)lo, false, false);
-  auto h2 = __builtin_amdgcn_permlane32_swap((unsigned)hi, (unsigned)hi, false, false);
-  return __hiloint2double((int)h2[0], (int)l2[0]) + __hiloint2double((int)h2[1], (int)l2[1]);
-}
-__device__ inline double wave_sum4d(double v0, double v1, double v2, double v3, int lane) {
-  const bool b3 = lane & 8, b2 = lane & 4;
-  double ka = b3 ? v2 : v0, kb = b3 ? v3 : v1;
-  double sa = b3 ? v0 : v2, sb = b3 ? v1 : v3;
-  double w0 = ka + dpp_d<DPP_MIRROR>(sa);
-  double w1 = kb + dpp_d<DPP_MIRROR>(sb);
-  double x = (b2 ? w1 : w0) + dpp_d<DPP_HALF_MIRROR>(b2 ? w0 : w1);
-  x += dpp_d<DPP_XOR1>(x);
-  x += dpp_d<DPP_XOR2>(x);
-  return cross_rows(x);
-}
-// Two values: lanes 0 and 8 end with the totals of v0 and v1.
-__device__ inline double wave_sum2d(double v0, double v1, int lane) {
-  const bool b3 = lane & 8;
-  double x = (b3 ? v1 : v0) + dpp_d<DPP_MIRROR>(b3 ? v0 : v1);
+// Every lane of a 16-lane row ends with the row's total of x (row mirror, half-row mirror, then
+// the two quad exchanges: each step pairs every lane with another).
+__device__ inline double row_sum_d(double x) {
+  x += dpp_d<DPP_MIRROR>(x);
   x += dpp_d<DPP_HALF_MIRROR>(x);
   x += dpp_d<DPP_XOR1>(x);
   x += dpp_d<DPP_XOR2>(x);
-  return cross_rows(x);
+  return x;
+}
+// a + b after exchanging the upper wave half of a with the lower half of b (v_permlane32_swap:
+// lanes 0-31 end with a's sum over l and l + 32, lanes 32-63 with b's), or the odd rows of a with
+// the even rows of b (v_permlane16_swap: even rows end with a's sum over rows 2r and 2r + 1, odd
+// rows with b's).
+__device__ inline void swap_add(double &a, double b, bool rows16) {
+  const unsigned alo = __double2loint(a), ahi = __double2hiint(a);
+  const unsigned blo = __double2loint(b), bhi = __double2hiint(b);
+  auto lo = rows16 ? __builtin_amdgcn_permlane16_swap(alo, blo, false, false)
+                   : __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+  auto hi = rows16 ? __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false)
+                   : __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+  a = __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+// Wavefront sums by recursive halving. Four values: the total of v_k ends in every lane of row k
+// (lanes 16k .. 16k + 15).
+__device__ inline double wave_sum4d(double v0, double v1, double v2, double v3) {
+  swap_add(v0, v2, false);   // upper half: v2
+  swap_add(v1, v3, false);   // upper half: v3
+  swap_add(v0, v1, true);    // rows 0, 1, 2, 3: v0, v1, v2, v3
+  return row_sum_d(v0);
+}
+// Two values: v0's total in lanes 0-31, v1's in lanes 32-63.
+__device__ inline double wave_sum2d(double v0, double v1) {
+  swap_add(v0, v1, false);
+  swap_add(v0, v0, true);
+  return row_sum_d(v0);
+}
+// Sixteen values: swaps of half the values across the wave halves and across row pairs leave
+// four per lane, each summed over its four rows; a row mirror and a half-row mirror, each
+// exchanging one of two selected values, and the quad exchanges finish within the row. Lane l
+// ends with the total of x[l >> 2] (lanes 0, 4, ..., 60 hold x[0] ... x[15]).
+__device__ inline double wave_sum16d(double (&x)[16], int lane) {
+#pragma unroll
+  for (int p = 0; p < 8; p++) swap_add(x[p], x[p + 8], false);   // lanes >= 32: x[p + 8]
+#pragma unroll
+  for (int p = 0; p < 4; p++) swap_add(x[p], x[p + 4], true);    // odd rows: x[p + 4]
+  const bool b3 = lane & 8, b2 = lane & 4;
+  const double w0 = (b3 ? x[2] : x[0]) + dpp_d<DPP_MIRROR>(b3 ? x[0] : x[2]);
+  const double w1 = (b3 ? x[3] : x[1]) + dpp_d<DPP_MIRROR>(b3 ? x[1] : x[3]);
+  double s = (b2 ? w1 : w0) + dpp_d<DPP_HALF_MIRROR>(b2 ? w0 : w1);
+  s += dpp_d<DPP_XOR1>(s);
+  s += dpp_d<DPP_XOR2>(s);
+  return s;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1095,35 +1120,34 @@ __device__ inline void sweep_f(SweepF &s, const RTm &l, float dbt, float &Cc, fl
   s.prdnd = prdnd;
 }
 
-// Wavefront sums in FP32 (the DPP pattern of wave_sum4d / wave_sum2d, one 32-bit move a value).
+// Wavefront sums in FP32, as wave_sum4d / wave_sum2d (one 32-bit exchange a value).
 template <int CTRL>
 __device__ inline float dpp_f(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
 }
-__device__ inline float cross_rows_f(float x) {
-  x += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x401F));
-  auto p = __builtin_amdgcn_permlane32_swap((unsigned)__float_as_int(x), (unsigned)__float_as_int(x),
-                                            false, false);
-  return __int_as_float((int)p[0]) + __int_as_float((int)p[1]);
-}
-__device__ inline float wave_sum4f(float v0, float v1, float v2, float v3, int lane) {
-  const bool b3 = lane & 8, b2 = lane & 4;
-  float ka = b3 ? v2 : v0, kb = b3 ? v3 : v1;
-  float sa = b3 ? v0 : v2, sb = b3 ? v1 : v3;
-  float w0 = ka + dpp_f<DPP_MIRROR>(sa);
-  float w1 = kb + dpp_f<DPP_MIRROR>(sb);
-  float x = (b2 ? w1 : w0) + dpp_f<DPP_HALF_MIRROR>(b2 ? w0 : w1);
-  x += dpp_f<DPP_XOR1>(x);
-  x += dpp_f<DPP_XOR2>(x);
-  return cross_rows_f(x);
-}
-__device__ inline float wave_sum2f(float v0, float v1, int lane) {
-  const bool b3 = lane & 8;
-  float x = (b3 ? v1 : v0) + dpp_f<DPP_MIRROR>(b3 ? v0 : v1);
+__device__ inline float row_sum_f(float x) {
+  x += dpp_f<DPP_MIRROR>(x);
   x += dpp_f<DPP_HALF_MIRROR>(x);
   x += dpp_f<DPP_XOR1>(x);
   x += dpp_f<DPP_XOR2>(x);
-  return cross_rows_f(x);
+  return x;
+}
+__device__ inline void swap_add_f(float &a, float b, bool rows16) {
+  const unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+  auto p = rows16 ? __builtin_amdgcn_permlane16_swap(ua, ub, false, false)
+                  : __builtin_amdgcn_permlane32_swap(ua, ub, false, false);
+  a = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ inline float wave_sum4f(float v0, float v1, float v2, float v3) {
+  swap_add_f(v0, v2, false);
+  swap_add_f(v1, v3, false);
+  swap_add_f(v0, v1, true);
+  return row_sum_f(v0);
+}
+__device__ inline float wave_sum2f(float v0, float v1) {
+  swap_add_f(v0, v1, false);
+  swap_add_f(v0, v0, true);
+  return row_sum_f(v0);
 }
 
 // ------------------------------------------------------------------------------------------

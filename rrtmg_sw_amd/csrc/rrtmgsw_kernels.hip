@@ -77,9 +77,9 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, float *redf, const float 
   float ut = (albdir * stf.tdbt + albdif * (stf.ztdn - stf.tdbt)) * rcpf(1.f - albdif * stf.prdnd);
   {
     const float dc = scf.ztdn + scf.prdnd * uc, dt = stf.ztdn + stf.prdnd * ut;
-    const float s = wave_sum4f(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
+    const float s = wave_sum4f(zinc * ut, zinc * dt, zinc * uc, zinc * dc);
     float *r = redf + (wave * (L + 1) + L) * 6;
-    if (lane < 16 && (lane & 3) == 0) r[lane >> 2] = s;
+    if ((lane & 15) == 0) r[lane >> 4] = s;
     if (lane == 0) r[4] = 0.f, r[5] = 0.f;
   }
   struct Co {
@@ -101,11 +101,11 @@ __device__ inline void rt_pass2_mixed(const RtArgs &a, float *redf, const float 
     const float bu_t = (tot ? o.tB : o.cB) * ut, ad_t = tot ? o.tA : o.cA;
     uc = o.cC + o.cE * uc;
     const float bu_c = o.cB * uc;
-    const float s4 = wave_sum4f(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
-    const float s2 = wave_sum2f(zinc * (ad_t * (bu_t + ub_t)), zinc * (o.cA * (bu_c + ub_c)), lane);
+    const float s4 = wave_sum4f(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c);
+    const float s2 = wave_sum2f(zinc * (ad_t * (bu_t + ub_t)), zinc * (o.cA * (bu_c + ub_c)));
     float *r = redf + (wave * (L + 1) + j) * 6;
-    if (lane < 16 && (lane & 3) == 0) r[lane >> 2] = s4;
-    if (lane == 0 || lane == 8) r[4 + (lane >> 3)] = s2;
+    if ((lane & 15) == 0) r[lane >> 4] = s4;
+    if ((lane & 31) == 0) r[4 + (lane >> 5)] = s2;
   };
   Co r0 = fetch(L - 1), r1 = fetch(L - 2), r2 = fetch(L - 3), r3 = fetch(L - 4);
   int j = L - 1;
@@ -533,9 +533,9 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       pend_j = j, pend_t = jfirst <= j;
       // sums over g of the incident-flux-weighted ztdn and pass-1 absorbed flux, both skies
       const float zf = float(zinc);
-      const float sa = wave_sum4f(zf * At, zf * Ac, zf * Pt, zf * Pc, lane);
-      if (lane < 16 && (lane & 3) == 0) {
-        const int q = (wave * L + j) * 4 + (lane >> 2);
+      const float sa = wave_sum4f(zf * At, zf * Ac, zf * Pt, zf * Pc);
+      if ((lane & 15) == 0) {
+        const int q = (wave * L + j) * 4 + (lane >> 4);
         if (gsum) reinterpret_cast<float *>(greda)[q] = sa;
         else v.redaf[q] = sa;
       }
@@ -564,9 +564,9 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     }
     pend_j = j, pend_t = jfirst <= j;
     // sum over g of the incident-flux-weighted ztdn of level j, both skies
-    const double sa = wave_sum2d(zinc * At, zinc * Ac, lane);
-    if (lane == 0 || lane == 8) {
-      const int q = (wave * L + j) * 2 + (lane >> 3);
+    const double sa = wave_sum2d(zinc * At, zinc * Ac);
+    if ((lane & 31) == 0) {
+      const int q = (wave * L + j) * 2 + (lane >> 5);
       if (gsum) greda[q] = sa;
       else v.reda[q] = sa;
     }
@@ -609,9 +609,9 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   double ut = (albdir * st_.tdbt + albdif * (st_.ztdn - st_.tdbt)) / (1. - albdif * st_.prdnd);
   double dt = st_.ztdn + st_.prdnd * ut;
   {
-    const double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc, lane);
-    if (lane < 16 && (lane & 3) == 0) {
-      const int q = (wave * (L + 1) + L) * 4 + (lane >> 2);
+    const double s = wave_sum4d(zinc * ut, zinc * dt, zinc * uc, zinc * dc);
+    if ((lane & 15) == 0) {
+      const int q = (wave * (L + 1) + L) * 4 + (lane >> 4);
       if (gsum) gred[q] = s;
       else v.red[q] = s;
     }
@@ -647,9 +647,9 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
       gs[(size_t)(j * 4 + 2) * NGPT] = zinc * ut;
       gs[(size_t)(j * 4 + 3) * NGPT] = zinc * bu_t;
     }
-    const double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c, lane);
-    if (lane < 16 && (lane & 3) == 0) {
-      const int q = (wave * (L + 1) + j) * 4 + (lane >> 2);
+    const double s = wave_sum4d(zinc * ut, zinc * bu_t, zinc * uc, zinc * bu_c);
+    if ((lane & 15) == 0) {
+      const int q = (wave * (L + 1) + j) * 4 + (lane >> 4);
       if (gsum) gred[q] = s;
       else v.red[q] = s;
     }
