@@ -416,11 +416,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   const bool active = lane < WAVE_G;
 
   // ---- prologue (as the forward) + adjoint accumulators and level weights ----
-  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_LAYLOW] = 0;
+  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_LAYLOW] = 0, v.misc[M_CLDTOP] = -1;
   for (int i = tid; i < L * AC_N; i += blockDim.x) acc[i] = 0.0;
   if (tid < 4) albacc[tid] = 0.0;
   __syncthreads();
-  for (int lay = tid; lay < L; lay += blockDim.x) setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+  for (int lay = tid; lay < L; lay += blockDim.x) {
+    setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
+    if (a.sc.icld >= 1 && a.cldfrac[c + (size_t)C * lay] >= CLDMIN) atomicMax(&v.misc[M_CLDTOP], lay);
+  }
   for (int j = tid; j <= L; j += blockDim.x) {
     // top-based level j is bottom-based level i = L - j; swhr(k) = (net(k+1) - net(k)) hf / pdp(k)
     // for k < L - 1 (the top layer's heating rate is zero)
@@ -450,14 +453,15 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     // workgroup past the 40 KB at which four fit a CU at 137 layers
     err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac + c,
                        a.alpha ? a.alpha + c : nullptr, (size_t)C,
-                       a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
+                       a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask, v.misc[M_CLDTOP]);
   __syncthreads();
   // cloud optics and their water-path derivatives for each (layer, band) with a cloudy sub-column
   const int K = A.ckpt;
   double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
   double *CJ = UB + adj_work_doubles(L, K) - (size_t)L * NBND * CJ_N;
   if (a.sc.icld >= 1) {
-    for (int idx = tid; idx < L * NBND; idx += blockDim.x) {
+    const int nlc = a.mc_irng == 0 ? v.misc[M_CLDTOP] + 1 : L;   // KISS masks: clear above the cloud top
+    for (int idx = tid; idx < nlc * NBND; idx += blockDim.x) {
       const int lay = idx / NBND, b = idx - lay * NBND;
       uint32_t any = 0;
       for (int gg = first[b]; gg < first[b + 1]; gg++) any |= v.mask[(lay >> 5) * NGPT + gg];

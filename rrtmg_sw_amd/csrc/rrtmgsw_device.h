@@ -81,6 +81,7 @@ struct ColView {
 constexpr int M_LAYTROP = 0, M_LAYSOL = 1;   // laysolfr of band b at misc[M_LAYSOL + b]
 constexpr int M_PARTIAL = 15;                 // non-McICA: a layer has a partial cloud fraction
 constexpr int M_LAYLOW = 16;                  // setcoef_sw's laylow (layers with log p >= 6.62)
+constexpr int M_CLDTOP = 17;                  // highest layer with cloud fraction >= cldmin, or -1
 
 // ------------------------------------------------------------------------------------------
 // inatm_sw + setcoef_sw for one layer (src/rrtmg_sw_rad.f90:1531-1586,
@@ -977,11 +978,8 @@ __device__ inline void gas_finish(const GasLane &q, const GasFetch<R> &f, ColVie
   taur = colmol * ray;
 }
 
-// Wavefront sums by DPP. Four values: two transposing exchanges inside each row of 16 lanes
-// (mirror: lanes i and 15-i, half mirror: i and 7-i) leave lane i with quantity (i >> 2) & 3
-// summed over 4 lanes of its row, two quad exchanges complete the row sums, and the rows are
-// combined by a swizzle (rows 0-1, 2-3) and a swap of the wave halves. Lanes 0, 4, 8, 12 end
-// with the totals of v0, v1, v2, v3 (every lane holds the total of its quantity).
+// Wavefront sums: exchanges across the wave halves and row pairs (v_permlane32_swap,
+// v_permlane16_swap), then DPP exchanges inside each row of 16 lanes.
 template <int CTRL>
 __device__ inline double dpp_d(double x) {
   int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, false);
@@ -1194,9 +1192,14 @@ __device__ inline double mt_real(uint32_t u) { return (double)u / (4294967296.0 
 // column reads an MT19937 stream of its own, mt + c * mt_stride, indexed as a single column.
 // cf[l * cs] and al[l * cs] are the column's cloud fraction and overlap parameter of layer l
 // (the caller's arrays at column c with stride ncol, or the copies staged in LDS with stride 1).
+// ltop is the highest layer whose fraction is at least cldmin (-1 if none, L - 1 if not known).
+// KISS draws lie strictly inside (0, 1), so a layer below cldmin (fraction taken as 0) is never
+// cloudy and the overlap recurrence above ltop reaches no cloudy layer: the KISS path draws up
+// to ltop and leaves the layers above clear. A Mersenne Twister draw can be exactly 1, which
+// makes a clear layer cloudy (cdf >= 1 - 0), so that path draws every layer.
 __device__ static int column_masks(int icld, int irng, int permuteseed, int C, int L, int c, int g,
                                    const double *play, const double *cf, const double *al, size_t cs,
-                                   const uint32_t *mt, const KissJump *jumps, uint32_t *mask,
+                                   const uint32_t *mt, const KissJump *jumps, uint32_t *mask, int ltop,
                                    const KissJump *col_jump = nullptr, size_t mt_stride = 0) {
 #pragma clang fp contract(off)
   Kiss k = {0, 0, 0, 0};
@@ -1216,6 +1219,7 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
   if (icld == 3) r3 = irng == 0 ? kiss_draw(k) : mt_real(mt[(size_t)g * mC + mc]);
   uint32_t word = 0;
   double cldf_prev = 0.0;
+  const int lend = irng == 0 ? min(ltop, L - 1) : L - 1;   // the last layer drawn
   // one layer: its draws (cdf, cdf2), the overlap recurrence, the mask bit
   auto layer = [&](int l, double cdf, double cdf2) {
     double cldf = cf[l * cs];
@@ -1232,13 +1236,14 @@ __device__ static int column_masks(int icld, int irng, int permuteseed, int C, i
     cdf_prev = cdf;
     cldf_prev = cldf;
     if (cdf >= 1. - cldf) word |= 1u << (l & 31);
-    if ((l & 31) == 31 || l == L - 1) {
+    if ((l & 31) == 31 || l == lend) {
       mask[(l >> 5) * NGPT + g] = word;
       word = 0;
     }
   };
+  for (int w = (lend + 32) >> 5; w < (L + 31) >> 5; w++) mask[w * NGPT + g] = 0u;   // above lend
   if (irng == 0 || icld == 3) {
-    for (int l = 0; l < L; l++) {
+    for (int l = 0; l <= lend; l++) {
       double cdf = r3, cdf2 = 0.0;
       if (icld != 3) {
         cdf = kiss_draw(k);

@@ -206,15 +206,16 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   const int gi = active ? g : wave * WAVE_G;
 
   // ---- prologue: inatm_sw + setcoef_sw per layer, laytrop, laysolfr per band, masks ----
-  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0, v.misc[M_LAYLOW] = 0;
+  if (tid == 0) v.misc[M_LAYTROP] = 0, v.misc[M_PARTIAL] = 0, v.misc[M_LAYLOW] = 0, v.misc[M_CLDTOP] = -1;
   __syncthreads();
   for (int lay = tid; lay < L; lay += blockDim.x) {
     setcoef_layer(a, T, c, lay, v, &v.misc[M_LAYTROP]);
     // the column's cloud fractions and overlap parameters, for the mask drawing below (its
-    // layer-by-layer reads then wait on LDS, not on a load from memory)
+    // layer-by-layer reads then wait on LDS, not on a load from memory), and its cloud top
     if (FUSED && a.sc.icld >= 1) {
       v.cf[lay] = a.cldfrac[c + (size_t)C * lay];
       v.al[lay] = a.mc_icld >= 4 ? a.alpha[c + (size_t)C * lay] : 0.0;
+      if (v.cf[lay] >= CLDMIN) atomicMax(&v.misc[M_CLDTOP], lay);
     }
   }
   __syncthreads();
@@ -228,10 +229,11 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
     if (SPEC)   // the column program's McICA samples
       err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, v.cf, v.al, 1,
                          a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask,
-                         reinterpret_cast<const KissJump *>(a.col_jump), a.mt_stride);
+                         v.misc[M_CLDTOP], reinterpret_cast<const KissJump *>(a.col_jump), a.mt_stride);
     else
       err = column_masks(a.mc_icld, a.mc_irng, a.permuteseed, C, L, c, g, a.play, v.cf, v.al, 1,
-                         a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask);
+                         a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask,
+                         v.misc[M_CLDTOP]);
   }
   if (LAYERCLD && a.sc.icld >= 1)   // src/rrtmg_sw_rad.nomcica.f90:617-622
     for (int lay = tid; lay < L; lay += blockDim.x) {
@@ -266,7 +268,8 @@ __device__ __attribute__((always_inline)) inline void rt_body(const RtArgs &a, c
   }
   if (FUSED && a.sc.icld >= 1) {
     const int first[NBND + 1] = {0, 6, 18, 26, 34, 44, 54, 56, 66, 74, 80, 86, 94, 100, NGPT};
-    for (int idx = tid; idx < L * NBND; idx += blockDim.x) {
+    const int nlc = a.mc_irng == 0 ? v.misc[M_CLDTOP] + 1 : L;   // KISS masks: clear above the cloud top
+    for (int idx = tid; idx < nlc * NBND; idx += blockDim.x) {
       const int lay = idx / NBND, b = idx - lay * NBND;
       uint32_t any = 0;
       for (int gg = first[b]; gg < first[b + 1]; gg++) any |= v.mask[(lay >> 5) * NGPT + gg];
@@ -825,7 +828,7 @@ __global__ void __launch_bounds__(128) mcica_kernel(McicaArgs a, DeviceTables T)
   if (active) {
     int e = column_masks(a.icld, a.irng, a.permuteseed, C, L, c, g, a.play, a.cldfrac + c,
                          a.alpha ? a.alpha + c : nullptr, (size_t)C, a.mt_stream,
-                         reinterpret_cast<const KissJump *>(T.kiss_jump), mask);
+                         reinterpret_cast<const KissJump *>(T.kiss_jump), mask, L - 1);
     if (e) atomicCAS(a.status, 0, e);
     // band of g (ngb, src/rrtmg_sw_init.f90:340-353)
     int ibnd = T.gp[g].ibnd;

@@ -165,3 +165,72 @@ def test_ecmwf_aerosols_iaer6_fused_device(gpu_lib):
     ref = ob.run_rrtmg(cols, mc, icld=2)
     print("iaer6 fused", {k: float(np.abs(got[k] - ref[k]).max()) for k in FLUX + HR})
     assert_close(got, ref, "iaer=6 fused")
+
+
+def _clouds_at_all_heights(cols, seed):
+    """Clouds in any layer up to the top one, with a clear column, a column cloudy only in its
+    top layer, one only in its bottom layer, and fractions at and just below cldmin = 1e-20
+    (src/mcica_subcol_gen_sw.f90:487 treats the latter as clear)."""
+    from rrtmg_sw_amd.workloads import alpha_for
+    rng = np.random.default_rng(seed)
+    C, L = cols.ncol, cols.nlay
+    f = np.where(rng.uniform(size=(C, L)) < 0.3, rng.uniform(0.05, 1.0, (C, L)), 0.0)
+    f[0] = 0.0
+    f[1] = 0.0
+    f[1, L - 1] = 0.7
+    f[2] = 0.0
+    f[2, 0] = 0.4
+    f[3, L - 1], f[3, L - 2] = 1.e-20, 0.9e-20
+    f[4, L - 1] = 0.9e-20
+    cols.cldfrac = np.asfortranarray(f)
+    cloudy = f > 0.0
+    cols.clwp = np.asfortranarray(np.where(cloudy, rng.lognormal(np.log(40.0), 1.0, (C, L)), 0.0))
+    cols.ciwp = np.asfortranarray(np.where(cloudy, rng.lognormal(np.log(15.0), 1.0, (C, L)), 0.0))
+    cols.rel = np.asfortranarray(rng.uniform(5.0, 20.0, (C, L)))
+    cols.rei = np.asfortranarray(rng.uniform(20.0, 120.0, (C, L)))
+    if cols.icld in (4, 5):
+        cols.alpha = np.asfortranarray(alpha_for(cols))
+    return cols
+
+
+@pytest.mark.parametrize("icld,irng", [(1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (2, 1), (3, 1), (5, 1)])
+def test_fused_masks_with_clouds_at_all_heights(gpu_lib, icld, irng):
+    """The fused entry draws KISS masks only up to the column's cloud top (every draw lies in
+    (0, 1), so no layer above it can be cloudy) and Mersenne Twister masks in every layer: both
+    give the oracle's fluxes with clouds anywhere in the column."""
+    import torch
+    from rrtmg_sw_amd import api
+    cols = _clouds_at_all_heights(make_columns(40, 48, icld=icld, iaer=0, seed=600 + icld), 610 + icld)
+    cols.irng = irng
+    d = api.device_inputs(cols)
+    out = api.mcica_rrtmg_sw(cols, d)
+    torch.cuda.synchronize()
+    got = {k: api.to_host(v) for k, v in out.items()}
+    mc, _ = ob.run_mcica(cols)
+    ref = ob.run_rrtmg(cols, mc)
+    print(icld, irng, {k: float(np.abs(got[k] - ref[k]).max()) for k in FLUX + HR})
+    assert_close(got, ref, f"clouds at all heights icld={icld} irng={irng}")
+
+
+@pytest.mark.parametrize("irng", [0, 1])
+def test_adjoint_masks_with_clouds_at_all_heights(gpu_lib, irng):
+    """The adjoint draws its masks the same way: its water-path gradients are zero exactly at
+    the cells where no sub-column of the oracle's masks is cloudy, and nonzero at nearly all
+    cloudy cells of sunlit columns."""
+    import torch
+    from rrtmg_sw_amd import api
+    cols = _clouds_at_all_heights(make_columns(40, 48, icld=2, iaer=0, seed=620), 621)
+    cols.irng = irng
+    d = api.device_inputs(cols)
+    st = torch.ones(cols.ncol, dtype=torch.float64).cuda()
+    sh = torch.zeros(cols.nlay, cols.ncol, dtype=torch.float64).cuda()
+    out = api.mcica_rrtmg_sw_adjoint(cols, d, st, sh)
+    torch.cuda.synchronize()
+    dl = api.to_host(out["clwp"])
+    mc, _ = ob.run_mcica(cols)
+    anyc = (np.asarray(mc["cldfmcl"]) > 0).any(axis=0)          # (ncol, nlay)
+    assert (dl[~anyc] == 0.0).all()
+    day = np.asarray(cols.coszen) > 0.05
+    sel = anyc & day[:, None]
+    assert sel.sum() > 100
+    assert (dl[sel] != 0.0).mean() > 0.95
