@@ -508,42 +508,40 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     if (K > 0) rec(j, 5) = ly.r, rec(j, 6) = ly.rd, rec(j, 7) = ly.t, rec(j, 8) = ly.td, rec(j, 9) = ly.dbt;
   };
 
-  // Gas optics of layer j with the lane's constants of the loop's regime (q), or of the other
-  // regime's, read from the g-point record, for a layer on the other side of laytrop (checkpoint
-  // segments that straddle it).
   const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
-  auto gas_at = [&](auto lower_tag, const GasLane &q, int j, double &taug, double &taur) {
-    constexpr bool LOWER = decltype(lower_tag)::value;
-    const int lay = L - 1 - j;
-    if ((j >= jsplit) == LOWER) {
-      gas_finish<LOWER, double>(q, gas_fetch<LOWER, double>(q, T.tab, v, lay), v, lay, taug, taur);
-    } else {
-      const GasLane o = gas_lane<!LOWER>(gp, L);
-      gas_finish<!LOWER, double>(o, gas_fetch<!LOWER, double>(o, T.tab, v, lay), v, lay, taug, taur);
-    }
-  };
-  auto layer_at = [&](auto lower_tag, const GasLane &q, int j, LayerIn &in, double &taug, double &taur) {
-    gas_at(lower_tag, q, j, taug, taur);
-    in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - j, active);
-    return total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
-  };
 
   // ---- F1: top -> bottom: sweep states and ubar_j = dJ/dU_j. The back substitution
   // U_j = C_j + E_j U_(j+1) feeds J through U_j and D_j = ztdn_j + prdnd_j U_j, so
   // ubar_j = ubar_(j-1) E_(j-1) + zinc (wU_j + wD_j prdnd_j), known as soon as layer j is. ----
   Sweep st = {1.0, 1.0, 0.0};
   double carry = 0.0;
+  // A layer's record and ubar are stored once the next layer's table reads have been issued (the
+  // vector memory counter retires in issue order, so earlier stores would lengthen those waits).
+  double pr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int pj = -1;
+  auto flush_rec = [&]() {
+    if (active && pj >= 0) {
+      if (K == 0)
+        rec(pj, 0) = pr[0], rec(pj, 1) = pr[1], rec(pj, 2) = pr[2], rec(pj, 3) = pr[3], rec(pj, 4) = pr[4];
+      UB[(size_t)pj * NGPT + g] = pr[5];
+    }
+  };
   auto f1_layer = [&](auto lower_tag, const GasLane &q, int j) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
     if (active && K > 0 && j % K == 0) ck(j / K, 0) = st.tdbt, ck(j / K, 1) = st.ztdn, ck(j / K, 2) = st.prdnd;
-    LayerIn in;
+    const int lay = L - 1 - j;
+    const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
+    flush_rec();
     double taug, taur;
-    LayerV ly = layer_at(lower_tag, q, j, in, taug, taur);
-    if (active && K == 0) put_rec(j, st, taug, taur, ly);
+    gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
+    const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
+    LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
+    pr[0] = st.tdbt, pr[1] = st.ztdn, pr[2] = st.prdnd, pr[3] = taug, pr[4] = taur;
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     double Aa, Bb, Cc, Ee;
     sweep_layer(st, l, ly.dbt, Aa, Bb, Cc, Ee);
     double ub = carry + zinc * (wU[j] + wD[j] * Bb);
-    if (active) UB[(size_t)j * NGPT + g] = ub;
+    pr[5] = ub, pj = j;
     carry = ub * Ee;
   };
   {
@@ -554,6 +552,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     const GasLane q = gas_lane<true>(gp, L);
     for (int j = jsplit; j < L; j++) f1_layer(std::integral_constant<bool, true>(), q, j);
   }
+  flush_rec();
   if (err) atomicCAS(a.status, 0, err);
 
   // ---- surface: U_L = (adir tdbt + adif (ztdn - tdbt)) / (1 - adif prdnd), D_L = ztdn + prdnd U_L ----
