@@ -6,7 +6,7 @@
 //
 // The oracle's source is compiled unchanged as C++ with `double` standing for `cd`, a value type
 // of the same size and layout whose +, -, *, / (binary, compound) and sqrt add one to fp_ops and
-// whose exp, log and pow add one to fp_trans before computing in double exactly as the plain
+// whose exp, expm1, log and pow add one to fp_trans before computing in double exactly as the plain
 // build does (same operands, same order, -ffp-contract=off), so its outputs are bit for bit the
 // plain oracle's (tests/test_flops.py checks this). Negation, comparisons, min/max, fabs, table
 // gathers and integer work are not counted. Arrays passed through the C interface are arrays of
@@ -51,6 +51,7 @@ inline cd operator-(cd a) { return cd(-a.v); }
 inline cd sqrt(cd a) { orc_fp_ops++; return cd(std::sqrt(a.v)); }
 inline cd exp(cd a) { orc_fp_trans++; return cd(std::exp(a.v)); }
 inline cd log(cd a) { orc_fp_trans++; return cd(std::log(a.v)); }
+inline cd expm1(cd a) { orc_fp_trans++; return cd(std::expm1(a.v)); }
 inline cd pow(cd a, cd b) { orc_fp_trans++; return cd(std::pow(a.v, b.v)); }
 inline cd pow(cd a, double b) { orc_fp_trans++; return cd(std::pow(a.v, b)); }
 inline cd pow(double a, cd b) { orc_fp_trans++; return cd(std::pow(a, b.v)); }

@@ -1007,6 +1007,35 @@ static inline double tbl_exp(double ze) {
   return EXP_TBL[itind];
 }
 
+/* Exact-exponential mode only: the general solution where k mu0 is within KMU_WIN of 1, its
+ * removable singularity. With eps = 1 - k mu0, 1 - (k mu0)^2 = eps (2 - eps), and the numerators
+ * of R and T (multiplied through by e1) are eps times smooth terms plus 2B e1 (e2 - e1) and
+ * 2C (e2 - e1), where e2 - e1 = e1 expm1(-z), z = k tau eps / (1 - eps) = tau / mu0 - k tau:
+ *   R = w [A + B e1^2 - 2 a2 e1 e2 - 2 B e1^2 q] / ((2 - eps) D),
+ *   T = e2 - w [2 a1 e1 - C e2 - E e1^2 e2 - 2 C e1 q] / ((2 - eps) D),
+ * A = a2 + k g3, B = a2 - k g3, C = a1 + k g4, E = a1 - k g4, D = (k + g1) + (k - g1) e1^2,
+ * q = k tau phi(-z) / (1 - eps), phi(x) = expm1(x) / x. The reference's expressions lose
+ * eps / |1 - (k mu0)^2| of their precision there (and its 1e-8 guard replaces R by 1e-8 where
+ * the denominator vanishes); this form has no cancellation, so the exact forward is smooth
+ * through k mu0 = 1. The diffuse Rd and Td have no such factor. */
+#define KMU_WIN 1.e-3
+static void exact_near_resonance(double zw, double g1, double g3, double g4, double a1, double a2,
+                                 double k, double zeps, double tau, double e1, double e2,
+                                 double *pref, double *ptra) {
+  double z = k * tau * zeps / (1. - zeps);
+  double phi = 1.;
+  if (z != 0.) phi = expm1(-z) / (-z);
+  double q = k * tau * phi / (1. - zeps);
+  double ee = e1 * e1;
+  double A = a2 + k * g3, B = a2 - k * g3, C = a1 + k * g4, E = a1 - k * g4;
+  double D = (k + g1) + (k - g1) * ee;
+  double nr = A + B * ee - 2. * a2 * e1 * e2 - 2. * B * ee * q;
+  double nt = 2. * a1 * e1 - C * e2 - E * ee * e2 - 2. * C * e1 * q;
+  double den = (2. - zeps) * D;
+  *pref = zw * nr / den;
+  *ptra = e2 - zw * nt / den;
+}
+
 /* reftra_sw -- src/rrtmg_sw_reftra.f90:48-329 (kmodts = 2, PIFM). */
 static void reftra(int n, const int *lrtchk, const double *pgg, double prmuz, const double *ptau,
                    const double *pw, double *pref, double *prefd, double *ptra, double *ptrad) {
@@ -1063,7 +1092,10 @@ static void reftra(int n, const int *lrtchk, const double *pgg, double prmuz, co
       double zem2 = tbl_exp(ze2), zep2 = 1. / zem2;
       double zdenr = zr4 * zep1 + zr5 * zem1;
       double zdent = zt4 * zep1 + zt5 * zem1;
-      if (zdenr >= -eps && zdenr <= eps) {
+      if (g_exact_exp && fabs(1. - zrp) < KMU_WIN && ze1 < 500. && ze2 < 500.) {
+        exact_near_resonance(zw, zgamma1, zgamma3, zgamma4, za1, za2, zrk, 1. - zrp, zto1, zem1,
+                             zem2, &pref[jk], &ptra[jk]);
+      } else if (zdenr >= -eps && zdenr <= eps) {
         pref[jk] = eps;
         ptra[jk] = zem2;
       } else {

@@ -120,112 +120,101 @@ __device__ inline void gas_adj_r(const GasLane &q, const double *__restrict__ ta
   acc[AC_COL + 5] += mol_b;
 }
 
+// Everything a layer of one sub-column needs besides the gas optics (aerosol and cloud optics),
+// as the forward computes it. Cloud optics come from the per-(layer, band) table CJ built in the
+// prologue: every cloudy sub-column cell of a band has the same cloud inputs.
+struct LayerIn {
+  double ta, wa, ga, cf, ct, cw, cg;
+};
 // Layer optics of one sub-column (src/rrtmg_sw_spcvmc.f90:442-505): clear sky with delta
-// scaling (g0 = ga wa ta / s and w0 = s / tau0 share 1 / (s tau0); (w0 - w0 f)/(1 - w0 f) and
-// (g0 - f)/(1 - f) = g0 / (1 + g0) share 1 / ((1 - w0 f)(1 + g0)), f = g0^2), and the total sky
-// (clear + delta-scaled cloud), as the forward kernel evaluates them.
+// scaling (g0 = ga wa ta / s and w0 = s / tau0 share r1 = 1 / (s tau0); (w0 - w0 f)/(1 - w0 f) and
+// (g0 - f)/(1 - f) = g0 / (1 + g0) share r2 = 1 / ((1 - w0 f)(1 + g0)), f = g0^2), and the total
+// sky (clear + delta-scaled cloud), as the forward kernel evaluates them.
 struct SkyOpt {
   double tau, w, g;
 };
-__device__ inline SkyOpt clear_optics(double taug, double taur, double ta, double wa, double ga) {
+__device__ inline SkyOpt clear_optics(double taug, double taur, double ta, double wa, double ga, double &r1,
+                                      double &r2) {
 #pragma clang fp contract(off)
   const double tau0 = taur + taug + ta;
   const double sct = taur * 1.0 + ta * wa;
-  const double r1 = frcp(sct * tau0);
+  r1 = frcp(sct * tau0);
   const double g0 = ga * wa * ta * tau0 * r1;
   const double w0 = sct * sct * r1;
   const double zwf = w0 * (g0 * g0);
-  const double r2 = frcp((1.0 - zwf) * (1.0 + g0));
+  r2 = frcp((1.0 - zwf) * (1.0 + g0));
   return {(1.0 - zwf) * tau0, (w0 - zwf) * (1.0 + g0) * r2, g0 * (1.0 - zwf) * r2};
 }
-__device__ inline SkyOpt total_optics(const SkyOpt &c, double ct, double cw, double cg) {
-#pragma clang fp contract(off)
-  const double tauo = c.tau + ct;
-  const double so = c.tau * c.w + ct * cw;
-  const double r = frcp(so * tauo);
-  return {tauo, so * so * r, (ct * cw * cg + c.tau * c.w * c.g) * tauo * r};
-}
 
-// Total-sky layer (r, rd, t, td, direct-beam transmittance) of the exact-exponential forward:
-// blend of the clear and cloudy solutions by the sub-column's cloud fraction (:539-549); for
-// McICA cloud fractions of 0 or 1 only one of the two is evaluated.
+// The layer of one sub-column cell (r, rd, t, td, direct-beam transmittance) in the
+// exact-exponential forward. McICA cells are clear (cf = 0) or overcast (cf = 1), so the
+// total-sky layer is the clear solution or the cloudy one and each lane evaluates one reftra;
+// the forward kernel's blend (1 - cf) rc + cf ro (:539-549) gives the same bits (1 * rc, or ro
+// alone). The tape keeps what the reverse sweep needs.
 struct LayerV {
   double r, rd, t, td, dbt;
 };
-__device__ inline LayerV total_layer_exact(double taug, double taur, const double ta, double wa,
-                                           double ga, double cf, double ct, double cw, double cg,
-                                           double mu0, double rmu) {
+struct CellTape {
+  SkyOpt c;            // clear optics
+  SkyOpt s;            // reftra's inputs: the clear optics, or the total-sky optics of a cloudy cell
+  double r1, r2, r3;   // the delta scaling's reciprocals, and 1 / (so tauo) of a cloudy cell
+  RtTape rt;
+};
+__device__ inline LayerV cell_layer(double taug, double taur, const LayerIn &in, double mu0, double rmu,
+                                    CellTape &tp) {
 #pragma clang fp contract(off)
-  SkyOpt c = clear_optics(taug, taur, ta, wa, ga);
-  LayerV o = {0., 0., 1., 1., 1.};
-  double zcl = 1.0 - cf;
-  if (zcl != 0.0) {
-    RTx rc = reftra_x(c.tau, c.w, c.g, mu0, rmu);
-    o = {zcl * rc.r, zcl * rc.rd, zcl * rc.t, zcl * rc.td, zcl * rc.e2};
+  tp.c = clear_optics(taug, taur, in.ta, in.wa, in.ga, tp.r1, tp.r2);
+  tp.s = tp.c;
+  tp.r3 = 0.;
+  if (in.cf > 0.0) {
+    // total-sky optics (:500-505): go = (ct cw cg + tauc omc gcc) / so, omo = so / tauo
+    const double tauo = tp.c.tau + in.ct;
+    const double so = tp.c.tau * tp.c.w + in.ct * in.cw;
+    const double r = frcp(so * tauo);
+    tp.s = {tauo, so * so * r, (in.ct * in.cw * in.cg + tp.c.tau * tp.c.w * tp.c.g) * tauo * r};
+    tp.r3 = r;
   }
-  if (cf > 0.0) {
-    SkyOpt t = total_optics(c, ct, cw, cg);
-    RTx ro = reftra_x(t.tau, t.w, t.g, mu0, rmu);
-    double dbo = ro.e2;
-    if (!(cf > REPCLC)) ro.r = 0., ro.rd = 0., ro.t = 1., ro.td = 1.;
-    if (zcl == 0.0) {
-      o = {ro.r, ro.rd, ro.t, ro.td, dbo};
-    } else {
-      o.r += cf * ro.r, o.rd += cf * ro.rd, o.t += cf * ro.t, o.td += cf * ro.td, o.dbt += cf * dbo;
-    }
-  }
-  return o;
+  const RTx o = reftra_xt(tp.s.tau, tp.s.w, tp.s.g, mu0, rmu, tp.rt);
+  return {o.r, o.rd, o.t, o.td, o.e2};
 }
 
-// Adjoint of total_layer_exact: given the adjoints lb of (r, rd, t, td, dbt), returns in xb the
-// adjoints of (taug, taur, ct, cw, cg): reftra in reverse mode (reftra_exact_adj), then the
-// reverse of the total-sky combination and of the clear-sky delta scaling, written out:
+// Adjoint of cell_layer from its tape: given the adjoints lb of (r, rd, t, td, dbt), returns in
+// xb the adjoints of (taug, taur, ct, cw, cg): reftra in reverse mode, then the reverse of the
+// total-sky combination (cloudy cells) and of the clear-sky delta scaling, written out:
 //   clear: tauc0 = taur + taug + ta, s = taur + ta wa, g0 = ga wa ta / s, w0 = s / tauc0,
 //          f = g0^2, tauc = (1 - w0 f) tauc0, wc = w0 (1 - f) / (1 - w0 f), gc = g0 / (1 + g0);
 //   total: tauo = tauc + ct, so = tauc wc + ct cw, go = (ct cw cg + tauc wc gc) / so,
 //          wo = so / tauo.
-__device__ inline void total_layer_adj(double taug, double taur, double ta, double wa, double ga,
-                                       double cf, double ct, double cw, double cg, double mu0,
-                                       double rmu, const double *lb, double *xb) {
-  const double tauc0 = taur + taug + ta;
-  const double sc = taur * 1.0 + ta * wa;
-  const double r1 = frcp(sc * tauc0), rsc = tauc0 * r1, rtau0 = sc * r1;
-  const double g0 = ga * wa * ta * rsc;
+__device__ inline void cell_layer_adj(const CellTape &tp, double taug, double taur, const LayerIn &in,
+                                      double mu0, double rmu, const double *lb, double *xb) {
+  double sbt, sbw, sbg;
+  reftra_xt_adj(tp.rt, tp.s.tau, tp.s.w, tp.s.g, mu0, rmu, lb[0], lb[1], lb[2], lb[3], lb[4], sbt, sbw, sbg);
+  double tcb = sbt, wcb = sbw, gcb = sbg;     // adjoints of the clear optics (tauc, wc, gc)
+  double ctb = 0., cwb = 0., cgb = 0.;
+  if (in.cf > 0.0) {
+    const SkyOpt &c = tp.c, &t = tp.s;
+    const double ct = in.ct, cw = in.cw, cg = in.cg;
+    const double so = c.tau * c.w + ct * cw;
+    const double rtt = so * tp.r3, rso = t.tau * tp.r3;   // 1 / tauo, 1 / so
+    const double taob = sbt - sbw * t.w * rtt;
+    const double Nb = sbg * rso;
+    const double sob = sbw * rtt - sbg * t.g * rso;
+    ctb = taob + sob * cw + Nb * cw * cg;
+    cwb = sob * ct + Nb * ct * cg;
+    cgb = Nb * ct * cw;
+    tcb = taob + sob * c.w + Nb * c.w * c.g;
+    wcb = sob * c.tau + Nb * c.tau * c.g;
+    gcb = Nb * c.tau * c.w;
+  }
+  // clear-sky delta scaling: 1 / s = tauc0 r1, 1 / tauc0 = s r1, 1 / (1 + g0) = r2 D, 1 / D = r2 (1 + g0)
+  const double tauc0 = taur + taug + in.ta;
+  const double sc = taur * 1.0 + in.ta * in.wa;
+  const double rsc = tauc0 * tp.r1, rtau0 = sc * tp.r1;
+  const double g0 = in.ga * in.wa * in.ta * rsc;
   const double w0 = sc * rtau0;
   const double f = g0 * g0;
   const double D = 1.0 - w0 * f;
-  SkyOpt c = clear_optics(taug, taur, ta, wa, ga);
-  double tcb = 0., wcb = 0., gcb = 0.;     // adjoints of the clear optics (tauc, wc, gc)
-  double ctb = 0., cwb = 0., cgb = 0.;
-  const double zcl = 1.0 - cf;
-  if (zcl != 0.0) {
-    double tb, wb, gb;
-    reftra_exact_adj(c.tau, c.w, c.g, mu0, rmu, zcl * lb[0], zcl * lb[1], zcl * lb[2], zcl * lb[3],
-                     zcl * lb[4], tb, wb, gb);
-    tcb += tb, wcb += wb, gcb += gb;
-  }
-  if (cf > 0.0) {
-    SkyOpt t = total_optics(c, ct, cw, cg);
-    // no cloudy reftra when the cloud fraction is tiny: r = 0, t = 1, only the direct beam counts
-    const bool rt = cf > REPCLC;
-    double tob, wob, gob;
-    reftra_exact_adj(t.tau, t.w, t.g, mu0, rmu, rt ? cf * lb[0] : 0., rt ? cf * lb[1] : 0.,
-                     rt ? cf * lb[2] : 0., rt ? cf * lb[3] : 0., cf * lb[4], tob, wob, gob);
-    const double so = c.tau * c.w + ct * cw;
-    const double rts = frcp(t.tau * so), rtt = so * rts, rso = t.tau * rts;
-    const double sob0 = wob * rtt;
-    const double taob = tob - wob * t.w * rtt;
-    const double Nb = gob * rso;
-    const double sob = sob0 - gob * t.g * rso;
-    ctb += taob + sob * cw + Nb * cw * cg;
-    cwb += sob * ct + Nb * ct * cg;
-    cgb += Nb * ct * cw;
-    tcb += taob + sob * c.w + Nb * c.w * c.g;
-    wcb += sob * c.tau + Nb * c.tau * c.g;
-    gcb += Nb * c.tau * c.w;
-  }
-  // clear-sky delta scaling
-  const double rg = frcp(1.0 + g0), rD = frcp(D);
+  const double rg = tp.r2 * D, rD = tp.r2 * (1.0 + g0);
   double g0b = gcb * rg * rg;
   double w0b = wcb * (1.0 - f) * rD * rD - tcb * f * tauc0;
   double fb = -wcb * w0 * (1.0 - w0) * rD * rD - tcb * w0 * tauc0;
@@ -238,12 +227,6 @@ __device__ inline void total_layer_adj(double taug, double taur, double ta, doub
   xb[2] = ctb, xb[3] = cwb, xb[4] = cgb;
 }
 
-// Everything a layer of one sub-column needs besides the gas optics (aerosol and cloud optics),
-// as the forward computes it. Cloud optics come from the per-(layer, band) table CJ built in the
-// prologue: every cloudy sub-column cell of a band has the same cloud inputs.
-struct LayerIn {
-  double ta, wa, ga, cf, ct, cw, cg;
-};
 // per (layer, band): cloud tau, ssa, asymmetry, then their derivatives with respect to the
 // in-cloud liquid water path and to the ice water path
 constexpr int CJ_N = 9;
@@ -383,17 +366,15 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
   q_b -= coldry_b * coldry * ((amw - amd) / amm + 1. / (1. + q));
 }
 
-// Doubles of work area per column: (UB (L) + per-layer states and gas optics (5 L, store mode) or
-// checkpoints (3 ceil(L/K)) and a segment buffer (3 K)) times NGPT, then the cloud table CJ
+// Doubles of work area per column: (UB (L) + per-layer records (5 L, store mode) or checkpoints
+// (3 ceil(L/K)) and the records of one segment (5 K)) times NGPT, then the cloud table CJ
 // (L, NBND, CJ_N).
 // A layer's record for the reverse sweep, per g-point: the sweep state at its top (tdbt, ztdn,
-// prdnd) and its gas optics (taug, taur); in checkpoint mode also its two-stream solution
-// (r, rd, t, td, dbt), which the segment's re-sweep has just formed. (In store mode the reverse
-// sweep recomputes the solution: keeping it for every layer measured slower, the extra HBM
-// traffic costing more than the recomputation.)
-constexpr int REC_S = 5, REC_N = 10;
+// prdnd) and its gas optics (taug, taur). The reverse sweep forms the layer's two-stream
+// solution again from them, keeping the intermediates its own reverse needs.
+constexpr int REC_S = 5;
 __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
-  size_t states = K == 0 ? (size_t)L * REC_S : (size_t)((L + K - 1) / K) * 3 + (size_t)K * REC_N;
+  size_t states = K == 0 ? (size_t)L * REC_S : (size_t)((L + K - 1) / K) * 3 + (size_t)K * REC_S;
   return ((size_t)L + states) * NGPT + (size_t)L * NBND * CJ_N;
 }
 
@@ -495,17 +476,16 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 
   // Per-column work area: UB (L, NGPT) = dJ/dU_j, then either every layer's record
   // S (L, REC_S, NGPT) (store mode) or one sweep state per K layers CK (ceil(L/K), 3, NGPT)
-  // plus the records of one K-layer segment SG (K, REC_N, NGPT) (checkpoint mode, K = A.ckpt).
+  // plus the records of one K-layer segment SG (K, REC_S, NGPT) (checkpoint mode, K = A.ckpt).
   double *S = UB + (size_t)L * NGPT;
   double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
   auto ck = [&](int i, int k) -> double & { return S[((size_t)i * 3 + k) * NGPT + g]; };
   // record k of layer j: S in store mode, SG (segment-relative) in checkpoint mode
   auto rec = [&](int j, int k) -> double & {
-    return K == 0 ? S[((size_t)j * REC_S + k) * NGPT + g] : SG[((size_t)(j % K) * REC_N + k) * NGPT + g];
+    return K == 0 ? S[((size_t)j * REC_S + k) * NGPT + g] : SG[((size_t)(j % K) * REC_S + k) * NGPT + g];
   };
-  auto put_rec = [&](int j, const Sweep &s, double taug, double taur, const LayerV &ly) {
+  auto put_rec = [&](int j, const Sweep &s, double taug, double taur) {
     rec(j, 0) = s.tdbt, rec(j, 1) = s.ztdn, rec(j, 2) = s.prdnd, rec(j, 3) = taug, rec(j, 4) = taur;
-    if (K > 0) rec(j, 5) = ly.r, rec(j, 6) = ly.rd, rec(j, 7) = ly.t, rec(j, 8) = ly.td, rec(j, 9) = ly.dbt;
   };
 
   const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
@@ -535,7 +515,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     double taug, taur;
     gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
     const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
-    LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
+    CellTape tp;
+    const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
     pr[0] = st.tdbt, pr[1] = st.ztdn, pr[2] = st.prdnd, pr[3] = taug, pr[4] = taur;
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     double Aa, Bb, Cc, Ee;
@@ -598,30 +579,26 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
       for (int jj = j0; jj <= j; jj++) {
         double taug, taur;
         gas_rec(jj, taug, taur);
-        const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - jj, active);
-        LayerV ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
-        if (active) put_rec(jj, ss, taug, taur, ly);
+        if (active) put_rec(jj, ss, taug, taur);
         if (jj < j) {
+          const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - jj, active);
+          CellTape tp;
+          const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
           RT4 l = {ly.r, ly.rd, ly.t, ly.td};
           double Aa, Bb, Cc, Ee;
           sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
         }
       }
     }
-    // the layer's record: sweep state at its top, gas optics, and in checkpoint mode the
-    // two-stream solution (store mode recomputes it with the same function, so both modes see
-    // the same values)
+    // the layer's record (sweep state at its top, gas optics); its two-stream solution formed
+    // again with the intermediates the reverse needs (taped)
     const Sweep sj = {active ? rec(j, 0) : 1.0, active ? rec(j, 1) : 1.0, active ? rec(j, 2) : 0.0};
     const double taug = active ? rec(j, 3) : 1., taur = active ? rec(j, 4) : 0.1;
     const double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
     const int lay = L - 1 - j;
     const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
-    LayerV ly;
-    if (K == 0)
-      ly = total_layer_exact(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu);
-    else
-      ly = {active ? rec(j, 5) : 0., active ? rec(j, 6) : 0., active ? rec(j, 7) : 1.,
-            active ? rec(j, 8) : 1., active ? rec(j, 9) : 1.};
+    CellTape tp;
+    const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
     RT4 l = {ly.r, ly.rd, ly.t, ly.td};
     // C_j, E_j and U_j from the layer
     Sweep tmp = sj;
@@ -632,7 +609,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     phi += wU[j] * u + wD[j] * (sj.ztdn + sj.prdnd * u);
     double lb[5], xb[5];
     sweep_layer_adj(sj, l, ly.dbt, sb, ub, ub * Uj1, zinc * wD[j], zinc * wD[j] * u, lb);
-    total_layer_adj(taug, taur, in.ta, in.wa, in.ga, in.cf, in.ct, in.cw, in.cg, mu0, rmu, lb, xb);
+    cell_layer_adj(tp, taug, taur, in, mu0, rmu, lb, xb);
     double la[AC_N];
 #pragma unroll
     for (int k = 0; k < AC_N; k++) la[k] = 0.0;

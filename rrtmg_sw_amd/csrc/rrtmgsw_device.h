@@ -536,9 +536,6 @@ __device__ inline double exp_cw(double y) {
   p = fma(p, r, 1.0);
   return ldexp(p, (int)k);
 }
-__device__ inline double dsqrt(double x) { return ::sqrt(x); }
-__device__ inline double dexp(double x) { return exp_cw(x); }
-__device__ inline double dmin(double a, double b) { return ::fmin(a, b); }
 // reftra_sw with exp() in place of the lookup table and its expansion: the reference's branches
 // (src/rrtmg_sw_reftra.f90:164-322), the general solution multiplied through by e1 as in
 // reftra_e. e2 = exp(-min(tau / mu0, 500)) is also the layer's direct-beam transmittance (the
@@ -546,7 +543,48 @@ __device__ inline double dmin(double a, double b) { return ::fmin(a, b); }
 struct RTx {
   double r, rd, t, td, e2;
 };
-__device__ inline RTx reftra_x(double zto1, double zw, double zg, double prmuz, double rmu) {
+// phi(x) = (exp(x) - 1) / x and its derivative for |x| <= 0.51, by their Taylor series to degree
+// 15 (truncation below 1e-18): sum x^n / (n + 1)! and sum n x^(n - 1) / (n + 1)!.
+__device__ inline double phi_em1(double x) {
+  double p = 1.0 / 20922789888000.0;             // 1/16!
+  const double c[15] = {1.0 / 1307674368000.0, 1.0 / 87178291200.0, 1.0 / 6227020800.0, 1.0 / 479001600.0,
+                        1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
+                        1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0};   // 1/15! .. 1/1!
+#pragma unroll
+  for (int i = 0; i < 15; i++) p = fma(p, x, c[i]);
+  return p;
+}
+__device__ inline double dphi_em1(double x) {
+  // n / (n + 1)! for n = 15 .. 1
+  double p = 15.0 / 20922789888000.0;
+  const double c[14] = {14.0 / 1307674368000.0, 13.0 / 87178291200.0, 12.0 / 6227020800.0, 11.0 / 479001600.0,
+                        10.0 / 39916800.0, 9.0 / 3628800.0, 8.0 / 362880.0, 7.0 / 40320.0, 6.0 / 5040.0,
+                        5.0 / 720.0, 4.0 / 120.0, 3.0 / 24.0, 2.0 / 6.0, 1.0 / 2.0};
+#pragma unroll
+  for (int i = 0; i < 14; i++) p = fma(p, x, c[i]);
+  return p;
+}
+// The exact-exponential forms evaluate the general solution in a cancellation-free form where
+// 1 - k mu0 lies within this window of 0 (the removable singularity of the direct-beam particular
+// solution): with eps = 1 - k mu0, 1 - (k mu0)^2 = eps (2 - eps), and the numerators of R and T
+// (multiplied through by e1) are eps times smooth terms plus 2B e1 (e2 - e1) and 2C (e2 - e1),
+// where e2 - e1 = -e1 z phi(-z), z = k tau eps / (1 - eps) (= tau / mu0 - k tau):
+//   R = w [A + B e1^2 - 2 a2 e1 e2 - 2 B e1^2 q] / ((2 - eps) D),
+//   T = e2 - w [2 a1 e1 - C e2 - E e1^2 e2 - 2 C e1 q] / ((2 - eps) D),
+// A = a2 + k g3, B = a2 - k g3, C = a1 + k g4, E = a1 - k g4, q = k tau phi(-z) / (1 - eps).
+// The closed form loses about 1e-16 / |1 - (k mu0)^2| of relative precision there (and the
+// reference's 1e-8 guard sets R = 1e-8 where its denominator vanishes); this form keeps the
+// exact forward smooth, so that finite differences and the adjoint see the same function.
+constexpr double KMU_WIN = 1.e-3;
+
+// Intermediates of one reftra_xt evaluation, kept for its reverse sweep (reftra_xt_adj). The
+// conservative branch keeps za1 in a1, zgt in a2 and 1 / (1 + zgt) in rD; the near-resonance form
+// (reg) keeps 1 / ((2 - eps) D) in rD, 1 / D in P, eps in eps and its numerators in NR, NT.
+struct RtTape {
+  double g1, g2, g3, a1, a2, k, e1, e2, D, P, rD, NR, NT, eps;
+  bool cons, tiny, reg;
+};
+__device__ inline RTx reftra_xt(double zto1, double zw, double zg, double prmuz, double rmu, RtTape &tp) {
 #pragma clang fp contract(off)
   const double zwcrit = 0.9999995, eps = 1.e-08;
   RTx o;
@@ -560,6 +598,8 @@ __device__ inline RTx reftra_x(double zto1, double zw, double zg, double prmuz, 
   bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
   const double e2 = exp_cw(-fmin(zto1 * rmu, 500.));
   o.e2 = e2;
+  tp.g1 = g1, tp.g2 = g2, tp.g3 = g3, tp.e2 = e2, tp.cons = cons, tp.tiny = false, tp.reg = false;
+  tp.k = 0., tp.e1 = 1., tp.D = 1., tp.P = 1., tp.NR = 0., tp.NT = 0., tp.eps = 0.;
   if (cons) {
     double za1 = g1 * prmuz - g3;
     double zgt = g1 * zto1;
@@ -569,151 +609,203 @@ __device__ inline RTx reftra_x(double zto1, double zw, double zg, double prmuz, 
     o.rd = zgt * inv;
     o.td = 1. - o.rd;
     if (e2 == 1.0) o.r = 0., o.t = 1., o.rd = 0., o.td = 1.;
+    tp.a1 = za1, tp.a2 = zgt, tp.rD = inv;
   } else {
     double a1 = g1 * g4 + g2 * g3;
     double a2 = g1 * g3 + g2 * g4;
     double k = sqrt(g1 * g1 - g2 * g2);
     double kmu = k * prmuz;
-    double e1 = exp_cw(-fmin(k * zto1, 500.));
+    const double x1 = k * zto1;
+    double e1 = exp_cw(-fmin(x1, 500.));
     double ee = e1 * e1;
     double D = (k + g1) - (g1 - k) * ee;
-    double P = 1. - kmu * kmu;
-    bool tiny = fabs(P * D) <= eps * e1;
-    P = tiny ? 1. : P;
-    double rD = frcp(P * D);
-    double NR = (1. - kmu) * (a2 + k * g3) - (1. + kmu) * (a2 - k * g3) * ee - 2. * k * (g3 - a2 * prmuz) * (e1 * e2);
-    double NT = e2 * ((1. + kmu) * (a1 + k * g4) - (1. - kmu) * (a1 - k * g4) * ee) - 2. * k * (g4 + a1 * prmuz) * e1;
-    o.r = tiny ? eps : zw * NR * rD;
-    o.t = tiny ? e2 : e2 - zw * NT * rD;
-    double PrD = P * rD;
-    o.rd = g2 * (1. - ee) * PrD;
-    o.td = 2. * k * e1 * PrD;
+    const double ke = 1. - kmu;
+    tp.a1 = a1, tp.a2 = a2, tp.k = k, tp.e1 = e1, tp.D = D, tp.eps = ke;
+    if (fabs(ke) < KMU_WIN && x1 < 500. && zto1 * rmu < 500.) {
+      // the near-resonance form (KMU_WIN)
+      const double s1 = frcp(1. - ke);
+      const double q = x1 * phi_em1(-(x1 * ke * s1)) * s1;
+      const double B = a2 - k * g3, C = a1 + k * g4;
+      const double NR = (a2 + k * g3) + B * ee - 2. * a2 * e1 * e2 - 2. * B * ee * q;
+      const double NT = 2. * a1 * e1 - C * e2 - (a1 - k * g4) * ee * e2 - 2. * C * e1 * q;
+      const double rDd = frcp(D);
+      const double rW = rDd * frcp(2. - ke);
+      o.r = zw * NR * rW;
+      o.t = e2 - zw * NT * rW;
+      o.rd = g2 * (1. - ee) * rDd;
+      o.td = 2. * k * e1 * rDd;
+      tp.P = rDd, tp.rD = rW, tp.NR = NR, tp.NT = NT, tp.reg = true;
+    } else {
+      double P = 1. - kmu * kmu;
+      bool tiny = fabs(P * D) <= eps * e1;
+      P = tiny ? 1. : P;
+      double rD = frcp(P * D);
+      double NR = (1. - kmu) * (a2 + k * g3) - (1. + kmu) * (a2 - k * g3) * ee - 2. * k * (g3 - a2 * prmuz) * (e1 * e2);
+      double NT = e2 * ((1. + kmu) * (a1 + k * g4) - (1. - kmu) * (a1 - k * g4) * ee) - 2. * k * (g4 + a1 * prmuz) * e1;
+      o.r = tiny ? eps : zw * NR * rD;
+      o.t = tiny ? e2 : e2 - zw * NT * rD;
+      double PrD = P * rD;
+      o.rd = g2 * (1. - ee) * PrD;
+      o.td = 2. * k * e1 * PrD;
+      tp.P = P, tp.rD = rD, tp.NR = NR, tp.NT = NT, tp.tiny = tiny;
+    }
   }
   return o;
 }
+__device__ inline RTx reftra_x(double zto1, double zw, double zg, double prmuz, double rmu) {
+  RtTape tp;
+  return reftra_xt(zto1, zw, zg, prmuz, rmu, tp);
+}
 
-// Reverse-mode derivative of reftra_x: given the adjoints of (r, rd, t, td, e2), returns the
-// adjoints of (tau, w, g). The forward expressions are recomputed, then swept in reverse.
-__device__ inline void reftra_exact_adj(double zto1, double zw, double zg, double prmuz, double rmu,
-                                        double rb, double rdb, double tb, double tdb, double e2b,
-                                        double &taub, double &wb, double &gb) {
-  const double zwcrit = 0.9999995, eps = 1.e-08;
-  const double zg3 = 3. * zg;
-  const double g1 = (8. - zw * (5. + zg3)) * 0.25;
-  const double g2 = 3. * (zw * (1. - zg)) * 0.25;
-  const double g3 = (2. - zg3 * prmuz) * 0.25;
-  const double g4 = 1. - g3;
-  double omg = 1. - zg;
-  double den2 = omg * omg - (1. - zw) * zg * zg;
-  bool cons = zg != 1. ? (zw > 0. && den2 > 0. && zw * (omg * omg) >= zwcrit * den2) : (zw >= zwcrit);
+// Reverse-mode derivative of reftra_xt from its tape: given the adjoints of (r, rd, t, td, e2),
+// returns the adjoints of (tau, w, g). General branch, with X = (1 + kmu)(a1 + k g4) -
+// (1 - kmu)(a1 - k g4) ee, Y = g4 + a1 mu0, Z = g3 - a2 mu0 (so NT = e2 X - 2k Y e1 and
+// NR = (1 - kmu)(a2 + k g3) - (1 + kmu)(a2 - k g3) ee - 2k Z e1 e2): the adjoints flow from
+// (r, t) = (w NR rD, e2 - w NT rD) and (rd, td) = (g2 (1 - ee), 2k e1) P rD back through
+// rD = 1 / (P D), P = 1 - kmu^2 (a constant 1 where the denominator is tiny), D, NR, NT,
+// ee = e1^2, e1 = exp(-min(k tau, 500)), k = sqrt(g1^2 - g2^2), a1, a2 and the gammas; within
+// KMU_WIN of the resonance, through the near-resonance form's expressions instead.
+__device__ inline void reftra_xt_adj(const RtTape &tp, double zto1, double zw, double zg, double prmuz,
+                                     double rmu, double rb, double rdb, double tb, double tdb, double e2b,
+                                     double &taub, double &wb, double &gb) {
+  const double g1 = tp.g1, g2 = tp.g2, g3 = tp.g3, g4 = 1. - g3;
   double g1b = 0., g2b = 0., g3b = 0., g4b = 0.;
   taub = 0., wb = 0., gb = 0.;
-  if (cons) {
-    double za1 = g1 * prmuz - g3;
-    double zgt = g1 * zto1;
-    double xa = zto1 * rmu;
-    double x2 = dmin(xa, 500.);
-    double ze2 = dexp(-x2);
-    double inv = frcp(1. + zgt);
-    double ze2b = e2b;
+  if (tp.cons) {
+    const double za1 = tp.a1, zgt = tp.a2, inv = tp.rD, ze2 = tp.e2;
     if (ze2 != 1.0) {
-      double Reff = rb - tb, Rdeff = rdb - tdb;   // t = 1 - r, td = 1 - rd
-      double Rn = zgt - za1 * (1. - ze2);
-      double Rnb = Reff * inv, invb = Reff * Rn;
-      double zgtb = Rdeff * inv;
-      invb += Rdeff * zgt;
-      zgtb += Rnb;
-      double za1b = -Rnb * (1. - ze2);
-      ze2b += Rnb * za1;
-      zgtb -= invb * inv * inv;
-      g1b += zgtb * zto1;
+      // r = (zgt - za1 (1 - e2)) inv, t = 1 - r, rd = zgt inv, td = 1 - rd
+      const double Reff = rb - tb, Rdeff = rdb - tdb;
+      const double Rn = zgt - za1 * (1. - ze2);
+      const double Rnb = Reff * inv;
+      const double invb = Reff * Rn + Rdeff * zgt;
+      const double zgtb = Rdeff * inv + Rnb - invb * inv * inv;
+      const double za1b = -Rnb * (1. - ze2);
+      e2b += Rnb * za1;
+      g1b += zgtb * zto1 + za1b * prmuz;
       taub += zgtb * g1;
-      g1b += za1b * prmuz;
       g3b -= za1b;
     }
-    double x2b = -ze2b * ze2;
-    if (xa < 500.) taub += x2b * rmu;
   } else {
-    const double a1 = g1 * g4 + g2 * g3, a2 = g1 * g3 + g2 * g4;
-    const double k = dsqrt(g1 * g1 - g2 * g2);
-    const double rp = k * prmuz, rp1 = 1. + rp, rm1 = 1. - rp, k2 = 2. * k, rpp = 1. - rp * rp;
-    const double rkg = k + g1;
-    const double r1 = rm1 * (a2 + k * g3), r2 = rp1 * (a2 - k * g3), r3 = k2 * (g3 - a2 * prmuz);
-    const double r4 = rpp * rkg, r5 = rpp * (k - g1);
-    const double t1 = rp1 * (a1 + k * g4), t2 = rm1 * (a1 - k * g4), t3 = k2 * (g4 + a1 * prmuz);
-    const double xk = k * zto1, xa = zto1 * rmu;
-    const double x1 = dmin(xk, 500.), x2 = dmin(xa, 500.);
-    const double em1 = dexp(-x1), em2 = dexp(-x2);
-    const double ep1 = frcp(em1), ep2 = frcp(em2);
-    const double denr = r4 * ep1 + r5 * em1;
-    const double emm = em1 * em1;
-    const double Dd = rkg - (g1 - k) * emm, dend = frcp(Dd);
-    double kb = 0., k2b = 0., em1b = 0., em2b = e2b, emmb = 0., dendb = 0.;
-    double ep1b = 0., ep2b = 0., rkgb = 0.;
-    double r1b = 0., r2b = 0., r3b = 0., r4b = 0., r5b = 0., t1b = 0., t2b = 0., t3b = 0.;
-    // td = k2 em1 dend ; rd = g2 (1 - emm) dend
-    k2b += tdb * em1 * dend;
-    em1b += tdb * k2 * dend;
-    dendb += tdb * k2 * em1;
-    g2b += rdb * (1. - emm) * dend;
-    emmb -= rdb * g2 * dend;
-    dendb += rdb * g2 * (1. - emm);
-    double Ddb = -dendb * dend * dend;
-    rkgb += Ddb;
-    g1b -= Ddb * emm;
-    kb += Ddb * emm;
-    emmb -= Ddb * (g1 - k);
-    em1b += emmb * 2. * em1;
-    double denrb = 0.;
-    if (denr >= -eps && denr <= eps) {
-      em2b += tb;   // r = eps, t = em2
+    const double k = tp.k, e1 = tp.e1, e2 = tp.e2, a1 = tp.a1, a2 = tp.a2;
+    const double ee = e1 * e1, D = tp.D;
+    double kb = 0., e1b = 0., eeb = 0., a1b = 0., a2b = 0., Db = 0.;
+    if (tp.reg) {
+      // the near-resonance form: R = w NR rW, T = e2 - w NT rW, rW = 1 / ((2 - eps) D),
+      // (rd, td) = (g2 (1 - ee), 2k e1) / D, NR = A + B ee - 2 a2 e1 e2 - 2 B ee q,
+      // NT = 2 a1 e1 - C e2 - E ee e2 - 2 C e1 q, q = x phi(-z) s, z = x eps s, s = 1 / (1 - eps),
+      // x = k tau, eps = 1 - k mu0
+      const double ke = tp.eps, x1 = k * zto1, s1 = frcp(1. - ke), z = x1 * ke * s1;
+      const double ph = phi_em1(-z), dph = dphi_em1(-z), q = x1 * ph * s1;
+      const double A = a2 + k * g3, B = a2 - k * g3, C = a1 + k * g4, E = a1 - k * g4;
+      const double rDd = tp.P, rW = tp.rD, r2e = frcp(2. - ke);
+      wb += (rb * tp.NR - tb * tp.NT) * rW;
+      const double NRb = rb * zw * rW, NTb = -tb * zw * rW, rWb = zw * (rb * tp.NR - tb * tp.NT);
+      e2b += tb;
+      double rDdb = rdb * g2 * (1. - ee) + tdb * 2. * k * e1 + rWb * r2e;
+      double epsb = rWb * rW * r2e;
+      g2b += rdb * (1. - ee) * rDd;
+      eeb -= rdb * g2 * rDd;
+      kb += tdb * 2. * e1 * rDd;
+      e1b += tdb * 2. * k * rDd;
+      Db = -rDdb * rDd * rDd;
+      // NR
+      const double Ab = NRb, Bb0 = NRb * ee * (1. - 2. * q);
+      eeb += NRb * B * (1. - 2. * q);
+      a2b -= NRb * 2. * e1 * e2;
+      e1b -= NRb * 2. * a2 * e2;
+      e2b -= NRb * 2. * a2 * e1;
+      double qb = -NRb * 2. * B * ee;
+      // NT
+      a1b += NTb * 2. * e1;
+      e1b += NTb * 2. * (a1 - C * q);
+      const double Cb = -NTb * (e2 + 2. * e1 * q), Eb = -NTb * ee * e2;
+      e2b -= NTb * (C + E * ee);
+      eeb -= NTb * E * e2;
+      qb -= NTb * 2. * C * e1;
+      // q
+      const double x1b = qb * s1 * (ph - z * dph);
+      epsb += qb * x1 * s1 * s1 * (ph - x1 * s1 * dph);
+      kb += x1b * zto1, taub += x1b * k;
+      // A, B, C, E, eps
+      a2b += Ab + Bb0;
+      kb += (Ab - Bb0) * g3;
+      g3b += (Ab - Bb0) * k;
+      a1b += Cb + Eb;
+      kb += (Cb - Eb) * g4;
+      g4b += (Cb - Eb) * k;
+      kb -= epsb * prmuz;
     } else {
-      const double rden = frcp(denr);
-      const double NR = r1 * ep1 - r2 * em1 - r3 * em2;
-      const double NT = t1 * ep1 - t2 * em1 - t3 * ep2;
-      em2b += tb * (1. - zw * NT * rden);
-      wb -= tb * em2 * NT * rden;
-      double NTb = -tb * em2 * zw * rden;
-      double rdenb = -tb * em2 * zw * NT;
-      wb += rb * NR * rden;
-      double NRb = rb * zw * rden;
-      rdenb += rb * zw * NR;
-      t1b += NTb * ep1, ep1b += NTb * t1, t2b -= NTb * em1, em1b -= NTb * t2, t3b -= NTb * ep2, ep2b -= NTb * t3;
-      r1b += NRb * ep1, ep1b += NRb * r1, r2b -= NRb * em1, em1b -= NRb * r2, r3b -= NRb * em2, em2b -= NRb * r3;
-      denrb = -rdenb * rden * rden;
+      const double kmu = k * prmuz, P = tp.P, rD = tp.rD, PrD = P * rD;
+      double kmub = 0., rDb = 0., NRb = 0., NTb = 0.;
+      // rd = g2 (1 - ee) P rD, td = 2k e1 P rD
+      const double PrDb = rdb * g2 * (1. - ee) + tdb * 2. * k * e1;
+      g2b += rdb * (1. - ee) * PrD;
+      eeb -= rdb * g2 * PrD;
+      kb += tdb * 2. * e1 * PrD;
+      e1b += tdb * 2. * k * PrD;
+      // r = w NR rD, t = e2 - w NT rD (r = eps, t = e2 where the denominator is tiny)
+      if (!tp.tiny) {
+        wb += (rb * tp.NR - tb * tp.NT) * rD;
+        NRb = rb * zw * rD;
+        NTb = -tb * zw * rD;
+        rDb = zw * (rb * tp.NR - tb * tp.NT);
+      }
+      e2b += tb;
+      rDb += PrDb * P;
+      const double PDb = -rDb * rD * rD;
+      Db = PDb * P;
+      if (!tp.tiny) kmub -= (PrDb * rD + PDb * D) * 2. * kmu;
+      // NT = e2 X - 2k Y e1
+      const double Xp = a1 + k * g4, Xm = a1 - k * g4;
+      const double Y = g4 + a1 * prmuz;
+      e2b += NTb * ((1. + kmu) * Xp - (1. - kmu) * Xm * ee);
+      const double Xb = NTb * e2, Yb = -NTb * 2. * k * e1;
+      kb -= NTb * 2. * Y * e1;
+      e1b -= NTb * 2. * k * Y;
+      const double xs = (1. + kmu) + (1. - kmu) * ee;
+      kmub += Xb * (Xp + Xm * ee);
+      a1b += Xb * ((1. + kmu) - (1. - kmu) * ee) + Yb * prmuz;
+      kb += Xb * g4 * xs;
+      g4b += Xb * k * xs + Yb;
+      eeb -= Xb * (1. - kmu) * Xm;
+      // NR = (1 - kmu)(a2 + k g3) - (1 + kmu)(a2 - k g3) ee - 2k Z e1 e2
+      const double Rp = a2 + k * g3, Rm = a2 - k * g3, Z = g3 - a2 * prmuz;
+      const double rs = (1. - kmu) + (1. + kmu) * ee;
+      const double Zb = -NRb * 2. * k * e1 * e2;
+      kmub -= NRb * (Rp + Rm * ee);
+      a2b += NRb * ((1. - kmu) - (1. + kmu) * ee) - Zb * prmuz;
+      kb += NRb * (g3 * rs - 2. * Z * e1 * e2);
+      g3b += NRb * k * rs + Zb;
+      eeb -= NRb * (1. + kmu) * Rm;
+      e1b -= NRb * 2. * k * Z * e2;
+      e2b -= NRb * 2. * k * Z * e1;
+      kb += kmub * prmuz;
     }
-    r4b += denrb * ep1, ep1b += denrb * r4, r5b += denrb * em1, em1b += denrb * r5;
-    em1b -= ep1b * ep1 * ep1;
-    em2b -= ep2b * ep2 * ep2;
-    double x1b = -em1b * em1, x2b = -em2b * em2;
-    if (xk < 500.) kb += x1b * zto1, taub += x1b * k;
-    if (xa < 500.) taub += x2b * rmu;
-    double a1b = 0., a2b = 0., rpb = 0., rppb = 0., rm1b = 0., rp1b = 0.;
-    k2b += t3b * (g4 + a1 * prmuz), g4b += t3b * k2, a1b += t3b * k2 * prmuz;
-    rm1b += t2b * (a1 - k * g4), a1b += t2b * rm1, kb -= t2b * rm1 * g4, g4b -= t2b * rm1 * k;
-    rp1b += t1b * (a1 + k * g4), a1b += t1b * rp1, kb += t1b * rp1 * g4, g4b += t1b * rp1 * k;
-    rppb += r5b * (k - g1), kb += r5b * rpp, g1b -= r5b * rpp;
-    rppb += r4b * rkg, rkgb += r4b * rpp;
-    k2b += r3b * (g3 - a2 * prmuz), g3b += r3b * k2, a2b -= r3b * k2 * prmuz;
-    rp1b += r2b * (a2 - k * g3), a2b += r2b * rp1, kb -= r2b * rp1 * g3, g3b -= r2b * rp1 * k;
-    rm1b += r1b * (a2 + k * g3), a2b += r1b * rm1, kb += r1b * rm1 * g3, g3b += r1b * rm1 * k;
-    kb += rkgb, g1b += rkgb;
-    rpb -= rppb * 2. * rp;
-    kb += 2. * k2b;
-    rpb += rp1b - rm1b;
-    kb += rpb * prmuz;
+    // D = (k + g1) - (g1 - k) ee, ee = e1^2, e1 = exp(-min(k tau, 500))
+    kb += Db * (1. + ee);
+    g1b += Db * (1. - ee);
+    eeb -= Db * (g1 - k);
+    e1b += eeb * 2. * e1;
+    const double x1b = -e1b * e1;
+    if (k * zto1 < 500.) kb += x1b * zto1, taub += x1b * k;
+    // k = sqrt(g1^2 - g2^2); a1 = g1 g4 + g2 g3, a2 = g1 g3 + g2 g4
     if (k > 0.) {
       const double rk = frcp(k);
       g1b += kb * g1 * rk, g2b -= kb * g2 * rk;
     }
-    g1b += a2b * g3 + a1b * g4, g3b += a2b * g1 + a1b * g2;
-    g2b += a2b * g4 + a1b * g3, g4b += a2b * g2 + a1b * g1;
+    g1b += a1b * g4 + a2b * g3, g3b += a1b * g2 + a2b * g1;
+    g2b += a1b * g3 + a2b * g4, g4b += a1b * g1 + a2b * g2;
   }
+  // e2 = exp(-min(tau / mu0, 500))
+  const double x2b = -e2b * tp.e2;
+  if (zto1 * rmu < 500.) taub += x2b * rmu;
+  // the gammas: g4 = 1 - g3, g3 = (2 - 3 g mu0) / 4, g2 = 3 w (1 - g) / 4, g1 = (8 - w (5 + 3g)) / 4
   g3b -= g4b;
-  double g3p_b = -g3b * prmuz * 0.25;      // d/d(zg3) through gamma3
+  double g3p_b = -g3b * prmuz * 0.25;      // d/d(3g) through gamma3
   wb += g2b * 3. * (1. - zg) * 0.25;
   gb -= g2b * 3. * zw * 0.25;
-  wb -= g1b * (5. + zg3) * 0.25;
+  wb -= g1b * (5. + 3. * zg) * 0.25;
   g3p_b -= g1b * zw * 0.25;
   gb += 3. * g3p_b;
 }

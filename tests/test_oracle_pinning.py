@@ -217,6 +217,80 @@ def test_reftra_exact_exponential_matches_two_stream_solution():
     assert worst > 0.0
 
 
+def _two_stream_beam(tau, w, g, mu):
+    """The same layer response from the system augmented with the direct beam, y = (U, D, S),
+    dS/dt = -S/mu: one 3 x 3 matrix exponential, with no particular-solution denominator, so it
+    holds at the resonance k mu = 1 too. Returns R, T (collimated, T = D(tau) + S(tau))."""
+    g1 = (8.0 - w * (5.0 + 3.0 * g)) / 4.0
+    g2 = 3.0 * w * (1.0 - g) / 4.0
+    g3 = (2.0 - 3.0 * g * mu) / 4.0
+    g4 = 1.0 - g3
+    A = np.array([[g1, -g2, -w * g3 / mu], [g2, -g1, w * g4 / mu], [0.0, 0.0, -1.0 / mu]])
+    M = scipy.linalg.expm(A * tau)
+    U0 = -M[0, 2] / M[0, 0]   # U(tau) = 0 with D(0) = 0, S(0) = 1
+    y = M @ np.array([U0, 0.0, 1.0])
+    return U0, y[1] + y[2]
+
+
+def _closed_form_mp(tau, w, g, mu, digits=60):
+    """The reference's closed-form general solution (src/rrtmg_sw_reftra.f90:233-322: zr1..zr5,
+    zt1..zt3, zdenr) in 60-digit arithmetic: its cancellation near k mu = 1 then costs digits the
+    result does not need. Returns R, T."""
+    import mpmath as mp
+    mp.mp.dps = digits
+    tau, w, g, mu = mp.mpf(tau), mp.mpf(w), mp.mpf(g), mp.mpf(mu)
+    g1 = (8 - w * (5 + 3 * g)) / 4
+    g2 = 3 * w * (1 - g) / 4
+    g3 = (2 - 3 * g * mu) / 4
+    g4 = 1 - g3
+    a1, a2 = g1 * g4 + g2 * g3, g1 * g3 + g2 * g4
+    k = mp.sqrt(g1 * g1 - g2 * g2)
+    rp = k * mu
+    r1, r2, r3 = (1 - rp) * (a2 + k * g3), (1 + rp) * (a2 - k * g3), 2 * k * (g3 - a2 * mu)
+    r4, r5 = (1 - rp * rp) * (k + g1), (1 - rp * rp) * (k - g1)
+    t1, t2, t3 = (1 + rp) * (a1 + k * g4), (1 - rp) * (a1 - k * g4), 2 * k * (g4 + a1 * mu)
+    em1, em2 = mp.exp(-k * tau), mp.exp(-tau / mu)
+    den = r4 / em1 + r5 * em1
+    R = w * (r1 / em1 - r2 * em1 - r3 * em2) / den
+    T = em2 - em2 * w * (t1 / em1 - t2 * em1 - t3 / em2) / den
+    return float(R), float(T)
+
+
+def test_reftra_exact_exponential_through_the_resonance():
+    """At k mu0 = 1 the closed form's particular solution divides by 1 - (k mu0)^2, which vanishes
+    with its numerators. The exact-exponential mode evaluates the general solution there in a
+    form without the cancellation (|1 - k mu0| < 1e-3), so it is the two-stream solution to
+    rounding at the resonance, next to it, and on both sides of the window's edge: against the
+    augmented matrix exponential (an independent solution of the ODEs, for layers it resolves,
+    k tau <= 8) and against the closed form in 60-digit arithmetic (every thickness; at the
+    resonance itself 1e-30 from it). The reference's expressions in FP64 lose about
+    1e-16 / |1 - (k mu0)^2| there (1e-8 at 1e-9 from the resonance, 2e-11 at 1e-6) and return
+    R = 1e-8 where the denominator vanishes."""
+    ob.orc().orc_set_exact_exp(1)
+    try:
+        worst, n = 0.0, 0
+        for w0, g0 in ((0.5, 0.5), (0.2, 0.3), (0.4, 0.6)):
+            _, w, g = _delta_scaled(1.0, w0, g0)
+            g1 = (8 - w * (5 + 3 * g)) / 4
+            k = math.sqrt(g1 ** 2 - (3 * w * (1 - g) / 4) ** 2)
+            for tau in (1e-3, 0.1, 1.0, 4.0, 30.0, 250.0):
+                for d in (0.0, 1e-12, -1e-12, 1e-9, -1e-9, 1e-6, -1e-6, 1e-4, -1e-4,
+                          1e-3 - 1e-9, -1e-3 + 1e-9, 1e-3 + 1e-9, -1e-3 - 1e-9, 1e-2, -1e-2):
+                    mu = (1.0 + d) / k
+                    R, _, T, _ = [v[0] for v in _reftra([tau], [w], [g], mu)]
+                    refs = [_closed_form_mp(tau, w, g, mu if d != 0.0 else (1.0 + 1e-30) / k)]
+                    if k * tau <= 8.0:
+                        refs.append(_two_stream_beam(tau, w, g, mu))
+                    for eR, eT in refs:
+                        err = max(abs(R - eR), abs(T - eT))
+                        worst = max(worst, err)
+                        n += 1
+                        assert err < 1e-11, (w, g, tau, d, (R, eR), (T, eT))
+    finally:
+        ob.orc().orc_set_exact_exp(0)
+    assert n == 3 * 15 * (6 + 4) and worst > 0.0
+
+
 def test_exact_exponential_forward_near_table_forward():
     """End to end, the exact-exponential forward differs from the reference's table forward only
     by the table's quantisation: under 5 W/m2, and not at all where the mode is off again."""
