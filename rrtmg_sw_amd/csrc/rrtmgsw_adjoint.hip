@@ -57,7 +57,7 @@ __device__ inline void key_spec_adj(double c1, double c2, double strrat, double 
   if (bin) {
     // only binned bands use the ratio; the others can have speccomb = 0 (no water vapour in a
     // band keyed on it alone), where its reciprocal is infinite
-    const double rsc = frcp(speccomb);
+    const double rsc = frcp1(speccomb);
     const double specparm = c1 * rsc;
     const double specparm_b = specparm < ONEMINUS ? fs_b * mult : 0.0;
     speccomb_b -= specparm_b * c1 * rsc * rsc;
@@ -138,11 +138,11 @@ __device__ inline SkyOpt clear_optics(double taug, double taur, double ta, doubl
 #pragma clang fp contract(off)
   const double tau0 = taur + taug + ta;
   const double sct = taur * 1.0 + ta * wa;
-  r1 = frcp(sct * tau0);
+  r1 = frcp1(sct * tau0);
   const double g0 = ga * wa * ta * tau0 * r1;
   const double w0 = sct * sct * r1;
   const double zwf = w0 * (g0 * g0);
-  r2 = frcp((1.0 - zwf) * (1.0 + g0));
+  r2 = frcp1((1.0 - zwf) * (1.0 + g0));
   return {(1.0 - zwf) * tau0, (w0 - zwf) * (1.0 + g0) * r2, g0 * (1.0 - zwf) * r2};
 }
 
@@ -170,7 +170,7 @@ __device__ inline LayerV cell_layer(double taug, double taur, const LayerIn &in,
     // total-sky optics (:500-505): go = (ct cw cg + tauc omc gcc) / so, omo = so / tauo
     const double tauo = tp.c.tau + in.ct;
     const double so = tp.c.tau * tp.c.w + in.ct * in.cw;
-    const double r = frcp(so * tauo);
+    const double r = frcp1(so * tauo);
     tp.s = {tauo, so * so * r, (in.ct * in.cw * in.cg + tp.c.tau * tp.c.w * tp.c.g) * tauo * r};
     tp.r3 = r;
   }
@@ -263,16 +263,23 @@ __device__ static LayerIn layer_inputs(const RtArgs &a, const DeviceTables &T, i
   return in;
 }
 
-// Adjoint of one elimination step (sweep_layer): given the adjoints of the new state sb (in,
-// replaced by the adjoint of the old state) and of C, E, A = ztdn, B = prdnd, returns the
-// adjoints of the layer's r, rd, t, td, dbt in lb[0..4].
-__device__ inline void sweep_layer_adj(const Sweep &s, const RT4 &l, double dbt, Sweep &sb,
-                                       double C_b, double E_b, double A_b, double B_b, double *lb) {
+// One step of the back substitution and its adjoint. With the sweep state s at the top of layer
+// j and its solution l: C_j = P rho, E_j = td rho (rho = 1 / (1 - rd prdnd), P = rd a + r tdbt,
+// a = ztdn - tdbt) give U_j = C_j + E_j U_(j+1) (u: U_(j+1) in, U_j out). J reaches the step
+// through U_j (adjoint ub, the part carried from above), through U_j's weight in D_j = ztdn +
+// prdnd U_j (zwD = zinc wD_j, adjoints A = zwD of ztdn and B = zwD U_j of prdnd) and through the
+// new state (adjoint sb, replaced by the adjoint of s). Returns the adjoints of the layer's
+// r, rd, t, td, dbt in lb[0..4].
+__device__ inline void sweep_step_adj(const Sweep &s, const RT4 &l, double dbt, double ub, double zwD,
+                                      double &u, Sweep &sb, double *lb) {
   const double t = s.tdbt, z = s.ztdn, p = s.prdnd;
   const double a = z - t;
-  const double den = 1. - l.rd * p, rho = frcp(den);
+  const double den = 1. - l.rd * p, rho = frcp1(den);
   const double P = l.rd * a + l.r * t;
   const double Q = a + t * l.r * p;
+  const double Uj1 = u;
+  u = P * rho + l.td * rho * Uj1;
+  const double C_b = ub, E_b = ub * Uj1, A_b = zwD, B_b = zwD * u;
   double t_b = 0., z_b = A_b, p_b = B_b, a_b = 0., rho_b = 0.;
   double r_b = 0., rd_b = 0., tr_b = 0., td_b = 0., dbt_b = 0.;
   // tdbt' = dbt t
@@ -378,6 +385,8 @@ __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
   return ((size_t)L + states) * NGPT + (size_t)L * NBND * CJ_N;
 }
 
+// CKPT: checkpoint mode (every K-th sweep state kept, K = A.ckpt >= 1), else store mode.
+template <bool CKPT>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) adj_kernel(AdjArgs A, DeviceTables T) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const RtArgs &a = A.f;
@@ -437,7 +446,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
                        a.mt_stream, reinterpret_cast<const KissJump *>(T.kiss_jump), v.mask, v.misc[M_CLDTOP]);
   __syncthreads();
   // cloud optics and their water-path derivatives for each (layer, band) with a cloudy sub-column
-  const int K = A.ckpt;
+  const int K = CKPT ? A.ckpt : 0;
   double *UB = A.work + (size_t)cl * adj_work_doubles(L, K);
   double *CJ = UB + adj_work_doubles(L, K) - (size_t)L * NBND * CJ_N;
   if (a.sc.icld >= 1) {
@@ -480,12 +489,12 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   double *S = UB + (size_t)L * NGPT;
   double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
   auto ck = [&](int i, int k) -> double & { return S[((size_t)i * 3 + k) * NGPT + g]; };
-  // record k of layer j: S in store mode, SG (segment-relative) in checkpoint mode
-  auto rec = [&](int j, int k) -> double & {
-    return K == 0 ? S[((size_t)j * REC_S + k) * NGPT + g] : SG[((size_t)(j % K) * REC_S + k) * NGPT + g];
-  };
-  auto put_rec = [&](int j, const Sweep &s, double taug, double taur) {
-    rec(j, 0) = s.tdbt, rec(j, 1) = s.ztdn, rec(j, 2) = s.prdnd, rec(j, 3) = taug, rec(j, 4) = taur;
+  // the lane's record in slot i (fields k * NGPT apart): layer i of S in store mode, layer
+  // jseg0 + i of the current segment in SG in checkpoint mode (a uniform base plus the lane)
+  auto recp = [&](int i) -> double * { return (CKPT ? SG : S) + (size_t)i * REC_S * NGPT + g; };
+  auto put_rec = [&](int i, const Sweep &s, double taug, double taur) {
+    double *r = recp(i);
+    r[0] = s.tdbt, r[NGPT] = s.ztdn, r[2 * NGPT] = s.prdnd, r[3 * NGPT] = taug, r[4 * NGPT] = taur;
   };
 
   const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
@@ -501,14 +510,21 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   int pj = -1;
   auto flush_rec = [&]() {
     if (active && pj >= 0) {
-      if (K == 0)
-        rec(pj, 0) = pr[0], rec(pj, 1) = pr[1], rec(pj, 2) = pr[2], rec(pj, 3) = pr[3], rec(pj, 4) = pr[4];
+      if (!CKPT) {
+        double *r = recp(pj);
+        r[0] = pr[0], r[NGPT] = pr[1], r[2 * NGPT] = pr[2], r[3 * NGPT] = pr[3], r[4 * NGPT] = pr[4];
+      }
       UB[(size_t)pj * NGPT + g] = pr[5];
     }
   };
+  int jk = 0, ci = 0;   // checkpoint mode: layer within the segment, checkpoint index
   auto f1_layer = [&](auto lower_tag, const GasLane &q, int j) {
     constexpr bool LOWER = decltype(lower_tag)::value;
-    if (active && K > 0 && j % K == 0) ck(j / K, 0) = st.tdbt, ck(j / K, 1) = st.ztdn, ck(j / K, 2) = st.prdnd;
+    if (CKPT) {
+      if (jk == 0 && active) ck(ci, 0) = st.tdbt, ck(ci, 1) = st.ztdn, ck(ci, 2) = st.prdnd;
+      ci += jk == 0;
+      jk = jk + 1 == K ? 0 : jk + 1;
+    }
     const int lay = L - 1 - j;
     const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
     flush_rec();
@@ -560,28 +576,31 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 
   // ---- R1: bottom -> top. The lane's gas-optics constants are read from its g-point record
   // where used: held in registers across this sweep they would not fit beside its state. ----
-  auto gas_rec = [&](int j, double &taug, double &taur) {
-    const int lay = L - 1 - j;
-    if (j >= jsplit) {
-      const GasLane o = gas_lane<true>(gp, L);
-      gas_finish<true, double>(o, gas_fetch<true, double>(o, T.tab, v, lay), v, lay, taug, taur);
-    } else {
-      const GasLane o = gas_lane<false>(gp, L);
-      gas_finish<false, double>(o, gas_fetch<false, double>(o, T.tab, v, lay), v, lay, taug, taur);
-    }
-  };
-  for (int j = L - 1; j >= 0; j--) {
-    if (K > 0 && (j == L - 1 || (j + 1) % K == 0)) {
-      // checkpoint mode: the records of this segment, re-swept from its checkpoint with the
-      // forward sweep's own functions (the same values as F1's, so the same as store mode's)
-      const int j0 = (j / K) * K;
-      Sweep ss = {active ? ck(j0 / K, 0) : 1.0, active ? ck(j0 / K, 1) : 1.0, active ? ck(j0 / K, 2) : 0.0};
-      for (int jj = j0; jj <= j; jj++) {
+  // Checkpoint mode: the records of the segment jseg0..j, re-swept from its checkpoint with the
+  // forward sweep's own functions (the same values as F1's, so the same as store mode's), one
+  // loop per regime with the lane's constants resolved once; a record is stored once the next
+  // layer's table reads have been issued, as in F1.
+  // checkpoint mode: the segment being reversed starts at layer jseg0 = ci K
+  ci = CKPT ? (L - 1) / max(K, 1) : 0;
+  int jseg0 = ci * K;
+  auto resweep = [&](int j) {
+    Sweep ss = {active ? ck(ci, 0) : 1.0, active ? ck(ci, 1) : 1.0, active ? ck(ci, 2) : 0.0};
+    int pend = -1;
+    Sweep ps = ss;
+    double ptg = 0.0, ptr = 0.0;
+    auto part = [&](auto lower_tag, int ja, int jb) {
+      constexpr bool LOWER = decltype(lower_tag)::value;
+      if (ja >= jb) return;
+      const GasLane q = gas_lane<LOWER>(gp, L);
+      for (int jj = ja; jj < jb; jj++) {
+        const int lay = L - 1 - jj;
+        const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
+        if (pend >= 0 && active) put_rec(pend - jseg0, ps, ptg, ptr);
         double taug, taur;
-        gas_rec(jj, taug, taur);
-        if (active) put_rec(jj, ss, taug, taur);
+        gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
+        pend = jj, ps = ss, ptg = taug, ptr = taur;
         if (jj < j) {
-          const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - jj, active);
+          const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
           CellTape tp;
           const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
           RT4 l = {ly.r, ly.rd, ly.t, ly.td};
@@ -589,26 +608,32 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
           sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
         }
       }
+    };
+    part(std::integral_constant<bool, false>(), jseg0, min(j + 1, jsplit));
+    part(std::integral_constant<bool, true>(), max(jseg0, jsplit), j + 1);
+    if (pend >= 0 && active) put_rec(pend - jseg0, ps, ptg, ptr);
+  };
+  for (int j = L - 1; j >= 0; j--) {
+    if (CKPT && (j == L - 1 || j < jseg0)) {
+      if (j < jseg0) ci -= 1, jseg0 -= K;
+      resweep(j);
     }
     // the layer's record (sweep state at its top, gas optics); its two-stream solution formed
     // again with the intermediates the reverse needs (taped)
-    const Sweep sj = {active ? rec(j, 0) : 1.0, active ? rec(j, 1) : 1.0, active ? rec(j, 2) : 0.0};
-    const double taug = active ? rec(j, 3) : 1., taur = active ? rec(j, 4) : 0.1;
+    const double *r = recp(CKPT ? j - jseg0 : j);
+    const Sweep sj = {active ? r[0] : 1.0, active ? r[NGPT] : 1.0, active ? r[2 * NGPT] : 0.0};
+    const double taug = active ? r[3 * NGPT] : 1., taur = active ? r[4 * NGPT] : 0.1;
     const double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
     const int lay = L - 1 - j;
     const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
     CellTape tp;
     const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
-    RT4 l = {ly.r, ly.rd, ly.t, ly.td};
-    // C_j, E_j and U_j from the layer
-    Sweep tmp = sj;
-    double Aa, Bb, Cc, Ee;
-    sweep_layer(tmp, l, ly.dbt, Aa, Bb, Cc, Ee);
-    const double Uj1 = u;
-    u = Cc + Ee * Uj1;
-    phi += wU[j] * u + wD[j] * (sj.ztdn + sj.prdnd * u);
+    const RT4 l = {ly.r, ly.rd, ly.t, ly.td};
+    // U_j by the back substitution, the objective per unit incident flux, and the step's adjoint
+    const double zwD = zinc * wD[j];
     double lb[5], xb[5];
-    sweep_layer_adj(sj, l, ly.dbt, sb, ub, ub * Uj1, zinc * wD[j], zinc * wD[j] * u, lb);
+    sweep_step_adj(sj, l, ly.dbt, ub, zwD, u, sb, lb);
+    phi += wU[j] * u + wD[j] * (sj.ztdn + sj.prdnd * u);
     cell_layer_adj(tp, taug, taur, in, mu0, rmu, lb, xb);
     double la[AC_N];
 #pragma unroll
@@ -695,7 +720,10 @@ size_t adj_work_bytes(int nlay, int ckpt) { return adj_work_doubles(nlay, ckpt) 
 
 hipError_t launch_adjoint(const AdjArgs &a, const DeviceTables &t, hipStream_t s) {
   int per = (a.f.ncol_chunk + 7) / 8;
-  hipLaunchKernelGGL(adj_kernel, dim3(per * 8), dim3(128), adj_lds_bytes(a.f.nlay), s, a, t);
+  if (a.ckpt > 0)
+    hipLaunchKernelGGL(adj_kernel<true>, dim3(per * 8), dim3(128), adj_lds_bytes(a.f.nlay), s, a, t);
+  else
+    hipLaunchKernelGGL(adj_kernel<false>, dim3(per * 8), dim3(128), adj_lds_bytes(a.f.nlay), s, a, t);
   return hipGetLastError();
 }
 

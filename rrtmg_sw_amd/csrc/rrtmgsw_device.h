@@ -603,7 +603,7 @@ __device__ inline RTx reftra_xt(double zto1, double zw, double zg, double prmuz,
   if (cons) {
     double za1 = g1 * prmuz - g3;
     double zgt = g1 * zto1;
-    double inv = frcp(1. + zgt);
+    double inv = frcp1(1. + zgt);
     o.r = (zgt - za1 * (1. - e2)) * inv;
     o.t = 1. - o.r;
     o.rd = zgt * inv;
@@ -623,13 +623,13 @@ __device__ inline RTx reftra_xt(double zto1, double zw, double zg, double prmuz,
     tp.a1 = a1, tp.a2 = a2, tp.k = k, tp.e1 = e1, tp.D = D, tp.eps = ke;
     if (fabs(ke) < KMU_WIN && x1 < 500. && zto1 * rmu < 500.) {
       // the near-resonance form (KMU_WIN)
-      const double s1 = frcp(1. - ke);
+      const double s1 = frcp1(1. - ke);
       const double q = x1 * phi_em1(-(x1 * ke * s1)) * s1;
       const double B = a2 - k * g3, C = a1 + k * g4;
       const double NR = (a2 + k * g3) + B * ee - 2. * a2 * e1 * e2 - 2. * B * ee * q;
       const double NT = 2. * a1 * e1 - C * e2 - (a1 - k * g4) * ee * e2 - 2. * C * e1 * q;
-      const double rDd = frcp(D);
-      const double rW = rDd * frcp(2. - ke);
+      const double rDd = frcp1(D);
+      const double rW = rDd * frcp1(2. - ke);
       o.r = zw * NR * rW;
       o.t = e2 - zw * NT * rW;
       o.rd = g2 * (1. - ee) * rDd;
@@ -639,7 +639,7 @@ __device__ inline RTx reftra_xt(double zto1, double zw, double zg, double prmuz,
       double P = 1. - kmu * kmu;
       bool tiny = fabs(P * D) <= eps * e1;
       P = tiny ? 1. : P;
-      double rD = frcp(P * D);
+      double rD = frcp1(P * D);
       double NR = (1. - kmu) * (a2 + k * g3) - (1. + kmu) * (a2 - k * g3) * ee - 2. * k * (g3 - a2 * prmuz) * (e1 * e2);
       double NT = e2 * ((1. + kmu) * (a1 + k * g4) - (1. - kmu) * (a1 - k * g4) * ee) - 2. * k * (g4 + a1 * prmuz) * e1;
       o.r = tiny ? eps : zw * NR * rD;
@@ -695,10 +695,10 @@ __device__ inline void reftra_xt_adj(const RtTape &tp, double zto1, double zw, d
       // (rd, td) = (g2 (1 - ee), 2k e1) / D, NR = A + B ee - 2 a2 e1 e2 - 2 B ee q,
       // NT = 2 a1 e1 - C e2 - E ee e2 - 2 C e1 q, q = x phi(-z) s, z = x eps s, s = 1 / (1 - eps),
       // x = k tau, eps = 1 - k mu0
-      const double ke = tp.eps, x1 = k * zto1, s1 = frcp(1. - ke), z = x1 * ke * s1;
+      const double ke = tp.eps, x1 = k * zto1, s1 = frcp1(1. - ke), z = x1 * ke * s1;
       const double ph = phi_em1(-z), dph = dphi_em1(-z), q = x1 * ph * s1;
-      const double A = a2 + k * g3, B = a2 - k * g3, C = a1 + k * g4, E = a1 - k * g4;
-      const double rDd = tp.P, rW = tp.rD, r2e = frcp(2. - ke);
+      const double B = a2 - k * g3, C = a1 + k * g4, E = a1 - k * g4;
+      const double rDd = tp.P, rW = tp.rD, r2e = frcp1(2. - ke);
       wb += (rb * tp.NR - tb * tp.NT) * rW;
       const double NRb = rb * zw * rW, NTb = -tb * zw * rW, rWb = zw * (rb * tp.NR - tb * tp.NT);
       e2b += tb;
@@ -791,7 +791,7 @@ __device__ inline void reftra_xt_adj(const RtTape &tp, double zto1, double zw, d
     if (k * zto1 < 500.) kb += x1b * zto1, taub += x1b * k;
     // k = sqrt(g1^2 - g2^2); a1 = g1 g4 + g2 g3, a2 = g1 g3 + g2 g4
     if (k > 0.) {
-      const double rk = frcp(k);
+      const double rk = frcp1(k);
       g1b += kb * g1 * rk, g2b -= kb * g2 * rk;
     }
     g1b += a1b * g4 + a2b * g3, g3b += a1b * g2 + a2b * g1;
@@ -824,7 +824,7 @@ __device__ inline void sweep_layer(Sweep &s, const RT4 &l, double dbt, double &A
   A = s.ztdn;
   B = s.prdnd;
   double a = s.ztdn - s.tdbt;
-  double zreflect = frcp(1. - l.rd * s.prdnd);
+  double zreflect = frcp1(1. - l.rd * s.prdnd);
   Cc = (l.rd * a + l.r * s.tdbt) * zreflect;
   E = l.td * zreflect;
   double ztdn = s.tdbt * l.t + (l.td * (a + s.tdbt * l.r * s.prdnd)) * zreflect;
