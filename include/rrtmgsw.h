@@ -53,6 +53,32 @@ extern "C" {
  * uploads them to the current GPU. Must precede every other call. */
 int rrtmgsw_init(double cpdair);
 
+/* Replaces rrtmg_sw_ini(cpdair) of a build with src/rrtmg_sw_read_nc.f90 in place of
+ * src/rrtmg_sw_k_g.f90: the k-distribution and solar-source data of the fourteen bands are read
+ * from the netCDF file `path` (NULL or "": "rrtmg_sw.nc" in the working directory, the name
+ * read_nc opens) with the start/count of each nf90_get_var there (sw_kgb16 .. sw_kgb29,
+ * :46-1047); the rest as rrtmgsw_init. Classic, 64-bit-offset and 64-bit-data files are read; a
+ * netCDF-4 (HDF5) file is refused with a message. A missing variable or a dimension too short
+ * for a read fails with "Error reading band NN variables from file", read_nc's STOP text. The
+ * tables of one process come from one source: rrtmgsw_init and rrtmgsw_init_nc with another
+ * source fail until rrtmgsw_finalize. */
+int rrtmgsw_init_nc(double cpdair, const char *path);
+
+/* Host-only (no GPU call): builds the reduced tables from `ktable_file` (NULL: the compiled-in
+ * data) and writes up to `cap` doubles of them to `out` -- the k, solar, cloud and aerosol table
+ * image, then per g-point (112) the Rayleigh coefficients of the lower and upper atmosphere and
+ * the minor-absorber coefficients of both -- and their total count to *n. No counterpart in the
+ * reference, whose tables are module variables a Fortran program reads directly; it lets a
+ * k-table file be checked without a GPU. */
+int rrtmgsw_host_tables(const char *ktable_file, double *out, size_t cap, size_t *n);
+
+/* Host-only: reads the hyperslab start[d] .. start[d] + count[d] - 1 (dimensions slowest first,
+ * 0-based) of variable `name` of the netCDF file `path` as doubles, row-major over count -- the
+ * nc_get_vara_double under the nf90_get_var calls of src/rrtmg_sw_read_nc.f90, for the classic
+ * formats that rrtmgsw_init_nc reads. `ndims` must equal the variable's rank. */
+int rrtmgsw_nc_get_vara_double(const char *path, const char *name, int ndims, const size_t *start,
+                               const size_t *count, double *out);
+
 /* Frees the device tables (no counterpart in the reference, whose tables are module state). */
 int rrtmgsw_finalize(void);
 

@@ -36,6 +36,10 @@ def lib():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rrtmg_sw_amd.build`")
     L = C.CDLL(LIB_PATH)
     L.rrtmgsw_init.argtypes = [C.c_double]
+    L.rrtmgsw_init_nc.argtypes = [C.c_double, C.c_char_p]
+    L.rrtmgsw_host_tables.argtypes = [C.c_char_p, _dp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.rrtmgsw_nc_get_vara_double.argtypes = [C.c_char_p, C.c_char_p, C.c_int,
+                                             C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _dp]
     L.rrtmgsw_finalize.argtypes = []
     L.rrtmgsw_last_error.restype = C.c_char_p
     sw = ([C.c_int, C.c_int, _ip, _ip] + [_dp] * 16 + [C.c_double, C.c_int, C.c_double, C.c_int,
@@ -65,7 +69,8 @@ def lib():
     return L
 
 
-EXPORTED = ["rrtmgsw_init", "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_sw",
+EXPORTED = ["rrtmgsw_init", "rrtmgsw_init_nc", "rrtmgsw_host_tables", "rrtmgsw_nc_get_vara_double",
+            "rrtmgsw_finalize", "rrtmgsw_last_error", "rrtmgsw_sw",
             "rrtmgsw_sw_dev", "rrtmgsw_sw_nomcica", "rrtmgsw_sw_nomcica_dev", "rrtmgsw_column_sw",
             "rrtmgsw_column_mcica_sw",
             "rrtmgsw_mcica_subcol", "rrtmgsw_mcica_subcol_dev",

@@ -16,6 +16,7 @@ contiguous (nlay, ncol) tensor). Outputs come back in the same form as the input
 Errors raise RRTMGSWError carrying the reference's STOP message.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -75,9 +76,43 @@ def to_host(t, fshape=None):
     return np.asfortranarray(a.T) if fshape is None else np.asfortranarray(a.T.reshape(fshape))
 
 
-def rrtmg_sw_ini(cpdair=1004.64):
-    """Build and upload the 112-point tables (src/rrtmg_sw_init.f90:47)."""
-    check(_lib.lib().rrtmgsw_init(C.c_double(cpdair)))
+def rrtmg_sw_ini(cpdair=1004.64, ktable_file=None):
+    """Build and upload the 112-point tables (src/rrtmg_sw_init.f90:47). With `ktable_file`, the
+    k-distribution and solar-source data come from that netCDF file, read as
+    src/rrtmg_sw_read_nc.f90 reads rrtmg_sw.nc ("" names rrtmg_sw.nc in the working directory)."""
+    if ktable_file is None:
+        check(_lib.lib().rrtmgsw_init(C.c_double(cpdair)))
+    else:
+        check(_lib.lib().rrtmgsw_init_nc(C.c_double(cpdair), os.fsencode(ktable_file)))
+
+
+def rrtmg_sw_finalize():
+    """Free the device tables; the next rrtmg_sw_ini may take its tables from another source."""
+    check(_lib.lib().rrtmgsw_finalize())
+
+
+def host_tables(ktable_file=None):
+    """The reduced tables built on the host (no GPU): the table image followed by four
+    coefficients per g-point (rrtmgsw_host_tables)."""
+    L = _lib.lib()
+    path = None if ktable_file is None else os.fsencode(ktable_file)
+    n = C.c_size_t(0)
+    check(L.rrtmgsw_host_tables(path, None, 0, C.byref(n)))
+    out = np.empty(n.value, dtype=np.float64)
+    check(L.rrtmgsw_host_tables(path, out.ctypes.data_as(C.POINTER(C.c_double)), n.value,
+                                C.byref(n)))
+    return out
+
+
+def nc_get_vara(path, name, start, count):
+    """Hyperslab of a netCDF classic-format variable as float64, shape `count` (C order)."""
+    start = (C.c_size_t * len(start))(*start)
+    cnt = (C.c_size_t * len(count))(*count)
+    out = np.empty(int(np.prod(count, dtype=np.int64)), dtype=np.float64)
+    check(_lib.lib().rrtmgsw_nc_get_vara_double(os.fsencode(path), name.encode(), len(count),
+                                                start, cnt,
+                                                out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out.reshape(tuple(count))
 
 
 def rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr,

@@ -26,7 +26,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", os.path.join(ROOT, "include")]
 SOURCES = ["rrtmgsw_kernels.hip", "rrtmgsw_adjoint.hip", "rrtmgsw_api.cpp", "rrtmgsw_tables.cpp",
-           "rrtmgsw_mtjump.cpp"]
+           "rrtmgsw_mtjump.cpp", "rrtmgsw_nc.cpp"]
 
 
 def _run(cmd, verbose):

@@ -19,6 +19,7 @@
 
 #include "../../include/rrtmgsw.h"
 #include "rrtmgsw_internal.h"
+#include "rrtmgsw_nc.h"
 #include "rrtmgsw_tables.h"
 
 using namespace rrtmgsw;
@@ -54,6 +55,7 @@ struct Context {
   int device = 0;
   double heatfac = 0.0;
   HostTables host;
+  std::string source;             // k-table file the tables came from ("" = compiled-in data)
   DeviceTables dev{};
   double *d_tab = nullptr, *d_exp = nullptr, *d_pthr = nullptr;
   void *d_kisspow = nullptr;      // KissPow: the KISS maps of 2^k draws
@@ -656,13 +658,24 @@ extern "C" {
 
 const char *rrtmgsw_last_error(void) { return g_err.c_str(); }
 
-int rrtmgsw_init(double cpdair) {
+}  // extern "C"
+
+namespace {
+
+// rrtmgsw_init / rrtmgsw_init_nc. `source`: the k-table file, or "" for the compiled-in tables.
+int init_tables(double cpdair, const std::string &source) {
   std::lock_guard<std::mutex> lk(ctx.mu);
   if (!(cpdair > 0.0)) return fail(RRTMGSW_ERR_ARGUMENT, "cpdair must be positive");
+  if (ctx.ready && ctx.source != source)
+    return fail(RRTMGSW_ERR_ARGUMENT,
+                "the tables are already initialised from " +
+                    (ctx.source.empty() ? std::string("the compiled-in data") : ctx.source) +
+                    ": call rrtmgsw_finalize first");
   ctx.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);  // src/rrtmg_sw_init.f90:258
   if (ctx.ready) return 0;
   std::string err;
-  if (!build_tables(ctx.host, err)) return fail(RRTMGSW_ERR_ARGUMENT, err);
+  if (!build_tables(ctx.host, err, source.empty() ? nullptr : source.c_str()))
+    return fail(RRTMGSW_ERR_ARGUMENT, err);
   HIPCHK(hipGetDevice(&ctx.device));
   const HostTables &H = ctx.host;
   HIPCHK(hipMalloc(&ctx.d_tab, H.tab.size() * sizeof(double)));
@@ -692,7 +705,47 @@ int rrtmgsw_init(double cpdair) {
     HIPCHK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
     ctx.noalloc[ctx.device] = p;
   }
+  ctx.source = source;
   ctx.ready = true;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rrtmgsw_init(double cpdair) { return init_tables(cpdair, std::string()); }
+
+int rrtmgsw_init_nc(double cpdair, const char *path) {
+  return init_tables(cpdair, path && *path ? std::string(path) : std::string("rrtmg_sw.nc"));
+}
+
+int rrtmgsw_host_tables(const char *ktable_file, double *out, size_t cap, size_t *n) {
+  HostTables H;
+  std::string err;
+  if (!build_tables(H, err, ktable_file)) return fail(RRTMGSW_ERR_ARGUMENT, err);
+  std::vector<double> v(H.tab);
+  for (const GPoint &g : H.gp)
+    v.insert(v.end(), {g.rayl_lo, g.rayl_hi, g.lo_minor_coef, g.hi_minor_coef});
+  if (n) *n = v.size();
+  if (out) std::copy(v.begin(), v.begin() + std::min(cap, v.size()), out);
+  return 0;
+}
+
+int rrtmgsw_nc_get_vara_double(const char *path, const char *name, int ndims, const size_t *start,
+                               const size_t *count, double *out) {
+  if (!path || !name || ndims < 0 || (ndims && (!start || !count)))
+    return fail(RRTMGSW_ERR_ARGUMENT, "rrtmgsw_nc_get_vara_double: null argument");
+  NcFile nc;
+  std::string err;
+  if (!nc.open(path, err)) return fail(RRTMGSW_ERR_ARGUMENT, err);
+  const NcFile::Var *v = nc.var(name);
+  if (!v) return fail(RRTMGSW_ERR_ARGUMENT, std::string(path) + ": no variable " + name);
+  if ((int)v->shape.size() != ndims)
+    return fail(RRTMGSW_ERR_ARGUMENT, std::string(path) + ": variable " + name + " has " +
+                                          std::to_string(v->shape.size()) + " dimensions");
+  std::vector<uint64_t> s(start, start + ndims), c(count, count + ndims);
+  if (!nc.get(*v, s.data(), c.data(), out, err)) return fail(RRTMGSW_ERR_ARGUMENT, err);
   return 0;
 }
 

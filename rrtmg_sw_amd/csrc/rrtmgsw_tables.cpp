@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "rrtmgsw_internal.h"
+#include "rrtmgsw_nc.h"
 #include "rrtmgsw_tables.h"
 
 extern "C" const unsigned char rrtmgsw_blob_start[];
@@ -74,6 +75,155 @@ std::map<std::string, Rec> parse_blob() {
   return m;
 }
 
+// ---- the k-table file (rrtmg_sw.nc) ----
+// One nf90_get_var of src/rrtmg_sw_read_nc.f90: the file variable, the Fortran-order start
+// (1-based) and count of its dimensions before (band, g-point set), and the band-module array it
+// fills -- whole, or rows row0+1 .. row0+nrow of a (rows, 16) array where two variables share
+// forrefo.
+struct NcRead {
+  int band;
+  const char *rec, *var;
+  int n, start[4], count[4];
+  int row0 = 0, nrow = 0;
+};
+
+#define KEY_L "KeySpeciesAbsorptionCoefficientsLowerAtmos"
+#define KEY_U "KeySpeciesAbsorptionCoefficientsUpperAtmos"
+#define SELF "H2OSelfAbsorptionCoefficients"
+#define FOR_L "H2OForeignAbsorptionCoefficientsLowerAtmos"
+#define FOR_U "H2OForeignAbsorptionCoefficientsUpperAtmos"
+#define SSF_L "SolarSourceFunctionLowerAtmos"
+#define SSF_U "SolarSourceFunctionUpperAtmos"
+#define QS_L "NRLSSI2SSFQuietSunLowerAtmos"
+#define QS_U "NRLSSI2SSFQuietSunUpperAtmos"
+#define FAC_L "NRLSSI2SSFFacularLowerAtmos"
+#define FAC_U "NRLSSI2SSFFacularUpperAtmos"
+#define SPOT_L "NRLSSI2SSFSunspotLowerAtmos"
+#define SPOT_U "NRLSSI2SSFSunspotUpperAtmos"
+#define RAY_L "RayleighExtinctionCoefficientsLowerAtmos"
+#define RAY_U "RayleighExtinctionCoefficientsUpperAtmos"
+#define ABS_L "AbsorptionCoefficientsLowerAtmos"
+#define ABS_U "AbsorptionCoefficientsUpperAtmos"
+// absorber index of the minor-absorber variables (getAbsorberIndex over AbsorberNames,
+// modules/rrsw_ncpar.f90)
+enum { AB_H2O = 6, AB_CO2 = 7, AB_O3 = 8, AB_CH4 = 11 };
+
+// Solar source and its three NRLSSI2 components: (16 g, nkey) at key 1 of one atmosphere half.
+#define SOLAR(b, sflux_var, lo, nkey)                                                        \
+  {b, "sfluxrefo", sflux_var, 2, {1, 1}, {16, nkey}},                                      \
+  {b, "irradnceo", lo ? QS_L : QS_U, 2, {1, 1}, {16, nkey}},                               \
+  {b, "facbrghto", lo ? FAC_L : FAC_U, 2, {1, 1}, {16, nkey}},                             \
+  {b, "snsptdrko", lo ? SPOT_L : SPOT_U, 2, {1, 1}, {16, nkey}}
+#define RAYL(b) {b, "rayl", RAY_L, 2, {1, 1}, {1, 1}}
+#define RAYLO(b) {b, "raylo", RAY_L, 2, {1, 1}, {16, 1}}
+#define KAO(b, nkey) {b, "kao", KEY_L, 4, {1, 1, 1, 1}, {nkey, 5, 13, 16}}
+#define KBO(b, nkey) {b, "kbo", KEY_U, 4, {1, 1, 1, 1}, {nkey, 5, 47, 16}}
+#define SELFREF(b) {b, "selfrefo", SELF, 2, {1, 1}, {10, 16}}
+#define FORREF3(b) {b, "forrefo", FOR_L, 2, {1, 1}, {3, 16}}
+#define FORREF4(b)                                                                           \
+  {b, "forrefo", FOR_L, 2, {1, 1}, {3, 16}, 0, 3}, {b, "forrefo", FOR_U, 2, {2, 1}, {1, 16}, 3, 1}
+#define MINOR(b, rec, var, ab) {b, rec, var, 4, {1, 1, 1, ab}, {1, 1, 16, 1}}
+
+const NcRead NC_READS[] = {
+    // sw_kgb16, src/rrtmg_sw_read_nc.f90:46-113
+    SOLAR(16, SSF_L, 0, 1), RAYL(16), KAO(16, 9), KBO(16, 1), SELFREF(16), FORREF3(16),
+    // sw_kgb17 :117-189
+    SOLAR(17, SSF_U, 0, 5), RAYL(17), KAO(17, 9), KBO(17, 5), SELFREF(17), FORREF4(17),
+    // sw_kgb18 :193-261
+    SOLAR(18, SSF_L, 1, 9), RAYL(18), KAO(18, 9), KBO(18, 1), SELFREF(18), FORREF3(18),
+    // sw_kgb19 :265-333
+    SOLAR(19, SSF_L, 1, 9), RAYL(19), KAO(19, 9), KBO(19, 1), SELFREF(19), FORREF3(19),
+    // sw_kgb20 :337-417
+    SOLAR(20, SSF_L, 1, 1), RAYL(20), KAO(20, 1), KBO(20, 1), SELFREF(20), FORREF4(20),
+    MINOR(20, "absch4o", ABS_L, AB_CH4),
+    // sw_kgb21 :421-493
+    SOLAR(21, SSF_L, 1, 9), RAYL(21), KAO(21, 9), KBO(21, 5), SELFREF(21), FORREF4(21),
+    // sw_kgb22 :497-565
+    SOLAR(22, SSF_L, 1, 9), RAYL(22), KAO(22, 9), KBO(22, 1), SELFREF(22), FORREF3(22),
+    // sw_kgb23 :569-628
+    SOLAR(23, SSF_L, 1, 1), RAYLO(23), KAO(23, 1), SELFREF(23), FORREF3(23),
+    // sw_kgb24 :632-714
+    SOLAR(24, SSF_L, 1, 9), {24, "raylao", RAY_L, 2, {1, 1}, {16, 9}},
+    {24, "raylbo", RAY_U, 2, {1, 1}, {16, 1}}, KAO(24, 9), KBO(24, 1), SELFREF(24), FORREF3(24),
+    MINOR(24, "abso3ao", ABS_L, AB_O3), MINOR(24, "abso3bo", ABS_U, AB_O3),
+    // sw_kgb25 :718-781
+    SOLAR(25, SSF_L, 1, 1), RAYLO(25), KAO(25, 1), MINOR(25, "abso3ao", ABS_L, AB_O3),
+    MINOR(25, "abso3bo", ABS_U, AB_O3),
+    // sw_kgb26 :785-831
+    SOLAR(26, SSF_L, 1, 1), RAYLO(26),
+    // sw_kgb27 :835-891
+    SOLAR(27, SSF_L, 0, 1), RAYLO(27), KAO(27, 1), KBO(27, 1),
+    // sw_kgb28 :895-955
+    SOLAR(28, SSF_U, 0, 5), RAYL(28), KAO(28, 9), KBO(28, 5),
+    // sw_kgb29 :959-1047
+    SOLAR(29, SSF_L, 0, 1), RAYL(29), KAO(29, 1), KBO(29, 1), SELFREF(29), FORREF4(29),
+    MINOR(29, "absh2oo", ABS_L, AB_H2O), MINOR(29, "absco2o", ABS_L, AB_CO2),
+};
+static_assert(sizeof NC_READS / sizeof NC_READS[0] == 126, "read_nc makes 126 nf90_get_var calls");
+
+// Replaces the kgNN_* records of `recs` by the values read from the file; `store` owns them. Every
+// band-module array must be filled completely (the read list covers each kgNN_* record).
+bool read_ktable_file(const char *path, std::map<std::string, Rec> &recs,
+                      std::vector<std::vector<double>> &store, std::string &err) {
+  NcFile nc;
+  if (!nc.open(path, err)) return false;
+  std::map<std::string, std::vector<double>> out;
+  std::map<std::string, int> filled;
+  std::vector<double> tmp;
+  for (const NcRead &r : NC_READS) {
+    char name[64];
+    snprintf(name, sizeof name, "kg%d_%s", r.band, r.rec);
+    auto fail = [&](const std::string &why) {
+      err = "Error reading band " + std::to_string(r.band) + " variables from file " + path + ": " + why;
+      return false;
+    };
+    auto it = recs.find(name);
+    if (it == recs.end()) return fail(std::string("no module array ") + name);
+    const Rec &target = it->second;
+    const NcFile::Var *v = nc.var(r.var);
+    if (!v) return fail(std::string("no variable ") + r.var);
+    if ((int)v->shape.size() != r.n + 2)
+      return fail(std::string("variable ") + r.var + " has " + std::to_string(v->shape.size()) +
+                  " dimensions, not " + std::to_string(r.n + 2));
+    // netCDF order is the Fortran order reversed: (g-point set, band, ..., first)
+    uint64_t start[6], count[6], total = 1;
+    start[0] = 0, count[0] = 1;                        // gPointSetNumber = 1
+    start[1] = (uint64_t)(r.band - 16), count[1] = 1;  // bandNumber = band - 15 (1-based)
+    for (int d = 0; d < r.n; d++) {
+      start[2 + d] = (uint64_t)(r.start[r.n - 1 - d] - 1);
+      count[2 + d] = (uint64_t)r.count[r.n - 1 - d];
+      total *= count[2 + d];
+    }
+    tmp.assign(total, 0.0);
+    std::string e;
+    if (!nc.get(*v, start, count, tmp.data(), e)) return fail(e);
+    std::vector<double> &dst = out[name];
+    const int n = target.count();
+    dst.resize(n);
+    if (r.nrow == 0) {
+      if ((int)total != n)
+        return fail(std::string(name) + " holds " + std::to_string(n) + " values, the read gives " +
+                    std::to_string(total));
+      std::copy(tmp.begin(), tmp.end(), dst.begin());
+    } else {  // rows row0 .. row0 + nrow - 1 of a (rows, 16) array
+      const int rows = n / 16;
+      for (int g = 0; g < 16; g++)
+        for (int q = 0; q < r.nrow; q++) dst[r.row0 + q + rows * g] = tmp[q + r.nrow * g];
+    }
+    filled[name] += (int)total;
+  }
+  for (auto &kv : recs)
+    if (!kv.first.compare(0, 2, "kg") && filled[kv.first] != kv.second.count()) {
+      err = "k-table file reader: " + kv.first + " is not filled by the read list";
+      return false;
+    }
+  for (auto &kv : out) {
+    store.push_back(std::move(kv.second));
+    recs[kv.first].data = store.back().data();
+  }
+  return true;
+}
+
 }  // namespace
 
 // Weights of the original g-points inside their merged reduced g-point:
@@ -111,12 +261,14 @@ static std::vector<double> merge(const Rec &r, int b, bool gfirst, bool weighted
   return out;
 }
 
-bool build_tables(HostTables &H, std::string &err) {
+bool build_tables(HostTables &H, std::string &err, const char *ktable_file) {
   auto recs = parse_blob();
   if (recs.empty()) {
     err = "embedded table blob is missing or corrupt";
     return false;
   }
+  std::vector<std::vector<double>> file_data;  // owns the records read from the k-table file
+  if (ktable_file && !read_ktable_file(ktable_file, recs, file_data, err)) return false;
   auto get = [&](const std::string &n) -> const Rec * {
     auto it = recs.find(n);
     return it == recs.end() ? nullptr : &it->second;

@@ -17,7 +17,11 @@ struct HostTables {
   int32_t abari, bbari, cbari, dbari, ebari, fbari, rsrtaua, rsrpiza, rsrasya;
 };
 
-bool build_tables(HostTables &h, std::string &err);
+// Builds the tables from the data compiled into the library or, when `ktable_file` is given,
+// with the k-distribution and solar-source data read from that netCDF file as
+// src/rrtmg_sw_read_nc.f90 reads rrtmg_sw.nc (cloud and aerosol optics stay compiled in, as the
+// reference keeps them in module data under either build).
+bool build_tables(HostTables &h, std::string &err, const char *ktable_file = nullptr);
 // The pressures (hPa) at which setcoef_sw's jp, troposphere and lowest-layer decisions switch,
 // on the host libm's log (layout PTHR_*).
 bool pressure_thresholds(std::vector<double> &pthr, std::string &err);

@@ -2,9 +2,10 @@
 ! reference: rrtmg_sw_ini once, then mcica_subcol_sw and rrtmg_sw per chunk of columns
 ! (README.md of the reference, "GCM" usage). Inputs are read from a raw stream file and the six
 ! outputs written to another, so that tests can drive it:
-!   example_gcm <input.bin> <output.bin> [pad]
+!   example_gcm <input.bin> <output.bin> [pad [ktable.nc]]
 ! With pad > 0 every array is re-homed in one with pcols = ncol + pad columns (the extra columns
 ! hold junk), as a GCM's pcols-sized arrays are, and the outputs' first ncol columns are written.
+! With ktable.nc the k-tables come from that netCDF file (rrtmg_sw_ini's ktable_file).
 ! input: 11 int32 (ncol nlay icld iaer permuteseed irng dyofyr isolvar inflgsw iceflgsw liqflgsw),
 !        2 real64 (adjes scon), then play plev tlay tlev tsfc h2o o3 co2 ch4 n2o o2 asdir asdif
 !        aldir aldif coszen cldfrac ciwp clwp rei rel tauc ssac asmc fsfc alpha tauaer ssaaer
@@ -73,7 +74,12 @@ program example_gcm
   cldfmcl = 0; ciwpmcl = 0; clwpmcl = 0; taucmcl = 0; ssacmcl = 1; asmcmcl = 0; fsfcmcl = 0
   reicmcl = 0; relqmcl = 0
 
-  call rrtmg_sw_ini(1004.64_rb)
+  if (command_argument_count() >= 4) then   ! a k-table file, read as read_nc reads rrtmg_sw.nc
+    call get_command_argument(4, arg)
+    call rrtmg_sw_ini(1004.64_rb, ktable_file=trim(arg))
+  else
+    call rrtmg_sw_ini(1004.64_rb)
+  end if
   if (icld > 0) then
     call mcica_subcol_sw(1, ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, &
                          rei, rel, tauc, ssac, asmc, fsfc, alpha, cldfmcl, ciwpmcl, clwpmcl, &

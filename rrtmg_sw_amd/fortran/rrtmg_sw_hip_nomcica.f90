@@ -18,6 +18,11 @@ module rrtmgsw_capi_nomcica
       import :: c_int, c_double
       real(c_double), value :: cpdair
     end function
+    integer(c_int) function c_rrtmgsw_init_nc(cpdair, path) bind(C, name="rrtmgsw_init_nc")
+      import :: c_int, c_double, c_char
+      real(c_double), value :: cpdair
+      character(kind=c_char) :: path(*)
+    end function
     type(c_ptr) function c_rrtmgsw_last_error() bind(C, name="rrtmgsw_last_error")
       import :: c_ptr
     end function
@@ -62,9 +67,16 @@ module rrtmg_sw_init
   implicit none
   integer, parameter, private :: rb = selected_real_kind(12)
 contains
-  subroutine rrtmg_sw_ini(cpdair)
+  ! ktable_file (an addition, optional): read the k-distribution and solar-source data from that
+  ! netCDF file as the reference's src/rrtmg_sw_read_nc.f90 build reads 'rrtmg_sw.nc'.
+  subroutine rrtmg_sw_ini(cpdair, ktable_file)
     real(kind=rb), intent(in) :: cpdair
-    call rrtmgsw_check(c_rrtmgsw_init(real(cpdair, c_double)))
+    character(len=*), intent(in), optional :: ktable_file
+    if (present(ktable_file)) then
+      call rrtmgsw_check(c_rrtmgsw_init_nc(real(cpdair, c_double), trim(ktable_file) // c_null_char))
+    else
+      call rrtmgsw_check(c_rrtmgsw_init(real(cpdair, c_double)))
+    end if
   end subroutine rrtmg_sw_ini
 end module rrtmg_sw_init
 

@@ -6,6 +6,7 @@ one of pcols = ncol + pad columns whose extra columns hold junk (a GCM's pcols-s
 drop-in reads the leading ncol columns, as the reference does."""
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -120,3 +121,29 @@ def test_fortran_host_program(tmp_path, icld, iaer, pad):
     for k in names:
         d = np.abs(got[k] - ref[k]).max()
         assert d <= (0.01 if k.startswith("swhr") else 0.1), (k, d)
+
+
+def test_fortran_host_program_ktable_file(tmp_path):
+    """rrtmg_sw_ini(cpdair, ktable_file=...) of the Fortran module: a k-table file holding the
+    compiled-in data gives the same bits as the compiled-in tables."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_ktable_nc as mk
+    cols = make_columns(12, 40, icld=2, iaer=10, seed=77)
+    fin = tmp_path / "in.bin"
+    hdr = np.array([cols.ncol, cols.nlay, 2, 10, cols.permuteseed, cols.irng, cols.dyofyr,
+                    cols.isolvar, cols.inflgsw, cols.iceflgsw, cols.liqflgsw], dtype=np.int32)
+    with open(fin, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.array([cols.adjes, cols.scon], dtype=np.float64).tobytes())
+        for k in ORDER:
+            f.write(np.asarray(getattr(cols, k), dtype=np.float64).ravel(order="F").tobytes())
+    nc = tmp_path / "rrtmg_sw.nc"
+    mk.write(str(nc))
+    outs = []
+    for extra in ([], [str(nc)]):
+        fout = tmp_path / f"out{len(extra)}.bin"
+        r = subprocess.run([EXE, str(fin), str(fout), "0", *extra], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append(open(fout, "rb").read())
+    assert len(outs[0]) > 8 + 8 * 12 * 40 and outs[0] == outs[1]

@@ -1,0 +1,335 @@
+"""The k-table file reader (rrtmgsw_init_nc; src/rrtmg_sw_read_nc.f90), on the CPU.
+
+* The netCDF reader returns what scipy's independent reader returns, for every numeric type,
+  record and fixed variables, random hyperslabs, classic and 64-bit-offset files.
+* The list of reads the fixture writer follows is the one src/rrtmg_sw_read_nc.f90 makes (parsed
+  from that source where the reference is present).
+* Tables built from a file holding the compiled-in data, placed where read_nc reads it and junk
+  everywhere else, equal the compiled-in tables bit for bit; misplaced data does not.
+* Malformed or incomplete files fail with read_nc's message before any GPU call.
+"""
+import os
+import re
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "rrtmg_sw_amd", "librrtmgsw_hip.so")
+READ_NC = "/root/reference/src/rrtmg_sw_read_nc.f90"
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+import make_ktable_nc as mk  # noqa: E402
+
+needs_lib = pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+
+
+@pytest.fixture(scope="module")
+def compiled():
+    from rrtmg_sw_amd import api
+    return api.host_tables(None)
+
+
+@pytest.fixture(scope="module")
+def classic(tmp_path_factory):
+    p = str(tmp_path_factory.mktemp("kt") / "rrtmg_sw.nc")
+    mk.write(p, version=1)
+    return p
+
+
+def _same(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+# ---- the netCDF reader against scipy's ----
+
+def _random_file(path, version, rng):
+    from scipy.io import netcdf_file
+    f = netcdf_file(path, "w", version=version)
+    dims = {"a": 3, "b": 5, "c": 7, "d": 2}
+    f.createDimension("rec", None)
+    for d, n in dims.items():
+        f.createDimension(d, n)
+    kinds = {"b": np.int8, "h": np.int16, "i": np.int32, "f": np.float32, "d": np.float64}
+    names = []
+    for k, t in kinds.items():
+        for rank in (0, 1, 3):
+            shp = tuple(rng.choice(list(dims), rank, replace=False))
+            v = f.createVariable(f"fix_{k}{rank}", k, shp)
+            data = rng.standard_normal([dims[s] for s in shp]) * 100
+            v[...] = data.astype(t) if rank else np.asarray(data, dtype=t)
+            v.note = "x" * int(rng.integers(1, 9))     # attribute lengths that need padding
+            names.append(v)
+        v = f.createVariable(f"rec_{k}", k, ("rec", "c", "a"))
+        v[:4] = (rng.standard_normal((4, 7, 3)) * 100).astype(t)
+    f.close()
+
+
+@needs_lib
+@pytest.mark.parametrize("version", [1, 2])
+def test_reader_matches_scipy(tmp_path, version):
+    from scipy.io import netcdf_file
+    from rrtmg_sw_amd import api
+    rng = np.random.default_rng(version)
+    p = str(tmp_path / "r.nc")
+    _random_file(p, version, rng)
+    ref = netcdf_file(p, "r", mmap=False)
+    n = 0
+    for name, var in ref.variables.items():
+        full = np.asarray(var[...], dtype=np.float64) if var.shape else \
+            np.asarray(var.getValue(), dtype=np.float64).reshape(())
+        shape = full.shape
+        for _ in range(6):
+            start = [int(rng.integers(0, s)) if s else 0 for s in shape]
+            count = [int(rng.integers(0, s - a + 1)) if s else 0 for s, a in zip(shape, start)]
+            got = api.nc_get_vara(p, name, start, count)
+            want = full[tuple(slice(a, a + c) for a, c in zip(start, count))]
+            assert got.shape == want.shape and np.array_equal(got, want), (name, start, count)
+            n += 1
+        got = api.nc_get_vara(p, name, [0] * len(shape), list(shape))
+        assert np.array_equal(got, full), name
+    ref.close()
+    assert n > 60
+
+
+def _cdf5(path):
+    """A small 64-bit-data (CDF-5) file written from the published layout: 8-byte counts,
+    dimension ids, sizes and offsets, and the CDF-5 types int64 / uint16."""
+    def name(s):
+        b = s.encode()
+        return struct.pack(">q", len(b)) + b + b"\0" * (-len(b) % 4)
+    vals_i64 = (np.arange(6).reshape(2, 3) * -(2 ** 40)).astype(">i8")
+    vals_u16 = np.array([[1, 65535, 7], [9, 3, 40000]], dtype=">u2")
+    vals_d = np.array([1.5, -2.25, 1e300], dtype=">f8")
+    hdr = b"CDF\x05" + struct.pack(">q", 0)
+    hdr += struct.pack(">iq", 0x0A, 2) + name("y") + struct.pack(">q", 2) + name("x") + \
+        struct.pack(">q", 3)
+    hdr += struct.pack(">iq", 0, 0)   # no global attributes
+    body = [("big", 10, (0, 1), vals_i64), ("u", 8, (0, 1), vals_u16), ("d", 6, (1,), vals_d)]
+    var_hdr_len = 0
+    parts = []
+    for nm, t, dimids, v in body:
+        ent = name(nm) + struct.pack(">q", len(dimids)) + b"".join(struct.pack(">q", i) for i in dimids)
+        ent += struct.pack(">iq", 0, 0) + struct.pack(">i", t) + struct.pack(">q", v.nbytes)
+        parts.append(ent)
+        var_hdr_len += len(ent) + 8
+    base = len(hdr) + 12 + var_hdr_len
+    out, data, off = hdr + struct.pack(">iq", 0x0B, len(body)), b"", base
+    for (nm, t, dimids, v), ent in zip(body, parts):
+        out += ent + struct.pack(">q", off)
+        blob = v.tobytes() + b"\0" * (-v.nbytes % 4)
+        data += blob
+        off += len(blob)
+    assert len(out) == base
+    open(path, "wb").write(out + data)
+    return {"big": vals_i64.astype(np.float64), "u": vals_u16.astype(np.float64),
+            "d": vals_d.astype(np.float64)}
+
+
+@needs_lib
+def test_reader_cdf5(tmp_path):
+    """CDF-5 has no independent writer here: the file is built by the test from the format
+    description, which exercises the 8-byte header fields and the CDF-5 types."""
+    from rrtmg_sw_amd import api
+    p = str(tmp_path / "c5.nc")
+    want = _cdf5(p)
+    for nm, a in want.items():
+        assert np.array_equal(api.nc_get_vara(p, nm, [0] * a.ndim, list(a.shape)), a), nm
+    assert np.array_equal(api.nc_get_vara(p, "big", [1, 1], [1, 2]), want["big"][1:, 1:])
+
+
+# ---- the read list against src/rrtmg_sw_read_nc.f90 ----
+
+def _parse_read_nc(text):
+    """(band, array, variable, start, count, rows) of every nf90_get_var in the source."""
+    par = dict(mk.DIMS, Tdiff=5, keylower=9, keyupper=5, plower=13, pupper=47, Tself=10,
+               Tforeignlower=3, Tforeignupper=2, numGPoints=16, gPointSetNumber=1)
+    out = []
+    for m in re.finditer(r"subroutine sw_kgb(\d+)(.*?)end subroutine", text, re.S):
+        band, body = int(m.group(1)), m.group(2)
+        body = re.sub(r"&\s*\n\s*", " ", body)
+        bandnum = int(re.search(r"bandNumber\s*=\s*(\d+)", body).group(1))
+        assert bandnum == band - 15
+        ab = None
+        var = None
+        for line in body.splitlines():
+            g = re.search(r"getAbsorberIndex\('(\w+)'", line)
+            if g:
+                ab = mk.ABSORBERS.index(g.group(1)) + 1
+            g = re.search(r'nf90_inq_varid\(ncid,\s*"(\w+)"', line)
+            if g:
+                var = g.group(1)
+            g = re.search(r"nf90_get_var\(ncid,\s*varID,\s*([\w:(),]+?)\s*,\s*start\s*=\s*\(/(.*?)/\)"
+                          r"\s*,\s*count\s*=\s*\(/(.*?)/\)", line)
+            if not g:
+                continue
+
+            def ev(tok):
+                tok = tok.strip()
+                if tok.endswith("_im"):
+                    return int(tok[:-3])
+                if tok == "bandNumber":
+                    return bandnum
+                if tok == "ab":
+                    return ab
+                return par[tok]
+            start = [ev(t) for t in g.group(2).split(",")]
+            count = [ev(t) for t in g.group(3).split(",")]
+            assert start[-2:] == [band - 15, 1] and count[-2:] == [1, 1]
+            target = g.group(1)
+            rows = None
+            r = re.match(r"(\w+)\((\d+):(\d+),:\)", target)
+            if r:
+                target, rows = r.group(1), (int(r.group(2)) - 1, int(r.group(3)))
+            if target == "ncrayl":
+                target = "rayl"
+            out.append((band, target, var, tuple(start[:-2]), tuple(count[:-2]), rows))
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(READ_NC), reason="reference source not present")
+def test_read_list_is_read_nc():
+    parsed = _parse_read_nc(open(READ_NC).read())
+    assert len(parsed) == len(mk.READS) == 126
+    assert sorted(parsed, key=repr) == sorted(mk.READS, key=repr)
+
+
+# ---- tables from a file ----
+
+@needs_lib
+def test_tables_from_file_equal_compiled(classic, compiled):
+    from rrtmg_sw_amd import api
+    assert compiled.size > 90000
+    assert _same(api.host_tables(classic), compiled)
+
+
+@needs_lib
+def test_tables_from_64bit_offset_file_with_extras(tmp_path, compiled):
+    """Attributes, character, integer and float variables and record variables beside the
+    tables; 64-bit offsets."""
+    from rrtmg_sw_amd import api
+    p = str(tmp_path / "k2.nc")
+    mk.write(p, version=2, extras=True, seed=7)
+    assert _same(api.host_tables(p), compiled)
+
+
+def _write_arrays(path, arrays, drop=None, dims=None):
+    from scipy.io import netcdf_file
+    dims = dict(mk.DIMS, **(dims or {}))
+    f = netcdf_file(path, "w")
+    for d, n in dims.items():
+        f.createDimension(d, n)
+    for v, a in arrays.items():
+        if v == drop:
+            continue
+        vd = mk.VARS[v]
+        a = a[tuple(slice(0, dims[d]) for d in vd)]
+        f.createVariable(v, "d", tuple(reversed(vd)))[...] = a.T
+    f.close()
+
+
+@needs_lib
+@pytest.mark.parametrize("plant", ["key2", "gpointset2", "absorber", "band", "forrefo_row"])
+def test_misplaced_data_changes_tables(tmp_path, compiled, plant):
+    """Mistakes a reader could make -- another key, g-point set, absorber, band, or the other
+    temperature of the upper foreign table -- planted in the file: the tables differ."""
+    from rrtmg_sw_amd import api
+    arrays = mk.build_arrays()
+    if plant == "key2":          # band 24 raylbo moved to key 2 of the upper Rayleigh table
+        a = arrays["RayleighExtinctionCoefficientsUpperAtmos"]
+        a[:, 1, 8, 0], a[:, 0, 8, 0] = a[:, 0, 8, 0].copy(), a[:, 1, 8, 0].copy()
+    elif plant == "gpointset2":  # the self-continuum data swapped into set 2
+        a = arrays["H2OSelfAbsorptionCoefficients"]
+        a[..., 0], a[..., 1] = a[..., 1].copy(), a[..., 0].copy()
+    elif plant == "absorber":    # band 25 ozone at the O2 index
+        a = arrays["AbsorptionCoefficientsLowerAtmos"]
+        a[0, 0, :, 11, 9], a[0, 0, :, 7, 9] = a[0, 0, :, 7, 9].copy(), a[0, 0, :, 11, 9].copy()
+    elif plant == "band":        # bands 18 and 19 of the lower key-species table exchanged
+        a = arrays["KeySpeciesAbsorptionCoefficientsLowerAtmos"]
+        a[..., 2, 0], a[..., 3, 0] = a[..., 3, 0].copy(), a[..., 2, 0].copy()
+    else:                        # forrefo(4,:) from temperature 1 of the upper foreign table
+        a = arrays["H2OForeignAbsorptionCoefficientsUpperAtmos"]
+        a[0], a[1] = a[1].copy(), a[0].copy()
+    p = str(tmp_path / "bad.nc")
+    _write_arrays(p, arrays)
+    assert not _same(api.host_tables(p), compiled)
+
+
+# ---- failures ----
+
+def _err(fn, *a):
+    from rrtmg_sw_amd.api import RRTMGSWError
+    with pytest.raises(RRTMGSWError) as e:
+        fn(*a)
+    return e.value.message
+
+
+@needs_lib
+def test_missing_variable_stops_like_read_nc(tmp_path):
+    from rrtmg_sw_amd import api
+    p = str(tmp_path / "nosun.nc")
+    _write_arrays(p, mk.build_arrays(), drop="NRLSSI2SSFSunspotUpperAtmos")
+    msg = _err(api.host_tables, p)
+    assert msg.startswith("Error reading band 16 variables from file") and "NRLSSI2SSFSunspotUpperAtmos" in msg
+
+
+@needs_lib
+def test_short_dimension_stops(tmp_path):
+    from rrtmg_sw_amd import api
+    arrays = mk.build_arrays()
+    p = str(tmp_path / "short.nc")
+    _write_arrays(p, arrays, dims={"keyupper": 3})
+    msg = _err(api.host_tables, p)
+    assert msg.startswith("Error reading band 17 variables from file") and "out of range" in msg
+
+
+@needs_lib
+def test_bad_files(tmp_path, classic):
+    from rrtmg_sw_amd import api
+    assert "cannot open" in _err(api.host_tables, str(tmp_path / "absent.nc"))
+    h5 = tmp_path / "h5.nc"
+    h5.write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 64)
+    assert "netCDF-4" in _err(api.host_tables, str(h5)) and "nccopy" in _err(api.host_tables, str(h5))
+    txt = tmp_path / "t.nc"
+    txt.write_text("netcdf rrtmg_sw { }")
+    assert "not a netCDF file" in _err(api.host_tables, str(txt))
+    raw = open(classic, "rb").read()
+    for cut in (6, 100, 2000, len(raw) // 2, len(raw) - 8):
+        t = tmp_path / f"cut{cut}.nc"
+        t.write_bytes(raw[:cut])
+        msg = _err(api.host_tables, str(t))
+        assert "malformed" in msg or "past the end" in msg, (cut, msg)
+
+
+@needs_lib
+def test_init_nc_reports_file_errors_before_the_gpu(tmp_path):
+    """rrtmgsw_init_nc reads the file before any device call, so a bad file gives read_nc's
+    message on a machine without a GPU as well."""
+    import ctypes as C
+    from rrtmg_sw_amd import _lib
+    L = _lib.lib()
+    assert L.rrtmgsw_init_nc(C.c_double(1004.64), str(tmp_path / "none.nc").encode()) != 0
+    assert b"cannot open" in L.rrtmgsw_last_error()
+    assert L.rrtmgsw_init_nc(C.c_double(-1.0), None) != 0
+    assert b"cpdair" in L.rrtmgsw_last_error()
+
+
+@needs_lib
+def test_reader_single_record_variable(tmp_path):
+    """A file whose only record variable is not a multiple of 4 bytes per record stores its
+    records unpadded."""
+    from scipy.io import netcdf_file
+    from rrtmg_sw_amd import api
+    p = str(tmp_path / "one.nc")
+    f = netcdf_file(p, "w")
+    f.createDimension("rec", None)
+    f.createDimension("x", 5)
+    v = f.createVariable("r", "b", ("rec", "x"))
+    data = (np.arange(35).reshape(7, 5) - 17).astype(np.int8)
+    v[:7] = data
+    f.createVariable("s", "d", ("x",))[...] = np.arange(5.0)
+    f.close()
+    assert np.array_equal(api.nc_get_vara(p, "r", [0, 0], [7, 5]), data)
+    assert np.array_equal(api.nc_get_vara(p, "r", [3, 1], [4, 3]), data[3:, 1:4])
+    assert np.array_equal(api.nc_get_vara(p, "s", [0], [5]), np.arange(5.0))
