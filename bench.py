@@ -240,6 +240,10 @@ def adjoint_ratio(args, name="expran4096x60"):
             args.warmup)
     res["adjoint_over_forward_fp64"] = res["adjoint_ckpt0_ms"] / res["forward_fp64_ms"]
     res["adjoint_over_forward_exact"] = res["adjoint_ckpt0_ms"] / res["forward_exact_ms"]
+    res["adjoint_ckpt16_over_forward_exact"] = res["adjoint_ckpt16_ms"] / res["forward_exact_ms"]
+    from rrtmg_sw_amd import _lib
+    for ck in (0, 16):
+        res[f"work_bytes_per_column_ckpt{ck}"] = int(_lib.lib().rrtmgsw_adjoint_work_bytes(cols.nlay, ck))
     return res
 
 

@@ -311,10 +311,10 @@ int rrtmgsw_mcica_sw_dev_noalloc(int ncol, int nlay, int icld, int iaer, int per
  * o3vmr, clwp and ciwp (the in-cloud water paths of mcica_subcol_sw), each (ncol, nlay), and
  * dJ/d (asdir, asdif, aldir, aldif) as d_alb (ncol, 4). McICA masks are held fixed. ckpt = 0
  * stores every layer's gas and Rayleigh optical depths, adding-method state and state adjoint
- * (6 doubles per layer and g-point); ckpt = K > 0 keeps one state per K layers and re-sweeps
- * each K-layer segment in the reverse sweep, keeping that segment's gas optics and two-stream
- * solutions (1 + 3/K doubles per layer and g-point plus 10 K per g-point, one more layer
- * evaluation per layer). Both add 9 doubles per layer and band. No counterpart
+ * (6 doubles per layer and g-point); ckpt = K > 0 stores the optical depths and the state
+ * adjoint, keeps the adding-method state of one layer in K and re-sweeps each K-layer segment
+ * from it in the reverse sweep (3 + 3/K doubles per layer and g-point plus 3 K per g-point, one
+ * more two-stream evaluation per layer). Both add 9 doubles per layer and band. No counterpart
  * in the reference; inputs as rrtmgsw_mcica_sw_dev (same argument order). */
 int rrtmgsw_mcica_sw_adjoint_dev(int ncol, int nlay, int icld, int iaer, int permuteseed, int *irng,
                                  const double *play, const double *plev, const double *tlay,

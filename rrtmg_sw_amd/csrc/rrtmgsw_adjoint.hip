@@ -373,15 +373,16 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
   q_b -= coldry_b * coldry * ((amw - amd) / amm + 1. / (1. + q));
 }
 
-// Doubles of work area per column: (UB (L) + per-layer records (5 L, store mode) or checkpoints
-// (3 ceil(L/K)) and the records of one segment (5 K)) times NGPT, then the cloud table CJ
-// (L, NBND, CJ_N).
 // A layer's record for the reverse sweep, per g-point: the sweep state at its top (tdbt, ztdn,
 // prdnd) and its gas optics (taug, taur). The reverse sweep forms the layer's two-stream
 // solution again from them, keeping the intermediates its own reverse needs.
+// Doubles of work area per column, times NGPT: UB (L), then in store mode every layer's record
+// (5 L); in checkpoint mode every layer's gas optics (2 L), the sweep state at the top of every
+// K-th layer (3 ceil(L/K)) and the sweep states of one K-layer segment (3 K). Then the cloud
+// table CJ (L, NBND, CJ_N).
 constexpr int REC_S = 5;
 __host__ __device__ inline size_t adj_work_doubles(int L, int K) {
-  size_t states = K == 0 ? (size_t)L * REC_S : (size_t)((L + K - 1) / K) * 3 + (size_t)K * REC_S;
+  size_t states = K == 0 ? (size_t)L * REC_S : (size_t)L * 2 + (size_t)((L + K - 1) / K) * 3 + (size_t)K * 3;
   return ((size_t)L + states) * NGPT + (size_t)L * NBND * CJ_N;
 }
 
@@ -484,17 +485,22 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   const double zinc = active ? incident_flux(gp, T.tab, v, a.sc, mu0) : 0.0;
 
   // Per-column work area: UB (L, NGPT) = dJ/dU_j, then either every layer's record
-  // S (L, REC_S, NGPT) (store mode) or one sweep state per K layers CK (ceil(L/K), 3, NGPT)
-  // plus the records of one K-layer segment SG (K, REC_S, NGPT) (checkpoint mode, K = A.ckpt).
+  // S (L, REC_S, NGPT) (store mode) or every layer's gas optics GT (L, 2, NGPT), one sweep state
+  // per K layers CK (ceil(L/K), 3, NGPT) and the sweep states of one K-layer segment
+  // SG (K, 3, NGPT) (checkpoint mode, K = A.ckpt).
   double *S = UB + (size_t)L * NGPT;
-  double *SG = S + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
-  auto ck = [&](int i, int k) -> double & { return S[((size_t)i * 3 + k) * NGPT + g]; };
-  // the lane's record in slot i (fields k * NGPT apart): layer i of S in store mode, layer
-  // jseg0 + i of the current segment in SG in checkpoint mode (a uniform base plus the lane)
-  auto recp = [&](int i) -> double * { return (CKPT ? SG : S) + (size_t)i * REC_S * NGPT + g; };
-  auto put_rec = [&](int i, const Sweep &s, double taug, double taur) {
-    double *r = recp(i);
-    r[0] = s.tdbt, r[NGPT] = s.ztdn, r[2 * NGPT] = s.prdnd, r[3 * NGPT] = taug, r[4 * NGPT] = taur;
+  double *GT = S;
+  double *CK = GT + (size_t)L * 2 * NGPT;
+  double *SG = CK + (size_t)((L + K - 1) / max(K, 1)) * 3 * NGPT;
+  auto ck = [&](int i, int k) -> double & { return CK[((size_t)i * 3 + k) * NGPT + g]; };
+  // the lane's sweep state for layer j (fields NGPT apart): layer j of S in store mode, slot
+  // j - jseg0 of the current segment in SG in checkpoint mode (a uniform base plus the lane)
+  auto statep = [&](int i) -> double * { return (CKPT ? SG + (size_t)i * 3 * NGPT : S + (size_t)i * REC_S * NGPT) + g; };
+  // the lane's gas optics of layer j: in its record (store mode) or in GT (checkpoint mode)
+  auto gasp = [&](int j) -> double * { return (CKPT ? GT + (size_t)j * 2 * NGPT : S + ((size_t)j * REC_S + 3) * NGPT) + g; };
+  auto put_state = [&](int i, const Sweep &s) {
+    double *r = statep(i);
+    r[0] = s.tdbt, r[NGPT] = s.ztdn, r[2 * NGPT] = s.prdnd;
   };
 
   const int jsplit = L - laytrop;   // layers j >= jsplit lie below laytrop
@@ -511,9 +517,11 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   auto flush_rec = [&]() {
     if (active && pj >= 0) {
       if (!CKPT) {
-        double *r = recp(pj);
-        r[0] = pr[0], r[NGPT] = pr[1], r[2 * NGPT] = pr[2], r[3 * NGPT] = pr[3], r[4 * NGPT] = pr[4];
+        double *r = statep(pj);
+        r[0] = pr[0], r[NGPT] = pr[1], r[2 * NGPT] = pr[2];
       }
+      double *t = gasp(pj);
+      t[0] = pr[3], t[NGPT] = pr[4];
       UB[(size_t)pj * NGPT + g] = pr[5];
     }
   };
@@ -576,42 +584,25 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
 
   // ---- R1: bottom -> top. The lane's gas-optics constants are read from its g-point record
   // where used: held in registers across this sweep they would not fit beside its state. ----
-  // Checkpoint mode: the records of the segment jseg0..j, re-swept from its checkpoint with the
-  // forward sweep's own functions (the same values as F1's, so the same as store mode's), one
-  // loop per regime with the lane's constants resolved once; a record is stored once the next
-  // layer's table reads have been issued, as in F1.
-  // checkpoint mode: the segment being reversed starts at layer jseg0 = ci K
+  // Checkpoint mode: the sweep states of the segment jseg0..j, re-swept from its checkpoint with
+  // the forward sweep's own functions on the gas optics F1 kept (the same values as F1's, so the
+  // same as store mode's); a state is stored once the next layer's gas optics have been read.
   ci = CKPT ? (L - 1) / max(K, 1) : 0;
   int jseg0 = ci * K;
   auto resweep = [&](int j) {
     Sweep ss = {active ? ck(ci, 0) : 1.0, active ? ck(ci, 1) : 1.0, active ? ck(ci, 2) : 0.0};
-    int pend = -1;
-    Sweep ps = ss;
-    double ptg = 0.0, ptr = 0.0;
-    auto part = [&](auto lower_tag, int ja, int jb) {
-      constexpr bool LOWER = decltype(lower_tag)::value;
-      if (ja >= jb) return;
-      const GasLane q = gas_lane<LOWER>(gp, L);
-      for (int jj = ja; jj < jb; jj++) {
-        const int lay = L - 1 - jj;
-        const GasFetch<double> gf = gas_fetch<LOWER, double>(q, T.tab, v, lay);
-        if (pend >= 0 && active) put_rec(pend - jseg0, ps, ptg, ptr);
-        double taug, taur;
-        gas_finish<LOWER, double>(q, gf, v, lay, taug, taur);
-        pend = jj, ps = ss, ptg = taug, ptr = taur;
-        if (jj < j) {
-          const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
-          CellTape tp;
-          const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
-          RT4 l = {ly.r, ly.rd, ly.t, ly.td};
-          double Aa, Bb, Cc, Ee;
-          sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
-        }
-      }
-    };
-    part(std::integral_constant<bool, false>(), jseg0, min(j + 1, jsplit));
-    part(std::integral_constant<bool, true>(), max(jseg0, jsplit), j + 1);
-    if (pend >= 0 && active) put_rec(pend - jseg0, ps, ptg, ptr);
+    for (int jj = jseg0; jj < j; jj++) {
+      const double *t = gasp(jj);
+      const double taug = active ? t[0] : 1., taur = active ? t[NGPT] : 0.1;
+      if (active) put_state(jj - jseg0, ss);
+      const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, L - 1 - jj, active);
+      CellTape tp;
+      const LayerV ly = cell_layer(taug, taur, in, mu0, rmu, tp);
+      RT4 l = {ly.r, ly.rd, ly.t, ly.td};
+      double Aa, Bb, Cc, Ee;
+      sweep_layer(ss, l, ly.dbt, Aa, Bb, Cc, Ee);
+    }
+    if (active) put_state(j - jseg0, ss);
   };
   for (int j = L - 1; j >= 0; j--) {
     if (CKPT && (j == L - 1 || j < jseg0)) {
@@ -620,9 +611,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     }
     // the layer's record (sweep state at its top, gas optics); its two-stream solution formed
     // again with the intermediates the reverse needs (taped)
-    const double *r = recp(CKPT ? j - jseg0 : j);
+    const double *r = statep(CKPT ? j - jseg0 : j), *t = gasp(j);
     const Sweep sj = {active ? r[0] : 1.0, active ? r[NGPT] : 1.0, active ? r[2 * NGPT] : 0.0};
-    const double taug = active ? r[3 * NGPT] : 1., taur = active ? r[4 * NGPT] : 0.1;
+    const double taug = active ? t[0] : 1., taur = active ? t[NGPT] : 0.1;
     const double ub = active ? UB[(size_t)j * NGPT + g] : 0.0;
     const int lay = L - 1 - j;
     const LayerIn in = layer_inputs(a, T, gp.ibnd, v, CJ, c, g, lay, active);
