@@ -375,7 +375,8 @@ __device__ static void setcoef_adj(const RtArgs &a, int c, int lay, const double
 
 // A layer's record for the reverse sweep, per g-point: the sweep state at its top (tdbt, ztdn,
 // prdnd) and its gas optics (taug, taur). The reverse sweep forms the layer's two-stream
-// solution again from them, keeping the intermediates its own reverse needs.
+// solution again from them, keeping the intermediates its own reverse needs, and leaves the
+// adjoints of taug and taur in place of the gas optics for the gas-optics pass that follows.
 // Doubles of work area per column, times NGPT: UB (L), then in store mode every layer's record
 // (5 L); in checkpoint mode every layer's gas optics (2 L), the sweep state at the top of every
 // K-th layer (3 ceil(L/K)) and the sweep states of one K-layer segment (3 K). Then the cloud
@@ -626,25 +627,46 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     sweep_step_adj(sj, l, ly.dbt, ub, zwD, u, sb, lb);
     phi += wU[j] * u + wD[j] * (sj.ztdn + sj.prdnd * u);
     cell_layer_adj(tp, taug, taur, in, mu0, rmu, lb, xb);
-    double la[AC_N];
-#pragma unroll
-    for (int k = 0; k < AC_N; k++) la[k] = 0.0;
+    // the adjoints of taug and taur replace the layer's gas optics in its slot, for R2
     if (active) {
-      if (j >= jsplit)
-        gas_adj_r<true>(gas_lane<true>(gp, L), T.tab, v, lay, xb[0], xb[1], la);
-      else
-        gas_adj_r<false>(gas_lane<false>(gp, L), T.tab, v, lay, xb[0], xb[1], la);
-      if (in.cf > 0.0) {
-        const double *o = CJ + ((size_t)lay * NBND + gp.ibnd) * CJ_N;
-        la[AC_CLWP] = xb[2] * o[3] + xb[3] * o[4] + xb[4] * o[5];
-        la[AC_CIWP] = xb[2] * o[6] + xb[3] * o[7] + xb[4] * o[8];
-      }
+      double *t = gasp(j);
+      t[0] = xb[0], t[NGPT] = xb[1];
     }
-    static_assert(AC_N == 16, "one wave_sum16d per layer");
-    const double s = wave_sum16d(la, lane);
+    double lw = 0.0, li = 0.0;
+    if (active && in.cf > 0.0) {
+      const double *o = CJ + ((size_t)lay * NBND + gp.ibnd) * CJ_N;
+      lw = xb[2] * o[3] + xb[3] * o[4] + xb[4] * o[5];
+      li = xb[2] * o[6] + xb[3] * o[7] + xb[4] * o[8];
+    }
+    static_assert(AC_CIWP == AC_CLWP + 1, "water paths adjacent");
+    const double s = wave_sum2d(lw, li);
     // one term per wave and entry, onto zero: a + b rounds as b + a, whichever wave is first
-    if ((lane & 3) == 0) atomicAdd(&acc[(size_t)lay * AC_N + (lane >> 2)], s);
+    if ((lane & 31) == 0) atomicAdd(&acc[(size_t)lay * AC_N + AC_CLWP + (lane >> 5)], s);
   }
+
+  // ---- R2: the gas-optics adjoint of every layer (onto the setcoef outputs), from the adjoints
+  // of taug and taur R1 left in the layers' slots. Its layers are independent of one another, so
+  // their table reads overlap; one regime at a time with the lane's constants resolved once. ----
+  auto r2 = [&](auto lower_tag, int j0, int j1) {
+    constexpr bool LOWER = decltype(lower_tag)::value;
+    if (j0 >= j1) return;
+    const GasLane q = gas_lane<LOWER>(gp, L);
+    for (int j = j0; j < j1; j++) {
+      const int lay = L - 1 - j;
+      double la[AC_N];
+#pragma unroll
+      for (int k = 0; k < AC_N; k++) la[k] = 0.0;
+      if (active) {
+        const double *t = gasp(j);
+        gas_adj_r<LOWER>(q, T.tab, v, lay, t[0], t[NGPT], la);
+      }
+      static_assert(AC_N == 16 && AC_CLWP == 14, "one wave_sum16d per layer, gas terms in 0..13");
+      const double s = wave_sum16d(la, lane);
+      if ((lane & 3) == 0 && (lane >> 2) < AC_CLWP) atomicAdd(&acc[(size_t)lay * AC_N + (lane >> 2)], s);
+    }
+  };
+  r2(std::integral_constant<bool, false>(), 0, jsplit);
+  r2(std::integral_constant<bool, true>(), jsplit, L);
 
   // ---- incident flux: zinc = adjflux src mu0, src interpolated in the key-species parameter
   // at laysolfr for some bands (src/rrtmg_sw_taumol.f90, e.g. :495-520). The g-points of a band
