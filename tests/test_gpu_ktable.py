@@ -57,6 +57,15 @@ def test_ktable_file_drives_the_gpu_path(gpu_lib, tmp_path, monkeypatch):
         d = np.abs(np.asarray(alt["swdflx"]) - np.asarray(ref["swdflx"]))
         print("doubled band-18/19 lower-atmosphere absorption: max swdflx change %.3f W/m2" % d.max())
         assert d.max() > 0.5
+        # a file that fails to read leaves the library uninitialised, free to start again
+        api.rrtmg_sw_finalize()
+        with pytest.raises(api.RRTMGSWError) as e:
+            api.rrtmg_sw_ini(1004.64, ktable_file=str(tmp_path / "missing.nc"))
+        assert "cannot open" in e.value.message
+        with pytest.raises(api.RRTMGSWError) as e:
+            _run(api, cols)
+        assert e.value.code == 1        # RRTMGSW_ERR_NOT_INITIALIZED
     finally:
         api.rrtmg_sw_finalize()
         api.rrtmg_sw_ini(1004.64)
+    assert all(np.array_equal(np.asarray(_run(api, cols)[k]), np.asarray(ref[k])) for k in OUT)
