@@ -222,12 +222,15 @@ bool NcFile::get(const Var &v, const uint64_t *start, const uint64_t *count, dou
   for (int d = R - 2; d >= 0; d--) stride[d] = stride[d + 1] * v.shape[d + 1];
   if (v.record) stride[0] = recsize_;
   std::vector<uint64_t> idx(R ? R : 1, 0);
-  const uint64_t run = R ? count[R - 1] : 1;
+  // the innermost dimension is read as one contiguous run, unless it is the record dimension
+  const bool contig = R > 0 && !(v.record && R == 1);
+  const uint64_t run = contig ? count[R - 1] : 1;
+  const int outer = contig ? R - 1 : R;   // dimensions the odometer steps
   for (uint64_t done = 0; done < total; done += run) {
     uint64_t o = v.begin;
     for (int d = 0; d < R; d++) o += (start[d] + idx[d]) * stride[d];
     for (uint64_t k = 0; k < run; k++) out[done + k] = decode(buf_.data() + o + k * es, v.type);
-    for (int d = R - 2; d >= 0; d--) {  // odometer over all but the innermost dimension
+    for (int d = outer - 1; d >= 0; d--) {
       if (++idx[d] < count[d]) break;
       idx[d] = 0;
     }

@@ -63,6 +63,10 @@ def _random_file(path, version, rng):
             names.append(v)
         v = f.createVariable(f"rec_{k}", k, ("rec", "c", "a"))
         v[:4] = (rng.standard_normal((4, 7, 3)) * 100).astype(t)
+        v = f.createVariable(f"rec1_{k}", k, ("rec",))           # the record dimension innermost
+        v[:4] = (rng.standard_normal(4) * 100).astype(t)
+        v = f.createVariable(f"rec2_{k}", k, ("rec", "b"))
+        v[:4] = (rng.standard_normal((4, 5)) * 100).astype(t)
     f.close()
 
 
@@ -333,3 +337,25 @@ def test_reader_single_record_variable(tmp_path):
     assert np.array_equal(api.nc_get_vara(p, "r", [0, 0], [7, 5]), data)
     assert np.array_equal(api.nc_get_vara(p, "r", [3, 1], [4, 3]), data[3:, 1:4])
     assert np.array_equal(api.nc_get_vara(p, "s", [0], [5]), np.arange(5.0))
+
+
+@needs_lib
+def test_reader_streaming_record_count(tmp_path):
+    """A file written while streaming carries numrecs = 0xFFFFFFFF; the record count then follows
+    from the file size."""
+    from scipy.io import netcdf_file
+    from rrtmg_sw_amd import api
+    p = tmp_path / "s.nc"
+    f = netcdf_file(str(p), "w")
+    f.createDimension("rec", None)
+    f.createDimension("x", 3)
+    f.createVariable("a", "d", ("rec", "x"))[:5] = np.arange(15.0).reshape(5, 3)
+    f.createVariable("b", "i", ("rec",))[:5] = np.arange(5) * 7
+    f.close()
+    raw = bytearray(p.read_bytes())
+    assert raw[4:8] == b"\0\0\0\x05"
+    raw[4:8] = b"\xff\xff\xff\xff"
+    q = tmp_path / "stream.nc"
+    q.write_bytes(bytes(raw))
+    assert np.array_equal(api.nc_get_vara(str(q), "a", [0, 0], [5, 3]), np.arange(15.0).reshape(5, 3))
+    assert np.array_equal(api.nc_get_vara(str(q), "b", [2], [3]), np.array([14.0, 21.0, 28.0]))
