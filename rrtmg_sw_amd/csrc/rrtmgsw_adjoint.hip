@@ -14,22 +14,24 @@
 //       :144-159 as U_j = C_j + E_j U_(j+1), D_j = ztdn_j + prdnd_j U_j); since J reaches U_j
 //       through U_j itself, D_j and U_(j-1), ubar_j = dJ/dU_j = ubar_(j-1) E_(j-1) +
 //       zinc (wU_j + wD_j prdnd_j) is known as soon as layer j is: stored per layer, with the
-//       sweep state s_j = (tdbt, ztdn, prdnd) and gas optics (taug, taur) of every layer
-//       (store mode) or the state of every K-th layer (checkpoint mode); then the surface
+//       gas optics (taug, taur) of every layer, with the sweep state s_j = (tdbt, ztdn, prdnd)
+//       of every layer (store mode) or of every K-th layer (checkpoint mode); then the surface
 //       adjoint (albedos, s_L);
-//   R1  bottom -> top: s_j (loaded, or recomputed per K-layer segment from its checkpoint), the
-//       layer optics and reftra recomputed (cheaper than storing and reloading them), U_j by
-//       the back substitution and the objective per unit incident flux Phi on the way up, the
-//       adjoint of the elimination step, reftra_sw's reverse-mode derivative, the reverse of
-//       the layer-optics combination, then the hand-written adjoints of the gas optics
-//       (taumol template) and of cldprmc_sw (per-(layer, band) water-path Jacobians from the
-//       prologue); per-layer sums over g go to LDS accumulators;
+//   R1  bottom -> top: s_j (loaded, or recomputed per K-layer segment from its checkpoint and
+//       the kept gas optics), the layer optics and reftra recomputed (cheaper than storing and
+//       reloading them), U_j by the back substitution and the objective per unit incident flux
+//       Phi on the way up, the adjoint of the elimination step, reftra_sw's reverse-mode
+//       derivative, the reverse of the layer-optics combination, and the adjoint of cldprmc_sw
+//       (per-(layer, band) water-path Jacobians from the prologue); the adjoints of taug and
+//       taur are left in the layer's slot;
+//   R2  every layer: the hand-written adjoint of the gas optics (taumol template) from them;
+//       per-layer sums over g go to LDS accumulators;
 //   end the incident-flux adjoint (solar source interpolated at laysolfr), and per layer the
 //       adjoint of inatm_sw + setcoef_sw onto T, H2O, O3.
 // Memory per (column, layer, g-point): ubar_j, s_j and the gas and Rayleigh optical depths, 6
-// doubles (store mode), or ubar_j and 3/K for the checkpoints plus a 3K segment buffer per
-// column (checkpoint mode, every layer's gas optics and sweep recomputed); plus 9 doubles per
-// (layer, band) for the cloud table.
+// doubles (store mode), or ubar_j, the optical depths and 3/K for the checkpoints plus a 3K
+// segment buffer per column (checkpoint mode, every layer's sweep recomputed); plus 9 doubles
+// per (layer, band) for the cloud table.
 #include <type_traits>
 
 #include "rrtmgsw_device.h"
