@@ -68,3 +68,19 @@ def test_random_case(gpu_lib, seed):
                 dmax = float(np.abs(fused[k] - ref[k]).max())
                 assert dmax <= (0.01 if k in HR else 0.1), (label, precision, k, dmax)
     print("  max |d|", {k: float(np.abs(np.asarray(got[k]) - ref[k]).max()) for k in FLUX + HR})
+
+
+@pytest.mark.parametrize("seed", range(32))
+def test_random_nomcica_case(gpu_lib, seed):
+    """The same draws through the non-McICA driver, every cloudy layer made overcast (the driver
+    stops on partial clouds)."""
+    from rrtmg_sw_amd.workloads import overcast
+    from test_gpu_nomcica import nm_args
+    cols = overcast(_case(seed))
+    label = (f"seed={seed} ncol={cols.ncol} nlay={cols.nlay} icld={cols.icld} iaer={cols.iaer} "
+             f"iceflag={cols.iceflgsw} isolvar={cols.isolvar}")
+    print(label, flush=True)
+    got = gpu_lib.rrtmg_sw_nomcica(**nm_args(cols))
+    ref, ic, ia = ob.run_nomcica(cols)
+    assert (got["icld"], got["iaer"]) == (ic, ia), label
+    assert_close(got, ref, label + " nomcica")
