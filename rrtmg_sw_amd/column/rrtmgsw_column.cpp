@@ -2,7 +2,10 @@
 // library: reads INPUT_RRTM, IN_CLD_RRTM (icld >= 1) and IN_AER_RRTM (iaer = 10) from a
 // directory, runs rrtmgsw_column_sw, and writes OUTPUT_RRTM there in the reference's layout.
 //
-//   rrtmgsw_column [directory]          (default: the current directory)
+//   rrtmgsw_column [--ktable rrtmg_sw.nc] [directory]   (default: the current directory)
+//
+// --ktable takes the k-distribution and solar-source data from that netCDF file, as the
+// reference's build with src/rrtmg_sw_read_nc.f90 does.
 //
 // Covered: layer input given directly (IATM = 0; src/rrtmg_sw.1col.f90:1693-1766), clear or
 // overcast clouds without McICA (imca = 0; readcld :1884-1958), McICA sampling (imca = 1: 200
@@ -391,9 +394,17 @@ void dump(const char *name, const double *v, size_t n) {
 }
 
 int main(int argc, char **argv) {
-  // --dump: print what was read (no GPU work), for checking the readers
-  const bool dump_only = argc > 1 && std::strcmp(argv[1], "--dump") == 0;
-  const int a0 = dump_only ? 2 : 1;
+  // --dump: print what was read (no GPU work), for checking the readers;
+  // --ktable FILE: the k-tables from a netCDF file, as the reference's rrtmg_sw_read_nc.f90
+  // build reads rrtmg_sw.nc (else the compiled-in data); then the input directory, if any
+  bool dump_only = false;
+  const char *ktable = nullptr;
+  int a0 = 1;
+  for (; a0 < argc; a0++) {
+    if (std::strcmp(argv[a0], "--dump") == 0) dump_only = true;
+    else if (std::strcmp(argv[a0], "--ktable") == 0 && a0 + 1 < argc) ktable = argv[++a0];
+    else break;
+  }
   const std::string dir = argc > a0 ? std::string(argv[a0]) + "/" : std::string();
   Profile p = read_input(dir + "INPUT_RRTM");
   const int L = p.nlay;
@@ -426,7 +437,7 @@ int main(int argc, char **argv) {
   std::vector<double> alb(NB);
   for (int b = 0; b < NB; b++) alb[b] = 1.0 - p.semiss[b];
   const double cosz = p.zenith;
-  if (rrtmgsw_init(CPDAIR)) die(rrtmgsw_last_error());
+  if (ktable ? rrtmgsw_init_nc(CPDAIR, ktable) : rrtmgsw_init(CPDAIR)) die(rrtmgsw_last_error());
   const size_t nlev = (size_t)L + 1;
   std::vector<double> fup(nlev * 15), fdn(nlev * 15), fdir(nlev * 15);
   const bool c = p.icld >= 1;

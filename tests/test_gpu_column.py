@@ -50,3 +50,23 @@ def test_column_program_example_suite(tmp_path, name):
         assert hr <= 0.5e-5 + 2e-7, (name, b, hr)
     print(f"{name}: {len(got)} blocks, max flux diff {worst[0]:.2e} W/m2, max HR diff {worst[1]:.2e} K/day",
           flush=True)
+
+
+def test_column_program_ktable_file(tmp_path):
+    """--ktable: a k-table file holding the compiled-in data gives the same OUTPUT_RRTM, byte for
+    byte, as the compiled-in tables."""
+    import make_ktable_nc as mk
+    name = str(KATS["cases"][0])
+    outs = []
+    nc = tmp_path / "rrtmg_sw.nc"
+    mk.write(str(nc))
+    for extra in ([], ["--ktable", str(nc)]):
+        d = tmp_path / f"run{len(extra)}"
+        d.mkdir()
+        src = os.path.join(INPUTS, name)
+        for f in os.listdir(src):
+            shutil.copyfile(os.path.join(src, f), d / f)
+        r = subprocess.run([EXE, *extra, str(d)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append((d / "OUTPUT_RRTM").read_bytes())
+    assert len(outs[0]) > 1000 and outs[0] == outs[1]
