@@ -585,8 +585,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
     if ((lane & 15) == 0) atomicAdd(&albacc[lane >> 4], s);
   }
 
-  // ---- R1: bottom -> top. The lane's gas-optics constants are read from its g-point record
-  // where used: held in registers across this sweep they would not fit beside its state. ----
+  // ---- R1: bottom -> top: the reverse of the two-stream layers and of the elimination; the
+  // gas optics' adjoints wait in the layers' slots for R2. ----
   // Checkpoint mode: the sweep states of the segment jseg0..j, re-swept from its checkpoint with
   // the forward sweep's own functions on the gas optics F1 kept (the same values as F1's, so the
   // same as store mode's); a state is stored once the next layer's gas optics have been read.
