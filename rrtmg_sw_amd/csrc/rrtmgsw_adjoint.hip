@@ -594,6 +594,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) a
   int jseg0 = ci * K;
   auto resweep = [&](int j) {
     Sweep ss = {active ? ck(ci, 0) : 1.0, active ? ck(ci, 1) : 1.0, active ? ck(ci, 2) : 0.0};
+    // two layers a step: their two-stream solutions are independent, only the sweep is serial
+    // (measured 1.5 % faster at 137 layers than one a step; three or four are no faster)
+#pragma unroll 2
     for (int jj = jseg0; jj < j; jj++) {
       const double *t = gasp(jj);
       const double taug = active ? t[0] : 1., taur = active ? t[NGPT] : 0.1;
