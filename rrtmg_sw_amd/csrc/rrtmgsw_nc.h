@@ -24,6 +24,7 @@ class NcFile {
   // Reads and parses the whole file; false with a message on any malformed or unsupported input.
   bool open(const std::string &path, std::string &err);
   const Var *var(const std::string &name) const;
+  const std::vector<Var> &vars() const { return vars_; }
   // Values start[i] .. start[i] + count[i] - 1 along each dimension (slowest first, 0-based), as
   // doubles in row-major order over `count` -- nc_get_vara_double. Integer and float types are
   // converted exactly; char variables are refused.

@@ -10,7 +10,9 @@
 """
 import os
 import re
+import shutil
 import struct
+import subprocess
 import sys
 
 import numpy as np
@@ -359,3 +361,32 @@ def test_reader_streaming_record_count(tmp_path):
     q.write_bytes(bytes(raw))
     assert np.array_equal(api.nc_get_vara(str(q), "a", [0, 0], [5, 3]), np.arange(15.0).reshape(5, 3))
     assert np.array_equal(api.nc_get_vara(str(q), "b", [2], [3]), np.array([14.0, 21.0, 28.0]))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
+def test_reader_survives_damaged_files(tmp_path):
+    """The reader, built with AddressSanitizer and UndefinedBehaviorSanitizer, on 4000 damaged
+    copies of a small k-table-like file: each is refused or read without a fault."""
+    from scipy.io import netcdf_file
+    src = tmp_path / "ok.nc"
+    f = netcdf_file(str(src), "w", version=2)
+    f.createDimension("rec", None)
+    f.createDimension("g", 16)
+    f.createDimension("k", 5)
+    f.title = "fuzz seed"
+    f.createVariable("kao", "d", ("k", "g"))[...] = np.arange(80.0).reshape(5, 16)
+    f.createVariable("n", "i", ("g",))[...] = np.arange(16)
+    f.createVariable("r", "f", ("rec", "k"))[:3] = np.ones((3, 5))
+    f.createVariable("s", "h", ("rec",))[:3] = [1, 2, 3]
+    f.close()
+    exe = str(tmp_path / "nc_fuzz_check")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-I", os.path.join(ROOT, "rrtmg_sw_amd", "csrc"),
+                    "-o", exe, os.path.join(ROOT, "tests", "nc_fuzz_check.cpp"),
+                    os.path.join(ROOT, "rrtmg_sw_amd", "csrc", "rrtmgsw_nc.cpp")], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:verify_asan_link_order=0")
+    r = subprocess.run([exe, str(src), str(tmp_path / "mut.nc"), "4000"], capture_output=True,
+                       text=True, timeout=300, env=env)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+    assert "mutations=4000" in r.stdout
